@@ -1,0 +1,176 @@
+"""Benchmark of the DynaMOSP update hot path on one MI355X.
+
+Workload (default): the road-like 1400x1400 grid (1.96M vertices, ~5.56M
+arcs: a random spanning tree kept, 30% of the other grid edges dropped, 2%
+diagonal shortcuts, both directions), K=3 int32 weights U[1,100], a batch of
+50,000 connectivity-safe changes (50% insertions), Pref=(1,1,1), source 0,
+all seeded.  The graph, the old trees, the batch and the outputs are resident
+in HBM before timing; one step = one MOSP update (K SOSP updates + combined
+graph + its SOSP search) in one persistent cooperative launch, host wall
+clock around the synchronous call (the reference's gpu_compute_ms scope).
+
+Prints ONE JSON line.  `roofline` uses the algorithmic bytes of one update
+(mosp_amd/roofline.py) over the HIP-event duration of the persistent kernel
+(recorded by the library on the launch stream).  `cpu_baseline` times the
+oracle's restatement of the reference's CPU path (sequential SOSP update per
+objective + host combined graph + Dijkstra) on one host core over the same
+workload, and its outputs are compared with the GPU's.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import pathlib
+import platform
+import sys
+import time
+
+import numpy as np
+
+ROOT = pathlib.Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT / "mosp-cuda_amd"))
+
+PEAK_HBM_GBS = 8000.0
+
+
+def log(msg: str) -> None:
+    print(msg, file=sys.stderr, flush=True)
+
+
+def cpu_model() -> str:
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return platform.processor() or "unknown"
+
+
+def cpu_baseline(g, b, d0, p0, gpu_result) -> dict:
+    """The oracle (CPU restatement of the reference's sequential path) on one
+    pinned core, over the same workload; also checks the GPU outputs."""
+    sys.path.insert(0, str(ROOT / "tests"))
+    import oracle_lib as O
+
+    try:
+        os.sched_setaffinity(0, {sorted(os.sched_getaffinity(0))[0]})
+    except (AttributeError, OSError):
+        pass
+    og = O.Graph(g.n, g.K, g.row, g.col, g.w)
+    ob = O.Batch(g.K, b.ins_from, b.ins_to, b.ins_w, b.del_from, b.del_to)
+    t0 = time.perf_counter()
+    r = O.mosp_update(og, ob, d0, p0, g.K, 0, [])
+    wall = (time.perf_counter() - t0) * 1e3
+    compute_ms = float(r["times_ms"][1] + r["times_ms"][2])  # updates + combined
+    same = all(np.array_equal(gpu_result[k], r[k]) for k in
+               ("dist", "parents", "comb_dist", "comb_parent"))
+    return {"value": (b.n_ins + b.n_del) / (compute_ms / 1e3),
+            "unit": "changes/s", "cores": 1, "kind": "port",
+            "sample": ("full workload once: K=3 sequential SOSP updates (Phases 1-2) + host "
+                       "combined graph + Dijkstra, apply excluded "
+                       f"(apply {r['times_ms'][0]:.1f} ms, updates {r['times_ms'][1]:.1f} ms, "
+                       f"combined {r['times_ms'][2]:.1f} ms, wall {wall:.0f} ms)"),
+            "compute_ms": compute_ms, "cpu": cpu_model(),
+            "host_threads_visible": len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity")
+            else os.cpu_count(),
+            "outputs_equal_gpu": bool(same)}
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--width", type=int, default=1400)
+    ap.add_argument("--height", type=int, default=1400)
+    ap.add_argument("--changes", type=int, default=50_000)
+    ap.add_argument("--grid", type=int, default=0, help="workgroups of the persistent launch")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--traffic", default=str(ROOT / "profiles" / "traffic_road1400_k3.json"),
+                    help="measured HBM bytes per launch (rocprofv3 FETCH_SIZE/WRITE_SIZE)")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one GPU: this build runs the update on a single MI355X "
+              "(multi-GPU sharding is described, not built); use --gpus 1", file=sys.stderr)
+        return 2
+
+    import torch
+
+    import mosp_amd as M
+    from mosp_amd.roofline import alg_bytes
+
+    torch.cuda.set_device(0)
+    t = time.perf_counter()
+    g = M.gen_road_grid(args.width, args.height, 3, seed=7)
+    b = M.gen_batch(g, args.changes, 50.0, seed=8, safe=True)
+    log(f"workload: n={g.n} m={g.m} K=3 inserts={b.n_ins} deletes={b.n_del} "
+        f"({time.perf_counter() - t:.1f} s to generate)")
+    d0, p0 = M.initial_trees(g, 3)
+    sess = M.Session(g, b, d0, p0, grid=args.grid)
+    log(f"resident in HBM; persistent grid = {sess.ws.grid} workgroups x 256")
+
+    for _ in range(args.warmup):
+        sess.run()
+    torch.cuda.synchronize()
+    kernel_ms, sosp_ms, comb_ms = [], [], []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        sess.run()
+        kernel_ms.append(sess.stats[0].kernel_ms)
+        sosp_ms.append(sess.cstats.sosp_phase_ms)
+        comb_ms.append(sess.cstats.combined_phase_ms)
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    ms_per_step = elapsed * 1e3 / args.steps
+    changes = b.n_ins + b.n_del
+    res = sess.results()
+    log(f"{ms_per_step:.3f} ms/step; kernel {np.mean(kernel_ms):.3f} ms "
+        f"(K updates {np.mean(sosp_ms):.3f} ms, combined {np.mean(comb_ms):.3f} ms); "
+        f"stats {res['stats']}")
+
+    # Algorithmic bytes of one update.
+    changed = [M.changed_lists(b, sess.mask, k) for k in range(3)]
+    in_row = np.zeros(g.n + 1, np.int64)
+    np.add.at(in_row, sess.updated.col.astype(np.int64) + 1, 1)
+    in_row = np.cumsum(in_row)
+    ab = alg_bytes(g.n, sess.updated.row, in_row, 0, p0, d0, res["dist"], changed,
+                   b.ins_to, res["combined_edges"])
+    k_ms = float(np.mean(kernel_ms))
+    achieved = ab / (k_ms * 1e-3) / 1e9
+    traffic = None
+    tp = pathlib.Path(args.traffic)
+    if tp.exists():
+        traffic = json.loads(tp.read_text()).get("bytes_per_launch")
+
+    line = {
+        "metric": "changes/s + gpu_compute_ms, 2M-vertex grid K=3 50k batch; %HBM peak; p99 ms",
+        "value": changes / (ms_per_step / 1e3),
+        "unit": "changes/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "gpu_compute_ms": ms_per_step,
+        "kernel_ms": k_ms,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "int64", "data": "synthetic (seeded road-like grid + batch, GPU-computed "
+                                  "canonical initial trees)",
+        "config": {"workload": f"road-like grid {args.width}x{args.height} "
+                               f"(n={g.n}, m={g.m}), K=3 w~U[1,100], {changes} safe changes "
+                               "(50% inserts), Pref=(1,1,1), source 0",
+                   "n": g.n, "m": g.m, "K": 3, "batch": changes,
+                   "grid_workgroups": sess.ws.grid},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS,
+                     "unit": "GB/s", "frac": achieved / PEAK_HBM_GBS, "traffic": traffic,
+                     "alg_bytes": ab, "kernel": "sosp_persistent (one launch per update)"},
+    }
+    if not args.no_cpu_baseline:
+        log("cpu baseline: oracle CPU path on one core (about 20 s) ...")
+        line["cpu_baseline"] = cpu_baseline(g, b, d0, p0, res)
+    sess.close()
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,261 @@
+/*
+ * mosp_hip.h -- C ABI of the MI355X (gfx950) DynaMOSP update engine.
+ *
+ * Plain C: POD structs, raw pointers and sizes, int status codes (1 = ok,
+ * 0 = error with the message in mosp_last_error()).  Device pointers are HIP
+ * device addresses (hipMalloc or any allocator); `stream` is a hipStream_t
+ * passed as void* (NULL = the default stream).  Every compute call is
+ * synchronous on its stream, like the reference's entry points, and returns
+ * after its results are in device memory.
+ *
+ * Each declaration names the reference interface it replaces.  The
+ * reference-named C++ API (csrGraph.h, sospUpdateGpu.h, combinedGraphGpu.h,
+ * deviceGraph.h, mospUpdate.h in this directory) is a thin layer over these.
+ */
+#ifndef MOSP_HIP_H
+#define MOSP_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---------------------------------------------------------------- types -- */
+
+/* One-weight CSR view in device memory.
+ * Replaces DeviceCsr (headers/sospUpdateGpu.cuh:14-20). */
+typedef struct mosp_csr {
+  int32_t n;                 /* vertices */
+  int32_t m;                 /* edges */
+  const int32_t *row_ptr;    /* n + 1 */
+  const int32_t *col_ind;    /* m */
+  const int32_t *weights;    /* m (one objective) */
+} mosp_csr;
+
+/* The change batch as seen by one objective, device memory.
+ * Replaces DeviceChanges (headers/sospUpdateGpu.cuh:22-32). */
+typedef struct mosp_changes {
+  const int32_t *changed_from;  /* deleted or weight-increased edges (u,v) */
+  const int32_t *changed_to;
+  int32_t n_changed;
+  const int32_t *insert_heads;  /* heads of inserted edges */
+  int32_t n_insert_heads;
+} mosp_changes;
+
+/* Counters of one update.  Replaces SospStats (sospUpdateGpu.cuh:35-45);
+ * kernel_ms is new (HIP-event time of the persistent launch). */
+typedef struct mosp_sosp_stats {
+  int32_t invalidated;
+  int32_t jump_rounds;
+  int32_t iterations;
+  int32_t epochs;
+  int64_t pushes;
+  int32_t packed_parents;    /* 0: distance-only words + parent recovery */
+  float kernel_ms;
+} mosp_sosp_stats;
+
+/* Replaces CombineStats (headers/combinedGraphGpu.cuh:15-20). */
+typedef struct mosp_combine_stats {
+  int32_t n_edges;
+  int64_t scale;             /* L: combined distances in units of 1/L */
+  int64_t delta;
+  mosp_sosp_stats search;
+  /* Device-clock split of a fused launch (mosp_update_fused): time of the
+   * K updates and of the combined step inside the one kernel. */
+  float sosp_phase_ms;
+  float combined_phase_ms;
+} mosp_combine_stats;
+
+/* Device scratch of the engine; replaces SospWorkspace
+ * (sospUpdateGpu.cuh:53-78) and CombineWorkspace (combinedGraphGpu.cuh:25-46). */
+typedef struct mosp_ws mosp_ws;
+
+/* A multi-objective graph resident in HBM: out-CSR + in-CSR with one weight
+ * column per objective.  Replaces DeviceGraph (headers/deviceGraph.cuh:16-27). */
+typedef struct mosp_device_graph mosp_device_graph;
+
+/* ------------------------------------------------------------ errors ----- */
+
+/* Message of the last failed call on this thread ("" if none). */
+const char *mosp_last_error(void);
+
+/* --------------------------------------------------------- workspace ----- */
+
+/* Replaces SospWorkspace::SospWorkspace / reserve / release
+ * (src/sospUpdateGpu.cu:527-595).  slots = objectives updated per launch. */
+mosp_ws *mosp_ws_create(void);
+int mosp_ws_reserve(mosp_ws *ws, int32_t capacity, int32_t slots);
+void mosp_ws_destroy(mosp_ws *ws);
+/* Co-resident workgroups of the persistent launch (0 = occupancy maximum). */
+int mosp_ws_set_grid(mosp_ws *ws, int32_t workgroups);
+int32_t mosp_ws_grid(const mosp_ws *ws);
+
+/* Replaces defaultDelta (src/sospUpdateGpu.cu:607-616). */
+int64_t mosp_default_delta(int64_t m, int32_t n, int64_t weight_sum);
+
+/* --------------------------------------------------------- SOSP engine --- */
+
+/* Incremental SOSP update of one objective; d_dist / d_parent hold the old
+ * canonical tree on entry and the new one on return.
+ * Replaces sospUpdateGpu (src/sospUpdateGpu.cu:685-717). */
+int mosp_sosp_update(const mosp_csr *out, const mosp_csr *in,
+                     const mosp_changes *changes, int32_t source, int64_t delta,
+                     int64_t max_weight, mosp_ws *ws, int64_t *d_dist,
+                     int32_t *d_parent, mosp_sosp_stats *stats, void *stream);
+
+/* K objectives updated in ONE persistent launch (no reference counterpart;
+ * it replaces the K serial sospUpdateGpu calls of mospUpdate.cu:170-194).
+ * Arrays of length k; d_dist_in may equal d_dist_out (in place). */
+int mosp_sosp_update_multi(int32_t k, const mosp_csr *out, const mosp_csr *in,
+                           const mosp_changes *changes, int32_t source,
+                           const int64_t *delta, const int64_t *max_weight,
+                           mosp_ws *ws, const int64_t *const *d_dist_in,
+                           const int32_t *const *d_parent_in,
+                           int64_t *const *d_dist_out,
+                           int32_t *const *d_parent_out, mosp_sosp_stats *stats,
+                           void *stream);
+
+/* Single-source shortest paths from scratch (canonical tree).
+ * Replaces sospFromScratchGpu (src/sospUpdateGpu.cu:719-745). */
+int mosp_sosp_from_scratch(const mosp_csr *out, int32_t source, int64_t delta,
+                           int64_t max_weight, mosp_ws *ws, int64_t *d_dist,
+                           int32_t *d_parent, mosp_sosp_stats *stats,
+                           void *stream);
+
+/* Combined graph of K trees (d_parents objective-major, K*n) and its SOSP
+ * tree; delta <= 0 selects the default.  pref: K values >= 1, or n_pref = 0
+ * for all ones.  Replaces combinedGraphSospGpu (src/combinedGraphGpu.cu:251-317). */
+int mosp_combined_sosp(const int32_t *d_parents, int32_t n, int32_t k,
+                       const int32_t *pref, int32_t n_pref, int32_t source,
+                       int64_t delta, mosp_ws *ws, int64_t *d_dist,
+                       int32_t *d_parent, mosp_combine_stats *stats,
+                       void *stream);
+
+/* The whole device part of the MOSP update in ONE launch: K objective updates
+ * + combined graph + its search.  Replaces the K sospUpdateGpu calls and the
+ * combinedGraphSospGpu call of mospUpdate (src/mospUpdate.cu:167-209).
+ * d_dist/d_parents: K*n objective-major, old trees in (unless d_dist_in /
+ * d_parents_in are given: then those are read and d_dist/d_parents written),
+ * new trees out. */
+int mosp_update_fused(const mosp_device_graph *g, int32_t k,
+                      const mosp_changes *changes, int32_t source,
+                      const int64_t *delta, const int64_t *max_weight,
+                      const int32_t *pref, int32_t n_pref, mosp_ws *ws,
+                      const int64_t *d_dist_in, const int32_t *d_parents_in,
+                      int64_t *d_dist, int32_t *d_parents, int64_t *d_comb_dist,
+                      int32_t *d_comb_parent, mosp_sosp_stats *stats,
+                      mosp_combine_stats *cstats, void *stream);
+
+/* ------------------------------------------------------ device graph ----- */
+
+/* Copy a host CSR (edge-major weights w[e*K + k], the Values.txt layout) to
+ * HBM once and build the objective-major weight columns and the reverse
+ * CSR there.  Replaces uploadDeviceGraph (src/deviceGraph.cu:87-135). */
+mosp_device_graph *mosp_device_graph_upload(int32_t n, int32_t k,
+                                            const int32_t *row_ptr,
+                                            const int32_t *col_ind,
+                                            const int32_t *weights);
+void mosp_device_graph_free(mosp_device_graph *g);
+/* Views of objective `obj`.  Replaces DeviceGraph::out/in
+ * (src/deviceGraph.cu:67-85). */
+int mosp_device_graph_view(const mosp_device_graph *g, int32_t obj,
+                           mosp_csr *out, mosp_csr *in);
+int32_t mosp_device_graph_objectives(const mosp_device_graph *g);
+
+/* MOSP path costs on the GPU: costs[v*K + k] = objective-k length of the
+ * MOSP tree path to v (first (p,v) edge of row p), DISTANCE_INF if
+ * unreachable; K = all objectives of the graph.  Fails if a tree edge is
+ * missing.  Replaces mospPathCosts (src/combinedGraphGpu.cu:161-208), which
+ * runs on the host in the reference. */
+int mosp_path_costs_gpu(const mosp_device_graph *g, const int32_t *d_parent,
+                        int32_t source, int64_t *d_costs, void *stream);
+
+/* --------------------------------------------------------- host helpers --- */
+
+/* Replaces preferenceScale (src/combinedGraphGpu.cu:129-147). */
+int64_t mosp_preference_scale(const int32_t *pref, int32_t n_pref, int32_t k);
+/* Replaces combinedEdgeWeight (src/combinedGraphGpu.cu:149-159). */
+int64_t mosp_combined_edge_weight(uint32_t tree_mask, const int32_t *pref,
+                                  int32_t n_pref, int32_t k, int64_t scale);
+
+/* Packing chosen for n vertices and max weight (src/sospUpdateGpu.cu:127-143,
+ * 670-681): returns 0 if (n-1)*max_weight exceeds 62 bits, else 1 with
+ * *parent_bits (0 = distance-only) and *bound = (n-1)*max_weight. */
+int mosp_choose_packing(int32_t n, int64_t max_weight, int32_t *parent_bits,
+                        uint64_t *bound);
+
+/* Apply a change batch on the host (src/csrGraph.cu:727-894).  Output arrays
+ * need capacity m + n_ins (col) and (m + n_ins)*K (w); inc_mask[n_ins].
+ * Returns the updated edge count, or -1 on error. */
+int64_t mosp_apply_batch(int32_t n, int32_t k, const int32_t *row_ptr,
+                         const int32_t *col_ind, const int32_t *weights,
+                         int32_t n_ins, const int32_t *ins_from,
+                         const int32_t *ins_to, const int32_t *ins_w,
+                         int32_t n_del, const int32_t *del_from,
+                         const int32_t *del_to, int32_t *out_row_ptr,
+                         int32_t *out_col_ind, int32_t *out_weights,
+                         uint32_t *inc_mask);
+
+/* The in-memory MOSP update on host arrays (src/mospUpdate.cu:57-225):
+ * apply, upload, K updates + combined step on the GPU, download.
+ * Inputs: original graph (edge-major KG weights), batch, K*n initial trees
+ * (objective-major), options.  Outputs (caller-allocated): updated graph
+ * (capacity m + n_ins), K*n trees, combined dist/parent (n).
+ * fused != 0 runs the K updates and the combined step in one launch;
+ * fused == 0 runs K separate launches + one combined launch like the
+ * reference.  timings_ms[5] = apply, upload, sosp (all K), combined,
+ * download; gpu_compute = timings_ms[2] + timings_ms[3].
+ * stats[k] per objective; cstats for the combined step. */
+int mosp_update_host(int32_t n, int32_t kg, const int32_t *row_ptr,
+                     const int32_t *col_ind, const int32_t *weights,
+                     int32_t n_ins, const int32_t *ins_from,
+                     const int32_t *ins_to, const int32_t *ins_w, int32_t n_del,
+                     const int32_t *del_from, const int32_t *del_to,
+                     int32_t k_use, int32_t source, const int32_t *pref,
+                     int32_t n_pref, int64_t delta, int32_t fused,
+                     const int64_t *init_dist, const int32_t *init_parents,
+                     int64_t *out_m, int32_t *upd_row_ptr, int32_t *upd_col_ind,
+                     int32_t *upd_weights, int64_t *dist, int32_t *parents,
+                     int64_t *comb_dist, int32_t *comb_parent,
+                     mosp_sosp_stats *stats, mosp_combine_stats *cstats,
+                     double *timings_ms);
+
+/* ----------------------------------------------------- synthetic inputs --- */
+/* Seeded generators (xoshiro256** + unbiased bounded integers): identical
+ * instances on every host.  No reference counterpart (the reference's own
+ * generators live in src/changeGenerator.cu and src/mospTest.cu:64-106). */
+
+/* Road-like W x H grid: random spanning tree kept, `drop` of the grid edges
+ * removed, `diag` * n diagonal shortcuts added; both directions stored.
+ * Call with row_ptr == NULL to get *n / *m, then with buffers of capacity
+ * >= *m edges (weights edge-major, K per edge). */
+int mosp_gen_road_grid(int32_t width, int32_t height, int32_t k, int32_t wmin,
+                       int32_t wmax, double drop, double diag, uint64_t seed,
+                       int32_t *n, int64_t *m, int32_t *row_ptr,
+                       int32_t *col_ind, int32_t *weights, int64_t capacity);
+
+/* R-MAT (a, b, c; d = 1-a-b-c), 2^scale vertices, edge_factor * 2^scale
+ * draws, self-loops and duplicates removed.  Two-call protocol as above. */
+int mosp_gen_rmat(int32_t scale, int32_t edge_factor, double a, double b,
+                  double c, int32_t k, int32_t wmin, int32_t wmax, uint64_t seed,
+                  int32_t *n, int64_t *m, int32_t *row_ptr, int32_t *col_ind,
+                  int32_t *weights, int64_t capacity);
+
+/* Change batch: round(changes * ins_pct / 100) insertions of random pairs
+ * u != v (weights in [wmin, wmax]) and the rest deletions of distinct
+ * existing arcs; safe != 0 keeps every vertex reachable from `source`
+ * (src/changeGenerator.cu:78-116 rule).  Buffers: ins_from/ins_to n_ins,
+ * ins_w n_ins * K, del_* n_del (capacity = changes). */
+int mosp_gen_batch(int32_t n, int32_t k, const int32_t *row_ptr,
+                   const int32_t *col_ind, const int32_t *weights,
+                   int32_t changes, double ins_pct, int32_t safe, int32_t wmin,
+                   int32_t wmax, uint64_t seed, int32_t source, int32_t *n_ins,
+                   int32_t *ins_from, int32_t *ins_to, int32_t *ins_w,
+                   int32_t *n_del, int32_t *del_from, int32_t *del_to);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MOSP_HIP_H */

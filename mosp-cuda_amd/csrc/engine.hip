@@ -1,0 +1,1059 @@
+// engine.hip -- host side of the gfx950 engine and the C ABI (mosp_hip.h).
+//
+// Workspace management, descriptor set-up and the cooperative launch of the
+// persistent kernel (sosp_kernel.hip); the device-graph build (reverse CSR
+// and objective-major weight columns, src/deviceGraph.cu:28-135 in the
+// reference) and the GPU path-cost walk (the reference's host
+// mospPathCosts, src/combinedGraphGpu.cu:161-208, moved to the device).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <climits>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "engine.hpp"
+#include "host_api.hpp"
+#include "mosp_hip.h"
+
+namespace mosp {
+const void *sosp_kernel_symbol();
+hipError_t launch_ctl_reset(Ctl *ctl, hipStream_t stream);
+}  // namespace mosp
+
+using namespace mosp;
+
+// ---------------------------------------------------------------------------
+// Errors
+// ---------------------------------------------------------------------------
+
+namespace {
+
+thread_local std::string g_error;
+
+int fail(const std::string &message) {
+  g_error = message;
+  return 0;
+}
+
+#define HIP_TRY(call)                                                      \
+  do {                                                                     \
+    hipError_t err_ = (call);                                              \
+    if (err_ != hipSuccess) {                                              \
+      return fail(std::string("HIP error: ") + hipGetErrorString(err_) +   \
+                  " at " + __FILE__ + ":" + std::to_string(__LINE__));     \
+    }                                                                      \
+  } while (0)
+
+hipStream_t as_stream(void *s) { return static_cast<hipStream_t>(s); }
+
+size_t align256(size_t bytes) { return (bytes + 255) & ~static_cast<size_t>(255); }
+
+}  // namespace
+
+extern "C" const char *mosp_last_error(void) { return g_error.c_str(); }
+
+// ---------------------------------------------------------------------------
+// Workspace
+// ---------------------------------------------------------------------------
+
+struct mosp_ws {
+  int capacity = 0;
+  int slots = 0;
+  void *slab = nullptr;
+  std::vector<ObjState> state;
+  ObjDesc *dDesc = nullptr;  // MOSP_MAX_OBJ + 1 descriptors
+  ObjDesc *hDesc = nullptr;  // pinned staging
+  Ctl *dCtl = nullptr;
+  Ctl *hCtl = nullptr;       // pinned readback
+  int *genDev = nullptr;
+  // combined graph
+  int cCapacity = 0;
+  int cK = 0;
+  int *cDegree = nullptr, *cRowPtr = nullptr, *cColInd = nullptr,
+      *cWeights = nullptr, *cPartials = nullptr;
+  double clockKhz = 100000;  // wall_clock64 rate
+  int maxBlocks = 0;        // cooperative residency limit
+  int gridBlocks = 0;       // workgroups per launch
+  int gridOverride = 0;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+};
+
+namespace {
+
+void ws_release_slab(mosp_ws *ws) {
+  hipFree(ws->slab);
+  ws->slab = nullptr;
+  ws->state.clear();
+  ws->capacity = 0;
+  ws->slots = 0;
+}
+
+void ws_release_combined(mosp_ws *ws) {
+  hipFree(ws->cDegree);
+  hipFree(ws->cRowPtr);
+  hipFree(ws->cColInd);
+  hipFree(ws->cWeights);
+  ws->cDegree = ws->cRowPtr = ws->cColInd = ws->cWeights = nullptr;
+  ws->cCapacity = 0;
+  ws->cK = 0;
+}
+
+int ws_init(mosp_ws *ws) {
+  if (ws->dCtl != nullptr) {
+    return 1;
+  }
+  int device = 0, cooperative = 0, cus = 0, perCu = 0;
+  HIP_TRY(hipGetDevice(&device));
+  HIP_TRY(hipDeviceGetAttribute(&cooperative, hipDeviceAttributeCooperativeLaunch,
+                                device));
+  if (!cooperative) {
+    return fail("Error: the GPU does not support cooperative launches.");
+  }
+  HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount,
+                                device));
+  HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(
+      &perCu, reinterpret_cast<const void *>(sosp_kernel_symbol()), BLOCK, 0));
+  if (perCu < 1) {
+    return fail("Error: the persistent kernel does not fit on a CU.");
+  }
+  ws->maxBlocks = perCu * cus;
+  int clockKhz = 0;
+  if (hipDeviceGetAttribute(&clockKhz, hipDeviceAttributeWallClockRate, device) ==
+          hipSuccess &&
+      clockKhz > 0) {
+    ws->clockKhz = clockKhz;
+  }
+  // Default: two workgroups per CU -- enough lanes for the frontier sweeps
+  // while keeping the grid barrier (one arrival per workgroup) cheap.
+  int blocks = std::min(perCu, 2) * cus;
+  if (const char *env = std::getenv("MOSP_GRID")) {
+    blocks = std::atoi(env);
+  }
+  if (ws->gridOverride > 0) {
+    blocks = ws->gridOverride;
+  }
+  ws->gridBlocks = std::max(1, std::min(blocks, ws->maxBlocks));
+  HIP_TRY(hipMalloc(&ws->dDesc, sizeof(ObjDesc) * (MOSP_MAX_OBJ + 1)));
+  HIP_TRY(hipHostMalloc(&ws->hDesc, sizeof(ObjDesc) * (MOSP_MAX_OBJ + 1),
+                        hipHostMallocDefault));
+  HIP_TRY(hipMalloc(&ws->dCtl, sizeof(Ctl)));
+  HIP_TRY(hipHostMalloc(&ws->hCtl, sizeof(Ctl), hipHostMallocDefault));
+  HIP_TRY(hipMalloc(&ws->genDev, sizeof(int)));
+  HIP_TRY(hipMemset(ws->genDev, 0, sizeof(int)));
+  HIP_TRY(hipMalloc(&ws->cPartials, sizeof(int) * ws->maxBlocks));
+  HIP_TRY(hipEventCreate(&ws->ev0));
+  HIP_TRY(hipEventCreate(&ws->ev1));
+  return 1;
+}
+
+int ws_reserve(mosp_ws *ws, int capacity, int slots) {
+  if (!ws_init(ws)) {
+    return 0;
+  }
+  slots = std::max(slots, 1);
+  if (capacity <= ws->capacity && slots <= ws->slots) {
+    return 1;
+  }
+  capacity = std::max(capacity, ws->capacity);
+  slots = std::max(slots, ws->slots);
+  ws_release_slab(ws);
+  const size_t n = static_cast<size_t>(std::max(capacity, 1));
+  const size_t wordBytes = align256(n * sizeof(u64));
+  const size_t intBytes = align256(n * sizeof(int));
+  const size_t perSlot = wordBytes + 9 * intBytes;
+  HIP_TRY(hipMalloc(&ws->slab, perSlot * slots));
+  // stamps, far flags and invalidation marks must start at zero
+  HIP_TRY(hipMemset(ws->slab, 0, perSlot * slots));
+  char *p = static_cast<char *>(ws->slab);
+  ws->state.resize(slots);
+  for (int s = 0; s < slots; ++s) {
+    ObjState &st = ws->state[s];
+    st.packed = reinterpret_cast<u64 *>(p);
+    p += wordBytes;
+    int **ints[9] = {&st.stamp, &st.inFar, &st.flag, &st.ancestor, &st.near[0],
+                     &st.near[1], &st.far[0], &st.far[1], &st.cand};
+    for (int i = 0; i < 9; ++i) {
+      *ints[i] = reinterpret_cast<int *>(p);
+      p += intBytes;
+    }
+  }
+  ws->capacity = capacity;
+  ws->slots = slots;
+  return 1;
+}
+
+int ws_reserve_combined(mosp_ws *ws, int n, int K) {
+  if (n <= ws->cCapacity && K <= ws->cK) {
+    return 1;
+  }
+  ws_release_combined(ws);
+  const size_t nn = static_cast<size_t>(std::max(n, 1));
+  HIP_TRY(hipMalloc(&ws->cDegree, (nn + 1) * sizeof(int)));
+  HIP_TRY(hipMalloc(&ws->cRowPtr, (nn + 1) * sizeof(int)));
+  HIP_TRY(hipMalloc(&ws->cColInd, nn * K * sizeof(int)));
+  HIP_TRY(hipMalloc(&ws->cWeights, nn * K * sizeof(int)));
+  ws->cCapacity = n;
+  ws->cK = K;
+  return 1;
+}
+
+// Packing of (distance << b | parent) words for n vertices
+// (src/sospUpdateGpu.cu:127-143, 670-681).
+struct PackChoice {
+  int bits;
+  u64 noParent;
+  u64 bound;
+};
+
+int choose_packing(int n, long long maxWeight, PackChoice &pc) {
+  const u64 outputMax = static_cast<u64>(DISTANCE_INF / 2 - 1);
+  const u64 weight = static_cast<u64>(std::max(maxWeight, 1LL));
+  const u64 hops = static_cast<u64>(std::max(n - 1, 1));
+  if (weight > outputMax / hops) {
+    return fail("Error: distances up to " + std::to_string(weight) + " * " +
+                std::to_string(hops) + " do not fit in 62 bits.");
+  }
+  pc.bound = weight * hops;
+  int bits = 1;
+  while ((1ULL << bits) - 1 < static_cast<u64>(n)) {
+    ++bits;
+  }
+  const u64 maxDistance = (PACKED_INF >> bits) - 1;
+  if (pc.bound <= maxDistance) {
+    pc.bits = bits;
+    pc.noParent = (1ULL << bits) - 1;
+  } else {
+    pc.bits = 0;
+    pc.noParent = 0;
+  }
+  return 1;
+}
+
+int jump_rounds(int n) {
+  int rounds = 0;
+  while ((1LL << rounds) < n) {
+    ++rounds;
+  }
+  return rounds + 1;
+}
+
+void fill_stats(const Ctl &c, int k, const ObjDesc &d, float ms,
+                mosp_sosp_stats *stats) {
+  if (stats == nullptr) {
+    return;
+  }
+  const ObjCtl &o = c.obj[k];
+  stats->invalidated = static_cast<int32_t>(o.invalidated);
+  stats->jump_rounds = static_cast<int32_t>(c.rounds);
+  stats->iterations = static_cast<int32_t>(o.iterations);
+  stats->epochs = static_cast<int32_t>(o.epochs);
+  stats->pushes = static_cast<int64_t>(o.pushes);
+  stats->packed_parents = d.parentBits > 0 ? 1 : 0;
+  stats->kernel_ms = ms;
+}
+
+// Upload descriptors, reset the control block, run the cooperative launch,
+// read the control block back.  Synchronous on `stream`.
+int run_launch(mosp_ws *ws, LaunchArgs &args, int descCount,
+               hipStream_t stream, float *kernelMs) {
+  args.ctl = ws->dCtl;
+  args.genDev = ws->genDev;
+  args.cPartials = ws->cPartials;
+  if (descCount > 0) {
+    HIP_TRY(hipMemcpyAsync(ws->dDesc, ws->hDesc, sizeof(ObjDesc) * descCount,
+                           hipMemcpyHostToDevice, stream));
+  }
+  HIP_TRY(launch_ctl_reset(ws->dCtl, stream));
+  void *params[] = {&args};
+  HIP_TRY(hipEventRecord(ws->ev0, stream));
+  HIP_TRY(hipLaunchCooperativeKernel(sosp_kernel_symbol(),
+                                     dim3(ws->gridBlocks), dim3(BLOCK), params,
+                                     0, stream));
+  HIP_TRY(hipEventRecord(ws->ev1, stream));
+  HIP_TRY(hipMemcpyAsync(ws->hCtl, ws->dCtl, sizeof(Ctl),
+                         hipMemcpyDeviceToHost, stream));
+  HIP_TRY(hipStreamSynchronize(stream));
+  float ms = 0;
+  HIP_TRY(hipEventElapsedTime(&ms, ws->ev0, ws->ev1));
+  if (kernelMs != nullptr) {
+    *kernelMs = ms;
+  }
+  for (int k = 0; k < args.nobj; ++k) {
+    if (ws->hCtl->obj[k].overflow) {
+      return fail("Error: an input distance is negative or larger than "
+                  "(n - 1) * maxWeight; the initial tree does not belong to "
+                  "this graph.");
+    }
+  }
+  return 1;
+}
+
+ObjDesc make_desc(const mosp_csr &out, const mosp_csr *in,
+                  const mosp_changes *ch, const PackChoice &pc, u64 delta,
+                  const ObjState &st) {
+  ObjDesc d{};
+  d.outRow = out.row_ptr;
+  d.outCol = out.col_ind;
+  d.outW = out.weights;
+  if (in != nullptr) {
+    d.inRow = in->row_ptr;
+    d.inCol = in->col_ind;
+    d.inW = in->weights;
+  }
+  if (ch != nullptr) {
+    d.chFrom = ch->changed_from;
+    d.chTo = ch->changed_to;
+    d.nChanged = ch->n_changed;
+    d.heads = ch->insert_heads;
+    d.nHeads = ch->n_insert_heads;
+  }
+  d.parentBits = pc.bits;
+  d.noParent = pc.noParent;
+  d.maxDistance = pc.bound;
+  d.delta = delta;
+  d.s = st;
+  return d;
+}
+
+}  // namespace
+
+extern "C" mosp_ws *mosp_ws_create(void) { return new mosp_ws(); }
+
+extern "C" int mosp_ws_reserve(mosp_ws *ws, int32_t capacity, int32_t slots) {
+  return ws_reserve(ws, capacity, slots);
+}
+
+extern "C" void mosp_ws_destroy(mosp_ws *ws) {
+  if (ws == nullptr) {
+    return;
+  }
+  ws_release_slab(ws);
+  ws_release_combined(ws);
+  hipFree(ws->dDesc);
+  hipHostFree(ws->hDesc);
+  hipFree(ws->dCtl);
+  hipHostFree(ws->hCtl);
+  hipFree(ws->genDev);
+  hipFree(ws->cPartials);
+  if (ws->ev0) hipEventDestroy(ws->ev0);
+  if (ws->ev1) hipEventDestroy(ws->ev1);
+  delete ws;
+}
+
+extern "C" int mosp_ws_set_grid(mosp_ws *ws, int32_t workgroups) {
+  ws->gridOverride = workgroups;
+  if (ws->maxBlocks > 0) {
+    int blocks = workgroups > 0 ? workgroups : ws->maxBlocks;
+    ws->gridBlocks = std::max(1, std::min(blocks, ws->maxBlocks));
+  }
+  return 1;
+}
+
+extern "C" int32_t mosp_ws_grid(const mosp_ws *ws) { return ws->gridBlocks; }
+
+extern "C" int64_t mosp_default_delta(int64_t m, int32_t n, int64_t weightSum) {
+  if (m <= 0 || n <= 0) {
+    return 1;
+  }
+  const double avgW = static_cast<double>(weightSum) / static_cast<double>(m);
+  const double avgDeg = static_cast<double>(m) / static_cast<double>(n);
+  return std::max<int64_t>(1, static_cast<int64_t>(32.0 * avgW / avgDeg));
+}
+
+extern "C" int mosp_choose_packing(int32_t n, int64_t maxWeight,
+                                   int32_t *parentBits, uint64_t *bound) {
+  PackChoice pc{};
+  if (!choose_packing(n, maxWeight, pc)) {
+    return 0;
+  }
+  *parentBits = pc.bits;
+  *bound = pc.bound;
+  return 1;
+}
+
+// ---------------------------------------------------------------------------
+// SOSP update / from scratch
+// ---------------------------------------------------------------------------
+
+extern "C" int mosp_sosp_update_multi(
+    int32_t K, const mosp_csr *out, const mosp_csr *in,
+    const mosp_changes *changes, int32_t source, const int64_t *delta,
+    const int64_t *maxWeight, mosp_ws *ws, const int64_t *const *distIn,
+    const int32_t *const *parentIn, int64_t *const *distOut,
+    int32_t *const *parentOut, mosp_sosp_stats *stats, void *stream) {
+  g_error.clear();
+  if (K <= 0 || K > MOSP_MAX_OBJ) {
+    return fail("Error: 1..32 objectives per launch.");
+  }
+  const int n = out[0].n;
+  for (int k = 0; k < K; ++k) {
+    if (stats != nullptr) {
+      stats[k] = mosp_sosp_stats{};
+      stats[k].packed_parents = 1;
+    }
+  }
+  if (n == 0) {
+    return 1;
+  }
+  if (source < 0 || source >= n) {
+    return fail("Error: source vertex out of range.");
+  }
+  if (!ws_reserve(ws, n, K)) {
+    return 0;
+  }
+  int anyFallback = 0;
+  for (int k = 0; k < K; ++k) {
+    if (delta[k] <= 0) {
+      return fail("Error: the near-far bucket width must be positive.");
+    }
+    PackChoice pc{};
+    if (!choose_packing(n, maxWeight[k], pc)) {
+      return 0;
+    }
+    ObjDesc d = make_desc(out[k], &in[k], &changes[k], pc,
+                          static_cast<u64>(delta[k]), ws->state[k]);
+    d.distIn = distIn[k];
+    d.parentIn = parentIn[k];
+    d.distOut = distOut[k];
+    d.parentOut = parentOut[k];
+    ws->hDesc[k] = d;
+    anyFallback |= pc.bits == 0 ? 1 : 0;
+  }
+  LaunchArgs args{};
+  args.objs = ws->dDesc;
+  args.nobj = K;
+  args.n = n;
+  args.source = source;
+  args.phases = PH_UPDATE;
+  args.maxRounds = jump_rounds(n);
+  args.anyFallback = anyFallback;
+  float ms = 0;
+  if (!run_launch(ws, args, K, as_stream(stream), &ms)) {
+    return 0;
+  }
+  for (int k = 0; k < K; ++k) {
+    fill_stats(*ws->hCtl, k, ws->hDesc[k], ms, stats ? &stats[k] : nullptr);
+  }
+  return 1;
+}
+
+extern "C" int mosp_sosp_update(const mosp_csr *out, const mosp_csr *in,
+                                const mosp_changes *changes, int32_t source,
+                                int64_t delta, int64_t maxWeight, mosp_ws *ws,
+                                int64_t *d_dist, int32_t *d_parent,
+                                mosp_sosp_stats *stats, void *stream) {
+  const int64_t *dIn[1] = {d_dist};
+  const int32_t *pIn[1] = {d_parent};
+  int64_t *dOut[1] = {d_dist};
+  int32_t *pOut[1] = {d_parent};
+  return mosp_sosp_update_multi(1, out, in, changes, source, &delta, &maxWeight,
+                                ws, dIn, pIn, dOut, pOut, stats, stream);
+}
+
+extern "C" int mosp_sosp_from_scratch(const mosp_csr *out, int32_t source,
+                                      int64_t delta, int64_t maxWeight,
+                                      mosp_ws *ws, int64_t *d_dist,
+                                      int32_t *d_parent, mosp_sosp_stats *stats,
+                                      void *stream) {
+  g_error.clear();
+  const int n = out->n;
+  if (stats != nullptr) {
+    *stats = mosp_sosp_stats{};
+    stats->packed_parents = 1;
+  }
+  if (n == 0) {
+    return 1;
+  }
+  if (delta <= 0 || source < 0 || source >= n) {
+    return fail("Error: invalid from-scratch parameters (delta > 0, source in range).");
+  }
+  if (!ws_reserve(ws, n, 1)) {
+    return 0;
+  }
+  PackChoice pc{};
+  if (!choose_packing(n, maxWeight, pc)) {
+    return 0;
+  }
+  ObjDesc d = make_desc(*out, nullptr, nullptr, pc, static_cast<u64>(delta),
+                        ws->state[0]);
+  d.distOut = d_dist;
+  d.parentOut = d_parent;
+  ws->hDesc[0] = d;
+  LaunchArgs args{};
+  args.objs = ws->dDesc;
+  args.nobj = 1;
+  args.n = n;
+  args.source = source;
+  args.phases = PH_SCRATCH;
+  args.maxRounds = jump_rounds(n);
+  float ms = 0;
+  if (!run_launch(ws, args, 1, as_stream(stream), &ms)) {
+    return 0;
+  }
+  fill_stats(*ws->hCtl, 0, d, ms, stats);
+  return 1;
+}
+
+// ---------------------------------------------------------------------------
+// Combined graph + its search
+// ---------------------------------------------------------------------------
+
+extern "C" int64_t mosp_preference_scale(const int32_t *pref, int32_t nPref,
+                                         int32_t K) {
+  return preference_scale(pref, nPref, K);
+}
+
+extern "C" int64_t mosp_combined_edge_weight(uint32_t mask, const int32_t *pref,
+                                             int32_t nPref, int32_t K,
+                                             int64_t scale) {
+  return combined_edge_weight(mask, pref, nPref, K, scale);
+}
+
+namespace {
+
+// Fill the combined-step fields of `args` and descriptor slot `descIndex`.
+int setup_combined(mosp_ws *ws, LaunchArgs &args, const int32_t *d_parents,
+                   int n, int K, const int32_t *pref, int32_t nPref,
+                   int64_t delta, int64_t *d_dist, int32_t *d_parent,
+                   int descIndex, mosp_combine_stats *stats, PackChoice &pc) {
+  if (K <= 0 || K > MOSP_MAX_OBJ || n <= 0) {
+    return fail("Error: invalid combined-graph parameters.");
+  }
+  const int64_t scale = preference_scale(pref, nPref, K);
+  if (scale == 0) {
+    return fail("Error: invalid preference vector (need K values >= 1).");
+  }
+  if (!ws_reserve_combined(ws, n, K)) {
+    return 0;
+  }
+  const int base = static_cast<int>(scale * (K + 1));
+  args.cParents = d_parents;
+  args.cK = K;
+  args.cBase = base;
+  for (int k = 0; k < K; ++k) {
+    args.cTerms[k] = static_cast<int>(nPref == 0 ? scale : scale / pref[k]);
+  }
+  args.cDelta = delta;
+  args.cDegree = ws->cDegree;
+  args.cRowPtr = ws->cRowPtr;
+  args.cColInd = ws->cColInd;
+  args.cWeights = ws->cWeights;
+  if (!choose_packing(n, base, pc)) {
+    return 0;
+  }
+  mosp_csr view{n, 0, ws->cRowPtr, ws->cColInd, ws->cWeights};
+  ObjDesc d = make_desc(view, nullptr, nullptr, pc, 0, ws->state[0]);
+  d.distOut = d_dist;
+  d.parentOut = d_parent;
+  ws->hDesc[descIndex] = d;
+  args.cDesc = ws->dDesc + descIndex;
+  if (stats != nullptr) {
+    stats->scale = scale;
+  }
+  return 1;
+}
+
+void fill_combined_stats(const mosp_ws *ws, const ObjDesc &d, float ms,
+                         mosp_combine_stats *stats) {
+  if (stats == nullptr) {
+    return;
+  }
+  const Ctl &c = *ws->hCtl;
+  stats->n_edges = static_cast<int32_t>(c.combEdges);
+  stats->delta = static_cast<int64_t>(c.combDelta);
+  fill_stats(c, MOSP_MAX_OBJ, d, ms, &stats->search);
+  stats->search.jump_rounds = 0;
+  stats->search.invalidated = 0;
+  if (c.clkSospEnd > c.clkStart && c.clkEnd >= c.clkSospEnd) {
+    stats->sosp_phase_ms = static_cast<float>((c.clkSospEnd - c.clkStart) / ws->clockKhz);
+    stats->combined_phase_ms = static_cast<float>((c.clkEnd - c.clkSospEnd) / ws->clockKhz);
+  } else {
+    stats->sosp_phase_ms = 0;
+    stats->combined_phase_ms = static_cast<float>((c.clkEnd - c.clkStart) / ws->clockKhz);
+  }
+}
+
+}  // namespace
+
+extern "C" int mosp_combined_sosp(const int32_t *d_parents, int32_t n,
+                                  int32_t K, const int32_t *pref,
+                                  int32_t nPref, int32_t source, int64_t delta,
+                                  mosp_ws *ws, int64_t *d_dist,
+                                  int32_t *d_parent, mosp_combine_stats *stats,
+                                  void *stream) {
+  g_error.clear();
+  if (stats != nullptr) {
+    *stats = mosp_combine_stats{};
+    stats->scale = 1;
+  }
+  if (K <= 0 || K > MOSP_MAX_OBJ || n <= 0 || source < 0 || source >= n) {
+    return fail("Error: invalid combined-graph parameters.");
+  }
+  if (!ws_reserve(ws, n, 1)) {
+    return 0;
+  }
+  LaunchArgs args{};
+  PackChoice pc{};
+  if (!setup_combined(ws, args, d_parents, n, K, pref, nPref, delta, d_dist,
+                      d_parent, 0, stats, pc)) {
+    return 0;
+  }
+  args.objs = ws->dDesc;
+  args.nobj = 0;
+  args.n = n;
+  args.source = source;
+  args.phases = PH_COMBINED;
+  args.maxRounds = jump_rounds(n);
+  float ms = 0;
+  if (!run_launch(ws, args, 1, as_stream(stream), &ms)) {
+    return 0;
+  }
+  fill_combined_stats(ws, ws->hDesc[0], ms, stats);
+  return 1;
+}
+
+// ---------------------------------------------------------------------------
+// Device graph (src/deviceGraph.cu:28-135) and path costs
+// ---------------------------------------------------------------------------
+
+struct mosp_device_graph {
+  int n = 0;
+  int m = 0;
+  int K = 0;
+  int *outRow = nullptr, *outCol = nullptr, *outW = nullptr;  // outW: K columns
+  int *inRow = nullptr, *inCol = nullptr, *inW = nullptr;
+};
+
+namespace {
+
+constexpr int TILE = 2048;  // scan tile: 256 threads x 8 items
+
+__global__ void split_weights_kernel(const int *edgeMajor, long long total,
+                                     int m, int K, int *columns) {
+  const long long i = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= total) {
+    return;
+  }
+  const long long e = i / K;
+  const int k = static_cast<int>(i - e * K);
+  columns[static_cast<size_t>(k) * m + e] = edgeMajor[i];
+}
+
+__global__ void in_degree_kernel(const int *colInd, int m, int *degree) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e < m) {
+    atomicAdd(&degree[colInd[e]], 1);
+  }
+}
+
+__global__ void tile_sums_kernel(const int *in, int count, int *sums) {
+  __shared__ int red[BLOCK];
+  const int base = blockIdx.x * TILE;
+  int s = 0;
+  for (int i = threadIdx.x; i < TILE; i += BLOCK) {
+    const int idx = base + i;
+    s += idx < count ? in[idx] : 0;
+  }
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int w = BLOCK / 2; w > 0; w >>= 1) {
+    if (threadIdx.x < w) {
+      red[threadIdx.x] += red[threadIdx.x + w];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    sums[blockIdx.x] = red[0];
+  }
+}
+
+// Exclusive scan of `sums` in place by one workgroup (any length).
+__global__ void scan_sums_kernel(int *sums, int count) {
+  __shared__ int buf[BLOCK];
+  __shared__ int carry;
+  if (threadIdx.x == 0) {
+    carry = 0;
+  }
+  __syncthreads();
+  for (int base = 0; base < count; base += BLOCK) {
+    const int idx = base + threadIdx.x;
+    const int v = idx < count ? sums[idx] : 0;
+    buf[threadIdx.x] = v;
+    __syncthreads();
+    for (int off = 1; off < BLOCK; off <<= 1) {
+      const int add = threadIdx.x >= off ? buf[threadIdx.x - off] : 0;
+      __syncthreads();
+      buf[threadIdx.x] += add;
+      __syncthreads();
+    }
+    if (idx < count) {
+      sums[idx] = carry + buf[threadIdx.x] - v;
+    }
+    __syncthreads();
+    if (threadIdx.x == BLOCK - 1) {
+      carry += buf[BLOCK - 1];
+    }
+    __syncthreads();
+  }
+}
+
+// out[i] = exclusive prefix of in[] (tile offsets from scan_sums_kernel).
+__global__ void tile_scan_kernel(const int *in, int count, const int *offsets,
+                                 int *out) {
+  __shared__ int buf[BLOCK];
+  constexpr int PER = TILE / BLOCK;
+  const int base = blockIdx.x * TILE + threadIdx.x * PER;
+  int vals[PER];
+  int mine = 0;
+#pragma unroll
+  for (int j = 0; j < PER; ++j) {
+    const int idx = base + j;
+    vals[j] = idx < count ? in[idx] : 0;
+    mine += vals[j];
+  }
+  buf[threadIdx.x] = mine;
+  __syncthreads();
+  for (int off = 1; off < BLOCK; off <<= 1) {
+    const int add = threadIdx.x >= off ? buf[threadIdx.x - off] : 0;
+    __syncthreads();
+    buf[threadIdx.x] += add;
+    __syncthreads();
+  }
+  int running = offsets[blockIdx.x] + buf[threadIdx.x] - mine;
+#pragma unroll
+  for (int j = 0; j < PER; ++j) {
+    const int idx = base + j;
+    if (idx < count) {
+      out[idx] = running;
+    }
+    running += vals[j];
+  }
+}
+
+__global__ void fill_reverse_kernel(int n, int m, int K, const int *rowPtr,
+                                    const int *colInd, const int *outW,
+                                    int *cursor, int *inCol, int *inW) {
+  const int u = blockIdx.x * blockDim.x + threadIdx.x;
+  if (u >= n) {
+    return;
+  }
+  for (int e = rowPtr[u]; e < rowPtr[u + 1]; ++e) {
+    const int pos = atomicAdd(&cursor[colInd[e]], 1);
+    inCol[pos] = u;
+    for (int k = 0; k < K; ++k) {
+      inW[static_cast<size_t>(k) * m + pos] = outW[static_cast<size_t>(k) * m + e];
+    }
+  }
+}
+
+int grid_for(long long work) {
+  return static_cast<int>((std::max(work, 1LL) + BLOCK - 1) / BLOCK);
+}
+
+// Exclusive scan of count ints (count >= 1) with a temporary of tiles ints.
+int exclusive_scan(const int *in, int *out, int count, hipStream_t stream) {
+  const int tiles = (count + TILE - 1) / TILE;
+  int *sums = nullptr;
+  HIP_TRY(hipMallocAsync(reinterpret_cast<void **>(&sums), sizeof(int) * tiles, stream));
+  hipLaunchKernelGGL(tile_sums_kernel, dim3(tiles), dim3(BLOCK), 0, stream, in,
+                     count, sums);
+  hipLaunchKernelGGL(scan_sums_kernel, dim3(1), dim3(BLOCK), 0, stream, sums,
+                     tiles);
+  hipLaunchKernelGGL(tile_scan_kernel, dim3(tiles), dim3(BLOCK), 0, stream, in,
+                     count, sums, out);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipFreeAsync(sums, stream));
+  return 1;
+}
+
+void free_graph(mosp_device_graph *g) {
+  hipFree(g->outRow);
+  hipFree(g->outCol);
+  hipFree(g->outW);
+  hipFree(g->inRow);
+  hipFree(g->inCol);
+  hipFree(g->inW);
+  delete g;
+}
+
+int build_graph(mosp_device_graph *g, const int32_t *rowPtr,
+                const int32_t *colInd, const int32_t *weights) {
+  const int n = g->n, m = g->m, K = g->K;
+  const size_t mm = static_cast<size_t>(std::max(m, 1));
+  int *edgeMajor = nullptr, *cursor = nullptr;
+  HIP_TRY(hipMalloc(&g->outRow, sizeof(int) * (n + 1)));
+  HIP_TRY(hipMalloc(&g->outCol, sizeof(int) * mm));
+  HIP_TRY(hipMalloc(&g->outW, sizeof(int) * mm * K));
+  HIP_TRY(hipMalloc(&g->inRow, sizeof(int) * (n + 1)));
+  HIP_TRY(hipMalloc(&g->inCol, sizeof(int) * mm));
+  HIP_TRY(hipMalloc(&g->inW, sizeof(int) * mm * K));
+  HIP_TRY(hipMalloc(&edgeMajor, sizeof(int) * mm * K));
+  HIP_TRY(hipMalloc(&cursor, sizeof(int) * (n + 1)));
+  HIP_TRY(hipMemcpy(g->outRow, rowPtr, sizeof(int) * (n + 1), hipMemcpyHostToDevice));
+  if (m > 0) {
+    HIP_TRY(hipMemcpy(g->outCol, colInd, sizeof(int) * m, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(edgeMajor, weights, sizeof(int) * static_cast<size_t>(m) * K,
+                      hipMemcpyHostToDevice));
+  }
+  const long long total = static_cast<long long>(m) * K;
+  hipLaunchKernelGGL(split_weights_kernel, dim3(grid_for(total)), dim3(BLOCK), 0,
+                     0, edgeMajor, total, m, K, g->outW);
+  HIP_TRY(hipMemset(cursor, 0, sizeof(int) * (n + 1)));
+  hipLaunchKernelGGL(in_degree_kernel, dim3(grid_for(m)), dim3(BLOCK), 0, 0,
+                     g->outCol, m, cursor);
+  if (!exclusive_scan(cursor, g->inRow, n + 1, 0)) {
+    return 0;
+  }
+  HIP_TRY(hipMemcpy(cursor, g->inRow, sizeof(int) * n, hipMemcpyDeviceToDevice));
+  hipLaunchKernelGGL(fill_reverse_kernel, dim3(grid_for(n)), dim3(BLOCK), 0, 0,
+                     n, m, K, g->outRow, g->outCol, g->outW, cursor, g->inCol,
+                     g->inW);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipDeviceSynchronize());
+  hipFree(edgeMajor);
+  hipFree(cursor);
+  return 1;
+}
+
+}  // namespace
+
+extern "C" mosp_device_graph *mosp_device_graph_upload(int32_t n, int32_t K,
+                                                       const int32_t *rowPtr,
+                                                       const int32_t *colInd,
+                                                       const int32_t *weights) {
+  g_error.clear();
+  if (n <= 0 || K <= 0 || K > MOSP_MAX_OBJ) {
+    fail("Error: invalid graph dimensions.");
+    return nullptr;
+  }
+  auto *g = new mosp_device_graph();
+  g->n = n;
+  g->m = rowPtr[n];
+  g->K = K;
+  if (!build_graph(g, rowPtr, colInd, weights)) {
+    const std::string keep = g_error;
+    free_graph(g);
+    g_error = keep.empty() ? "Error: could not allocate the graph on the GPU." : keep;
+    return nullptr;
+  }
+  return g;
+}
+
+extern "C" void mosp_device_graph_free(mosp_device_graph *g) {
+  if (g != nullptr) {
+    free_graph(g);
+  }
+}
+
+extern "C" int mosp_device_graph_view(const mosp_device_graph *g, int32_t obj,
+                                      mosp_csr *out, mosp_csr *in) {
+  if (obj < 0 || obj >= g->K) {
+    return fail("Error: objective out of range.");
+  }
+  const size_t off = static_cast<size_t>(obj) * g->m;
+  if (out != nullptr) {
+    *out = mosp_csr{g->n, g->m, g->outRow, g->outCol, g->outW + off};
+  }
+  if (in != nullptr) {
+    *in = mosp_csr{g->n, g->m, g->inRow, g->inCol, g->inW + off};
+  }
+  return 1;
+}
+
+extern "C" int32_t mosp_device_graph_objectives(const mosp_device_graph *g) {
+  return g->K;
+}
+
+namespace {
+
+// Path costs: every vertex starts with its tree edge's K weights (first (p,v)
+// edge of row p) and its parent as ancestor, then pointer jumping adds the
+// ancestor's partial sum and jumps (ping-pong buffers), ceil(log2 n)+1
+// rounds.
+__global__ void path_cost_init(int n, int K, int m, const int *rowPtr,
+                               const int *colInd, const int *wcols,
+                               const int *parent, int source, int *anc,
+                               long long *cost, int *missing) {
+  const int v = blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= n) {
+    return;
+  }
+  const size_t at = static_cast<size_t>(v) * K;
+  if (v == source) {
+    anc[v] = -1;
+    for (int k = 0; k < K; ++k) cost[at + k] = 0;
+    return;
+  }
+  const int p = parent[v];
+  if (p < 0) {
+    anc[v] = -1;
+    for (int k = 0; k < K; ++k) cost[at + k] = DISTANCE_INF;
+    return;
+  }
+  int edge = -1;
+  for (int e = rowPtr[p]; e < rowPtr[p + 1]; ++e) {
+    if (colInd[e] == v) {
+      edge = e;
+      break;
+    }
+  }
+  if (edge < 0) {
+    atomicOr(missing, 1);
+    anc[v] = -1;
+    for (int k = 0; k < K; ++k) cost[at + k] = DISTANCE_INF;
+    return;
+  }
+  anc[v] = p;
+  for (int k = 0; k < K; ++k) {
+    cost[at + k] = wcols[static_cast<size_t>(k) * m + edge];
+  }
+}
+
+__global__ void path_cost_jump(int n, int K, const int *ancIn,
+                               const long long *costIn, int *ancOut,
+                               long long *costOut) {
+  const int v = blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= n) {
+    return;
+  }
+  const size_t at = static_cast<size_t>(v) * K;
+  const int a = ancIn[v];
+  if (a < 0) {
+    ancOut[v] = -1;
+    for (int k = 0; k < K; ++k) costOut[at + k] = costIn[at + k];
+    return;
+  }
+  const size_t aa = static_cast<size_t>(a) * K;
+  for (int k = 0; k < K; ++k) {
+    costOut[at + k] = costIn[at + k] + costIn[aa + k];
+  }
+  ancOut[v] = ancIn[a];
+}
+
+}  // namespace
+
+extern "C" int mosp_path_costs_gpu(const mosp_device_graph *g,
+                                   const int32_t *d_parent, int32_t source,
+                                   int64_t *d_costs, void *stream) {
+  g_error.clear();
+  const int n = g->n, K = g->K;
+  hipStream_t s = as_stream(stream);
+  int *ancA = nullptr, *ancB = nullptr, *missing = nullptr;
+  long long *costB = nullptr;
+  HIP_TRY(hipMallocAsync(reinterpret_cast<void **>(&ancA), sizeof(int) * n, s));
+  HIP_TRY(hipMallocAsync(reinterpret_cast<void **>(&ancB), sizeof(int) * n, s));
+  HIP_TRY(hipMallocAsync(reinterpret_cast<void **>(&costB),
+                         sizeof(long long) * static_cast<size_t>(n) * K, s));
+  HIP_TRY(hipMallocAsync(reinterpret_cast<void **>(&missing), sizeof(int), s));
+  HIP_TRY(hipMemsetAsync(missing, 0, sizeof(int), s));
+  long long *costA = reinterpret_cast<long long *>(d_costs);
+  hipLaunchKernelGGL(path_cost_init, dim3(grid_for(n)), dim3(BLOCK), 0, s, n, K,
+                     g->m, g->outRow, g->outCol, g->outW, d_parent, source, ancA,
+                     costA, missing);
+  const int rounds = jump_rounds(n);
+  for (int r = 0; r < rounds; ++r) {
+    hipLaunchKernelGGL(path_cost_jump, dim3(grid_for(n)), dim3(BLOCK), 0, s, n,
+                       K, ancA, costA, ancB, costB);
+    std::swap(ancA, ancB);
+    std::swap(costA, costB);
+  }
+  if (costA != reinterpret_cast<long long *>(d_costs)) {
+    HIP_TRY(hipMemcpyAsync(d_costs, costA,
+                           sizeof(long long) * static_cast<size_t>(n) * K,
+                           hipMemcpyDeviceToDevice, s));
+    std::swap(costA, costB);  // costB is the scratch buffer again
+  }
+  int hostMissing = 0;
+  HIP_TRY(hipMemcpyAsync(&hostMissing, missing, sizeof(int),
+                         hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipFreeAsync(ancA, s));
+  HIP_TRY(hipFreeAsync(ancB, s));
+  HIP_TRY(hipFreeAsync(costB, s));
+  HIP_TRY(hipFreeAsync(missing, s));
+  HIP_TRY(hipStreamSynchronize(s));
+  if (hostMissing) {
+    return fail("Error: a MOSP tree edge is missing from the updated graph.");
+  }
+  return 1;
+}
+
+// ---------------------------------------------------------------------------
+// Fused device pipeline: K updates + combined graph + search, one launch.
+// ---------------------------------------------------------------------------
+
+extern "C" int mosp_update_fused(const mosp_device_graph *g, int32_t K,
+                                 const mosp_changes *changes, int32_t source,
+                                 const int64_t *delta, const int64_t *maxWeight,
+                                 const int32_t *pref, int32_t nPref,
+                                 mosp_ws *ws, const int64_t *d_dist_in,
+                                 const int32_t *d_parents_in, int64_t *d_dist,
+                                 int32_t *d_parents, int64_t *d_comb_dist,
+                                 int32_t *d_comb_parent, mosp_sosp_stats *stats,
+                                 mosp_combine_stats *cstats, void *stream) {
+  g_error.clear();
+  const int n = g->n;
+  if (K <= 0 || K > g->K) {
+    return fail("Error: invalid number of objectives.");
+  }
+  if (source < 0 || source >= n) {
+    return fail("Error: source vertex out of range.");
+  }
+  if (cstats != nullptr) {
+    *cstats = mosp_combine_stats{};
+    cstats->scale = 1;
+  }
+  if (!ws_reserve(ws, n, K)) {
+    return 0;
+  }
+  const int64_t *distIn = d_dist_in != nullptr ? d_dist_in : d_dist;
+  const int32_t *parIn = d_parents_in != nullptr ? d_parents_in : d_parents;
+  int anyFallback = 0;
+  for (int k = 0; k < K; ++k) {
+    if (delta[k] <= 0) {
+      return fail("Error: the near-far bucket width must be positive.");
+    }
+    PackChoice pc{};
+    if (!choose_packing(n, maxWeight[k], pc)) {
+      return 0;
+    }
+    mosp_csr out{}, in{};
+    mosp_device_graph_view(g, k, &out, &in);
+    ObjDesc d = make_desc(out, &in, &changes[k], pc, static_cast<u64>(delta[k]),
+                          ws->state[k]);
+    const size_t off = static_cast<size_t>(k) * n;
+    d.distIn = distIn + off;
+    d.parentIn = parIn + off;
+    d.distOut = d_dist + off;
+    d.parentOut = d_parents + off;
+    ws->hDesc[k] = d;
+    anyFallback |= pc.bits == 0 ? 1 : 0;
+  }
+  LaunchArgs args{};
+  PackChoice cpc{};
+  if (!setup_combined(ws, args, d_parents, n, K, pref, nPref, 0, d_comb_dist,
+                      d_comb_parent, K, cstats, cpc)) {
+    return 0;
+  }
+  args.objs = ws->dDesc;
+  args.nobj = K;
+  args.n = n;
+  args.source = source;
+  args.phases = PH_UPDATE | PH_COMBINED;
+  args.maxRounds = jump_rounds(n);
+  args.anyFallback = anyFallback;
+  float ms = 0;
+  if (!run_launch(ws, args, K + 1, as_stream(stream), &ms)) {
+    return 0;
+  }
+  for (int k = 0; k < K; ++k) {
+    fill_stats(*ws->hCtl, k, ws->hDesc[k], ms, stats ? &stats[k] : nullptr);
+  }
+  fill_combined_stats(ws, ws->hDesc[K], ms, cstats);
+  return 1;
+}

@@ -1,0 +1,117 @@
+// engine.hpp -- data structures shared by the host launch code and the gfx950
+// kernels of the SOSP / combined-graph engine.
+//
+// One persistent cooperative launch runs a whole update: Step 1 (pack, roots,
+// pointer-jumping subtree invalidation, pull) and Step 2 (near-far push) for
+// up to MOSP_MAX_OBJ objectives at once, optionally followed by the combined
+// graph build and its from-scratch search (Steps 2-3 of the MOSP update).
+// The semantics follow the reference's engine (src/sospUpdateGpu.cu:211-519,
+// src/combinedGraphGpu.cu:55-121); the schedule is this engine's own: all
+// objectives advance in lock-step with ONE grid barrier per near-far
+// iteration (rotating counter slots instead of the reference's two barriers),
+// and the far-pile minimum is tracked at append time instead of by an extra
+// reduction pass.
+#pragma once
+
+#include <stdint.h>
+
+namespace mosp {
+
+using u64 = unsigned long long;
+
+constexpr u64 PACKED_INF = ~0ULL;
+constexpr long long DISTANCE_INF = 0x7fffffffffffffffLL / 4;  // csrGraph.cuh:185
+constexpr int MOSP_MAX_OBJ = 32;     // K <= 32 (csrGraph.cu:736-739)
+constexpr int BLOCK = 256;           // threads per workgroup (4 wave64)
+
+// Phases of one launch.
+enum : int {
+  PH_UPDATE = 1,      // Step 1 + Step 2 on objs[0..nobj)
+  PH_SCRATCH = 2,     // from-scratch search on objs[0] (Step 3 engine)
+  PH_COMBINED = 4,    // combined build from cParents + from-scratch search
+};
+
+// Scratch of one objective (a slice of the workspace).
+struct ObjState {
+  u64 *packed;     // (distance << b | parent) words, n
+  int *stamp;      // generation a vertex was last listed in, n
+  int *inFar;      // vertex is in the far pile, n (left all-zero)
+  int *flag;       // invalidation marks, n (left all-zero)
+  int *ancestor;   // pointer-jumping ancestors, n
+  int *near[2];    // near frontiers (ping-pong), n each
+  int *far[2];     // far piles (ping-pong), n each
+  int *cand;       // pull candidates, n
+};
+
+// One objective of a launch (device-resident descriptor).
+struct ObjDesc {
+  const int *outRow, *outCol, *outW;  // out-CSR + this objective's weights
+  const int *inRow, *inCol, *inW;     // in-CSR + weights (update only)
+  const int *chFrom, *chTo;           // deleted / weight-increased edges
+  int nChanged;
+  const int *heads;                   // heads of inserted edges
+  int nHeads;
+  int parentBits;                     // 0: distance-only words
+  int pad0;
+  u64 noParent;                       // all-ones parent field
+  u64 maxDistance;                    // (n-1)*maxWeight: larger candidates dropped
+  u64 delta;                          // near-far bucket width
+  const int64_t *distIn;              // old tree (update) -- may alias distOut
+  const int *parentIn;
+  int64_t *distOut;                   // new tree
+  int *parentOut;
+  ObjState s;
+};
+
+// Per-objective device counters of one launch (zeroed by the reset kernel).
+struct ObjCtl {
+  unsigned nearCount[3];   // near frontier sizes, slot = iteration % 3
+  unsigned farCount[3];    // far pile sizes, slot = epoch % 3
+  u64 farMin[3];           // min distance appended to the far pile of an epoch
+  unsigned candCount;
+  unsigned invalidated;
+  unsigned overflow;       // an input distance does not fit the packing
+  unsigned iterations;
+  unsigned epochs;
+  unsigned pad;
+  u64 pushes;
+};
+
+struct Ctl {
+  unsigned barrier;        // monotonic grid-barrier arrival counter
+  unsigned jumpActive[3];  // pointer jumping: some vertex still jumps
+  unsigned rounds;
+  unsigned generationOut;  // last stamp generation used
+  u64 combEdges;           // combined graph: edge count
+  u64 combWeightSum;       // combined graph: weight sum
+  u64 combDelta;           // near-far width used for Step 3
+  u64 clkStart, clkSospEnd, clkEnd;  // wall_clock64 stamps (100 MHz)
+  ObjCtl obj[MOSP_MAX_OBJ + 1];
+};
+
+struct LaunchArgs {
+  const ObjDesc *objs;     // nobj descriptors (device memory)
+  int nobj;
+  int n;
+  int source;
+  int phases;
+  int maxRounds;           // pointer-jumping rounds: ceil(log2 n) + 1
+  int anyFallback;         // some objective of objs uses distance-only words
+  int *genDev;             // device-resident stamp generation (persists)
+  Ctl *ctl;
+  // Combined graph (PH_COMBINED): parents of cK trees, objective-major.
+  const int *cParents;
+  int cK;
+  int cBase;               // L * (K + 1)
+  int cTerms[MOSP_MAX_OBJ];// L / Pref_k
+  long long cDelta;        // > 0: use it; else the default rule on device
+  int *cDegree;            // n + 1 (counts, then fill cursors)
+  int *cRowPtr;            // n + 1 (written here, read through cDesc->outRow)
+  int *cColInd;            // K * n
+  int *cWeights;           // K * n
+  int *cPartials;          // one per workgroup (scan)
+  const ObjDesc *cDesc;    // search on the combined CSR: outRow = row pointers,
+                           // outCol / outW = edges, packing, scratch, outputs
+};
+
+}  // namespace mosp

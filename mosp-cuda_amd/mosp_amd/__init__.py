@@ -1,0 +1,540 @@
+"""Python bindings of libmosp_hip.so (the MI355X DynaMOSP update engine).
+
+Mirrors the reference's in-memory pipeline interface (src/mospUpdate.cu,
+src/sospUpdateGpu.cu, src/combinedGraphGpu.cu) over the C ABI declared in
+include/mosp_hip.h.  torch provides device buffers and the stream only; every
+computation runs in the HIP kernels of the library, and the module raises if
+the library is missing -- there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import pathlib
+from dataclasses import dataclass, field
+
+import numpy as np
+
+HERE = pathlib.Path(__file__).resolve().parent
+LIB_PATH = HERE.parent / "lib" / "libmosp_hip.so"
+DISTANCE_INF = (2**63 - 1) // 4
+
+_i32p = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+_i64p = np.ctypeslib.ndpointer(dtype=np.int64, flags="C_CONTIGUOUS")
+_u32p = np.ctypeslib.ndpointer(dtype=np.uint32, flags="C_CONTIGUOUS")
+_f64p = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+_I, _L, _VP = C.c_int32, C.c_int64, C.c_void_p
+
+
+class Csr(C.Structure):
+    """mosp_csr (include/mosp_hip.h) -- DeviceCsr of the reference."""
+    _fields_ = [("n", C.c_int32), ("m", C.c_int32), ("row_ptr", _VP),
+                ("col_ind", _VP), ("weights", _VP)]
+
+
+class Changes(C.Structure):
+    """mosp_changes -- DeviceChanges of the reference."""
+    _fields_ = [("changed_from", _VP), ("changed_to", _VP), ("n_changed", C.c_int32),
+                ("insert_heads", _VP), ("n_insert_heads", C.c_int32)]
+
+
+class SospStats(C.Structure):
+    """mosp_sosp_stats -- SospStats of the reference (+ kernel_ms)."""
+    _fields_ = [("invalidated", C.c_int32), ("jump_rounds", C.c_int32),
+                ("iterations", C.c_int32), ("epochs", C.c_int32), ("pushes", C.c_int64),
+                ("packed_parents", C.c_int32), ("kernel_ms", C.c_float)]
+
+    def as_dict(self) -> dict:
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
+class CombineStats(C.Structure):
+    """mosp_combine_stats -- CombineStats of the reference (+ phase split)."""
+    _fields_ = [("n_edges", C.c_int32), ("scale", C.c_int64), ("delta", C.c_int64),
+                ("search", SospStats), ("sosp_phase_ms", C.c_float),
+                ("combined_phase_ms", C.c_float)]
+
+
+EXPORTS = (
+    "mosp_last_error", "mosp_ws_create", "mosp_ws_reserve", "mosp_ws_destroy",
+    "mosp_ws_set_grid", "mosp_ws_grid", "mosp_default_delta", "mosp_sosp_update",
+    "mosp_sosp_update_multi", "mosp_sosp_from_scratch", "mosp_combined_sosp",
+    "mosp_update_fused", "mosp_device_graph_upload", "mosp_device_graph_free",
+    "mosp_device_graph_view", "mosp_device_graph_objectives", "mosp_path_costs_gpu",
+    "mosp_preference_scale", "mosp_combined_edge_weight", "mosp_choose_packing",
+    "mosp_apply_batch", "mosp_update_host", "mosp_gen_road_grid", "mosp_gen_rmat",
+    "mosp_gen_batch",
+)
+
+_LIB: C.CDLL | None = None
+
+
+def lib() -> C.CDLL:
+    """Load libmosp_hip.so (built by `make -C mosp-cuda_amd`); raise if absent."""
+    global _LIB
+    if _LIB is not None:
+        return _LIB
+    if not LIB_PATH.exists():
+        raise RuntimeError(f"{LIB_PATH} is missing: build it with `make -C mosp-cuda_amd` "
+                           "(python -c 'import __graft_entry__; __graft_entry__.build()')")
+    L = C.CDLL(str(LIB_PATH))
+    L.mosp_last_error.restype = C.c_char_p
+    L.mosp_ws_create.restype = _VP
+    L.mosp_ws_reserve.argtypes = [_VP, _I, _I]
+    L.mosp_ws_destroy.argtypes = [_VP]
+    L.mosp_ws_set_grid.argtypes = [_VP, _I]
+    L.mosp_ws_grid.argtypes = [_VP]
+    L.mosp_ws_grid.restype = _I
+    L.mosp_default_delta.argtypes = [_L, _I, _L]
+    L.mosp_default_delta.restype = _L
+    L.mosp_sosp_update.argtypes = [C.POINTER(Csr), C.POINTER(Csr), C.POINTER(Changes), _I, _L,
+                                   _L, _VP, _VP, _VP, C.POINTER(SospStats), _VP]
+    L.mosp_sosp_update_multi.argtypes = [_I, C.POINTER(Csr), C.POINTER(Csr),
+                                         C.POINTER(Changes), _I, C.POINTER(C.c_int64),
+                                         C.POINTER(C.c_int64), _VP, C.POINTER(_VP),
+                                         C.POINTER(_VP), C.POINTER(_VP), C.POINTER(_VP),
+                                         C.POINTER(SospStats), _VP]
+    L.mosp_sosp_from_scratch.argtypes = [C.POINTER(Csr), _I, _L, _L, _VP, _VP, _VP,
+                                         C.POINTER(SospStats), _VP]
+    L.mosp_combined_sosp.argtypes = [_VP, _I, _I, _i32p, _I, _I, _L, _VP, _VP, _VP,
+                                     C.POINTER(CombineStats), _VP]
+    L.mosp_update_fused.argtypes = [_VP, _I, C.POINTER(Changes), _I, C.POINTER(C.c_int64),
+                                    C.POINTER(C.c_int64), _i32p, _I, _VP, _VP, _VP, _VP, _VP,
+                                    _VP, _VP, C.POINTER(SospStats), C.POINTER(CombineStats),
+                                    _VP]
+    L.mosp_device_graph_upload.restype = _VP
+    L.mosp_device_graph_upload.argtypes = [_I, _I, _i32p, _i32p, _i32p]
+    L.mosp_device_graph_free.argtypes = [_VP]
+    L.mosp_device_graph_view.argtypes = [_VP, _I, C.POINTER(Csr), C.POINTER(Csr)]
+    L.mosp_device_graph_objectives.argtypes = [_VP]
+    L.mosp_device_graph_objectives.restype = _I
+    L.mosp_path_costs_gpu.argtypes = [_VP, _VP, _I, _VP, _VP]
+    L.mosp_preference_scale.argtypes = [_i32p, _I, _I]
+    L.mosp_preference_scale.restype = _L
+    L.mosp_combined_edge_weight.argtypes = [C.c_uint32, _i32p, _I, _I, _L]
+    L.mosp_combined_edge_weight.restype = _L
+    L.mosp_choose_packing.argtypes = [_I, _L, C.POINTER(C.c_int32), C.POINTER(C.c_uint64)]
+    L.mosp_apply_batch.restype = _L
+    L.mosp_apply_batch.argtypes = [_I, _I, _i32p, _i32p, _i32p, _I, _i32p, _i32p, _i32p, _I,
+                                   _i32p, _i32p, _i32p, _i32p, _i32p, _u32p]
+    L.mosp_update_host.argtypes = [_I, _I, _i32p, _i32p, _i32p, _I, _i32p, _i32p, _i32p, _I,
+                                   _i32p, _i32p, _I, _I, _i32p, _I, _L, _I, _i64p, _i32p,
+                                   C.POINTER(C.c_int64), _i32p, _i32p, _i32p, _i64p, _i32p,
+                                   _i64p, _i32p, C.POINTER(SospStats),
+                                   C.POINTER(CombineStats), _f64p]
+    gen_tail = [C.POINTER(C.c_int32), C.POINTER(C.c_int64), _VP, _VP, _VP, _L]
+    L.mosp_gen_road_grid.argtypes = [_I, _I, _I, _I, _I, C.c_double, C.c_double, C.c_uint64,
+                                     *gen_tail]
+    L.mosp_gen_rmat.argtypes = [_I, _I, C.c_double, C.c_double, C.c_double, _I, _I, _I,
+                                C.c_uint64, *gen_tail]
+    L.mosp_gen_batch.argtypes = [_I, _I, _i32p, _i32p, _i32p, _I, C.c_double, _I, _I, _I,
+                                 C.c_uint64, _I, C.POINTER(C.c_int32), _i32p, _i32p, _i32p,
+                                 C.POINTER(C.c_int32), _i32p, _i32p]
+    _LIB = L
+    return L
+
+
+def last_error() -> str:
+    return (lib().mosp_last_error() or b"").decode()
+
+
+def _check(ok: int, what: str) -> None:
+    if not ok:
+        raise RuntimeError(f"{what} failed: {last_error()}")
+
+
+def _i32(a) -> np.ndarray:
+    return np.ascontiguousarray(np.asarray(a, dtype=np.int32).reshape(-1))
+
+
+def _nz(a: np.ndarray) -> np.ndarray:
+    """numpy arrays passed through ndpointer must not be empty-with-None."""
+    return a if a.size else np.zeros(1, a.dtype)
+
+
+# --------------------------------------------------------------- host data --
+
+@dataclass
+class HostGraph:
+    """The reference's CsrGraph: rowPtr, colInd, edge-major weights."""
+    n: int
+    K: int
+    row: np.ndarray
+    col: np.ndarray
+    w: np.ndarray
+
+    @property
+    def m(self) -> int:
+        return int(self.row[-1])
+
+
+@dataclass
+class Batch:
+    """The reference's ChangeBatch (insertion-major weights)."""
+    K: int
+    ins_from: np.ndarray
+    ins_to: np.ndarray
+    ins_w: np.ndarray
+    del_from: np.ndarray
+    del_to: np.ndarray
+    mask: np.ndarray = field(default_factory=lambda: np.zeros(0, np.uint32))
+
+    @property
+    def n_ins(self) -> int:
+        return len(self.ins_from)
+
+    @property
+    def n_del(self) -> int:
+        return len(self.del_from)
+
+
+def gen_road_grid(width: int, height: int, K: int, seed: int, drop: float = 0.3,
+                  diag: float = 0.02, wmin: int = 1, wmax: int = 100) -> HostGraph:
+    L = lib()
+    n = C.c_int32(0)
+    m = C.c_int64(0)
+    args = (width, height, K, wmin, wmax, drop, diag, seed)
+    _check(L.mosp_gen_road_grid(*args, C.byref(n), C.byref(m), None, None, None, 0), "road grid")
+    row = np.zeros(n.value + 1, np.int32)
+    col = np.zeros(max(m.value, 1), np.int32)
+    w = np.zeros(max(m.value, 1) * K, np.int32)
+    _check(L.mosp_gen_road_grid(*args, C.byref(n), C.byref(m), row.ctypes.data,
+                                col.ctypes.data, w.ctypes.data, m.value), "road grid")
+    return HostGraph(n.value, K, row, col[: m.value].copy(), w[: m.value * K].copy())
+
+
+def gen_rmat(scale: int, edge_factor: int, K: int, seed: int, wmin: int = 1, wmax: int = 100,
+             a: float = 0.57, b: float = 0.19, c: float = 0.19) -> HostGraph:
+    L = lib()
+    n = C.c_int32(0)
+    m = C.c_int64(0)
+    args = (scale, edge_factor, a, b, c, K, wmin, wmax, seed)
+    _check(L.mosp_gen_rmat(*args, C.byref(n), C.byref(m), None, None, None, 0), "rmat")
+    row = np.zeros(n.value + 1, np.int32)
+    col = np.zeros(max(m.value, 1), np.int32)
+    w = np.zeros(max(m.value, 1) * K, np.int32)
+    _check(L.mosp_gen_rmat(*args, C.byref(n), C.byref(m), row.ctypes.data, col.ctypes.data,
+                           w.ctypes.data, m.value), "rmat")
+    return HostGraph(n.value, K, row, col[: m.value].copy(), w[: m.value * K].copy())
+
+
+def gen_batch(g: HostGraph, changes: int, ins_pct: float, seed: int, safe: bool = False,
+              wmin: int = 1, wmax: int = 100, source: int = 0) -> Batch:
+    cap = max(changes, 1)
+    f, t, dfm, dto = (np.zeros(cap, np.int32) for _ in range(4))
+    w = np.zeros(cap * g.K, np.int32)
+    ni, nd = C.c_int32(0), C.c_int32(0)
+    _check(lib().mosp_gen_batch(g.n, g.K, g.row, _nz(g.col), _nz(g.w), changes, ins_pct,
+                                int(safe), wmin, wmax, seed, source, C.byref(ni), f, t, w,
+                                C.byref(nd), dfm, dto), "batch generator")
+    a, b = ni.value, nd.value
+    return Batch(g.K, f[:a].copy(), t[:a].copy(), w[: a * g.K].copy(), dfm[:b].copy(),
+                 dto[:b].copy())
+
+
+def apply_batch(g: HostGraph, b: Batch) -> tuple[HostGraph, np.ndarray]:
+    cap = g.m + b.n_ins
+    row = np.zeros(g.n + 1, np.int32)
+    col = np.zeros(max(cap, 1), np.int32)
+    w = np.zeros(max(cap, 1) * g.K, np.int32)
+    mask = np.zeros(max(b.n_ins, 1), np.uint32)
+    m = lib().mosp_apply_batch(g.n, g.K, g.row, _nz(g.col), _nz(g.w), b.n_ins,
+                               _nz(_i32(b.ins_from)), _nz(_i32(b.ins_to)), _nz(_i32(b.ins_w)),
+                               b.n_del, _nz(_i32(b.del_from)), _nz(_i32(b.del_to)), row, col, w,
+                               mask)
+    if m < 0:
+        raise RuntimeError("applyChangeBatch failed")
+    return HostGraph(g.n, g.K, row, col[:m].copy(), w[: m * g.K].copy()), mask[: b.n_ins].copy()
+
+
+def changed_lists(b: Batch, mask: np.ndarray, k: int) -> tuple[np.ndarray, np.ndarray]:
+    """changed_k = deletions + insertions that raised objective k
+    (src/mospUpdate.cu:124-136)."""
+    sel = ((mask.astype(np.int64) >> k) & 1) == 1
+    return (np.concatenate([_i32(b.del_from), _i32(b.ins_from)[sel]]).astype(np.int32),
+            np.concatenate([_i32(b.del_to), _i32(b.ins_to)[sel]]).astype(np.int32))
+
+
+def preference_scale(pref, K: int) -> int:
+    a = _i32(pref)
+    return lib().mosp_preference_scale(_nz(a), len(a), K)
+
+
+def choose_packing(n: int, max_w: int):
+    bits = C.c_int32(0)
+    bound = C.c_uint64(0)
+    ok = lib().mosp_choose_packing(n, max_w, C.byref(bits), C.byref(bound))
+    return (bits.value, bound.value) if ok else None
+
+
+def default_delta(m: int, n: int, wsum: int) -> int:
+    return lib().mosp_default_delta(m, n, wsum)
+
+
+def objective_bounds(g: HostGraph, b: Batch, K: int):
+    """maxWeight (original + inserted weights) and weightSum (original edges)
+    per objective (src/mospUpdate.cu:104-122)."""
+    w = g.w.reshape(-1, g.K)[:, :K].astype(np.int64) if g.m else np.zeros((0, K), np.int64)
+    maxw = np.maximum(w.max(axis=0) if g.m else np.ones(K, np.int64), 1)
+    if b.n_ins:
+        maxw = np.maximum(maxw, _i32(b.ins_w).reshape(-1, g.K)[:, :K].max(axis=0))
+    return maxw.astype(np.int64), w.sum(axis=0).astype(np.int64)
+
+
+def update_host(g: HostGraph, b: Batch, init_dist: np.ndarray, init_parents: np.ndarray,
+                K: int = 0, source: int = 0, pref=(), delta: int = 0, fused: bool = True) -> dict:
+    """The in-memory MOSP update (src/mospUpdate.cu:57-225) on host arrays:
+    apply, upload, K updates + combined step on the GPU, download."""
+    Ku = K if K > 0 else g.K
+    Ku = min(Ku, g.K)
+    n = g.n
+    cap = g.m + b.n_ins
+    row = np.zeros(n + 1, np.int32)
+    col = np.zeros(max(cap, 1), np.int32)
+    w = np.zeros(max(cap, 1) * g.K, np.int32)
+    dist = np.zeros(Ku * n, np.int64)
+    par = np.zeros(Ku * n, np.int32)
+    cd = np.zeros(n, np.int64)
+    cp = np.zeros(n, np.int32)
+    st = (SospStats * Ku)()
+    cs = CombineStats()
+    tm = np.zeros(5, np.float64)
+    m = C.c_int64(0)
+    p = _i32(pref)
+    ok = lib().mosp_update_host(
+        n, g.K, g.row, _nz(g.col), _nz(g.w), b.n_ins, _nz(_i32(b.ins_from)),
+        _nz(_i32(b.ins_to)), _nz(_i32(b.ins_w)), b.n_del, _nz(_i32(b.del_from)),
+        _nz(_i32(b.del_to)), Ku, source, _nz(p), len(p), delta, int(fused),
+        np.ascontiguousarray(init_dist, np.int64).reshape(-1),
+        np.ascontiguousarray(init_parents, np.int32).reshape(-1), C.byref(m), row, col, w,
+        dist, par, cd, cp, st, C.byref(cs), tm)
+    if not ok:
+        raise RuntimeError(f"mospUpdate failed: {last_error()}")
+    mm = m.value
+    return {
+        "updated": HostGraph(n, g.K, row, col[:mm].copy(), w[: mm * g.K].copy()),
+        "dist": dist.reshape(Ku, n), "parents": par.reshape(Ku, n),
+        "comb_dist": cd, "comb_parent": cp,
+        "stats": [st[k].as_dict() for k in range(Ku)],
+        "combined_edges": cs.n_edges, "scale": cs.scale, "comb_delta": cs.delta,
+        "comb_stats": cs.search.as_dict(),
+        "timings_ms": {"apply": tm[0], "upload": tm[1], "sosp": tm[2], "combined": tm[3],
+                       "download": tm[4], "gpu_compute": tm[2] + tm[3]},
+    }
+
+
+# ------------------------------------------------------------- device side --
+
+def _torch():
+    import torch
+    if not torch.cuda.is_available():
+        raise RuntimeError("no GPU visible: the HIP engine needs an MI355X")
+    return torch
+
+
+def _stream():
+    return C.c_void_p(_torch().cuda.current_stream().cuda_stream)
+
+
+class Workspace:
+    """mosp_ws: scratch of the engine, reserved once and reused."""
+
+    def __init__(self, capacity: int = 0, slots: int = 1, grid: int = 0):
+        self.handle = lib().mosp_ws_create()
+        if grid:
+            lib().mosp_ws_set_grid(self.handle, grid)
+        if capacity:
+            self.reserve(capacity, slots)
+
+    def reserve(self, capacity: int, slots: int = 1) -> None:
+        _check(lib().mosp_ws_reserve(self.handle, capacity, slots), "workspace reserve")
+
+    def set_grid(self, workgroups: int) -> None:
+        lib().mosp_ws_set_grid(self.handle, workgroups)
+
+    @property
+    def grid(self) -> int:
+        return lib().mosp_ws_grid(self.handle)
+
+    def close(self) -> None:
+        if self.handle:
+            lib().mosp_ws_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class DeviceGraph:
+    """mosp_device_graph: out-CSR + in-CSR with K weight columns in HBM."""
+
+    def __init__(self, g: HostGraph):
+        _torch()
+        self.n, self.K, self.m = g.n, g.K, g.m
+        self.handle = lib().mosp_device_graph_upload(g.n, g.K, g.row, _nz(g.col), _nz(g.w))
+        if not self.handle:
+            raise RuntimeError(f"device graph upload failed: {last_error()}")
+
+    def view(self, k: int) -> tuple[Csr, Csr]:
+        out, inn = Csr(), Csr()
+        _check(lib().mosp_device_graph_view(self.handle, k, C.byref(out), C.byref(inn)), "view")
+        return out, inn
+
+    def close(self) -> None:
+        if self.handle:
+            lib().mosp_device_graph_free(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def _dev(a, dtype):
+    torch = _torch()
+    t = torch.as_tensor(np.ascontiguousarray(a))
+    return t.to(device="cuda", dtype=dtype).contiguous()
+
+
+def from_scratch(dg: DeviceGraph, k: int, source: int, delta: int, max_w: int, ws: Workspace):
+    """sospFromScratchGpu on objective k; returns (dist, parent, stats) on host."""
+    torch = _torch()
+    out, _ = dg.view(k)
+    d = torch.empty(dg.n, dtype=torch.int64, device="cuda")
+    p = torch.empty(dg.n, dtype=torch.int32, device="cuda")
+    st = SospStats()
+    _check(lib().mosp_sosp_from_scratch(C.byref(out), source, delta, max_w, ws.handle,
+                                        d.data_ptr(), p.data_ptr(), C.byref(st), _stream()),
+           "sospFromScratchGpu")
+    return d.cpu().numpy(), p.cpu().numpy(), st.as_dict()
+
+
+def initial_trees(g: HostGraph, K: int = 0, source: int = 0, ws: Workspace | None = None):
+    """Canonical trees of every objective by the GPU from-scratch search (the
+    product's replacement of `mospPrep init`)."""
+    K = K or g.K
+    dg = DeviceGraph(g)
+    ws = ws or Workspace(g.n)
+    w = g.w.reshape(-1, g.K).astype(np.int64)
+    dist = np.zeros((K, g.n), np.int64)
+    par = np.zeros((K, g.n), np.int32)
+    for k in range(K):
+        maxw = max(int(w[:, k].max()) if g.m else 1, 1)
+        delta = default_delta(g.m, g.n, int(w[:, k].sum()))
+        dist[k], par[k], _ = from_scratch(dg, k, source, delta, maxw, ws)
+    dg.close()
+    return dist, par
+
+
+def sosp_update(dg: DeviceGraph, k: int, changes_from, changes_to, heads, source: int,
+                delta: int, max_w: int, ws: Workspace, dist, parent):
+    """sospUpdateGpu on objective k with host arrays in/out."""
+    torch = _torch()
+    out, inn = dg.view(k)
+    cf, ct, hd = (_dev(_nz(_i32(a)), torch.int32) for a in (changes_from, changes_to, heads))
+    ch = Changes(cf.data_ptr(), ct.data_ptr(), len(changes_from), hd.data_ptr(), len(heads))
+    d = _dev(dist, torch.int64)
+    p = _dev(parent, torch.int32)
+    st = SospStats()
+    _check(lib().mosp_sosp_update(C.byref(out), C.byref(inn), C.byref(ch), source, delta, max_w,
+                                  ws.handle, d.data_ptr(), p.data_ptr(), C.byref(st), _stream()),
+           "sospUpdateGpu")
+    return d.cpu().numpy(), p.cpu().numpy(), st.as_dict()
+
+
+def combined_sosp(parents: np.ndarray, source: int, pref, ws: Workspace, delta: int = 0):
+    """combinedGraphSospGpu on K parent arrays (objective-major)."""
+    torch = _torch()
+    K, n = parents.shape
+    dp = _dev(parents.reshape(-1), torch.int32)
+    d = torch.empty(n, dtype=torch.int64, device="cuda")
+    p = torch.empty(n, dtype=torch.int32, device="cuda")
+    cs = CombineStats()
+    pa = _i32(pref)
+    _check(lib().mosp_combined_sosp(dp.data_ptr(), n, K, _nz(pa), len(pa), source, delta,
+                                    ws.handle, d.data_ptr(), p.data_ptr(), C.byref(cs),
+                                    _stream()), "combinedGraphSospGpu")
+    return d.cpu().numpy(), p.cpu().numpy(), {"n_edges": cs.n_edges, "scale": cs.scale,
+                                              "delta": cs.delta, **cs.search.as_dict()}
+
+
+def path_costs(g: HostGraph, parent, source: int) -> np.ndarray:
+    """mospPathCosts on the GPU: n x K costs (DISTANCE_INF if unreachable)."""
+    torch = _torch()
+    dg = DeviceGraph(g)
+    p = _dev(_i32(parent), torch.int32)
+    c = torch.empty(g.n * g.K, dtype=torch.int64, device="cuda")
+    _check(lib().mosp_path_costs_gpu(dg.handle, p.data_ptr(), source, c.data_ptr(), _stream()),
+           "mospPathCosts")
+    out = c.cpu().numpy().reshape(g.n, g.K)
+    dg.close()
+    return out
+
+
+class Session:
+    """A MOSP update resident in HBM: the updated graph, the old trees, the
+    batch and every output stay on the device; `run()` is one pass of the hot
+    path (K updates + combined graph + its search) in ONE persistent launch,
+    reading the old trees and writing new buffers, so it can be repeated."""
+
+    def __init__(self, original: HostGraph, batch: Batch, init_dist: np.ndarray,
+                 init_parents: np.ndarray, K: int = 0, source: int = 0, pref=(),
+                 delta: int = 0, grid: int = 0):
+        torch = _torch()
+        self.K = min(K or original.K, original.K)
+        self.n, self.source, self.pref = original.n, source, _i32(pref)
+        self.updated, self.mask = apply_batch(original, batch)
+        self.batch = batch
+        self.maxw, wsum = objective_bounds(original, batch, self.K)
+        m = max(original.m, 1)
+        self.delta = np.array([delta if delta > 0 else default_delta(m, self.n, int(wsum[k]))
+                               for k in range(self.K)], np.int64)
+        self.dg = DeviceGraph(self.updated)
+        self.ws = Workspace(self.n, self.K, grid)
+        self.heads = _dev(_nz(_i32(batch.ins_to)), torch.int32)
+        self._keep = []
+        changes = (Changes * self.K)()
+        for k in range(self.K):
+            cf, ct = changed_lists(batch, self.mask, k)
+            a, b = _dev(_nz(cf), torch.int32), _dev(_nz(ct), torch.int32)
+            self._keep += [a, b]
+            changes[k] = Changes(a.data_ptr(), b.data_ptr(), len(cf), self.heads.data_ptr(),
+                                 batch.n_ins)
+        self.changes = changes
+        self.init_dist = _dev(np.asarray(init_dist)[: self.K].reshape(-1), torch.int64)
+        self.init_par = _dev(np.asarray(init_parents)[: self.K].reshape(-1), torch.int32)
+        self.dist = torch.empty(self.K * self.n, dtype=torch.int64, device="cuda")
+        self.par = torch.empty(self.K * self.n, dtype=torch.int32, device="cuda")
+        self.cdist = torch.empty(self.n, dtype=torch.int64, device="cuda")
+        self.cpar = torch.empty(self.n, dtype=torch.int32, device="cuda")
+        self.stats = (SospStats * self.K)()
+        self.cstats = CombineStats()
+
+    def run(self, stream=None) -> None:
+        s = C.c_void_p(stream) if stream is not None else _stream()
+        dl = (C.c_int64 * self.K)(*[int(x) for x in self.delta])
+        mw = (C.c_int64 * self.K)(*[int(x) for x in self.maxw])
+        _check(lib().mosp_update_fused(self.dg.handle, self.K, self.changes, self.source, dl, mw,
+                                       _nz(self.pref), len(self.pref), self.ws.handle,
+                                       self.init_dist.data_ptr(), self.init_par.data_ptr(),
+                                       self.dist.data_ptr(), self.par.data_ptr(),
+                                       self.cdist.data_ptr(), self.cpar.data_ptr(), self.stats,
+                                       C.byref(self.cstats), s), "mosp_update_fused")
+
+    def results(self) -> dict:
+        return {"dist": self.dist.cpu().numpy().reshape(self.K, self.n),
+                "parents": self.par.cpu().numpy().reshape(self.K, self.n),
+                "comb_dist": self.cdist.cpu().numpy(), "comb_parent": self.cpar.cpu().numpy(),
+                "stats": [self.stats[k].as_dict() for k in range(self.K)],
+                "combined_edges": self.cstats.n_edges,
+                "kernel_ms": self.stats[0].kernel_ms,
+                "sosp_phase_ms": self.cstats.sosp_phase_ms,
+                "combined_phase_ms": self.cstats.combined_phase_ms}
+
+    def close(self) -> None:
+        self.dg.close()
+        self.ws.close()

@@ -1,0 +1,301 @@
+"""Parity of the HIP path (libmosp_hip.so via its C ABI) with the oracle.
+
+Bit-exact for every integer output: K updated trees (distances + lowest-id
+parents), the combined graph's distances (units of 1/L) and MOSP tree, and
+the MOSP path costs.  Cases follow the reference's oracle suite
+(src/mospTest.cu): thesis known answers, count-to-infinity regressions,
+graph families x batch kinds, large-weight fallback with ties, the packing
+boundary, determinism, plus the benchmark workloads.
+"""
+from __future__ import annotations
+
+import json
+import pathlib
+import sys
+
+import numpy as np
+import pytest
+
+import gen
+import oracle_lib as O
+from oracle_lib import INF, Batch, i32
+
+pytestmark = pytest.mark.gpu
+GOLDEN = pathlib.Path(__file__).resolve().parent / "golden"
+
+
+def M():
+    import mosp_amd
+    return mosp_amd
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def mgraph(g: O.Graph):
+    return M().HostGraph(g.n, g.K, i32(g.row), i32(g.col), i32(g.w))
+
+
+def mbatch(b: Batch):
+    return M().Batch(b.K, i32(b.ins_from), i32(b.ins_to), i32(b.ins_w), i32(b.del_from),
+                     i32(b.del_to))
+
+
+def check_pipeline(g: O.Graph, b: Batch, pref=(), K=0, fused=True, tag=""):
+    Ku = K or g.K
+    d0, p0 = gen.initial_trees(g, Ku)
+    say(f"case {tag} n={g.n} m={g.m} K={Ku} ins={b.n_ins} del={b.n_del} fused={fused}")
+    got = M().update_host(mgraph(g), mbatch(b), d0, p0, K=Ku, pref=list(pref), fused=fused)
+    want = O.mosp_update(g, b, d0, p0, Ku, 0, list(pref))
+    assert want["ok"]
+    upd = got["updated"]
+    assert np.array_equal(upd.row, want["updated"].row) and np.array_equal(upd.col,
+                                                                           want["updated"].col)
+    for k in range(Ku):
+        assert np.array_equal(got["dist"][k], want["dist"][k]), f"{tag} obj{k} distances"
+        assert np.array_equal(got["parents"][k], want["parents"][k]), f"{tag} obj{k} parents"
+    assert np.array_equal(got["comb_dist"], want["comb_dist"]), f"{tag} combined distances"
+    assert np.array_equal(got["comb_parent"], want["comb_parent"]), f"{tag} MOSP tree"
+    assert got["combined_edges"] == want["combined_edges"]
+    return got, want
+
+
+# ----------------------------------------------------------- known answers --
+
+def test_thesis_known_answer():
+    t = json.loads((GOLDEN / "thesis_example.json").read_text())
+    g = O.graph_from_edges(t["n"], t["K"], [(e[0], e[1], e[2:]) for e in t["edges"]])
+    b = Batch(3, i32([x[0] for x in t["insert"]]), i32([x[1] for x in t["insert"]]),
+              i32([w for x in t["insert"] for w in x[2:]]), i32([x[0] for x in t["delete"]]),
+              i32([x[1] for x in t["delete"]]))
+    d0, p0 = gen.initial_trees(g, 3)
+    for fused in (True, False):
+        for exp in t["mosp"]:
+            r = M().update_host(mgraph(g), mbatch(b), d0, p0, pref=exp["pref"], fused=fused)
+            assert r["parents"].tolist() == t["expected_trees"]
+            path = [6]
+            while r["comb_parent"][path[-1]] >= 0:
+                path.append(int(r["comb_parent"][path[-1]]))
+            assert path == exp["path_to_u7"]
+            costs = M().path_costs(r["updated"], r["comb_parent"], 0)
+            assert costs[6].tolist() == exp["cost_of_u7"]
+
+
+def test_regression_fixtures_single_objective():
+    cases = json.loads((GOLDEN / "regressions.json").read_text())["cases"]
+    for c in cases:
+        n, K = c["n"], c["K"]
+        upd = M().HostGraph(n, K, i32(c["updated_row"]), i32(c["updated_col"]),
+                            i32(c["updated_w"]))
+        b = Batch(K, i32(c["ins_from"]), i32(c["ins_to"]), i32(c["ins_w"]), i32(c["del_from"]),
+                  i32(c["del_to"]))
+        mask = np.array(c["increase_mask"], np.uint32)
+        dg = M().DeviceGraph(upd)
+        ws = M().Workspace(n)
+        for k in range(K):
+            say(f"regression {c['case']} obj{k}")
+            cf, ct = O.changed_lists(b, mask, k)
+            maxw = max(int(np.max(np.asarray(c["w"]).reshape(-1, K)[:, k])),
+                       int(np.max(b.ins_w.reshape(-1, K)[:, k])) if b.n_ins else 1)
+            d, p, st = M().sosp_update(dg, k, cf, ct, b.ins_to, 0, 7, maxw, ws,
+                                       np.array(c["init_dist"][k]), np.array(c["init_parent"][k]))
+            assert d.tolist() == c["expected_dist"][k] and p.tolist() == c["expected_parent"][k]
+
+
+# ------------------------------------------------- graph families x batches --
+
+GRAPHS = {
+    "random-tiny": lambda s: gen.random_graph(12, 18, 3, 50, s),
+    "random-small": lambda s: gen.random_graph(60, 180, 2, 50, s),
+    "random-medium": lambda s: gen.random_graph(800, 2400, 3, 100, s),
+    "grid-small": lambda s: gen.grid_graph(8, 8, 2, 50, 0.1, s),
+    "grid-medium": lambda s: gen.grid_graph(40, 30, 3, 100, 0.15, s),
+}
+CHANGES = {"random-tiny": 6, "random-small": 20, "random-medium": 200, "grid-small": 12,
+           "grid-medium": 150}
+
+
+@pytest.mark.parametrize("gname", list(GRAPHS))
+@pytest.mark.parametrize("kind", gen.BATCH_KINDS)
+def test_families_and_batch_kinds(gname, kind):
+    for rep in range(2):
+        seed = 1000 * rep + len(gname) + 31 * gen.BATCH_KINDS.index(kind)
+        g = GRAPHS[gname](seed)
+        b = gen.make_batch(kind, g, CHANGES[gname], seed)
+        pref = [] if rep == 0 else [1] + [2] * (g.K - 1)
+        check_pipeline(g, b, pref, fused=(rep == 0), tag=f"{gname}/{kind}/{rep}")
+
+
+def test_fused_equals_serial_and_is_deterministic():
+    g = gen.grid_graph(60, 50, 3, 100, 0.2, 5)
+    b = gen.uniform_batch(g, 400, 50, 6, wmax=100)
+    d0, p0 = gen.initial_trees(g, 3)
+    runs = [M().update_host(mgraph(g), mbatch(b), d0, p0, fused=f) for f in (True, True, False)]
+    for r in runs[1:]:
+        for key in ("dist", "parents", "comb_dist", "comb_parent"):
+            assert np.array_equal(r[key], runs[0][key]), key
+
+
+def test_objective_subset_and_skewed_pref():
+    g = gen.random_graph(300, 1200, 4, 60, 11)
+    b = gen.uniform_batch(g, 80, 50, 12, wmax=60)
+    check_pipeline(g, b, pref=[3, 1], K=2, tag="k2-of-4")
+    check_pipeline(g, b, pref=[1, 2, 3, 4], tag="pref-1234")
+
+
+# ------------------------------------------------------ engine entry points --
+
+@pytest.mark.parametrize("seed", range(6))
+def test_from_scratch_matches_dijkstra(seed):
+    g = gen.random_graph(500, 1500 + 100 * seed, 2, 1 if seed == 0 else 40, seed)
+    # a few vertices with no in-edges -> unreachable
+    dg = M().DeviceGraph(mgraph(g))
+    ws = M().Workspace(g.n)
+    for k in range(g.K):
+        for src in (0, 7):
+            say(f"from-scratch seed={seed} k={k} src={src}")
+            w = g.w.reshape(-1, g.K)[:, k]
+            d, p, st = M().from_scratch(dg, k, src, M().default_delta(g.m, g.n, int(w.sum())),
+                                        int(w.max()), ws)
+            rd, rp = O.dijkstra(g, k, src)
+            assert np.array_equal(d, rd) and np.array_equal(p, rp)
+
+
+def test_combined_standalone_and_path_costs():
+    g = gen.grid_graph(30, 25, 4, 50, 0.1, 3)
+    d0, p0 = gen.initial_trees(g, 4)
+    ws = M().Workspace(g.n)
+    for pref in ([], [1, 2, 3, 4], [5, 5, 1, 5]):
+        say(f"combined pref={pref}")
+        d, p, st = M().combined_sosp(p0, 0, pref, ws)
+        comb = O.combined_graph(p0, g.n, 4, 0, pref)
+        rd, rp = O.dijkstra(comb, 0, 0)
+        assert np.array_equal(d, rd) and np.array_equal(p, rp)
+        assert st["n_edges"] == comb.m
+        costs = M().path_costs(mgraph(g), p, 0)
+        want = O.path_costs(g, p, 0)
+        assert np.array_equal(costs, want)
+
+
+def test_grid_sizes_do_not_change_results():
+    """The persistent kernel with 1, 3 and the default number of workgroups."""
+    g = gen.grid_graph(40, 40, 1, 80, 0.2, 9)
+    b = gen.uniform_batch(g, 120, 50, 10, wmax=80)
+    upd, mask = O.apply_batch(g, b)
+    d0, p0 = O.dijkstra(g, 0, 0)
+    rd, rp = O.dijkstra(upd, 0, 0)
+    cf, ct = O.changed_lists(b, mask, 0)
+    dg = M().DeviceGraph(mgraph(upd))
+    for grid in (1, 3, 0):
+        ws = M().Workspace(upd.n, 1, grid)
+        say(f"grid={grid} -> {ws.grid} workgroups")
+        d, p, _ = M().sosp_update(dg, 0, cf, ct, b.ins_to, 0, 50, 80, ws, d0, p0)
+        assert np.array_equal(d, rd) and np.array_equal(p, rp)
+
+
+def test_packing_boundary():
+    """n = 2^17 - 1 path with W = 2^30 + 2^14 (src/mospTest.cu:723-815): the
+    packed format is used at its limit and over-bound candidates must drop."""
+    n = (1 << 17) - 1
+    W = (1 << 30) + (1 << 14)
+
+    def path(back):
+        edges = [(u, u + 1, [W]) for u in range(n - 1)]
+        if back:
+            edges.append((n - 1, 1, [W]))
+        return O.graph_from_edges(n, 1, edges)
+
+    for name, back, ins in (("pull", False, (n - 1, 1, W)), ("push", True, (n - 2, n - 1, W - 1)),
+                            ("from-scratch", True, None)):
+        say(f"packing-boundary/{name}")
+        orig = path(back)
+        b = Batch(1, i32([ins[0]] if ins else []), i32([ins[1]] if ins else []),
+                  i32([ins[2]] if ins else []), i32([]), i32([]))
+        upd, mask = O.apply_batch(orig, b)
+        d0, p0 = O.dijkstra(orig, 0, 0)
+        rd, rp = O.dijkstra(upd, 0, 0)
+        dg = M().DeviceGraph(mgraph(upd))
+        ws = M().Workspace(n)
+        delta = M().default_delta(upd.m, n, W * n)
+        if ins:
+            d, p, st = M().sosp_update(dg, 0, [], [], b.ins_to, 0, delta, W, ws, d0, p0)
+        else:
+            d, p, st = M().from_scratch(dg, 0, 0, delta, W, ws)
+        assert st["packed_parents"] == 1
+        assert np.array_equal(d, rd) and np.array_equal(p, rp)
+
+
+def test_large_weights_fallback_with_ties():
+    """(n-1) * maxW does not fit next to the parent ids: distance-only words
+    and lowest-id parent recovery (src/mospTest.cu:597-700)."""
+    for W, tag in ((2**31 - 1, "int-max"), (2_000_000_000, "ties")):
+        wmin = W if tag == "ties" else 1
+        g = gen.grid_graph(320, 320, 2, W, 0.1, 13, wmin=wmin)
+        b = gen.uniform_batch(g, 2000, 50, 15, wmax=W, wmin=wmin)
+        got, _ = check_pipeline(g, b, tag=f"large-weights/{tag}")
+        assert all(s["packed_parents"] == 0 for s in got["stats"])
+
+
+def test_edge_cases():
+    # empty batch; insert into the source; duplicate arcs; isolated vertices
+    g = O.graph_from_edges(6, 2, [(0, 1, [3, 1]), (0, 1, [1, 5]), (1, 2, [1, 1]),
+                                  (2, 0, [1, 1]), (3, 4, [2, 2])])
+    empty = Batch(2, i32([]), i32([]), i32([]), i32([]), i32([]))
+    check_pipeline(g, empty, tag="empty")
+    b = Batch(2, i32([2, 4]), i32([0, 5]), i32([1, 1, 9, 9]), i32([0]), i32([1]))
+    check_pipeline(g, b, tag="insert-into-source")
+
+
+# ------------------------------------------------------ benchmark workloads --
+
+def test_config1_small_grid_against_cpu_path():
+    """30x30 grid, K=3, 100 changes (50% inserts), Pref (1,1,1)."""
+    mg = M().gen_road_grid(30, 30, 3, seed=1, drop=0.0, diag=0.0)
+    assert mg.n == 900 and mg.m == 3480
+    mb = M().gen_batch(mg, 100, 50.0, seed=2, safe=True)
+    assert mb.n_ins == 50 and mb.n_del == 50
+    g = O.Graph(mg.n, 3, mg.row, mg.col, mg.w)
+    b = Batch(3, mb.ins_from, mb.ins_to, mb.ins_w, mb.del_from, mb.del_to)
+    check_pipeline(g, b, pref=[1, 1, 1], tag="config1")
+    check_pipeline(g, b, pref=[1, 1, 1], fused=False, tag="config1-serial")
+
+
+def test_config2_road_grid_full_size():
+    """1400x1400 road-like grid (1.96M vertices), K=3, 50k safe changes:
+    every tree equals canonical Dijkstra on the updated graph, the MOSP tree
+    equals Dijkstra on the host combined graph, and the GPU initial trees
+    equal Dijkstra on the original graph."""
+    mg = M().gen_road_grid(1400, 1400, 3, seed=7)
+    mb = M().gen_batch(mg, 50_000, 50.0, seed=8, safe=True)
+    g = O.Graph(mg.n, 3, mg.row, mg.col, mg.w)
+    say(f"config2 n={g.n} m={g.m} ins={mb.n_ins} del={mb.n_del}")
+    d0, p0 = M().initial_trees(mg, 3)
+    for k in range(3):
+        rd, rp = O.dijkstra(g, k, 0)
+        assert np.array_equal(d0[k], rd) and np.array_equal(p0[k], rp), f"initial tree {k}"
+    r = M().update_host(mg, mb, d0, p0, fused=True)
+    upd = O.Graph(mg.n, 3, r["updated"].row, r["updated"].col, r["updated"].w)
+    for k in range(3):
+        rd, rp = O.dijkstra(upd, k, 0)
+        assert np.array_equal(r["dist"][k], rd) and np.array_equal(r["parents"][k], rp)
+        assert (rd < INF // 2).all(), "safe batch keeps every vertex reachable"
+    comb = O.combined_graph(r["parents"], g.n, 3, 0, [])
+    cd, cp = O.dijkstra(comb, 0, 0)
+    assert np.array_equal(r["comb_dist"], cd) and np.array_equal(r["comb_parent"], cp)
+    say(f"config2 gpu_compute_ms={r['timings_ms']['gpu_compute']:.3f} stats={r['stats'][0]}")
+
+
+def test_session_repeats_out_of_place():
+    mg = M().gen_road_grid(200, 150, 3, seed=21)
+    mb = M().gen_batch(mg, 3000, 50.0, seed=22, safe=True)
+    d0, p0 = M().initial_trees(mg, 3)
+    ref = M().update_host(mg, mb, d0, p0)
+    s = M().Session(mg, mb, d0, p0)
+    for i in range(3):
+        s.run()
+        r = s.results()
+        for key in ("dist", "parents", "comb_dist", "comb_parent"):
+            assert np.array_equal(r[key], ref[key]), (i, key)
+    s.close()
