@@ -76,6 +76,11 @@ def lib() -> C.CDLL:
     if not LIB_PATH.exists():
         raise RuntimeError(f"{LIB_PATH} is missing: build it with `make -C mosp-cuda_amd` "
                            "(python -c 'import __graft_entry__; __graft_entry__.build()')")
+    # torch's HIP runtime must be the process's only one: importing torch
+    # first loads it (soname libamdhip64.so.7), and the library's dependency
+    # then binds to that same copy, so torch buffers and streams are valid
+    # here.  Loading the library first would bring a second runtime in.
+    import torch  # noqa: F401
     L = C.CDLL(str(LIB_PATH))
     L.mosp_last_error.restype = C.c_char_p
     L.mosp_ws_create.restype = _VP
