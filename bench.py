@@ -129,6 +129,7 @@ def main() -> int:
     log(f"{ms_per_step:.3f} ms/step; kernel {np.mean(kernel_ms):.3f} ms "
         f"(K updates {np.mean(sosp_ms):.3f} ms, combined {np.mean(comb_ms):.3f} ms); "
         f"stats {res['stats']}")
+    log(f"phase profile (last step, device clock): {sess.ws.last_profile()}")
 
     # Algorithmic bytes of one update.
     changed = [M.changed_lists(b, sess.mask, k) for k in range(3)]

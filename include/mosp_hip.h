@@ -91,6 +91,14 @@ void mosp_ws_destroy(mosp_ws *ws);
 int mosp_ws_set_grid(mosp_ws *ws, int32_t workgroups);
 int32_t mosp_ws_grid(const mosp_ws *ws);
 
+/* Device-clock profile of the last launch on this workspace (no reference
+ * counterpart; the reference's --timing stages are host-side): ms[i] = time
+ * of phase i (pack, jump, invalidate, pull, near-far, unpack, combined count,
+ * scan, fill, search, unpack; 0 if it did not run), *barriers = grid
+ * barriers executed.  n_phases <= 11.  Returns the number of phases filled. */
+int mosp_ws_last_profile(const mosp_ws *ws, double *ms, int32_t n_phases,
+                         int32_t *barriers);
+
 /* Replaces defaultDelta (src/sospUpdateGpu.cu:607-616). */
 int64_t mosp_default_delta(int64_t m, int32_t n, int64_t weight_sum);
 

@@ -80,6 +80,7 @@ struct mosp_ws {
   int maxBlocks = 0;        // cooperative residency limit
   int gridBlocks = 0;       // workgroups per launch
   int gridOverride = 0;
+  int laneBits = 2;         // 4 lanes per frontier vertex
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
 };
 
@@ -138,6 +139,9 @@ int ws_init(mosp_ws *ws) {
     blocks = ws->gridOverride;
   }
   ws->gridBlocks = std::max(1, std::min(blocks, ws->maxBlocks));
+  if (const char *env = std::getenv("MOSP_LANE_BITS")) {
+    ws->laneBits = std::max(0, std::min(5, std::atoi(env)));
+  }
   HIP_TRY(hipMalloc(&ws->dDesc, sizeof(ObjDesc) * (MOSP_MAX_OBJ + 1)));
   HIP_TRY(hipHostMalloc(&ws->hDesc, sizeof(ObjDesc) * (MOSP_MAX_OBJ + 1),
                         hipHostMallocDefault));
@@ -263,6 +267,7 @@ int run_launch(mosp_ws *ws, LaunchArgs &args, int descCount,
                hipStream_t stream, float *kernelMs) {
   args.ctl = ws->dCtl;
   args.genDev = ws->genDev;
+  args.laneBits = ws->laneBits;
   args.cPartials = ws->cPartials;
   if (descCount > 0) {
     HIP_TRY(hipMemcpyAsync(ws->dDesc, ws->hDesc, sizeof(ObjDesc) * descCount,
@@ -355,6 +360,28 @@ extern "C" int mosp_ws_set_grid(mosp_ws *ws, int32_t workgroups) {
 }
 
 extern "C" int32_t mosp_ws_grid(const mosp_ws *ws) { return ws->gridBlocks; }
+
+extern "C" int mosp_ws_last_profile(const mosp_ws *ws, double *ms, int32_t nPhases,
+                                    int32_t *barriers) {
+  if (ws == nullptr || ws->hCtl == nullptr) {
+    return 0;
+  }
+  const Ctl &c = *ws->hCtl;
+  const int count = std::min<int>(nPhases, PHASE_COUNT);
+  u64 prev = c.clkStart;
+  for (int i = 0; i < count; ++i) {
+    if (c.clk[i] != 0 && c.clk[i] >= prev) {
+      ms[i] = static_cast<double>(c.clk[i] - prev) / ws->clockKhz;
+      prev = c.clk[i];
+    } else {
+      ms[i] = 0;
+    }
+  }
+  if (barriers != nullptr) {
+    *barriers = static_cast<int32_t>(c.barriers);
+  }
+  return count;
+}
 
 extern "C" int64_t mosp_default_delta(int64_t m, int32_t n, int64_t weightSum) {
   if (m <= 0 || n <= 0) {

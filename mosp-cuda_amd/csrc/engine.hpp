@@ -31,6 +31,13 @@ enum : int {
   PH_COMBINED = 4,    // combined build from cParents + from-scratch search
 };
 
+// Phase stamps recorded by the persistent kernel (Ctl::clk index).
+enum : int {
+  PHASE_PACK = 0, PHASE_JUMP, PHASE_INVALIDATE, PHASE_PULL, PHASE_NEARFAR,
+  PHASE_UNPACK, PHASE_COMB_COUNT, PHASE_COMB_SCAN, PHASE_COMB_FILL,
+  PHASE_COMB_SEARCH, PHASE_COMB_UNPACK, PHASE_COUNT
+};
+
 // Scratch of one objective (a slice of the workspace).
 struct ObjState {
   u64 *packed;     // (distance << b | parent) words, n
@@ -77,15 +84,25 @@ struct ObjCtl {
   u64 pushes;
 };
 
+constexpr int BARRIER_GROUPS = 16;  // first-level arrival counters
+constexpr int LINE_WORDS = 32;      // one 128-byte line per counter
+
 struct Ctl {
-  unsigned barrier;        // monotonic grid-barrier arrival counter
+  // Two-level grid barrier: workgroup b arrives at sub[b % 16]; the last of
+  // each group arrives at top; the last group publishes the epoch in flag.
+  // All three only grow within a launch (zeroed by the reset kernel).
+  unsigned barrierSub[BARRIER_GROUPS * LINE_WORDS];
+  unsigned barrierTop[LINE_WORDS];
+  unsigned barrierFlag[LINE_WORDS];
   unsigned jumpActive[3];  // pointer jumping: some vertex still jumps
   unsigned rounds;
   unsigned generationOut;  // last stamp generation used
   u64 combEdges;           // combined graph: edge count
   u64 combWeightSum;       // combined graph: weight sum
   u64 combDelta;           // near-far width used for Step 3
-  u64 clkStart, clkSospEnd, clkEnd;  // wall_clock64 stamps (100 MHz)
+  u64 clkStart, clkSospEnd, clkEnd;  // wall_clock64 stamps
+  u64 clk[16];             // phase ends (PHASE_* below), wall_clock64
+  unsigned barriers;       // grid barriers executed by the launch
   ObjCtl obj[MOSP_MAX_OBJ + 1];
 };
 
@@ -97,6 +114,7 @@ struct LaunchArgs {
   int phases;
   int maxRounds;           // pointer-jumping rounds: ceil(log2 n) + 1
   int anyFallback;         // some objective of objs uses distance-only words
+  int laneBits;            // 2^laneBits lanes share a frontier vertex
   int *genDev;             // device-resident stamp generation (persists)
   Ctl *ctl;
   // Combined graph (PH_COMBINED): parents of cK trees, objective-major.

@@ -32,9 +32,6 @@ enum : int { MODE_DONE = 0, MODE_PUSH = 1, MODE_SPLIT = 2 };
 // Device helpers
 // ---------------------------------------------------------------------------
 
-__device__ __forceinline__ unsigned load_acquire(const unsigned *p) {
-  return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
-}
 __device__ __forceinline__ unsigned load_relaxed(const unsigned *p) {
   return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
@@ -49,22 +46,45 @@ __device__ __forceinline__ void store_relaxed(u64 *p, u64 v) {
 }
 
 // Grid-wide barrier for a cooperative launch (all workgroups co-resident).
-// The counter only grows: barrier number e completes when it reaches
-// e * gridDim.x, so it never needs a reset inside the launch.
+// Two levels so that no counter sees more than ~gridDim/16 same-address
+// atomics per barrier: workgroup b arrives at sub[b % 16]; the last arrival
+// of a group arrives at top; the last group stores the epoch in flag, which
+// every workgroup polls with relaxed loads (an acquire per poll would
+// invalidate the XCD's L2 on every poll) before one acquire fence.  The
+// counters only grow: barrier e completes when sub reaches e * groupSize and
+// top reaches e * groups, so nothing is reset inside a launch.  The last
+// arrivals re-fence (acquire + release) before passing the arrival on, so
+// every workgroup's writes happen-before every workgroup's later reads.
 struct GridBarrier {
-  unsigned *counter;
+  Ctl *ctl;
   unsigned epoch;
 
   __device__ void sync() {
     __syncthreads();
     ++epoch;
     if (threadIdx.x == 0) {
-      const unsigned target = epoch * gridDim.x;
-      __hip_atomic_fetch_add(counter, 1u, __ATOMIC_RELEASE,
-                             __HIP_MEMORY_SCOPE_AGENT);
-      while (load_acquire(counter) < target) {
+      const unsigned G = gridDim.x;
+      const unsigned groups = G < BARRIER_GROUPS ? G : BARRIER_GROUPS;
+      const unsigned g = blockIdx.x % BARRIER_GROUPS;
+      const unsigned groupSize =
+          G / BARRIER_GROUPS + (g < G % BARRIER_GROUPS ? 1u : 0u);
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+      const unsigned sub = __hip_atomic_fetch_add(
+          &ctl->barrierSub[g * LINE_WORDS], 1u, __ATOMIC_RELAXED,
+          __HIP_MEMORY_SCOPE_AGENT);
+      if (sub + 1 == epoch * groupSize) {
+        __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "agent");
+        const unsigned top = __hip_atomic_fetch_add(
+            &ctl->barrierTop[0], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (top + 1 == epoch * groups) {
+          __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "agent");
+          store_relaxed(&ctl->barrierFlag[0], epoch);
+        }
+      }
+      while (load_relaxed(&ctl->barrierFlag[0]) < epoch) {
         __builtin_amdgcn_s_sleep(1);
       }
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
     }
     __syncthreads();
   }
@@ -114,6 +134,13 @@ __device__ __forceinline__ u64 dist_of(u64 word, int bits) {
   return word >> bits;
 }
 
+// Stamp the end of a phase (called right after a grid barrier).
+__device__ __forceinline__ void stamp(Ctl *ctl, int phase, unsigned tid) {
+  if (tid == 0) {
+    ctl->clk[phase] = wall_clock64();
+  }
+}
+
 struct Threads {
   unsigned id;     // global thread index
   unsigned count;  // threads in the grid
@@ -130,16 +157,140 @@ struct NearFarShared {
   u64 thr[MOSP_MAX_OBJ];
   u64 newThr[MOSP_MAX_OBJ];
   u64 delta[MOSP_MAX_OBJ];
+  u64 farMin[MOSP_MAX_OBJ];   // this workgroup's far appends, this iteration
   unsigned cnt[MOSP_MAX_OBJ];
   int epoch[MOSP_MAX_OBJ];
   int mode[MOSP_MAX_OBJ];
+  int waveBegin[MOSP_MAX_OBJ];  // waves of the grid working on objective k
+  int waveEnd[MOSP_MAX_OBJ];
   int any;
 };
 
+// Split the grid's waves over the active objectives: one wave each, the rest
+// in proportion to their item counts, leftovers to the largest.  Integer
+// arithmetic only, so every workgroup derives the same partition.
+__device__ void partition_waves(NearFarShared &sh, int nobj, unsigned waves) {
+  u64 total = 0;
+  int active = 0, largest = -1;
+  for (int k = 0; k < nobj; ++k) {
+    sh.waveBegin[k] = sh.waveEnd[k] = 0;
+    if (sh.mode[k] != MODE_DONE) {
+      total += sh.cnt[k];
+      ++active;
+      if (largest < 0 || sh.cnt[k] > sh.cnt[largest]) largest = k;
+    }
+  }
+  if (active == 0) return;
+  const u64 spare = waves > static_cast<unsigned>(active) ? waves - active : 0;
+  unsigned next = 0, given = 0;
+  int extra[MOSP_MAX_OBJ];
+  for (int k = 0; k < nobj; ++k) {
+    extra[k] = 0;
+    if (sh.mode[k] != MODE_DONE) {
+      extra[k] = static_cast<int>(spare * sh.cnt[k] / (total > 0 ? total : 1));
+      given += 1 + extra[k];
+    }
+  }
+  if (given < waves) extra[largest] += static_cast<int>(waves - given);
+  for (int k = 0; k < nobj; ++k) {
+    if (sh.mode[k] == MODE_DONE) continue;
+    const unsigned w = 1 + extra[k];
+    sh.waveBegin[k] = static_cast<int>(next < waves ? next : waves - 1);
+    next += w;
+    sh.waveEnd[k] = static_cast<int>(next < waves ? next : waves);
+    if (sh.waveEnd[k] <= sh.waveBegin[k]) sh.waveEnd[k] = sh.waveBegin[k] + 1;
+  }
+}
+
+// Relax the out-edges of frontier vertices of one objective: groups of
+// 2^laneBits lanes share a vertex and stride over its edges.
+__device__ void push_items(const ObjDesc &D, ObjCtl &c, NearFarShared &sh, int k,
+                           unsigned it, int source, int gen, unsigned local,
+                           unsigned span, int laneBits) {
+  const unsigned cnt = sh.cnt[k];
+  const int e = sh.epoch[k];
+  const int bits = D.parentBits;
+  const u64 noParent = D.noParent;
+  const u64 thr = sh.thr[k];
+  const u64 maxDist = D.maxDistance;
+  u64 *packed = D.s.packed;
+  const int *in = D.s.near[it & 1];
+  int *outNear = D.s.near[(it + 1) & 1];
+  int *farBuf = D.s.far[e & 1];
+  unsigned *nextCnt = &c.nearCount[(it + 1) % 3];
+  unsigned *farCnt = &c.farCount[e % 3];
+  const unsigned lanes = 1u << laneBits;
+  const unsigned sub = local & (lanes - 1);
+  const unsigned groups = span >> laneBits;
+  for (unsigned i = local >> laneBits; i < cnt; i += groups) {
+    const int u = in[i];
+    const u64 word = packed[u];
+    if (word == PACKED_INF) {
+      continue;
+    }
+    const u64 du = dist_of(word, bits);
+    const int end = D.outRow[u + 1];
+    for (int x = D.outRow[u] + static_cast<int>(sub); x < end; x += lanes) {
+      const int w = D.outCol[x];
+      if (w == source) {
+        continue;  // the source is never relaxed
+      }
+      const u64 nd = du + static_cast<u64>(D.outW[x]);
+      if (nd > maxDist) {
+        continue;  // never shortest; would not fit the packed word
+      }
+      const u64 cand = pack(nd, u, bits, noParent);
+      if (cand >= packed[w]) {
+        continue;  // stale pre-check; the atomic below decides
+      }
+      const u64 prev = atomicMin(&packed[w], cand);
+      if (nd < dist_of(prev, bits)) {
+        if (nd < thr) {
+          if (D.s.stamp[w] != gen && claim(D.s.stamp, w, gen)) {
+            outNear[wave_append(nextCnt)] = w;
+          }
+        } else if (D.s.inFar[w] == 0 && atomicExch(&D.s.inFar[w], 1) == 0) {
+          farBuf[wave_append(farCnt)] = w;
+          atomicMin(&sh.farMin[k], nd);
+        }
+      }
+    }
+  }
+}
+
+// Raise the threshold past the far pile: move the entries below it to the
+// next near frontier, keep the rest in the other far buffer.
+__device__ void split_items(const ObjDesc &D, ObjCtl &c, NearFarShared &sh, int k,
+                            unsigned it, int gen, unsigned local, unsigned span) {
+  const unsigned cnt = sh.cnt[k];
+  const int e = sh.epoch[k];
+  const int bits = D.parentBits;
+  const u64 thr = sh.newThr[k];
+  const int *in = D.s.far[e & 1];
+  int *far2 = D.s.far[(e + 1) & 1];
+  int *outNear = D.s.near[(it + 1) & 1];
+  unsigned *nextCnt = &c.nearCount[(it + 1) % 3];
+  unsigned *far2Cnt = &c.farCount[(e + 1) % 3];
+  for (unsigned i = local; i < cnt; i += span) {
+    const int v = in[i];
+    const u64 word = D.s.packed[v];
+    const u64 d = word == PACKED_INF ? PACKED_INF : dist_of(word, bits);
+    if (d < thr) {
+      D.s.inFar[v] = 0;
+      if (claim(D.s.stamp, v, gen)) {
+        outNear[wave_append(nextCnt)] = v;
+      }
+    } else {
+      far2[wave_append(far2Cnt)] = v;
+      atomicMin(&sh.farMin[k], d);
+    }
+  }
+}
+
 __device__ int near_far(const ObjDesc *objs, int nobj, ObjCtl *ctls,
                         int source, int generation, u64 initThr,
-                        const u64 *deltaOverride, GridBarrier &grid,
-                        Threads t, NearFarShared &sh) {
+                        const u64 *deltaOverride, int laneBits,
+                        GridBarrier &grid, Threads t, NearFarShared &sh) {
   if (threadIdx.x < static_cast<unsigned>(nobj)) {
     const int k = threadIdx.x;
     sh.thr[k] = initThr;
@@ -147,8 +298,10 @@ __device__ int near_far(const ObjDesc *objs, int nobj, ObjCtl *ctls,
     sh.delta[k] = deltaOverride != nullptr ? load_relaxed(deltaOverride)
                                            : objs[k].delta;
   }
+  const unsigned waves = t.count / 64;
+  const unsigned wave = t.id / 64;
   for (unsigned it = 0;; ++it) {
-    const unsigned cur = it % 3, nxt = (it + 1) % 3, old = (it + 2) % 3;
+    const unsigned old = (it + 2) % 3;
     if (threadIdx.x == 0) {
       sh.any = 0;
     }
@@ -158,8 +311,9 @@ __device__ int near_far(const ObjDesc *objs, int nobj, ObjCtl *ctls,
     if (threadIdx.x < static_cast<unsigned>(nobj)) {
       const int k = threadIdx.x;
       ObjCtl &c = ctls[k];
-      const unsigned nc = load_relaxed(&c.nearCount[cur]);
+      const unsigned nc = load_relaxed(&c.nearCount[it % 3]);
       const int e = sh.epoch[k];
+      sh.farMin[k] = PACKED_INF;
       if (nc > 0) {
         sh.mode[k] = MODE_PUSH;
         sh.cnt[k] = nc;
@@ -182,6 +336,10 @@ __device__ int near_far(const ObjDesc *objs, int nobj, ObjCtl *ctls,
     if (!sh.any) {
       break;
     }
+    if (threadIdx.x == 0) {
+      partition_waves(sh, nobj, waves);
+    }
+    __syncthreads();
     const int gen = ++generation;
     if (t.id == 0) {
       // Slots that nobody reads or appends to in this iteration.
@@ -199,81 +357,34 @@ __device__ int near_far(const ObjDesc *objs, int nobj, ObjCtl *ctls,
         }
       }
     }
+    // This wave's objective (wave-uniform, so descriptor loads are scalar).
+    int mine = -1;
     for (int k = 0; k < nobj; ++k) {
-      const int mode = sh.mode[k];
-      if (mode == MODE_DONE) {
-        continue;
+      if (static_cast<int>(wave) >= sh.waveBegin[k] &&
+          static_cast<int>(wave) < sh.waveEnd[k] && sh.mode[k] != MODE_DONE) {
+        mine = k;
       }
-      const ObjDesc &D = objs[k];
-      ObjCtl &c = ctls[k];
-      const unsigned cnt = sh.cnt[k];
-      const int e = sh.epoch[k];
-      const int bits = D.parentBits;
-      const u64 noParent = D.noParent;
-      u64 *packed = D.s.packed;
-      int *outNear = D.s.near[(it + 1) & 1];
-      u64 localMin = PACKED_INF;
-      if (mode == MODE_PUSH) {
-        const u64 thr = sh.thr[k];
-        const u64 maxDist = D.maxDistance;
-        const int *in = D.s.near[it & 1];
-        int *farBuf = D.s.far[e & 1];
-        unsigned *farCnt = &c.farCount[e % 3];
-        for (unsigned i = t.id; i < cnt; i += t.count) {
-          const int u = in[i];
-          const u64 word = packed[u];
-          if (word == PACKED_INF) {
-            continue;
-          }
-          const u64 du = dist_of(word, bits);
-          const int begin = D.outRow[u], end = D.outRow[u + 1];
-          for (int x = begin; x < end; ++x) {
-            const int w = D.outCol[x];
-            if (w == source) {
-              continue;  // the source is never relaxed
-            }
-            const u64 nd = du + static_cast<u64>(D.outW[x]);
-            if (nd > maxDist) {
-              continue;  // never shortest; would not fit the packed word
-            }
-            const u64 cand = pack(nd, u, bits, noParent);
-            if (cand >= packed[w]) {
-              continue;  // stale pre-check; the atomic below decides
-            }
-            const u64 prev = atomicMin(&packed[w], cand);
-            if (nd < dist_of(prev, bits)) {
-              if (nd < thr) {
-                if (claim(D.s.stamp, w, gen)) {
-                  outNear[wave_append(&c.nearCount[nxt])] = w;
-                }
-              } else if (atomicExch(&D.s.inFar[w], 1) == 0) {
-                farBuf[wave_append(farCnt)] = w;
-                localMin = nd < localMin ? nd : localMin;
-              }
-            }
-          }
-        }
-        wave_min_into(localMin, &c.farMin[e % 3]);
-      } else {  // MODE_SPLIT: raise the threshold past the far pile
-        const u64 thr = sh.newThr[k];
-        const int *in = D.s.far[e & 1];
-        int *far2 = D.s.far[(e + 1) & 1];
-        unsigned *far2Cnt = &c.farCount[(e + 1) % 3];
-        for (unsigned i = t.id; i < cnt; i += t.count) {
-          const int v = in[i];
-          const u64 word = packed[v];
-          const u64 d = word == PACKED_INF ? PACKED_INF : dist_of(word, bits);
-          if (d < thr) {
-            D.s.inFar[v] = 0;
-            if (claim(D.s.stamp, v, gen)) {
-              outNear[wave_append(&c.nearCount[nxt])] = v;
-            }
-          } else {
-            far2[wave_append(far2Cnt)] = v;
-            localMin = d < localMin ? d : localMin;
-          }
-        }
-        wave_min_into(localMin, &c.farMin[(e + 1) % 3]);
+    }
+    mine = __builtin_amdgcn_readfirstlane(mine);
+    if (mine >= 0) {
+      const ObjDesc &D = objs[mine];
+      ObjCtl &c = ctls[mine];
+      const unsigned first = static_cast<unsigned>(sh.waveBegin[mine]) * 64;
+      const unsigned span = static_cast<unsigned>(sh.waveEnd[mine] - sh.waveBegin[mine]) * 64;
+      if (sh.mode[mine] == MODE_PUSH) {
+        push_items(D, c, sh, mine, it, source, gen, t.id - first, span, laneBits);
+      } else {
+        split_items(D, c, sh, mine, it, gen, t.id - first, span);
+      }
+    }
+    __syncthreads();
+    // Fold this workgroup's far minimum into the objective's slot.
+    if (threadIdx.x < static_cast<unsigned>(nobj)) {
+      const int k = threadIdx.x;
+      const u64 fm = sh.farMin[k];
+      if (fm != PACKED_INF) {
+        const int slot = sh.mode[k] == MODE_SPLIT ? (sh.epoch[k] + 1) % 3 : sh.epoch[k] % 3;
+        atomicMin(&ctls[k].farMin[slot], fm);
       }
     }
     grid.sync();
@@ -292,7 +403,11 @@ __device__ int near_far(const ObjDesc *objs, int nobj, ObjCtl *ctls,
 // the recovery pass (src/sospUpdateGpu.cu:478-497).
 __device__ void unpack(const ObjDesc &D, int n, int source, Threads t) {
   const int bits = D.parentBits;
+  const bool clearFlags = D.nChanged > 0 && D.s.flag != nullptr;
   for (int v = t.id; v < n; v += t.count) {
+    if (clearFlags) {
+      D.s.flag[v] = 0;  // invalidation marks are left clean for the next update
+    }
     const u64 word = D.s.packed[v];
     if (bits > 0) {
       if (word == PACKED_INF) {
@@ -381,6 +496,7 @@ __device__ int update_step1(const LaunchArgs &a, GridBarrier &grid, Threads t,
     }
   }
   grid.sync();
+  stamp(ctl, PHASE_PACK, t.id);
 
   // Subtree invalidation by pointer jumping.  Invariant for every x:
   // flag[x] == 1 implies a root among x and its ancestors; flag[x] == 0
@@ -424,31 +540,44 @@ __device__ int update_step1(const LaunchArgs &a, GridBarrier &grid, Threads t,
       }
     }
   }
+  stamp(ctl, PHASE_JUMP, t.id);
 
-  // Invalidate flagged vertices; candidates = invalidated + insert heads,
-  // de-duplicated by a stamp claim.  Flags are left clean.
+  // Invalidate flagged vertices; candidates = invalidated + insert heads.
+  // Each flagged vertex is visited by exactly one thread, so it is listed
+  // with a plain stamp store; insert heads that are not flagged are
+  // de-duplicated among themselves by a stamp claim.  Flags stay set until
+  // the unpack sweep clears them (they are read-only here).
   const int genInv = ++generation;
   for (int k = 0; k < a.nobj; ++k) {
     const ObjDesc &D = a.objs[k];
     ObjCtl &c = ctl->obj[k];
-    for (int v = t.id; v < n; v += t.count) {
-      if (D.s.flag[v]) {
-        D.s.flag[v] = 0;
-        D.s.packed[v] = PACKED_INF;
-        wave_append(&c.invalidated);
-        if (claim(D.s.stamp, v, genInv)) {
+    unsigned count = 0;
+    if (D.nChanged > 0) {
+      for (int v = t.id; v < n; v += t.count) {
+        if (D.s.flag[v]) {
+          D.s.packed[v] = PACKED_INF;
+          D.s.stamp[v] = genInv;
           D.s.cand[wave_append(&c.candCount)] = v;
+          ++count;
         }
       }
     }
     for (int i = t.id; i < D.nHeads; i += t.count) {
       const int v = D.heads[i];
-      if (v != a.source && claim(D.s.stamp, v, genInv)) {
+      if (v != a.source && !D.s.flag[v] && claim(D.s.stamp, v, genInv)) {
         D.s.cand[wave_append(&c.candCount)] = v;
       }
     }
+#pragma unroll
+    for (int offset = 32; offset > 0; offset >>= 1) {
+      count += __shfl_xor(count, offset);
+    }
+    if (__lane_id() == 0 && count > 0) {
+      atomicAdd(&c.invalidated, count);
+    }
   }
   grid.sync();
+  stamp(ctl, PHASE_INVALIDATE, t.id);
 
   // Pull: each candidate takes the best (distance, parent) over its
   // in-neighbours; improved ones start in the far pile of epoch 0 (the first
@@ -489,6 +618,7 @@ __device__ int update_step1(const LaunchArgs &a, GridBarrier &grid, Threads t,
     wave_min_into(localMin, &c.farMin[0]);
   }
   grid.sync();
+  stamp(ctl, PHASE_PULL, t.id);
   return generation;
 }
 
@@ -554,6 +684,7 @@ __device__ void combined_build(const LaunchArgs &a, GridBarrier &grid,
     atomicAdd(&ctl->combWeightSum, wsum);
   }
   grid.sync();
+  stamp(ctl, PHASE_COMB_COUNT, t.id);
 
   // Exclusive scan of the n + 1 degrees into row pointers: every workgroup
   // owns one contiguous chunk; chunk totals, then offsets, then local scans.
@@ -606,6 +737,7 @@ __device__ void combined_build(const LaunchArgs &a, GridBarrier &grid,
     running += d;
   }
   grid.sync();
+  stamp(ctl, PHASE_COMB_SCAN, t.id);
 
   // Fill (row order is irrelevant: Step 3 breaks ties by the lowest id).
   for (int v = t.id; v < n; v += t.count) {
@@ -647,7 +779,7 @@ __device__ void combined_build(const LaunchArgs &a, GridBarrier &grid,
 __global__ void __launch_bounds__(BLOCK) sosp_persistent(LaunchArgs a) {
   __shared__ NearFarShared sh;
   __shared__ int sScan[BLOCK];
-  GridBarrier grid{&a.ctl->barrier, 0};
+  GridBarrier grid{a.ctl, 0};
   const Threads t{blockIdx.x * BLOCK + threadIdx.x, gridDim.x * BLOCK};
   const int n = a.n;
   if (t.id == 0) {
@@ -674,7 +806,8 @@ __global__ void __launch_bounds__(BLOCK) sosp_persistent(LaunchArgs a) {
   if (a.phases & PH_UPDATE) {
     generation = update_step1(a, grid, t, generation);
     generation = near_far(a.objs, a.nobj, a.ctl->obj, a.source, generation, 0,
-                          nullptr, grid, t, sh);
+                          nullptr, a.laneBits, grid, t, sh);
+    stamp(a.ctl, PHASE_NEARFAR, t.id);
     for (int k = 0; k < a.nobj; ++k) {
       unpack(a.objs[k], n, a.source, t);
     }
@@ -688,6 +821,7 @@ __global__ void __launch_bounds__(BLOCK) sosp_persistent(LaunchArgs a) {
     }
     if (t.id == 0) {
       a.ctl->clkSospEnd = wall_clock64();
+      a.ctl->clk[PHASE_UNPACK] = a.ctl->clkSospEnd;
     }
   }
 
@@ -696,7 +830,7 @@ __global__ void __launch_bounds__(BLOCK) sosp_persistent(LaunchArgs a) {
     scratch_init(D, a.ctl->obj[0], n, a.source, t);
     grid.sync();
     generation = near_far(a.objs, 1, a.ctl->obj, a.source, generation,
-                          D.delta, nullptr, grid, t, sh);
+                          D.delta, nullptr, a.laneBits, grid, t, sh);
     unpack(D, n, a.source, t);
     if (D.parentBits == 0) {
       grid.sync();
@@ -713,9 +847,11 @@ __global__ void __launch_bounds__(BLOCK) sosp_persistent(LaunchArgs a) {
     combined_build(a, grid, t, sScan);
     scratch_init(C, cc, n, a.source, t);
     grid.sync();
+    stamp(a.ctl, PHASE_COMB_FILL, t.id);
     const u64 delta = load_relaxed(&a.ctl->combDelta);
     generation = near_far(a.cDesc, 1, &cc, a.source, generation, delta,
-                          &a.ctl->combDelta, grid, t, sh);
+                          &a.ctl->combDelta, a.laneBits, grid, t, sh);
+    stamp(a.ctl, PHASE_COMB_SEARCH, t.id);
     unpack(C, n, a.source, t);
     if (C.parentBits == 0) {
       grid.sync();
@@ -727,6 +863,8 @@ __global__ void __launch_bounds__(BLOCK) sosp_persistent(LaunchArgs a) {
     a.ctl->generationOut = static_cast<unsigned>(generation);
     *a.genDev = generation;
     a.ctl->clkEnd = wall_clock64();
+    a.ctl->clk[PHASE_COMB_UNPACK] = a.ctl->clkEnd;
+    a.ctl->barriers = grid.epoch;
   }
 }
 

@@ -47,14 +47,18 @@ int main(int argc, char **argv) {
   if (cmd == "road" || cmd == "rmat") {
     int32_t n = 0;
     int64_t m = 0;
-    const int K = std::atoi(argv[cmd == "road" ? 6 : 5]);
-    const uint64_t seed = std::strtoull(argv[cmd == "road" ? 7 : 6], nullptr, 10);
+    if (argc < 7) {
+      std::cerr << "usage: mosp_prep road <prefix> W H K seed | rmat <prefix> scale ef K seed\n";
+      return 2;
+    }
+    const int K = std::atoi(argv[5]);
+    const uint64_t seed = std::strtoull(argv[6], nullptr, 10);
     auto gen = [&](int32_t *row, int32_t *col, int32_t *w, int64_t cap) {
       if (cmd == "road") {
         return mosp_gen_road_grid(std::atoi(argv[3]), std::atoi(argv[4]), K,
-                                  static_cast<int>(argd(argc, argv, 10, 1)),
-                                  static_cast<int>(argd(argc, argv, 11, 100)),
-                                  argd(argc, argv, 8, 0.3), argd(argc, argv, 9, 0.02), seed, &n,
+                                  static_cast<int>(argd(argc, argv, 9, 1)),
+                                  static_cast<int>(argd(argc, argv, 10, 100)),
+                                  argd(argc, argv, 7, 0.3), argd(argc, argv, 8, 0.02), seed, &n,
                                   &m, row, col, w, cap);
       }
       return mosp_gen_rmat(std::atoi(argv[3]), std::atoi(argv[4]), 0.57, 0.19, 0.19, K,

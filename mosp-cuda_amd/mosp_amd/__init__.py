@@ -56,7 +56,7 @@ class CombineStats(C.Structure):
 
 EXPORTS = (
     "mosp_last_error", "mosp_ws_create", "mosp_ws_reserve", "mosp_ws_destroy",
-    "mosp_ws_set_grid", "mosp_ws_grid", "mosp_default_delta", "mosp_sosp_update",
+    "mosp_ws_set_grid", "mosp_ws_grid", "mosp_ws_last_profile", "mosp_default_delta", "mosp_sosp_update",
     "mosp_sosp_update_multi", "mosp_sosp_from_scratch", "mosp_combined_sosp",
     "mosp_update_fused", "mosp_device_graph_upload", "mosp_device_graph_free",
     "mosp_device_graph_view", "mosp_device_graph_objectives", "mosp_path_costs_gpu",
@@ -89,6 +89,7 @@ def lib() -> C.CDLL:
     L.mosp_ws_set_grid.argtypes = [_VP, _I]
     L.mosp_ws_grid.argtypes = [_VP]
     L.mosp_ws_grid.restype = _I
+    L.mosp_ws_last_profile.argtypes = [_VP, _f64p, _I, C.POINTER(C.c_int32)]
     L.mosp_default_delta.argtypes = [_L, _I, _L]
     L.mosp_default_delta.restype = _L
     L.mosp_sosp_update.argtypes = [C.POINTER(Csr), C.POINTER(Csr), C.POINTER(Changes), _I, _L,
@@ -359,6 +360,18 @@ class Workspace:
     @property
     def grid(self) -> int:
         return lib().mosp_ws_grid(self.handle)
+
+    PHASES = ("pack", "jump", "invalidate", "pull", "near_far", "unpack", "comb_count",
+              "comb_scan", "comb_fill", "comb_search", "comb_unpack")
+
+    def last_profile(self) -> dict:
+        """Device-clock phase times (ms) and barrier count of the last launch."""
+        ms = np.zeros(len(self.PHASES), np.float64)
+        b = C.c_int32(0)
+        lib().mosp_ws_last_profile(self.handle, ms, len(ms), C.byref(b))
+        out = {p: round(float(x), 4) for p, x in zip(self.PHASES, ms)}
+        out["barriers"] = b.value
+        return out
 
     def close(self) -> None:
         if self.handle:
