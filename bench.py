@@ -87,6 +87,8 @@ def main() -> int:
     ap.add_argument("--height", type=int, default=1400)
     ap.add_argument("--changes", type=int, default=50_000)
     ap.add_argument("--grid", type=int, default=0, help="workgroups of the persistent launch")
+    ap.add_argument("--delta-scale", type=float, default=1.0,
+                    help="multiply the default near-far width (tuning)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--traffic", default=str(ROOT / "profiles" / "traffic_road1400_k3.json"),
                     help="measured HBM bytes per launch (rocprofv3 FETCH_SIZE/WRITE_SIZE)")
@@ -108,7 +110,7 @@ def main() -> int:
     log(f"workload: n={g.n} m={g.m} K=3 inserts={b.n_ins} deletes={b.n_del} "
         f"({time.perf_counter() - t:.1f} s to generate)")
     d0, p0 = M.initial_trees(g, 3)
-    sess = M.Session(g, b, d0, p0, grid=args.grid)
+    sess = M.Session(g, b, d0, p0, grid=args.grid, delta_scale=args.delta_scale)
     log(f"resident in HBM; persistent grid = {sess.ws.grid} workgroups x 256")
 
     for _ in range(args.warmup):
@@ -130,6 +132,8 @@ def main() -> int:
         f"(K updates {np.mean(sosp_ms):.3f} ms, combined {np.mean(comb_ms):.3f} ms); "
         f"stats {res['stats']}")
     log(f"phase profile (last step, device clock): {sess.ws.last_profile()}")
+    log(f"delta {sess.delta.tolist()}; combined: edges {res['combined_edges']}, "
+        f"iterations {sess.cstats.search.iterations}, delta {sess.cstats.delta}")
 
     # Algorithmic bytes of one update.
     changed = [M.changed_lists(b, sess.mask, k) for k in range(3)]

@@ -129,9 +129,10 @@ int ws_init(mosp_ws *ws) {
       clockKhz > 0) {
     ws->clockKhz = clockKhz;
   }
-  // Default: two workgroups per CU -- enough lanes for the frontier sweeps
-  // while keeping the grid barrier (one arrival per workgroup) cheap.
-  int blocks = std::min(perCu, 2) * cus;
+  // Default: one 1024-thread workgroup per CU -- 262K lanes for the frontier
+  // sweeps with only one barrier arrival per CU (measured best on MI355X
+  // among 256/512/1024-thread workgroups at 1-2 per CU).
+  int blocks = cus;
   if (const char *env = std::getenv("MOSP_GRID")) {
     blocks = std::atoi(env);
   }

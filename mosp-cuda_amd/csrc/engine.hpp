@@ -22,7 +22,10 @@ using u64 = unsigned long long;
 constexpr u64 PACKED_INF = ~0ULL;
 constexpr long long DISTANCE_INF = 0x7fffffffffffffffLL / 4;  // csrGraph.cuh:185
 constexpr int MOSP_MAX_OBJ = 32;     // K <= 32 (csrGraph.cu:736-739)
-constexpr int BLOCK = 256;           // threads per workgroup (4 wave64)
+#ifndef MOSP_BLOCK
+#define MOSP_BLOCK 1024
+#endif
+constexpr int BLOCK = MOSP_BLOCK;    // threads per workgroup (wave64 multiple)
 
 // Phases of one launch.
 enum : int {
@@ -80,7 +83,7 @@ struct ObjCtl {
   unsigned overflow;       // an input distance does not fit the packing
   unsigned iterations;
   unsigned epochs;
-  unsigned pad;
+  unsigned jumpCount[3];   // pointer jumping: still-jumping list, slot = round % 3
   u64 pushes;
 };
 

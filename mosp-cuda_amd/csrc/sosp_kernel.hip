@@ -5,18 +5,27 @@
 // recovery) and src/combinedGraphGpu.cu:55-121 (combined graph).  The
 // schedule is this kernel's own (see engine.hpp):
 //   * up to 32 objectives are updated in ONE launch, in lock-step, so every
-//     grid barrier serves all of them;
+//     grid barrier serves all of them; in each near-far iteration the
+//     workgroups are split over the active objectives in proportion to their
+//     frontier sizes;
 //   * one grid barrier per near-far iteration: counters rotate over three
 //     slots (iteration % 3 for near frontiers, epoch % 3 for far piles) so a
 //     slot is cleared two iterations before it is reused;
-//   * the far-pile minimum is folded in when a vertex is appended (a wave
-//     reduction + one atomicMin per wave), so raising the threshold needs no
+//   * list appends are staged in LDS per workgroup (wave-aggregated LDS
+//     reservations) and published with ONE global atomicAdd per workgroup
+//     and list: same-address global atomics serialize (~10 ns each), so a
+//     per-wave or per-lane counter would dominate an iteration;
+//   * the far-pile minimum is folded in at append time (LDS atomicMin, one
+//     global atomicMin per workgroup), so raising the threshold needs no
 //     separate reduction pass and barrier;
-//   * list appends are wave64-aggregated (ballot + popcount + one atomicAdd
-//     per wave, slot broadcast with readfirstlane);
-//   * the grid barrier is a monotonic arrival counter with agent-scope
-//     release/acquire (no reset between barriers).
-// No MFMA: this is integer graph work bound by HBM latency and bandwidth.
+//   * 2^laneBits lanes share a frontier vertex and stride over its edges,
+//     shortening each lane's chain of dependent atomics;
+//   * pointer jumping compacts its still-jumping set every round; the pull
+//     pass sweeps the vertices instead of building a candidate list;
+//   * the grid barrier is two-level (16 arrival counters, then one) with
+//     agent-scope release/acquire fences and relaxed polling.
+// No MFMA: this is integer graph work bound by memory latency, atomics and
+// HBM bandwidth.
 #include <hip/hip_runtime.h>
 
 #include <climits>
@@ -27,6 +36,8 @@ namespace mosp {
 namespace {
 
 enum : int { MODE_DONE = 0, MODE_PUSH = 1, MODE_SPLIT = 2 };
+
+constexpr unsigned QCAP = 4096;  // LDS queue entries per list per workgroup
 
 // ---------------------------------------------------------------------------
 // Device helpers
@@ -90,13 +101,11 @@ struct GridBarrier {
   }
 };
 
-// Reserve one slot of a shared list for every active lane: one atomicAdd
-// per wave; the first active lane adds the lane count and its old value is
-// broadcast with readfirstlane.  The counter must be wave-uniform.
-__device__ __forceinline__ unsigned wave_append(unsigned *counter) {
+// One slot per active lane from a wave-uniform counter: one atomicAdd per
+// wave (LDS or global), slot broadcast from the first active lane.
+__device__ __forceinline__ unsigned wave_reserve(unsigned *counter) {
   const u64 active = __ballot(1);
-  const unsigned lane = __lane_id();
-  const unsigned rank = __popcll(active & ((1ull << lane) - 1ull));
+  const unsigned rank = __popcll(active & ((1ull << __lane_id()) - 1ull));
   unsigned base = 0;
   if (rank == 0) {
     base = atomicAdd(counter, static_cast<unsigned>(__popcll(active)));
@@ -106,16 +115,38 @@ __device__ __forceinline__ unsigned wave_append(unsigned *counter) {
   return base + rank;
 }
 
-// Minimum over a converged wave, folded into *target by lane 0.
-__device__ __forceinline__ void wave_min_into(u64 value, u64 *target) {
-#pragma unroll
-  for (int offset = 32; offset > 0; offset >>= 1) {
-    const u64 other = __shfl_xor(value, offset);
-    value = other < value ? other : value;
+// A list append staged in LDS: lanes reserve LDS slots; a workgroup's
+// entries are published by flush() with one global atomicAdd.  When the LDS
+// buffer is full the entry goes straight to the global list.
+struct BlockQueue {
+  int buf[QCAP];
+  unsigned count;
+  unsigned base;
+};
+
+__device__ __forceinline__ void queue_push(BlockQueue &q, int v, int *list,
+                                           unsigned *listCount) {
+  const unsigned slot = wave_reserve(&q.count);
+  if (slot < QCAP) {
+    q.buf[slot] = v;
+  } else {
+    list[wave_reserve(listCount)] = v;
   }
-  if (__lane_id() == 0 && value != PACKED_INF) {
-    atomicMin(target, value);
+}
+
+// Workgroup-uniform: publish the staged entries and empty the queue.
+__device__ void queue_flush(BlockQueue &q, int *list, unsigned *listCount) {
+  __syncthreads();
+  const unsigned count = q.count < QCAP ? q.count : QCAP;
+  if (threadIdx.x == 0) {
+    q.base = count > 0 ? atomicAdd(listCount, count) : 0;
+    q.count = 0;
   }
+  __syncthreads();
+  for (unsigned i = threadIdx.x; i < count; i += BLOCK) {
+    list[q.base + i] = q.buf[i];
+  }
+  __syncthreads();
 }
 
 __device__ __forceinline__ bool claim(int *stamp, int v, int generation) {
@@ -141,10 +172,96 @@ __device__ __forceinline__ void stamp(Ctl *ctl, int phase, unsigned tid) {
   }
 }
 
+// Workgroup sum of one value per thread (uses `red`, BLOCK entries).
+__device__ unsigned block_sum(unsigned value, unsigned *red) {
+#pragma unroll
+  for (int offset = 32; offset > 0; offset >>= 1) {
+    value += __shfl_xor(value, offset);
+  }
+  __syncthreads();
+  if (__lane_id() == 0) {
+    red[threadIdx.x / 64] = value;
+  }
+  __syncthreads();
+  unsigned total = 0;
+  for (unsigned w = 0; w < BLOCK / 64; ++w) {
+    total += red[w];
+  }
+  __syncthreads();
+  return total;
+}
+
 struct Threads {
   unsigned id;     // global thread index
   unsigned count;  // threads in the grid
 };
+
+// ---------------------------------------------------------------------------
+// Workgroup-shared state
+// ---------------------------------------------------------------------------
+
+struct Shared {
+  // near-far per objective
+  u64 thr[MOSP_MAX_OBJ];
+  u64 newThr[MOSP_MAX_OBJ];
+  u64 delta[MOSP_MAX_OBJ];
+  unsigned cnt[MOSP_MAX_OBJ];
+  int epoch[MOSP_MAX_OBJ];
+  int mode[MOSP_MAX_OBJ];
+  int blockBegin[MOSP_MAX_OBJ];  // workgroups working on objective k
+  int blockEnd[MOSP_MAX_OBJ];
+  int any;
+  u64 farMin;                    // this workgroup's far appends
+  unsigned red[BLOCK / 64];
+  int scan[BLOCK];
+  BlockQueue near;
+  BlockQueue far;
+};
+
+// Split the workgroups over the active objectives: one each, the rest in
+// proportion to their item counts, leftovers to the largest.  Integer
+// arithmetic only, so every workgroup derives the same partition.  With more
+// active objectives than workgroups, objective k goes to workgroup k % G.
+__device__ void partition_blocks(Shared &sh, int nobj, unsigned G) {
+  u64 total = 0;
+  unsigned active = 0;
+  int largest = -1;
+  for (int k = 0; k < nobj; ++k) {
+    sh.blockBegin[k] = sh.blockEnd[k] = 0;
+    if (sh.mode[k] != MODE_DONE) {
+      total += sh.cnt[k];
+      ++active;
+      if (largest < 0 || sh.cnt[k] > sh.cnt[largest]) largest = k;
+    }
+  }
+  if (active == 0) return;
+  if (active > G) {
+    for (int k = 0; k < nobj; ++k) {
+      if (sh.mode[k] != MODE_DONE) {
+        sh.blockBegin[k] = k % G;
+        sh.blockEnd[k] = k % G + 1;
+      }
+    }
+    return;
+  }
+  const u64 spare = G - active;
+  unsigned given = 0;
+  for (int k = 0; k < nobj; ++k) {
+    if (sh.mode[k] != MODE_DONE) {
+      sh.blockEnd[k] = 1 + static_cast<int>(spare * sh.cnt[k] / (total > 0 ? total : 1));
+      given += sh.blockEnd[k];
+    }
+  }
+  sh.blockEnd[largest] += static_cast<int>(G - given);
+  int next = 0;
+  for (int k = 0; k < nobj; ++k) {
+    if (sh.mode[k] == MODE_DONE) continue;
+    const int size = sh.blockEnd[k];
+    sh.blockBegin[k] = next;
+    next += size;
+    sh.blockEnd[k] = next;
+  }
+}
 
 // ---------------------------------------------------------------------------
 // Near-far propagation (Step 2), all objectives of `objs` in lock-step.
@@ -153,60 +270,12 @@ struct Threads {
 // farCount[0] / farMin[0]; `initThr` is the first threshold.
 // ---------------------------------------------------------------------------
 
-struct NearFarShared {
-  u64 thr[MOSP_MAX_OBJ];
-  u64 newThr[MOSP_MAX_OBJ];
-  u64 delta[MOSP_MAX_OBJ];
-  u64 farMin[MOSP_MAX_OBJ];   // this workgroup's far appends, this iteration
-  unsigned cnt[MOSP_MAX_OBJ];
-  int epoch[MOSP_MAX_OBJ];
-  int mode[MOSP_MAX_OBJ];
-  int waveBegin[MOSP_MAX_OBJ];  // waves of the grid working on objective k
-  int waveEnd[MOSP_MAX_OBJ];
-  int any;
-};
-
-// Split the grid's waves over the active objectives: one wave each, the rest
-// in proportion to their item counts, leftovers to the largest.  Integer
-// arithmetic only, so every workgroup derives the same partition.
-__device__ void partition_waves(NearFarShared &sh, int nobj, unsigned waves) {
-  u64 total = 0;
-  int active = 0, largest = -1;
-  for (int k = 0; k < nobj; ++k) {
-    sh.waveBegin[k] = sh.waveEnd[k] = 0;
-    if (sh.mode[k] != MODE_DONE) {
-      total += sh.cnt[k];
-      ++active;
-      if (largest < 0 || sh.cnt[k] > sh.cnt[largest]) largest = k;
-    }
-  }
-  if (active == 0) return;
-  const u64 spare = waves > static_cast<unsigned>(active) ? waves - active : 0;
-  unsigned next = 0, given = 0;
-  int extra[MOSP_MAX_OBJ];
-  for (int k = 0; k < nobj; ++k) {
-    extra[k] = 0;
-    if (sh.mode[k] != MODE_DONE) {
-      extra[k] = static_cast<int>(spare * sh.cnt[k] / (total > 0 ? total : 1));
-      given += 1 + extra[k];
-    }
-  }
-  if (given < waves) extra[largest] += static_cast<int>(waves - given);
-  for (int k = 0; k < nobj; ++k) {
-    if (sh.mode[k] == MODE_DONE) continue;
-    const unsigned w = 1 + extra[k];
-    sh.waveBegin[k] = static_cast<int>(next < waves ? next : waves - 1);
-    next += w;
-    sh.waveEnd[k] = static_cast<int>(next < waves ? next : waves);
-    if (sh.waveEnd[k] <= sh.waveBegin[k]) sh.waveEnd[k] = sh.waveBegin[k] + 1;
-  }
-}
-
-// Relax the out-edges of frontier vertices of one objective: groups of
-// 2^laneBits lanes share a vertex and stride over its edges.
-__device__ void push_items(const ObjDesc &D, ObjCtl &c, NearFarShared &sh, int k,
-                           unsigned it, int source, int gen, unsigned local,
-                           unsigned span, int laneBits) {
+// Relax the out-edges of the frontier vertices of objective k: groups of
+// 2^laneBits lanes share a vertex and stride over its edges.  Workgroup-
+// uniform chunks, so the LDS queues can be flushed between chunks.
+__device__ void push_items(const ObjDesc &D, ObjCtl &c, Shared &sh, int k,
+                           unsigned it, int source, int gen, unsigned blockRank,
+                           unsigned blocks, int laneBits) {
   const unsigned cnt = sh.cnt[k];
   const int e = sh.epoch[k];
   const int bits = D.parentBits;
@@ -220,48 +289,57 @@ __device__ void push_items(const ObjDesc &D, ObjCtl &c, NearFarShared &sh, int k
   unsigned *nextCnt = &c.nearCount[(it + 1) % 3];
   unsigned *farCnt = &c.farCount[e % 3];
   const unsigned lanes = 1u << laneBits;
-  const unsigned sub = local & (lanes - 1);
-  const unsigned groups = span >> laneBits;
-  for (unsigned i = local >> laneBits; i < cnt; i += groups) {
-    const int u = in[i];
-    const u64 word = packed[u];
-    if (word == PACKED_INF) {
-      continue;
-    }
-    const u64 du = dist_of(word, bits);
-    const int end = D.outRow[u + 1];
-    for (int x = D.outRow[u] + static_cast<int>(sub); x < end; x += lanes) {
-      const int w = D.outCol[x];
-      if (w == source) {
-        continue;  // the source is never relaxed
-      }
-      const u64 nd = du + static_cast<u64>(D.outW[x]);
-      if (nd > maxDist) {
-        continue;  // never shortest; would not fit the packed word
-      }
-      const u64 cand = pack(nd, u, bits, noParent);
-      if (cand >= packed[w]) {
-        continue;  // stale pre-check; the atomic below decides
-      }
-      const u64 prev = atomicMin(&packed[w], cand);
-      if (nd < dist_of(prev, bits)) {
-        if (nd < thr) {
-          if (D.s.stamp[w] != gen && claim(D.s.stamp, w, gen)) {
-            outNear[wave_append(nextCnt)] = w;
+  const unsigned sub = threadIdx.x & (lanes - 1);
+  const unsigned perBlock = BLOCK >> laneBits;
+  const unsigned stride = blocks * perBlock;
+  for (unsigned base = blockRank * perBlock; base < cnt; base += stride) {
+    const unsigned i = base + (threadIdx.x >> laneBits);
+    if (i < cnt) {
+      const int u = in[i];
+      const u64 word = packed[u];
+      if (word != PACKED_INF) {
+        const u64 du = dist_of(word, bits);
+        const int end = D.outRow[u + 1];
+        for (int x = D.outRow[u] + static_cast<int>(sub); x < end; x += lanes) {
+          const int w = D.outCol[x];
+          if (w == source) {
+            continue;  // the source is never relaxed
           }
-        } else if (D.s.inFar[w] == 0 && atomicExch(&D.s.inFar[w], 1) == 0) {
-          farBuf[wave_append(farCnt)] = w;
-          atomicMin(&sh.farMin[k], nd);
+          const u64 nd = du + static_cast<u64>(D.outW[x]);
+          if (nd > maxDist) {
+            continue;  // never shortest; would not fit the packed word
+          }
+          const u64 cand = pack(nd, u, bits, noParent);
+          if (cand >= packed[w]) {
+            continue;  // stale pre-check; the atomic below decides
+          }
+          const u64 prev = atomicMin(&packed[w], cand);
+          if (nd < dist_of(prev, bits)) {
+            if (nd < thr) {
+              if (D.s.stamp[w] != gen && claim(D.s.stamp, w, gen)) {
+                queue_push(sh.near, w, outNear, nextCnt);
+              }
+            } else if (D.s.inFar[w] == 0 && atomicExch(&D.s.inFar[w], 1) == 0) {
+              queue_push(sh.far, w, farBuf, farCnt);
+              atomicMin(&sh.farMin, nd);
+            }
+          }
         }
       }
     }
+    __syncthreads();
+    if (sh.near.count > QCAP / 2) queue_flush(sh.near, outNear, nextCnt);
+    if (sh.far.count > QCAP / 2) queue_flush(sh.far, farBuf, farCnt);
   }
+  queue_flush(sh.near, outNear, nextCnt);
+  queue_flush(sh.far, farBuf, farCnt);
 }
 
 // Raise the threshold past the far pile: move the entries below it to the
 // next near frontier, keep the rest in the other far buffer.
-__device__ void split_items(const ObjDesc &D, ObjCtl &c, NearFarShared &sh, int k,
-                            unsigned it, int gen, unsigned local, unsigned span) {
+__device__ void split_items(const ObjDesc &D, ObjCtl &c, Shared &sh, int k,
+                            unsigned it, int gen, unsigned blockRank,
+                            unsigned blocks) {
   const unsigned cnt = sh.cnt[k];
   const int e = sh.epoch[k];
   const int bits = D.parentBits;
@@ -271,26 +349,35 @@ __device__ void split_items(const ObjDesc &D, ObjCtl &c, NearFarShared &sh, int 
   int *outNear = D.s.near[(it + 1) & 1];
   unsigned *nextCnt = &c.nearCount[(it + 1) % 3];
   unsigned *far2Cnt = &c.farCount[(e + 1) % 3];
-  for (unsigned i = local; i < cnt; i += span) {
-    const int v = in[i];
-    const u64 word = D.s.packed[v];
-    const u64 d = word == PACKED_INF ? PACKED_INF : dist_of(word, bits);
-    if (d < thr) {
-      D.s.inFar[v] = 0;
-      if (claim(D.s.stamp, v, gen)) {
-        outNear[wave_append(nextCnt)] = v;
+  const unsigned stride = blocks * BLOCK;
+  for (unsigned base = blockRank * BLOCK; base < cnt; base += stride) {
+    const unsigned i = base + threadIdx.x;
+    if (i < cnt) {
+      const int v = in[i];
+      const u64 word = D.s.packed[v];
+      const u64 d = word == PACKED_INF ? PACKED_INF : dist_of(word, bits);
+      if (d < thr) {
+        D.s.inFar[v] = 0;
+        if (claim(D.s.stamp, v, gen)) {
+          queue_push(sh.near, v, outNear, nextCnt);
+        }
+      } else {
+        queue_push(sh.far, v, far2, far2Cnt);
+        atomicMin(&sh.farMin, d);
       }
-    } else {
-      far2[wave_append(far2Cnt)] = v;
-      atomicMin(&sh.farMin[k], d);
     }
+    __syncthreads();
+    if (sh.near.count > QCAP - BLOCK) queue_flush(sh.near, outNear, nextCnt);
+    if (sh.far.count > QCAP - BLOCK) queue_flush(sh.far, far2, far2Cnt);
   }
+  queue_flush(sh.near, outNear, nextCnt);
+  queue_flush(sh.far, far2, far2Cnt);
 }
 
 __device__ int near_far(const ObjDesc *objs, int nobj, ObjCtl *ctls,
                         int source, int generation, u64 initThr,
                         const u64 *deltaOverride, int laneBits,
-                        GridBarrier &grid, Threads t, NearFarShared &sh) {
+                        GridBarrier &grid, Threads t, Shared &sh) {
   if (threadIdx.x < static_cast<unsigned>(nobj)) {
     const int k = threadIdx.x;
     sh.thr[k] = initThr;
@@ -298,8 +385,10 @@ __device__ int near_far(const ObjDesc *objs, int nobj, ObjCtl *ctls,
     sh.delta[k] = deltaOverride != nullptr ? load_relaxed(deltaOverride)
                                            : objs[k].delta;
   }
-  const unsigned waves = t.count / 64;
-  const unsigned wave = t.id / 64;
+  if (threadIdx.x == 0) {
+    sh.near.count = 0;
+    sh.far.count = 0;
+  }
   for (unsigned it = 0;; ++it) {
     const unsigned old = (it + 2) % 3;
     if (threadIdx.x == 0) {
@@ -313,7 +402,6 @@ __device__ int near_far(const ObjDesc *objs, int nobj, ObjCtl *ctls,
       ObjCtl &c = ctls[k];
       const unsigned nc = load_relaxed(&c.nearCount[it % 3]);
       const int e = sh.epoch[k];
-      sh.farMin[k] = PACKED_INF;
       if (nc > 0) {
         sh.mode[k] = MODE_PUSH;
         sh.cnt[k] = nc;
@@ -337,7 +425,7 @@ __device__ int near_far(const ObjDesc *objs, int nobj, ObjCtl *ctls,
       break;
     }
     if (threadIdx.x == 0) {
-      partition_waves(sh, nobj, waves);
+      partition_blocks(sh, nobj, gridDim.x);
     }
     __syncthreads();
     const int gen = ++generation;
@@ -357,34 +445,29 @@ __device__ int near_far(const ObjDesc *objs, int nobj, ObjCtl *ctls,
         }
       }
     }
-    // This wave's objective (wave-uniform, so descriptor loads are scalar).
-    int mine = -1;
     for (int k = 0; k < nobj; ++k) {
-      if (static_cast<int>(wave) >= sh.waveBegin[k] &&
-          static_cast<int>(wave) < sh.waveEnd[k] && sh.mode[k] != MODE_DONE) {
-        mine = k;
+      const int b0 = sh.blockBegin[k], b1 = sh.blockEnd[k];
+      if (sh.mode[k] == MODE_DONE || static_cast<int>(blockIdx.x) < b0 ||
+          static_cast<int>(blockIdx.x) >= b1) {
+        continue;  // workgroup-uniform
       }
-    }
-    mine = __builtin_amdgcn_readfirstlane(mine);
-    if (mine >= 0) {
-      const ObjDesc &D = objs[mine];
-      ObjCtl &c = ctls[mine];
-      const unsigned first = static_cast<unsigned>(sh.waveBegin[mine]) * 64;
-      const unsigned span = static_cast<unsigned>(sh.waveEnd[mine] - sh.waveBegin[mine]) * 64;
-      if (sh.mode[mine] == MODE_PUSH) {
-        push_items(D, c, sh, mine, it, source, gen, t.id - first, span, laneBits);
+      const ObjDesc &D = objs[k];
+      ObjCtl &c = ctls[k];
+      if (threadIdx.x == 0) {
+        sh.farMin = PACKED_INF;
+      }
+      __syncthreads();
+      const unsigned rank = blockIdx.x - b0, blocks = b1 - b0;
+      int slot;
+      if (sh.mode[k] == MODE_PUSH) {
+        push_items(D, c, sh, k, it, source, gen, rank, blocks, laneBits);
+        slot = sh.epoch[k] % 3;
       } else {
-        split_items(D, c, sh, mine, it, gen, t.id - first, span);
+        split_items(D, c, sh, k, it, gen, rank, blocks);
+        slot = (sh.epoch[k] + 1) % 3;
       }
-    }
-    __syncthreads();
-    // Fold this workgroup's far minimum into the objective's slot.
-    if (threadIdx.x < static_cast<unsigned>(nobj)) {
-      const int k = threadIdx.x;
-      const u64 fm = sh.farMin[k];
-      if (fm != PACKED_INF) {
-        const int slot = sh.mode[k] == MODE_SPLIT ? (sh.epoch[k] + 1) % 3 : sh.epoch[k] % 3;
-        atomicMin(&ctls[k].farMin[slot], fm);
+      if (threadIdx.x == 0 && sh.farMin != PACKED_INF) {
+        atomicMin(&c.farMin[slot], sh.farMin);
       }
     }
     grid.sync();
@@ -460,8 +543,27 @@ __device__ void scratch_init(const ObjDesc &D, ObjCtl &c, int n, int source,
 // Step 1 of the update for every objective (src/sospUpdateGpu.cu:233-344)
 // ---------------------------------------------------------------------------
 
+// One pointer-jumping step for vertex v.  Invariant for every x: flag[x] == 1
+// implies a root among x and its ancestors; flag[x] == 0 implies no root on
+// the tree path from x up to (excluding) ancestor[x].  Only v's thread writes
+// (ancestor[v], flag[v]) and every value another thread can observe keeps the
+// invariant, so the update is in place.  Returns true if v still jumps.
+__device__ __forceinline__ bool jump(int *anc, int *flag, int v) {
+  const int x = anc[v];
+  if (x < 0 || flag[v]) {
+    return false;
+  }
+  if (flag[x]) {
+    flag[v] = 1;
+    return false;
+  }
+  const int next = anc[x];
+  anc[v] = next;
+  return next >= 0;
+}
+
 __device__ int update_step1(const LaunchArgs &a, GridBarrier &grid, Threads t,
-                            int generation) {
+                            int generation, Shared &sh) {
   const int n = a.n;
   Ctl *ctl = a.ctl;
   // Pack the old trees; ancestors; roots of changed tree edges.
@@ -483,7 +585,9 @@ __device__ int update_step1(const LaunchArgs &a, GridBarrier &grid, Threads t,
         word = pack(static_cast<u64>(d), p, D.parentBits, D.noParent);
       }
       D.s.packed[v] = word;
-      D.s.ancestor[v] = p;
+      if (D.nChanged > 0) {
+        D.s.ancestor[v] = p;
+      }
     }
     if (overflow) {
       atomicOr(&ctl->obj[k].overflow, 1u);
@@ -498,18 +602,17 @@ __device__ int update_step1(const LaunchArgs &a, GridBarrier &grid, Threads t,
   grid.sync();
   stamp(ctl, PHASE_PACK, t.id);
 
-  // Subtree invalidation by pointer jumping.  Invariant for every x:
-  // flag[x] == 1 implies a root among x and its ancestors; flag[x] == 0
-  // implies no root on the tree path from x up to (excluding) ancestor[x].
-  // Only x's own thread writes (ancestor[x], flag[x]) and every value another
-  // thread can observe keeps the invariant, so the update is in place.
+  // Subtree invalidation by pointer jumping.  Round 0 sweeps every vertex;
+  // the vertices still jumping are compacted into a list (near buffers are
+  // free before Step 2) that the next round walks, until no list is left.
   if (anyChanged) {
     for (int round = 0; round < a.maxRounds; ++round) {
       if (t.id == 0) {
-        store_relaxed(&ctl->jumpActive[(round + 1) % 3], 0u);
+        for (int k = 0; k < a.nobj; ++k) {
+          store_relaxed(&ctl->obj[k].jumpCount[(round + 2) % 3], 0u);
+        }
         ctl->rounds = round + 1;
       }
-      int jumping = 0;
       for (int k = 0; k < a.nobj; ++k) {
         const ObjDesc &D = a.objs[k];
         if (D.nChanged == 0) {
@@ -517,105 +620,112 @@ __device__ int update_step1(const LaunchArgs &a, GridBarrier &grid, Threads t,
         }
         int *anc = D.s.ancestor;
         int *flag = D.s.flag;
-        for (int v = t.id; v < n; v += t.count) {
-          const int x = anc[v];
-          if (x < 0 || flag[v]) {
-            continue;
+        int *outList = D.s.near[(round + 1) & 1];
+        unsigned *outCount = &ctl->obj[k].jumpCount[(round + 1) % 3];
+        const unsigned cnt =
+            round == 0 ? static_cast<unsigned>(n)
+                       : load_relaxed(&ctl->obj[k].jumpCount[round % 3]);
+        const int *inList = D.s.near[round & 1];
+        for (unsigned base = blockIdx.x * BLOCK; base < cnt; base += t.count) {
+          const unsigned i = base + threadIdx.x;
+          if (i < cnt) {
+            const int v = round == 0 ? static_cast<int>(i) : inList[i];
+            if (jump(anc, flag, v)) {
+              queue_push(sh.near, v, outList, outCount);
+            }
           }
-          if (flag[x]) {
-            flag[v] = 1;
-            continue;
-          }
-          const int next = anc[x];
-          anc[v] = next;
-          jumping |= next >= 0 ? 1 : 0;
+          __syncthreads();
+          if (sh.near.count > QCAP - BLOCK) queue_flush(sh.near, outList, outCount);
         }
-      }
-      if (__any(jumping) && __lane_id() == 0) {
-        store_relaxed(&ctl->jumpActive[round % 3], 1u);
+        queue_flush(sh.near, outList, outCount);
       }
       grid.sync();
-      if (load_relaxed(&ctl->jumpActive[round % 3]) == 0) {
+      unsigned left = 0;
+      for (int k = 0; k < a.nobj; ++k) {
+        left += load_relaxed(&ctl->obj[k].jumpCount[(round + 1) % 3]);
+      }
+      if (left == 0) {
         break;
       }
     }
   }
   stamp(ctl, PHASE_JUMP, t.id);
 
-  // Invalidate flagged vertices; candidates = invalidated + insert heads.
-  // Each flagged vertex is visited by exactly one thread, so it is listed
-  // with a plain stamp store; insert heads that are not flagged are
-  // de-duplicated among themselves by a stamp claim.  Flags stay set until
-  // the unpack sweep clears them (they are read-only here).
+  // Invalidate flagged vertices (INF); mark insert heads (stamp).  Flags stay
+  // set until the unpack sweep clears them.
   const int genInv = ++generation;
   for (int k = 0; k < a.nobj; ++k) {
     const ObjDesc &D = a.objs[k];
-    ObjCtl &c = ctl->obj[k];
     unsigned count = 0;
     if (D.nChanged > 0) {
       for (int v = t.id; v < n; v += t.count) {
         if (D.s.flag[v]) {
           D.s.packed[v] = PACKED_INF;
-          D.s.stamp[v] = genInv;
-          D.s.cand[wave_append(&c.candCount)] = v;
           ++count;
         }
       }
     }
     for (int i = t.id; i < D.nHeads; i += t.count) {
       const int v = D.heads[i];
-      if (v != a.source && !D.s.flag[v] && claim(D.s.stamp, v, genInv)) {
-        D.s.cand[wave_append(&c.candCount)] = v;
+      if (v != a.source) {
+        D.s.stamp[v] = genInv;
       }
     }
-#pragma unroll
-    for (int offset = 32; offset > 0; offset >>= 1) {
-      count += __shfl_xor(count, offset);
-    }
-    if (__lane_id() == 0 && count > 0) {
-      atomicAdd(&c.invalidated, count);
+    count = block_sum(count, sh.red);
+    if (threadIdx.x == 0 && count > 0) {
+      atomicAdd(&ctl->obj[k].invalidated, count);
     }
   }
   grid.sync();
   stamp(ctl, PHASE_INVALIDATE, t.id);
 
-  // Pull: each candidate takes the best (distance, parent) over its
-  // in-neighbours; improved ones start in the far pile of epoch 0 (the first
-  // near-far iteration splits it at min + delta).
-  const int genPull = ++generation;
+  // Pull: every invalidated vertex and insert head takes the best (distance,
+  // parent) over its in-neighbours; improved ones start in the far pile of
+  // epoch 0 (the first near-far iteration splits it at min + delta).
   for (int k = 0; k < a.nobj; ++k) {
     const ObjDesc &D = a.objs[k];
     ObjCtl &c = ctl->obj[k];
-    const unsigned cnt = load_relaxed(&c.candCount);
     const int bits = D.parentBits;
-    u64 localMin = PACKED_INF;
-    for (unsigned i = t.id; i < cnt; i += t.count) {
-      const int v = D.s.cand[i];
-      const u64 current = D.s.packed[v];
-      u64 best = current;
-      for (int x = D.inRow[v]; x < D.inRow[v + 1]; ++x) {
-        const int u = D.inCol[x];
-        const u64 word = D.s.packed[u];
-        if (word == PACKED_INF) {
-          continue;
-        }
-        const u64 nd = dist_of(word, bits) + static_cast<u64>(D.inW[x]);
-        if (nd <= D.maxDistance) {  // larger: never shortest, may not fit
-          const u64 cand = pack(nd, u, bits, D.noParent);
-          best = cand < best ? cand : best;
-        }
-      }
-      if (best < current) {
-        const u64 prev = atomicMin(&D.s.packed[v], best);
-        const u64 bd = dist_of(best, bits);
-        if (bd < dist_of(prev, bits) && claim(D.s.stamp, v, genPull)) {
-          D.s.inFar[v] = 1;
-          D.s.far[0][wave_append(&c.farCount[0])] = v;
-          localMin = bd < localMin ? bd : localMin;
-        }
-      }
+    if (threadIdx.x == 0) {
+      sh.farMin = PACKED_INF;
     }
-    wave_min_into(localMin, &c.farMin[0]);
+    __syncthreads();
+    for (unsigned base = blockIdx.x * BLOCK; base < static_cast<unsigned>(n);
+         base += t.count) {
+      const int v = static_cast<int>(base + threadIdx.x);
+      if (v < n && v != a.source &&
+          ((D.nChanged > 0 && D.s.flag[v]) || D.s.stamp[v] == genInv)) {
+        const u64 current = D.s.packed[v];
+        u64 best = current;
+        for (int x = D.inRow[v]; x < D.inRow[v + 1]; ++x) {
+          const int u = D.inCol[x];
+          const u64 word = D.s.packed[u];
+          if (word == PACKED_INF) {
+            continue;
+          }
+          const u64 nd = dist_of(word, bits) + static_cast<u64>(D.inW[x]);
+          if (nd <= D.maxDistance) {  // larger: never shortest, may not fit
+            const u64 cand = pack(nd, u, bits, D.noParent);
+            best = cand < best ? cand : best;
+          }
+        }
+        if (best < current) {
+          const u64 prev = atomicMin(&D.s.packed[v], best);
+          const u64 bd = dist_of(best, bits);
+          if (bd < dist_of(prev, bits)) {
+            D.s.inFar[v] = 1;
+            queue_push(sh.far, v, D.s.far[0], &c.farCount[0]);
+            atomicMin(&sh.farMin, bd);
+          }
+        }
+      }
+      __syncthreads();
+      if (sh.far.count > QCAP - BLOCK) queue_flush(sh.far, D.s.far[0], &c.farCount[0]);
+    }
+    queue_flush(sh.far, D.s.far[0], &c.farCount[0]);
+    if (threadIdx.x == 0 && sh.farMin != PACKED_INF) {
+      atomicMin(&c.farMin[0], sh.farMin);
+    }
   }
   grid.sync();
   stamp(ctl, PHASE_PULL, t.id);
@@ -777,13 +887,16 @@ __device__ void combined_build(const LaunchArgs &a, GridBarrier &grid,
 // ---------------------------------------------------------------------------
 
 __global__ void __launch_bounds__(BLOCK) sosp_persistent(LaunchArgs a) {
-  __shared__ NearFarShared sh;
-  __shared__ int sScan[BLOCK];
+  __shared__ Shared sh;
   GridBarrier grid{a.ctl, 0};
   const Threads t{blockIdx.x * BLOCK + threadIdx.x, gridDim.x * BLOCK};
   const int n = a.n;
   if (t.id == 0) {
     a.ctl->clkStart = wall_clock64();
+  }
+  if (threadIdx.x == 0) {
+    sh.near.count = 0;
+    sh.far.count = 0;
   }
   // Stamp generations persist in device memory across launches; restart
   // them (and clear the stamps) well before the counter could wrap.
@@ -802,9 +915,10 @@ __global__ void __launch_bounds__(BLOCK) sosp_persistent(LaunchArgs a) {
     generation = 0;
     grid.sync();
   }
+  __syncthreads();
 
   if (a.phases & PH_UPDATE) {
-    generation = update_step1(a, grid, t, generation);
+    generation = update_step1(a, grid, t, generation, sh);
     generation = near_far(a.objs, a.nobj, a.ctl->obj, a.source, generation, 0,
                           nullptr, a.laneBits, grid, t, sh);
     stamp(a.ctl, PHASE_NEARFAR, t.id);
@@ -844,7 +958,7 @@ __global__ void __launch_bounds__(BLOCK) sosp_persistent(LaunchArgs a) {
     }
     const ObjDesc &C = *a.cDesc;
     ObjCtl &cc = a.ctl->obj[MOSP_MAX_OBJ];
-    combined_build(a, grid, t, sScan);
+    combined_build(a, grid, t, sh.scan);
     scratch_init(C, cc, n, a.source, t);
     grid.sync();
     stamp(a.ctl, PHASE_COMB_FILL, t.id);

@@ -9,13 +9,16 @@ the library is missing -- there is no CPU fallback.
 from __future__ import annotations
 
 import ctypes as C
+import os
 import pathlib
 from dataclasses import dataclass, field
 
 import numpy as np
 
 HERE = pathlib.Path(__file__).resolve().parent
-LIB_PATH = HERE.parent / "lib" / "libmosp_hip.so"
+# MOSP_LIB_PATH selects another build of the same library (e.g. a variant
+# compiled with a different workgroup size); default: the in-tree build.
+LIB_PATH = pathlib.Path(os.environ.get("MOSP_LIB_PATH", HERE.parent / "lib" / "libmosp_hip.so"))
 DISTANCE_INF = (2**63 - 1) // 4
 
 _i32p = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
@@ -501,7 +504,7 @@ class Session:
 
     def __init__(self, original: HostGraph, batch: Batch, init_dist: np.ndarray,
                  init_parents: np.ndarray, K: int = 0, source: int = 0, pref=(),
-                 delta: int = 0, grid: int = 0):
+                 delta: int = 0, grid: int = 0, delta_scale: float = 1.0):
         torch = _torch()
         self.K = min(K or original.K, original.K)
         self.n, self.source, self.pref = original.n, source, _i32(pref)
@@ -509,7 +512,8 @@ class Session:
         self.batch = batch
         self.maxw, wsum = objective_bounds(original, batch, self.K)
         m = max(original.m, 1)
-        self.delta = np.array([delta if delta > 0 else default_delta(m, self.n, int(wsum[k]))
+        self.delta = np.array([delta if delta > 0 else
+                               max(1, int(default_delta(m, self.n, int(wsum[k])) * delta_scale))
                                for k in range(self.K)], np.int64)
         self.dg = DeviceGraph(self.updated)
         self.ws = Workspace(self.n, self.K, grid)
