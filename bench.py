@@ -8,6 +8,9 @@ all seeded.  The graph, the old trees, the batch and the outputs are resident
 in HBM before timing; one step = one MOSP update (K SOSP updates + combined
 graph + its SOSP search) in one persistent cooperative launch, host wall
 clock around the synchronous call (the reference's gpu_compute_ms scope).
+The timed region is bracketed by device synchronisations through the
+library (hipDeviceSynchronize); torch is not used, so the process holds one
+HIP runtime.
 
 Prints ONE JSON line.  `roofline` uses the algorithmic bytes of one update
 (mosp_amd/roofline.py) over the HIP-event duration of the persistent kernel
@@ -98,12 +101,10 @@ def main() -> int:
               "(multi-GPU sharding is described, not built); use --gpus 1", file=sys.stderr)
         return 2
 
-    import torch
-
     import mosp_amd as M
     from mosp_amd.roofline import alg_bytes
 
-    torch.cuda.set_device(0)
+    M.set_device(0)
     t = time.perf_counter()
     g = M.gen_road_grid(args.width, args.height, 3, seed=7)
     b = M.gen_batch(g, args.changes, 50.0, seed=8, safe=True)
@@ -115,7 +116,7 @@ def main() -> int:
 
     for _ in range(args.warmup):
         sess.run()
-    torch.cuda.synchronize()
+    M.synchronize()
     kernel_ms, sosp_ms, comb_ms = [], [], []
     t0 = time.perf_counter()
     for _ in range(args.steps):
@@ -123,7 +124,7 @@ def main() -> int:
         kernel_ms.append(sess.stats[0].kernel_ms)
         sosp_ms.append(sess.cstats.sosp_phase_ms)
         comb_ms.append(sess.cstats.combined_phase_ms)
-    torch.cuda.synchronize()
+    M.synchronize()
     elapsed = time.perf_counter() - t0
     ms_per_step = elapsed * 1e3 / args.steps
     changes = b.n_ins + b.n_del

@@ -75,6 +75,18 @@ typedef struct mosp_ws mosp_ws;
  * column per objective.  Replaces DeviceGraph (headers/deviceGraph.cuh:16-27). */
 typedef struct mosp_device_graph mosp_device_graph;
 
+/* ------------------------------------------------------------- device ----- */
+/* Plumbing for hosts without their own HIP allocator (no reference
+ * counterpart): select a device, allocate / free / copy device memory,
+ * wait for the device.  kind: 1 host->device, 2 device->host,
+ * 3 device->device. */
+int mosp_set_device(int32_t device);
+int32_t mosp_device_count(void);
+void *mosp_dev_alloc(uint64_t bytes);
+int mosp_dev_free(void *ptr);
+int mosp_memcpy(void *dst, const void *src, uint64_t bytes, int32_t kind);
+int mosp_device_synchronize(void);
+
 /* ------------------------------------------------------------ errors ----- */
 
 /* Message of the last failed call on this thread ("" if none). */

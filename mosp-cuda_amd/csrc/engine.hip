@@ -57,6 +57,49 @@ size_t align256(size_t bytes) { return (bytes + 255) & ~static_cast<size_t>(255)
 
 extern "C" const char *mosp_last_error(void) { return g_error.c_str(); }
 
+extern "C" int mosp_set_device(int32_t device) {
+  HIP_TRY(hipSetDevice(device));
+  return 1;
+}
+
+extern "C" int32_t mosp_device_count(void) {
+  int count = 0;
+  if (hipGetDeviceCount(&count) != hipSuccess) {
+    return 0;
+  }
+  return count;
+}
+
+extern "C" void *mosp_dev_alloc(uint64_t bytes) {
+  void *p = nullptr;
+  if (hipMalloc(&p, bytes > 0 ? bytes : 1) != hipSuccess) {
+    fail("Error: hipMalloc of " + std::to_string(bytes) + " bytes failed.");
+    return nullptr;
+  }
+  return p;
+}
+
+extern "C" int mosp_dev_free(void *ptr) {
+  HIP_TRY(hipFree(ptr));
+  return 1;
+}
+
+extern "C" int mosp_memcpy(void *dst, const void *src, uint64_t bytes, int32_t kind) {
+  if (bytes == 0) {
+    return 1;
+  }
+  const hipMemcpyKind k = kind == 1   ? hipMemcpyHostToDevice
+                          : kind == 2 ? hipMemcpyDeviceToHost
+                                      : hipMemcpyDeviceToDevice;
+  HIP_TRY(hipMemcpy(dst, src, bytes, k));
+  return 1;
+}
+
+extern "C" int mosp_device_synchronize(void) {
+  HIP_TRY(hipDeviceSynchronize());
+  return 1;
+}
+
 // ---------------------------------------------------------------------------
 // Workspace
 // ---------------------------------------------------------------------------
@@ -156,8 +199,21 @@ int ws_init(mosp_ws *ws) {
   return 1;
 }
 
+int run_launch(mosp_ws *ws, LaunchArgs &args, int descCount, hipStream_t stream,
+               float *kernelMs);
+
+// One empty persistent launch when the workspace is created: the first
+// cooperative launch of a process pays one-time set-up costs, which belong
+// to workspace creation, not to the first update.
+int ws_warm_up(mosp_ws *ws) {
+  LaunchArgs args{};
+  args.objs = ws->dDesc;
+  return run_launch(ws, args, 0, nullptr, nullptr);
+}
+
 int ws_reserve(mosp_ws *ws, int capacity, int slots) {
-  if (!ws_init(ws)) {
+  const bool fresh = ws->dCtl == nullptr;
+  if (!ws_init(ws) || (fresh && !ws_warm_up(ws))) {
     return 0;
   }
   slots = std::max(slots, 1);

@@ -4,6 +4,8 @@
 //   mosp_prep rmat  <prefix> scale edgeFactor K seed [wmin=1] [wmax=100]
 //   mosp_prep changes <prefix> <dir> changes insPct safe(0/1) seed [wmin] [wmax]
 //   mosp_prep init  <prefix> <dir> [source=0]
+//   mosp_prep copy  <MiB>      one device-to-device copy of MiB mebibytes
+//                              (calibrates rocprofv3 FETCH_SIZE / WRITE_SIZE)
 // `init` writes <dir>/obj<k>/distancesOriginal.txt + SSSPTreeOriginal.txt:
 // the canonical tree of every objective, computed by the GPU from-scratch
 // search (the role of the reference's `mospPrep init`, src/mospPrep.cu:
@@ -44,6 +46,18 @@ int main(int argc, char **argv) {
     return 2;
   }
   const std::string cmd = argv[1], prefix = argv[2];
+  if (cmd == "copy") {
+    const size_t bytes = static_cast<size_t>(std::atoll(argv[2])) << 20;
+    DeviceArray<char> a(bytes), b(bytes);
+    if (hipMemset(a.data(), 1, bytes) != hipSuccess ||
+        hipMemcpy(b.data(), a.data(), bytes, hipMemcpyDeviceToDevice) != hipSuccess ||
+        hipDeviceSynchronize() != hipSuccess) {
+      std::cerr << "copy failed\n";
+      return 1;
+    }
+    std::cout << "copied " << bytes << " bytes device to device\n";
+    return 0;
+  }
   if (cmd == "road" || cmd == "rmat") {
     int32_t n = 0;
     int64_t m = 0;

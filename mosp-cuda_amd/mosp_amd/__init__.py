@@ -2,7 +2,8 @@
 
 Mirrors the reference's in-memory pipeline interface (src/mospUpdate.cu,
 src/sospUpdateGpu.cu, src/combinedGraphGpu.cu) over the C ABI declared in
-include/mosp_hip.h.  torch provides device buffers and the stream only; every
+include/mosp_hip.h.  Device buffers come from the library itself
+(mosp_dev_alloc / mosp_memcpy), so no other GPU framework is needed; every
 computation runs in the HIP kernels of the library, and the module raises if
 the library is missing -- there is no CPU fallback.
 """
@@ -65,7 +66,8 @@ EXPORTS = (
     "mosp_device_graph_view", "mosp_device_graph_objectives", "mosp_path_costs_gpu",
     "mosp_preference_scale", "mosp_combined_edge_weight", "mosp_choose_packing",
     "mosp_apply_batch", "mosp_update_host", "mosp_gen_road_grid", "mosp_gen_rmat",
-    "mosp_gen_batch",
+    "mosp_gen_batch", "mosp_set_device", "mosp_device_count", "mosp_dev_alloc",
+    "mosp_dev_free", "mosp_memcpy", "mosp_device_synchronize",
 )
 
 _LIB: C.CDLL | None = None
@@ -79,12 +81,17 @@ def lib() -> C.CDLL:
     if not LIB_PATH.exists():
         raise RuntimeError(f"{LIB_PATH} is missing: build it with `make -C mosp-cuda_amd` "
                            "(python -c 'import __graft_entry__; __graft_entry__.build()')")
-    # torch's HIP runtime must be the process's only one: importing torch
-    # first loads it (soname libamdhip64.so.7), and the library's dependency
-    # then binds to that same copy, so torch buffers and streams are valid
-    # here.  Loading the library first would bring a second runtime in.
-    import torch  # noqa: F401
+    # A process must hold ONE HIP runtime.  If torch was imported before, its
+    # bundled runtime (soname libamdhip64.so.7) is already loaded and the
+    # library binds to it; otherwise the library loads the system runtime,
+    # and importing torch afterwards would bring in a second copy.
     L = C.CDLL(str(LIB_PATH))
+    L.mosp_set_device.argtypes = [_I]
+    L.mosp_device_count.restype = _I
+    L.mosp_dev_alloc.restype = _VP
+    L.mosp_dev_alloc.argtypes = [C.c_uint64]
+    L.mosp_dev_free.argtypes = [_VP]
+    L.mosp_memcpy.argtypes = [_VP, _VP, C.c_uint64, _I]
     L.mosp_last_error.restype = C.c_char_p
     L.mosp_ws_create.restype = _VP
     L.mosp_ws_reserve.argtypes = [_VP, _I, _I]
@@ -333,21 +340,61 @@ def update_host(g: HostGraph, b: Batch, init_dist: np.ndarray, init_parents: np.
 
 # ------------------------------------------------------------- device side --
 
-def _torch():
-    import torch
-    if not torch.cuda.is_available():
+def device_count() -> int:
+    return lib().mosp_device_count()
+
+
+def set_device(device: int = 0) -> None:
+    _check(lib().mosp_set_device(device), "hipSetDevice")
+
+
+def synchronize() -> None:
+    _check(lib().mosp_device_synchronize(), "hipDeviceSynchronize")
+
+
+def _require_gpu() -> None:
+    if device_count() < 1:
         raise RuntimeError("no GPU visible: the HIP engine needs an MI355X")
-    return torch
 
 
-def _stream():
-    return C.c_void_p(_torch().cuda.current_stream().cuda_stream)
+class DeviceBuffer:
+    """Device memory owned by Python (hipMalloc through the library)."""
+
+    def __init__(self, nbytes: int):
+        self.nbytes = int(nbytes)
+        self.ptr = lib().mosp_dev_alloc(max(self.nbytes, 1))
+        if not self.ptr:
+            raise RuntimeError(last_error())
+
+    @classmethod
+    def from_numpy(cls, a: np.ndarray) -> "DeviceBuffer":
+        a = np.ascontiguousarray(a)
+        buf = cls(a.nbytes)
+        _check(lib().mosp_memcpy(buf.ptr, a.ctypes.data, a.nbytes, 1), "copy to device")
+        return buf
+
+    def to_numpy(self, dtype, count: int) -> np.ndarray:
+        out = np.empty(count, dtype)
+        _check(lib().mosp_memcpy(out.ctypes.data, self.ptr, out.nbytes, 2), "copy to host")
+        return out
+
+    def free(self) -> None:
+        if self.ptr:
+            lib().mosp_dev_free(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
 
 
 class Workspace:
     """mosp_ws: scratch of the engine, reserved once and reused."""
 
     def __init__(self, capacity: int = 0, slots: int = 1, grid: int = 0):
+        _require_gpu()
         self.handle = lib().mosp_ws_create()
         if grid:
             lib().mosp_ws_set_grid(self.handle, grid)
@@ -392,7 +439,7 @@ class DeviceGraph:
     """mosp_device_graph: out-CSR + in-CSR with K weight columns in HBM."""
 
     def __init__(self, g: HostGraph):
-        _torch()
+        _require_gpu()
         self.n, self.K, self.m = g.n, g.K, g.m
         self.handle = lib().mosp_device_graph_upload(g.n, g.K, g.row, _nz(g.col), _nz(g.w))
         if not self.handle:
@@ -415,23 +462,19 @@ class DeviceGraph:
             pass
 
 
-def _dev(a, dtype):
-    torch = _torch()
-    t = torch.as_tensor(np.ascontiguousarray(a))
-    return t.to(device="cuda", dtype=dtype).contiguous()
+def _dev(a, dtype) -> DeviceBuffer:
+    return DeviceBuffer.from_numpy(np.ascontiguousarray(np.asarray(a, dtype=dtype)).reshape(-1))
 
 
 def from_scratch(dg: DeviceGraph, k: int, source: int, delta: int, max_w: int, ws: Workspace):
     """sospFromScratchGpu on objective k; returns (dist, parent, stats) on host."""
-    torch = _torch()
     out, _ = dg.view(k)
-    d = torch.empty(dg.n, dtype=torch.int64, device="cuda")
-    p = torch.empty(dg.n, dtype=torch.int32, device="cuda")
+    d = DeviceBuffer(8 * dg.n)
+    p = DeviceBuffer(4 * dg.n)
     st = SospStats()
     _check(lib().mosp_sosp_from_scratch(C.byref(out), source, delta, max_w, ws.handle,
-                                        d.data_ptr(), p.data_ptr(), C.byref(st), _stream()),
-           "sospFromScratchGpu")
-    return d.cpu().numpy(), p.cpu().numpy(), st.as_dict()
+                                        d.ptr, p.ptr, C.byref(st), None), "sospFromScratchGpu")
+    return d.to_numpy(np.int64, dg.n), p.to_numpy(np.int32, dg.n), st.as_dict()
 
 
 def initial_trees(g: HostGraph, K: int = 0, source: int = 0, ws: Workspace | None = None):
@@ -454,44 +497,38 @@ def initial_trees(g: HostGraph, K: int = 0, source: int = 0, ws: Workspace | Non
 def sosp_update(dg: DeviceGraph, k: int, changes_from, changes_to, heads, source: int,
                 delta: int, max_w: int, ws: Workspace, dist, parent):
     """sospUpdateGpu on objective k with host arrays in/out."""
-    torch = _torch()
     out, inn = dg.view(k)
-    cf, ct, hd = (_dev(_nz(_i32(a)), torch.int32) for a in (changes_from, changes_to, heads))
-    ch = Changes(cf.data_ptr(), ct.data_ptr(), len(changes_from), hd.data_ptr(), len(heads))
-    d = _dev(dist, torch.int64)
-    p = _dev(parent, torch.int32)
+    cf, ct, hd = (_dev(_nz(_i32(a)), np.int32) for a in (changes_from, changes_to, heads))
+    ch = Changes(cf.ptr, ct.ptr, len(changes_from), hd.ptr, len(heads))
+    d = _dev(dist, np.int64)
+    p = _dev(parent, np.int32)
     st = SospStats()
     _check(lib().mosp_sosp_update(C.byref(out), C.byref(inn), C.byref(ch), source, delta, max_w,
-                                  ws.handle, d.data_ptr(), p.data_ptr(), C.byref(st), _stream()),
-           "sospUpdateGpu")
-    return d.cpu().numpy(), p.cpu().numpy(), st.as_dict()
+                                  ws.handle, d.ptr, p.ptr, C.byref(st), None), "sospUpdateGpu")
+    return d.to_numpy(np.int64, dg.n), p.to_numpy(np.int32, dg.n), st.as_dict()
 
 
 def combined_sosp(parents: np.ndarray, source: int, pref, ws: Workspace, delta: int = 0):
     """combinedGraphSospGpu on K parent arrays (objective-major)."""
-    torch = _torch()
     K, n = parents.shape
-    dp = _dev(parents.reshape(-1), torch.int32)
-    d = torch.empty(n, dtype=torch.int64, device="cuda")
-    p = torch.empty(n, dtype=torch.int32, device="cuda")
+    dp = _dev(parents.reshape(-1), np.int32)
+    d = DeviceBuffer(8 * n)
+    p = DeviceBuffer(4 * n)
     cs = CombineStats()
     pa = _i32(pref)
-    _check(lib().mosp_combined_sosp(dp.data_ptr(), n, K, _nz(pa), len(pa), source, delta,
-                                    ws.handle, d.data_ptr(), p.data_ptr(), C.byref(cs),
-                                    _stream()), "combinedGraphSospGpu")
-    return d.cpu().numpy(), p.cpu().numpy(), {"n_edges": cs.n_edges, "scale": cs.scale,
+    _check(lib().mosp_combined_sosp(dp.ptr, n, K, _nz(pa), len(pa), source, delta, ws.handle,
+                                    d.ptr, p.ptr, C.byref(cs), None), "combinedGraphSospGpu")
+    return d.to_numpy(np.int64, n), p.to_numpy(np.int32, n), {"n_edges": cs.n_edges, "scale": cs.scale,
                                               "delta": cs.delta, **cs.search.as_dict()}
 
 
 def path_costs(g: HostGraph, parent, source: int) -> np.ndarray:
     """mospPathCosts on the GPU: n x K costs (DISTANCE_INF if unreachable)."""
-    torch = _torch()
     dg = DeviceGraph(g)
-    p = _dev(_i32(parent), torch.int32)
-    c = torch.empty(g.n * g.K, dtype=torch.int64, device="cuda")
-    _check(lib().mosp_path_costs_gpu(dg.handle, p.data_ptr(), source, c.data_ptr(), _stream()),
-           "mospPathCosts")
-    out = c.cpu().numpy().reshape(g.n, g.K)
+    p = _dev(_i32(parent), np.int32)
+    c = DeviceBuffer(8 * g.n * g.K)
+    _check(lib().mosp_path_costs_gpu(dg.handle, p.ptr, source, c.ptr, None), "mospPathCosts")
+    out = c.to_numpy(np.int64, g.n * g.K).reshape(g.n, g.K)
     dg.close()
     return out
 
@@ -505,7 +542,6 @@ class Session:
     def __init__(self, original: HostGraph, batch: Batch, init_dist: np.ndarray,
                  init_parents: np.ndarray, K: int = 0, source: int = 0, pref=(),
                  delta: int = 0, grid: int = 0, delta_scale: float = 1.0):
-        torch = _torch()
         self.K = min(K or original.K, original.K)
         self.n, self.source, self.pref = original.n, source, _i32(pref)
         self.updated, self.mask = apply_batch(original, batch)
@@ -517,40 +553,40 @@ class Session:
                                for k in range(self.K)], np.int64)
         self.dg = DeviceGraph(self.updated)
         self.ws = Workspace(self.n, self.K, grid)
-        self.heads = _dev(_nz(_i32(batch.ins_to)), torch.int32)
+        self.heads = _dev(_nz(_i32(batch.ins_to)), np.int32)
         self._keep = []
         changes = (Changes * self.K)()
         for k in range(self.K):
             cf, ct = changed_lists(batch, self.mask, k)
-            a, b = _dev(_nz(cf), torch.int32), _dev(_nz(ct), torch.int32)
+            a, b = _dev(_nz(cf), np.int32), _dev(_nz(ct), np.int32)
             self._keep += [a, b]
-            changes[k] = Changes(a.data_ptr(), b.data_ptr(), len(cf), self.heads.data_ptr(),
-                                 batch.n_ins)
+            changes[k] = Changes(a.ptr, b.ptr, len(cf), self.heads.ptr, batch.n_ins)
         self.changes = changes
-        self.init_dist = _dev(np.asarray(init_dist)[: self.K].reshape(-1), torch.int64)
-        self.init_par = _dev(np.asarray(init_parents)[: self.K].reshape(-1), torch.int32)
-        self.dist = torch.empty(self.K * self.n, dtype=torch.int64, device="cuda")
-        self.par = torch.empty(self.K * self.n, dtype=torch.int32, device="cuda")
-        self.cdist = torch.empty(self.n, dtype=torch.int64, device="cuda")
-        self.cpar = torch.empty(self.n, dtype=torch.int32, device="cuda")
+        self.init_dist = _dev(np.asarray(init_dist)[: self.K].reshape(-1), np.int64)
+        self.init_par = _dev(np.asarray(init_parents)[: self.K].reshape(-1), np.int32)
+        self.dist = DeviceBuffer(8 * self.K * self.n)
+        self.par = DeviceBuffer(4 * self.K * self.n)
+        self.cdist = DeviceBuffer(8 * self.n)
+        self.cpar = DeviceBuffer(4 * self.n)
         self.stats = (SospStats * self.K)()
         self.cstats = CombineStats()
 
     def run(self, stream=None) -> None:
-        s = C.c_void_p(stream) if stream is not None else _stream()
+        s = C.c_void_p(stream) if stream is not None else None
         dl = (C.c_int64 * self.K)(*[int(x) for x in self.delta])
         mw = (C.c_int64 * self.K)(*[int(x) for x in self.maxw])
         _check(lib().mosp_update_fused(self.dg.handle, self.K, self.changes, self.source, dl, mw,
                                        _nz(self.pref), len(self.pref), self.ws.handle,
-                                       self.init_dist.data_ptr(), self.init_par.data_ptr(),
-                                       self.dist.data_ptr(), self.par.data_ptr(),
-                                       self.cdist.data_ptr(), self.cpar.data_ptr(), self.stats,
+                                       self.init_dist.ptr, self.init_par.ptr, self.dist.ptr,
+                                       self.par.ptr, self.cdist.ptr, self.cpar.ptr, self.stats,
                                        C.byref(self.cstats), s), "mosp_update_fused")
 
     def results(self) -> dict:
-        return {"dist": self.dist.cpu().numpy().reshape(self.K, self.n),
-                "parents": self.par.cpu().numpy().reshape(self.K, self.n),
-                "comb_dist": self.cdist.cpu().numpy(), "comb_parent": self.cpar.cpu().numpy(),
+        n, K = self.n, self.K
+        return {"dist": self.dist.to_numpy(np.int64, K * n).reshape(K, n),
+                "parents": self.par.to_numpy(np.int32, K * n).reshape(K, n),
+                "comb_dist": self.cdist.to_numpy(np.int64, n),
+                "comb_parent": self.cpar.to_numpy(np.int32, n),
                 "stats": [self.stats[k].as_dict() for k in range(self.K)],
                 "combined_edges": self.cstats.n_edges,
                 "kernel_ms": self.stats[0].kernel_ms,
