@@ -130,7 +130,7 @@ struct mosp_ws {
 namespace {
 
 void ws_release_slab(mosp_ws *ws) {
-  hipFree(ws->slab);
+  (void)hipFree(ws->slab);
   ws->slab = nullptr;
   ws->state.clear();
   ws->capacity = 0;
@@ -138,10 +138,10 @@ void ws_release_slab(mosp_ws *ws) {
 }
 
 void ws_release_combined(mosp_ws *ws) {
-  hipFree(ws->cDegree);
-  hipFree(ws->cRowPtr);
-  hipFree(ws->cColInd);
-  hipFree(ws->cWeights);
+  (void)hipFree(ws->cDegree);
+  (void)hipFree(ws->cRowPtr);
+  (void)hipFree(ws->cColInd);
+  (void)hipFree(ws->cWeights);
   ws->cDegree = ws->cRowPtr = ws->cColInd = ws->cWeights = nullptr;
   ws->cCapacity = 0;
   ws->cK = 0;
@@ -337,11 +337,31 @@ int run_launch(mosp_ws *ws, LaunchArgs &args, int descCount,
                                      dim3(ws->gridBlocks), dim3(BLOCK), params,
                                      0, stream));
   HIP_TRY(hipEventRecord(ws->ev1, stream));
-  HIP_TRY(hipMemcpyAsync(ws->hCtl, ws->dCtl, sizeof(Ctl),
+  HIP_TRY(hipMemcpyAsync(ws->hCtl, ws->dCtl, CTL_READBACK,
                          hipMemcpyDeviceToHost, stream));
   HIP_TRY(hipStreamSynchronize(stream));
   float ms = 0;
   HIP_TRY(hipEventElapsedTime(&ms, ws->ev0, ws->ev1));
+#ifdef MOSP_SUBPROFILE
+  {
+    // Sums over workgroups of wall_clock64 ticks, per workgroup-iteration;
+    // "release" is the barrier time of the last workgroup to arrive.
+    const Ctl &h = *ws->hCtl;
+    const double tick = 1e3 / static_cast<double>(ws->clockKhz);  // us
+    const double G = static_cast<double>(ws->gridBlocks);
+    const double itU = std::max(1.0, h.prof[11] / G), itC = std::max(1.0, h.prof[15] / G);
+    std::fprintf(stderr,
+                 "subprof grid=%d barriers=%u update (%.0f iters): snapshot %.2f work %.2f "
+                 "wait %.2f release %.2f us/iter; push loop %.2f flush %.2f us/call (%.0f calls); "
+                 "combined (%.0f iters): snapshot %.2f work %.2f wait %.2f release %.2f\n",
+                 ws->gridBlocks, h.barriers, itU, h.prof[8] * tick / G / itU,
+                 h.prof[9] * tick / G / itU, h.prof[10] * tick / G / itU, h.prof[6] * tick / itU,
+                 h.prof[3] * tick / std::max<double>(1, h.prof[5]),
+                 h.prof[4] * tick / std::max<double>(1, h.prof[5]), static_cast<double>(h.prof[5]),
+                 itC, h.prof[12] * tick / G / itC, h.prof[13] * tick / G / itC,
+                 h.prof[14] * tick / G / itC, h.prof[7] * tick / itC);
+  }
+#endif
   if (kernelMs != nullptr) {
     *kernelMs = ms;
   }
@@ -396,14 +416,14 @@ extern "C" void mosp_ws_destroy(mosp_ws *ws) {
   }
   ws_release_slab(ws);
   ws_release_combined(ws);
-  hipFree(ws->dDesc);
-  hipHostFree(ws->hDesc);
-  hipFree(ws->dCtl);
-  hipHostFree(ws->hCtl);
-  hipFree(ws->genDev);
-  hipFree(ws->cPartials);
-  if (ws->ev0) hipEventDestroy(ws->ev0);
-  if (ws->ev1) hipEventDestroy(ws->ev1);
+  (void)hipFree(ws->dDesc);
+  (void)hipHostFree(ws->hDesc);
+  (void)hipFree(ws->dCtl);
+  (void)hipHostFree(ws->hCtl);
+  (void)hipFree(ws->genDev);
+  (void)hipFree(ws->cPartials);
+  if (ws->ev0) (void)hipEventDestroy(ws->ev0);
+  if (ws->ev1) (void)hipEventDestroy(ws->ev1);
   delete ws;
 }
 
@@ -856,12 +876,12 @@ int exclusive_scan(const int *in, int *out, int count, hipStream_t stream) {
 }
 
 void free_graph(mosp_device_graph *g) {
-  hipFree(g->outRow);
-  hipFree(g->outCol);
-  hipFree(g->outW);
-  hipFree(g->inRow);
-  hipFree(g->inCol);
-  hipFree(g->inW);
+  (void)hipFree(g->outRow);
+  (void)hipFree(g->outCol);
+  (void)hipFree(g->outW);
+  (void)hipFree(g->inRow);
+  (void)hipFree(g->inCol);
+  (void)hipFree(g->inW);
   delete g;
 }
 
@@ -899,8 +919,8 @@ int build_graph(mosp_device_graph *g, const int32_t *rowPtr,
                      g->inW);
   HIP_TRY(hipGetLastError());
   HIP_TRY(hipDeviceSynchronize());
-  hipFree(edgeMajor);
-  hipFree(cursor);
+  (void)hipFree(edgeMajor);
+  (void)hipFree(cursor);
   return 1;
 }
 

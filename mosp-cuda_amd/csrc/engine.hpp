@@ -13,6 +13,7 @@
 // reduction pass.
 #pragma once
 
+#include <stddef.h>
 #include <stdint.h>
 
 namespace mosp {
@@ -89,14 +90,21 @@ struct ObjCtl {
 
 constexpr int BARRIER_GROUPS = 16;  // first-level arrival counters
 constexpr int LINE_WORDS = 32;      // one 128-byte line per counter
+// Words a barrier can publish with its epoch: four per objective (near count,
+// far count, far minimum as two words) for MOSP_MAX_OBJ + 1 searches.
+constexpr int MAIL_WORDS = 4 * (MOSP_MAX_OBJ + 1);
+// One mailbox per barrier group: the epoch line, then the published words.
+constexpr int MAIL_STRIDE =
+    LINE_WORDS * ((2 * LINE_WORDS + MAIL_WORDS - 1) / LINE_WORDS);
 
 struct Ctl {
   // Two-level grid barrier: workgroup b arrives at sub[b % 16]; the last of
-  // each group arrives at top; the last group publishes the epoch in flag.
-  // All three only grow within a launch (zeroed by the reset kernel).
+  // each group arrives at top; the last arrival publishes the epoch (and the
+  // words the barrier carries) in every group's mailbox, which the group's
+  // workgroups poll.  All only grow within a launch (zeroed by the reset
+  // kernel).
   unsigned barrierSub[BARRIER_GROUPS * LINE_WORDS];
   unsigned barrierTop[LINE_WORDS];
-  unsigned barrierFlag[LINE_WORDS];
   unsigned jumpActive[3];  // pointer jumping: some vertex still jumps
   unsigned rounds;
   unsigned generationOut;  // last stamp generation used
@@ -106,8 +114,14 @@ struct Ctl {
   u64 clkStart, clkSospEnd, clkEnd;  // wall_clock64 stamps
   u64 clk[16];             // phase ends (PHASE_* below), wall_clock64
   unsigned barriers;       // grid barriers executed by the launch
+  u64 prof[16];            // sub-phase clock sums (MOSP_SUBPROFILE builds)
   ObjCtl obj[MOSP_MAX_OBJ + 1];
+  // Barrier mailboxes (device-only: not part of the host read-back).
+  alignas(256) unsigned mail[BARRIER_GROUPS * MAIL_STRIDE];
 };
+
+// Bytes of Ctl the host reads back after a launch (everything but mail).
+constexpr size_t CTL_READBACK = offsetof(Ctl, mail);
 
 struct LaunchArgs {
   const ObjDesc *objs;     // nobj descriptors (device memory)

@@ -56,48 +56,117 @@ __device__ __forceinline__ void store_relaxed(u64 *p, u64 v) {
   __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
+// Sub-phase clock sums for MOSP_SUBPROFILE builds: thread 0 of every
+// workgroup adds its own intervals to LDS slots, written out to ctl->prof at
+// the end of the launch (no contended atomics inside the timed loops).
+#ifdef MOSP_SUBPROFILE
+#define SUBPROF_MARK(var) const u64 var = wall_clock64()
+#define SUBPROF_ADD(slot, from, to)                          \
+  do {                                                       \
+    if (threadIdx.x == 0) subprof_lds()[slot] += (to) - (from); \
+  } while (0)
+__device__ __forceinline__ u64 *subprof_lds() {
+  __shared__ u64 slots[16];
+  return slots;
+}
+__device__ __forceinline__ int *subprof_last() {
+  __shared__ int last;
+  return &last;
+}
+#else
+#define SUBPROF_MARK(var) (void)0
+#define SUBPROF_ADD(slot, from, to) (void)0
+#endif
+
 // Grid-wide barrier for a cooperative launch (all workgroups co-resident).
-// Two levels so that no counter sees more than ~gridDim/16 same-address
-// atomics per barrier: workgroup b arrives at sub[b % 16]; the last arrival
-// of a group arrives at top; the last group stores the epoch in flag, which
-// every workgroup polls with relaxed loads (an acquire per poll would
-// invalidate the XCD's L2 on every poll) before one acquire fence.  The
-// counters only grow: barrier e completes when sub reaches e * groupSize and
-// top reaches e * groups, so nothing is reset inside a launch.  The last
-// arrivals re-fence (acquire + release) before passing the arrival on, so
-// every workgroup's writes happen-before every workgroup's later reads.
+// Two arrival levels so that no counter sees more than ~gridDim/16
+// same-address atomics per barrier: workgroup b arrives at sub[b % 16]; the
+// last arrival of a group arrives at top.  Coherent loads of one address
+// from many workgroups serialize like atomics (~20 ns each), so the release
+// is replicated: the last arrival's wave writes the epoch -- and up to
+// MAIL_WORDS words the caller names, read once after every arrival -- into
+// one mailbox per group, and each workgroup polls only its group's mailbox
+// (relaxed loads: an acquire per poll would invalidate the XCD's L2 on every
+// poll) before one acquire fence.  Counters and epochs only grow, so nothing
+// is reset inside a launch.  The last arrivals re-fence (acquire + release)
+// before passing the arrival on, so every workgroup's writes happen-before
+// every workgroup's later reads.
 struct GridBarrier {
   Ctl *ctl;
   unsigned epoch;
+#ifdef MOSP_SUBPROFILE
+  bool wasLast = false;  // this workgroup released the last barrier
+#endif
 
-  __device__ void sync() {
+  // src: nsrc word addresses in a workgroup-uniform LDS array (set before
+  // the call); dst: LDS array that receives their values as of the barrier.
+  __device__ void sync(unsigned *const *src = nullptr, int nsrc = 0,
+                       unsigned *dst = nullptr) {
     __syncthreads();
-    ++epoch;
-    if (threadIdx.x == 0) {
-      const unsigned G = gridDim.x;
+    // Locals: a store through a global pointer could alias *this, so every
+    // member access after one would be reloaded.
+    const unsigned e = ++epoch;
+    Ctl *const c = ctl;
+    const unsigned G = gridDim.x;
+    const unsigned g = blockIdx.x % BARRIER_GROUPS;
+    unsigned *const mail = c->mail;
+    unsigned *const box = mail + g * MAIL_STRIDE;
+    if (threadIdx.x < 64) {
+      const unsigned lane = threadIdx.x;
       const unsigned groups = G < BARRIER_GROUPS ? G : BARRIER_GROUPS;
-      const unsigned g = blockIdx.x % BARRIER_GROUPS;
-      const unsigned groupSize =
-          G / BARRIER_GROUPS + (g < G % BARRIER_GROUPS ? 1u : 0u);
-      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-      const unsigned sub = __hip_atomic_fetch_add(
-          &ctl->barrierSub[g * LINE_WORDS], 1u, __ATOMIC_RELAXED,
-          __HIP_MEMORY_SCOPE_AGENT);
-      if (sub + 1 == epoch * groupSize) {
-        __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "agent");
-        const unsigned top = __hip_atomic_fetch_add(
-            &ctl->barrierTop[0], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        if (top + 1 == epoch * groups) {
+      int last = 0;
+      if (lane == 0) {
+        const unsigned groupSize =
+            G / BARRIER_GROUPS + (g < G % BARRIER_GROUPS ? 1u : 0u);
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+        const unsigned sub = __hip_atomic_fetch_add(
+            &c->barrierSub[g * LINE_WORDS], 1u, __ATOMIC_RELAXED,
+            __HIP_MEMORY_SCOPE_AGENT);
+        if (sub + 1 == e * groupSize) {
           __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "agent");
-          store_relaxed(&ctl->barrierFlag[0], epoch);
+          const unsigned top = __hip_atomic_fetch_add(
+              &c->barrierTop[0], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          if (top + 1 == e * groups) {
+            __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "agent");
+            last = 1;
+          }
         }
       }
-      while (load_relaxed(&ctl->barrierFlag[0]) < epoch) {
-        __builtin_amdgcn_s_sleep(1);
+#ifdef MOSP_SUBPROFILE
+      if (lane == 0) *subprof_last() = last;
+#endif
+      if (__shfl(last, 0)) {
+        // Lane i copies word i % nsrc into mailbox i / nsrc (and further
+        // mailboxes in steps of 64 lanes).
+        if (nsrc > 0) {
+          for (int i = static_cast<int>(lane); i < nsrc * static_cast<int>(groups);
+               i += 64) {
+            const int word = i % nsrc, h = i / nsrc;
+            mail[h * MAIL_STRIDE + LINE_WORDS + word] = load_relaxed(src[word]);
+          }
+          __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+        }
+        if (lane < groups) {
+          store_relaxed(&mail[lane * MAIL_STRIDE], e);
+        }
       }
-      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+      if (lane == 0) {
+        while (load_relaxed(box) < e) {
+          __builtin_amdgcn_s_sleep(1);
+        }
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+      }
     }
     __syncthreads();
+#ifdef MOSP_SUBPROFILE
+    wasLast = *subprof_last() != 0;
+#endif
+    if (nsrc > 0) {
+      for (int i = static_cast<int>(threadIdx.x); i < nsrc; i += BLOCK) {
+        dst[i] = load_relaxed(&box[LINE_WORDS + i]);
+      }
+      __syncthreads();
+    }
   }
 };
 
@@ -149,9 +218,6 @@ __device__ void queue_flush(BlockQueue &q, int *list, unsigned *listCount) {
   __syncthreads();
 }
 
-__device__ __forceinline__ bool claim(int *stamp, int v, int generation) {
-  return atomicExch(&stamp[v], generation) != generation;
-}
 
 __device__ __forceinline__ u64 pack(u64 distance, int parent, int bits,
                                     u64 noParent) {
@@ -212,6 +278,11 @@ struct Shared {
   int blockEnd[MOSP_MAX_OBJ];
   int any;
   u64 farMin;                    // this workgroup's far appends
+  unsigned *src[MAIL_WORDS];     // counters the next grid barrier publishes
+  unsigned snap[MAIL_WORDS];     // their values after it
+  unsigned statIter[MOSP_MAX_OBJ];
+  unsigned statEpoch[MOSP_MAX_OBJ];
+  u64 statPush[MOSP_MAX_OBJ];
   unsigned red[BLOCK / 64];
   int scan[BLOCK];
   BlockQueue near;
@@ -275,14 +346,22 @@ __device__ void partition_blocks(Shared &sh, int nobj, unsigned G) {
 // uniform chunks, so the LDS queues can be flushed between chunks.
 __device__ void push_items(const ObjDesc &D, ObjCtl &c, Shared &sh, int k,
                            unsigned it, int source, int gen, unsigned blockRank,
-                           unsigned blocks, int laneBits) {
+                           unsigned blocks, int laneBits, Ctl *prof) {
+  (void)prof;
   const unsigned cnt = sh.cnt[k];
   const int e = sh.epoch[k];
   const int bits = D.parentBits;
   const u64 noParent = D.noParent;
   const u64 thr = sh.thr[k];
   const u64 maxDist = D.maxDistance;
-  u64 *packed = D.s.packed;
+  // Locals: D lives in global memory, and any store or atomic below could
+  // alias it, so field reads inside the loop would be reissued each time.
+  u64 *const packed = D.s.packed;
+  int *const stampArr = D.s.stamp;
+  int *const inFar = D.s.inFar;
+  const int *const outRow = D.outRow;
+  const int *const outCol = D.outCol;
+  const int *const outW = D.outW;
   const int *in = D.s.near[it & 1];
   int *outNear = D.s.near[(it + 1) & 1];
   int *farBuf = D.s.far[e & 1];
@@ -291,35 +370,45 @@ __device__ void push_items(const ObjDesc &D, ObjCtl &c, Shared &sh, int k,
   const unsigned lanes = 1u << laneBits;
   const unsigned sub = threadIdx.x & (lanes - 1);
   const unsigned perBlock = BLOCK >> laneBits;
-  const unsigned stride = blocks * perBlock;
-  for (unsigned base = blockRank * perBlock; base < cnt; base += stride) {
+  // Contiguous, even shares: a short frontier still spreads over every
+  // workgroup of the objective (random loads cost per-CU address cycles).
+  const unsigned lo = static_cast<unsigned>(static_cast<u64>(cnt) * blockRank / blocks);
+  const unsigned hi = static_cast<unsigned>(static_cast<u64>(cnt) * (blockRank + 1) / blocks);
+  SUBPROF_MARK(tLoop);
+  for (unsigned base = lo; base < hi; base += perBlock) {
     const unsigned i = base + (threadIdx.x >> laneBits);
-    if (i < cnt) {
+    if (i < hi) {
       const int u = in[i];
+      // Independent loads, issued together.
       const u64 word = packed[u];
+      const int begin = outRow[u];
+      const int end = outRow[u + 1];
       if (word != PACKED_INF) {
         const u64 du = dist_of(word, bits);
-        const int end = D.outRow[u + 1];
-        for (int x = D.outRow[u] + static_cast<int>(sub); x < end; x += lanes) {
-          const int w = D.outCol[x];
+        for (int x = begin + static_cast<int>(sub); x < end; x += lanes) {
+          const int w = outCol[x];
           if (w == source) {
             continue;  // the source is never relaxed
           }
-          const u64 nd = du + static_cast<u64>(D.outW[x]);
+          const u64 nd = du + static_cast<u64>(outW[x]);
           if (nd > maxDist) {
             continue;  // never shortest; would not fit the packed word
           }
           const u64 cand = pack(nd, u, bits, noParent);
-          if (cand >= packed[w]) {
-            continue;  // stale pre-check; the atomic below decides
-          }
           const u64 prev = atomicMin(&packed[w], cand);
           if (nd < dist_of(prev, bits)) {
+            // Claims without atomics: two workgroups may both append w (a
+            // duplicate entry only repeats work), but a vertex improved in
+            // this iteration is never missed: the stamp only ever holds this
+            // iteration's generation if w was already appended in it, and
+            // inFar is only set during push iterations of this epoch.
             if (nd < thr) {
-              if (D.s.stamp[w] != gen && claim(D.s.stamp, w, gen)) {
+              if (stampArr[w] != gen) {
+                stampArr[w] = gen;
                 queue_push(sh.near, w, outNear, nextCnt);
               }
-            } else if (D.s.inFar[w] == 0 && atomicExch(&D.s.inFar[w], 1) == 0) {
+            } else if (inFar[w] == 0) {
+              inFar[w] = 1;
               queue_push(sh.far, w, farBuf, farCnt);
               atomicMin(&sh.farMin, nd);
             }
@@ -331,8 +420,13 @@ __device__ void push_items(const ObjDesc &D, ObjCtl &c, Shared &sh, int k,
     if (sh.near.count > QCAP / 2) queue_flush(sh.near, outNear, nextCnt);
     if (sh.far.count > QCAP / 2) queue_flush(sh.far, farBuf, farCnt);
   }
+  SUBPROF_MARK(tFlush);
   queue_flush(sh.near, outNear, nextCnt);
   queue_flush(sh.far, farBuf, farCnt);
+  SUBPROF_MARK(tDone);
+  SUBPROF_ADD(3, tLoop, tFlush);
+  SUBPROF_ADD(4, tFlush, tDone);
+  SUBPROF_ADD(5, 0, 1);
 }
 
 // Raise the threshold past the far pile: move the entries below it to the
@@ -349,16 +443,21 @@ __device__ void split_items(const ObjDesc &D, ObjCtl &c, Shared &sh, int k,
   int *outNear = D.s.near[(it + 1) & 1];
   unsigned *nextCnt = &c.nearCount[(it + 1) % 3];
   unsigned *far2Cnt = &c.farCount[(e + 1) % 3];
-  const unsigned stride = blocks * BLOCK;
-  for (unsigned base = blockRank * BLOCK; base < cnt; base += stride) {
+  const u64 *const packed = D.s.packed;
+  int *const stampArr = D.s.stamp;
+  int *const inFar = D.s.inFar;
+  const unsigned lo = static_cast<unsigned>(static_cast<u64>(cnt) * blockRank / blocks);
+  const unsigned hi = static_cast<unsigned>(static_cast<u64>(cnt) * (blockRank + 1) / blocks);
+  for (unsigned base = lo; base < hi; base += BLOCK) {
     const unsigned i = base + threadIdx.x;
-    if (i < cnt) {
+    if (i < hi) {
       const int v = in[i];
-      const u64 word = D.s.packed[v];
+      const u64 word = packed[v];
       const u64 d = word == PACKED_INF ? PACKED_INF : dist_of(word, bits);
       if (d < thr) {
-        D.s.inFar[v] = 0;
-        if (claim(D.s.stamp, v, gen)) {
+        inFar[v] = 0;
+        if (stampArr[v] != gen) {  // duplicates of v in the pile: append once
+          stampArr[v] = gen;
           queue_push(sh.near, v, outNear, nextCnt);
         }
       } else {
@@ -377,19 +476,24 @@ __device__ void split_items(const ObjDesc &D, ObjCtl &c, Shared &sh, int k,
 __device__ int near_far(const ObjDesc *objs, int nobj, ObjCtl *ctls,
                         int source, int generation, u64 initThr,
                         const u64 *deltaOverride, int laneBits,
-                        GridBarrier &grid, Threads t, Shared &sh) {
+                        GridBarrier &grid, Threads t, Shared &sh,
+                        int profBase) {
+  (void)profBase;
   if (threadIdx.x < static_cast<unsigned>(nobj)) {
     const int k = threadIdx.x;
     sh.thr[k] = initThr;
     sh.epoch[k] = 0;
-    sh.delta[k] = deltaOverride != nullptr ? load_relaxed(deltaOverride)
-                                           : objs[k].delta;
+    sh.delta[k] = deltaOverride != nullptr ? *deltaOverride : objs[k].delta;
+    sh.statIter[k] = 0;
+    sh.statEpoch[k] = 0;
+    sh.statPush[k] = 0;
   }
   if (threadIdx.x == 0) {
     sh.near.count = 0;
     sh.far.count = 0;
   }
   for (unsigned it = 0;; ++it) {
+    SUBPROF_MARK(tTop);
     const unsigned old = (it + 2) % 3;
     if (threadIdx.x == 0) {
       sh.any = 0;
@@ -397,19 +501,29 @@ __device__ int near_far(const ObjDesc *objs, int nobj, ObjCtl *ctls,
     __syncthreads();
     // Snapshot: every workgroup reads the same stable counters, so every
     // thread of the grid takes the same decisions.
+    // Iteration 0 reads the counters; later ones get them from the barrier.
     if (threadIdx.x < static_cast<unsigned>(nobj)) {
       const int k = threadIdx.x;
       ObjCtl &c = ctls[k];
-      const unsigned nc = load_relaxed(&c.nearCount[it % 3]);
       const int e = sh.epoch[k];
+      unsigned nc, fc;
+      u64 fm;
+      if (it == 0) {
+        nc = load_relaxed(&c.nearCount[0]);
+        fc = load_relaxed(&c.farCount[e % 3]);
+        fm = load_relaxed(&c.farMin[e % 3]);
+      } else {
+        nc = sh.snap[4 * k];
+        fc = sh.snap[4 * k + 1];
+        fm = static_cast<u64>(sh.snap[4 * k + 2]) |
+             (static_cast<u64>(sh.snap[4 * k + 3]) << 32);
+      }
       if (nc > 0) {
         sh.mode[k] = MODE_PUSH;
         sh.cnt[k] = nc;
         sh.any = 1;
       } else {
-        const unsigned fc = load_relaxed(&c.farCount[e % 3]);
         if (fc > 0) {
-          const u64 fm = load_relaxed(&c.farMin[e % 3]);
           const u64 base = fm > sh.thr[k] && fm != PACKED_INF ? fm : sh.thr[k];
           sh.mode[k] = MODE_SPLIT;
           sh.cnt[k] = fc;
@@ -428,21 +542,21 @@ __device__ int near_far(const ObjDesc *objs, int nobj, ObjCtl *ctls,
       partition_blocks(sh, nobj, gridDim.x);
     }
     __syncthreads();
+    SUBPROF_MARK(tSnap);
     const int gen = ++generation;
-    if (t.id == 0) {
+    if (blockIdx.x == 0 && threadIdx.x < static_cast<unsigned>(nobj)) {
       // Slots that nobody reads or appends to in this iteration.
-      for (int k = 0; k < nobj; ++k) {
-        ObjCtl &c = ctls[k];
-        store_relaxed(&c.nearCount[old], 0u);
-        if (sh.mode[k] == MODE_SPLIT) {
-          const int e2 = (sh.epoch[k] + 2) % 3;
-          store_relaxed(&c.farCount[e2], 0u);
-          store_relaxed(&c.farMin[e2], PACKED_INF);
-          c.epochs += 1;
-        } else if (sh.mode[k] == MODE_PUSH) {
-          c.iterations += 1;
-          c.pushes += sh.cnt[k];
-        }
+      const int k = threadIdx.x;
+      ObjCtl &c = ctls[k];
+      store_relaxed(&c.nearCount[old], 0u);
+      if (sh.mode[k] == MODE_SPLIT) {
+        const int e2 = (sh.epoch[k] + 2) % 3;
+        store_relaxed(&c.farCount[e2], 0u);
+        store_relaxed(&c.farMin[e2], PACKED_INF);
+        sh.statEpoch[k] += 1;
+      } else if (sh.mode[k] == MODE_PUSH) {
+        sh.statIter[k] += 1;
+        sh.statPush[k] += sh.cnt[k];
       }
     }
     for (int k = 0; k < nobj; ++k) {
@@ -460,7 +574,8 @@ __device__ int near_far(const ObjDesc *objs, int nobj, ObjCtl *ctls,
       const unsigned rank = blockIdx.x - b0, blocks = b1 - b0;
       int slot;
       if (sh.mode[k] == MODE_PUSH) {
-        push_items(D, c, sh, k, it, source, gen, rank, blocks, laneBits);
+        push_items(D, c, sh, k, it, source, gen, rank, blocks, laneBits,
+                   grid.ctl);
         slot = sh.epoch[k] % 3;
       } else {
         split_items(D, c, sh, k, it, gen, rank, blocks);
@@ -470,7 +585,27 @@ __device__ int near_far(const ObjDesc *objs, int nobj, ObjCtl *ctls,
         atomicMin(&c.farMin[slot], sh.farMin);
       }
     }
-    grid.sync();
+    // The barrier publishes the counters the next iteration starts from.
+    if (threadIdx.x < static_cast<unsigned>(nobj)) {
+      const int k = threadIdx.x;
+      ObjCtl &c = ctls[k];
+      const int eNext = sh.epoch[k] + (sh.mode[k] == MODE_SPLIT ? 1 : 0);
+      unsigned *fm = reinterpret_cast<unsigned *>(&c.farMin[eNext % 3]);
+      sh.src[4 * k] = &c.nearCount[(it + 1) % 3];
+      sh.src[4 * k + 1] = &c.farCount[eNext % 3];
+      sh.src[4 * k + 2] = fm;
+      sh.src[4 * k + 3] = fm + 1;
+    }
+    SUBPROF_MARK(tWork);
+    grid.sync(sh.src, 4 * nobj, sh.snap);
+    SUBPROF_MARK(tSync);
+    SUBPROF_ADD(profBase + 0, tTop, tSnap);
+    SUBPROF_ADD(profBase + 1, tSnap, tWork);
+    SUBPROF_ADD(profBase + 2, tWork, tSync);
+#ifdef MOSP_SUBPROFILE
+    if (grid.wasLast) SUBPROF_ADD(profBase == 8 ? 6 : 7, tWork, tSync);
+#endif
+    SUBPROF_ADD(profBase + 3, 0, 1);
     if (threadIdx.x < static_cast<unsigned>(nobj)) {
       const int k = threadIdx.x;
       if (sh.mode[k] == MODE_SPLIT) {
@@ -478,6 +613,12 @@ __device__ int near_far(const ObjDesc *objs, int nobj, ObjCtl *ctls,
         sh.epoch[k] += 1;
       }
     }
+  }
+  if (blockIdx.x == 0 && threadIdx.x < static_cast<unsigned>(nobj)) {
+    const int k = threadIdx.x;
+    atomicAdd(&ctls[k].iterations, sh.statIter[k]);
+    atomicAdd(&ctls[k].epochs, sh.statEpoch[k]);
+    atomicAdd(&ctls[k].pushes, sh.statPush[k]);
   }
   return generation;
 }
@@ -622,9 +763,7 @@ __device__ int update_step1(const LaunchArgs &a, GridBarrier &grid, Threads t,
         int *flag = D.s.flag;
         int *outList = D.s.near[(round + 1) & 1];
         unsigned *outCount = &ctl->obj[k].jumpCount[(round + 1) % 3];
-        const unsigned cnt =
-            round == 0 ? static_cast<unsigned>(n)
-                       : load_relaxed(&ctl->obj[k].jumpCount[round % 3]);
+        const unsigned cnt = round == 0 ? static_cast<unsigned>(n) : sh.snap[k];
         const int *inList = D.s.near[round & 1];
         for (unsigned base = blockIdx.x * BLOCK; base < cnt; base += t.count) {
           const unsigned i = base + threadIdx.x;
@@ -639,10 +778,13 @@ __device__ int update_step1(const LaunchArgs &a, GridBarrier &grid, Threads t,
         }
         queue_flush(sh.near, outList, outCount);
       }
-      grid.sync();
+      if (threadIdx.x < static_cast<unsigned>(a.nobj)) {
+        sh.src[threadIdx.x] = &ctl->obj[threadIdx.x].jumpCount[(round + 1) % 3];
+      }
+      grid.sync(sh.src, a.nobj, sh.snap);
       unsigned left = 0;
       for (int k = 0; k < a.nobj; ++k) {
-        left += load_relaxed(&ctl->obj[k].jumpCount[(round + 1) % 3]);
+        left += sh.snap[k];
       }
       if (left == 0) {
         break;
@@ -898,6 +1040,9 @@ __global__ void __launch_bounds__(BLOCK) sosp_persistent(LaunchArgs a) {
     sh.near.count = 0;
     sh.far.count = 0;
   }
+#ifdef MOSP_SUBPROFILE
+  if (threadIdx.x < 16) subprof_lds()[threadIdx.x] = 0;
+#endif
   // Stamp generations persist in device memory across launches; restart
   // them (and clear the stamps) well before the counter could wrap.
   int generation = *a.genDev;
@@ -920,7 +1065,7 @@ __global__ void __launch_bounds__(BLOCK) sosp_persistent(LaunchArgs a) {
   if (a.phases & PH_UPDATE) {
     generation = update_step1(a, grid, t, generation, sh);
     generation = near_far(a.objs, a.nobj, a.ctl->obj, a.source, generation, 0,
-                          nullptr, a.laneBits, grid, t, sh);
+                          nullptr, a.laneBits, grid, t, sh, 8);
     stamp(a.ctl, PHASE_NEARFAR, t.id);
     for (int k = 0; k < a.nobj; ++k) {
       unpack(a.objs[k], n, a.source, t);
@@ -944,7 +1089,7 @@ __global__ void __launch_bounds__(BLOCK) sosp_persistent(LaunchArgs a) {
     scratch_init(D, a.ctl->obj[0], n, a.source, t);
     grid.sync();
     generation = near_far(a.objs, 1, a.ctl->obj, a.source, generation,
-                          D.delta, nullptr, a.laneBits, grid, t, sh);
+                          D.delta, nullptr, a.laneBits, grid, t, sh, 8);
     unpack(D, n, a.source, t);
     if (D.parentBits == 0) {
       grid.sync();
@@ -960,11 +1105,14 @@ __global__ void __launch_bounds__(BLOCK) sosp_persistent(LaunchArgs a) {
     ObjCtl &cc = a.ctl->obj[MOSP_MAX_OBJ];
     combined_build(a, grid, t, sh.scan);
     scratch_init(C, cc, n, a.source, t);
-    grid.sync();
+    if (threadIdx.x < 2) {
+      sh.src[threadIdx.x] = reinterpret_cast<unsigned *>(&a.ctl->combDelta) + threadIdx.x;
+    }
+    grid.sync(sh.src, 2, sh.snap);
     stamp(a.ctl, PHASE_COMB_FILL, t.id);
-    const u64 delta = load_relaxed(&a.ctl->combDelta);
-    generation = near_far(a.cDesc, 1, &cc, a.source, generation, delta,
-                          &a.ctl->combDelta, a.laneBits, grid, t, sh);
+    const u64 delta = static_cast<u64>(sh.snap[0]) | (static_cast<u64>(sh.snap[1]) << 32);
+    generation = near_far(a.cDesc, 1, &cc, a.source, generation, delta, &delta,
+                          a.laneBits, grid, t, sh, 12);
     stamp(a.ctl, PHASE_COMB_SEARCH, t.id);
     unpack(C, n, a.source, t);
     if (C.parentBits == 0) {
@@ -973,6 +1121,10 @@ __global__ void __launch_bounds__(BLOCK) sosp_persistent(LaunchArgs a) {
     }
   }
 
+#ifdef MOSP_SUBPROFILE
+  __syncthreads();
+  if (threadIdx.x < 16) atomicAdd(&a.ctl->prof[threadIdx.x], subprof_lds()[threadIdx.x]);
+#endif
   if (t.id == 0) {
     a.ctl->generationOut = static_cast<unsigned>(generation);
     *a.genDev = generation;
