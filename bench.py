@@ -90,16 +90,25 @@ def main() -> int:
     ap.add_argument("--height", type=int, default=1400)
     ap.add_argument("--changes", type=int, default=50_000)
     ap.add_argument("--grid", type=int, default=0, help="workgroups of the persistent launch")
-    ap.add_argument("--delta-scale", type=float, default=1.0,
+    ap.add_argument("--delta-scale", type=float, default=3.0,
                     help="multiply the default near-far width (tuning)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--traffic", default=str(ROOT / "profiles" / "traffic_road1400_k3.json"),
                     help="measured HBM bytes per launch (rocprofv3 FETCH_SIZE/WRITE_SIZE)")
+    ap.add_argument("--mode", choices=("plain", "batched"), default="plain",
+                    help="plain: the 1.96M-vertex update (the headline line); batched: 64 "
+                         "uneven instances in one launch vs 64 serial launches")
+    ap.add_argument("--instances", type=int, default=64)
+    ap.add_argument("--tune-cache", default=str(ROOT / "profiles" / "autotune_batched.json"),
+                    help="autotuned batched launch choices (read; --retune re-times)")
+    ap.add_argument("--retune", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: this build runs the update on a single MI355X "
               "(multi-GPU sharding is described, not built); use --gpus 1", file=sys.stderr)
         return 2
+    if args.mode == "batched":
+        return bench_batched(args)
 
     import mosp_amd as M
     from mosp_amd.roofline import alg_bytes
@@ -174,6 +183,111 @@ def main() -> int:
         log("cpu baseline: oracle CPU path on one core (about 20 s) ...")
         line["cpu_baseline"] = cpu_baseline(g, b, d0, p0, res)
     sess.close()
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+def bench_batched(args) -> int:
+    """64 independent update instances of uneven size (road-like grids with
+    n log-uniform in [10k, 500k], K uniform in {2..8}, batches log-uniform in
+    [1k, 50k] with 40% insertions, Pref random in [1, 4]) in ONE persistent
+    launch replayed as a HIP graph, against 64 serial plain launches of the
+    same instances; outputs compared bit for bit."""
+    import mosp_amd as M
+    from mosp_amd import autotune
+    from mosp_amd.batched import BatchedUpdate, build_instance, road_specs
+
+    M.set_device(0)
+    specs = road_specs(count=args.instances)
+    t = time.perf_counter()
+    built = []
+    for i, spec in enumerate(specs):
+        built.append(build_instance(spec))
+        if i % 8 == 7:
+            log(f"built {i + 1}/{len(specs)} instances ({time.perf_counter() - t:.0f} s)")
+    sessions = [s for _, s in built]
+    changes = sum(s.batch.n_ins + s.batch.n_del for s in sessions)
+    total_n = sum(s.n for s in sessions)
+    log(f"workload: {len(sessions)} instances, total n={total_n}, "
+        f"K={[s.K for s in sessions]}, changes={changes}")
+
+    bu = BatchedUpdate(sessions)
+    key = autotune.signature(sessions)
+    choice = None if args.retune else autotune.load(args.tune_cache, key)
+    tuned = None
+    if choice is None:
+        tuned = autotune.tune(bu, log=log)
+        choice = tuned["best"]
+        autotune.save(ROOT / "gpurun_out" / "autotune_batched.json", key, choice)
+    bu.configure(**choice)
+    log(f"batched launch configuration: {choice}")
+
+    for _ in range(max(args.warmup, 1)):
+        bu.run(use_graph=True)
+    bu.check()
+    kms = []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        kms.append(bu.run(use_graph=True))
+    batched_ms = (time.perf_counter() - t0) * 1e3 / args.steps
+    sizes, mode = bu.schedule()
+    batched = [bu.results(i) for i in range(len(sessions))]
+    log(f"batched: {batched_ms:.3f} ms per pass (device {np.mean(kms):.3f} ms), "
+        f"graph mode {mode}, group sizes {sizes}")
+    slow = sorted(range(len(sessions)), key=lambda i: -(batched[i]["sosp_phase_ms"] +
+                                                        batched[i]["combined_phase_ms"]))
+    for i in slow[:6]:
+        r, s = batched[i], sessions[i]
+        log(f"  instance {i}: n={s.n} K={s.K} changes={s.batch.n_ins + s.batch.n_del} "
+            f"group={sizes[i]} updates {r['sosp_phase_ms']:.2f} ms combined "
+            f"{r['combined_phase_ms']:.2f} ms iterations "
+            f"{max(x['iterations'] for x in r['stats'])}+{r['combined_iterations']}")
+
+    # Serial: one plain fused launch per instance, the plain path's settings.
+    for s in sessions:
+        s.set_delta_scale(3.0)
+    for s in sessions:
+        s.run()
+    M.synchronize()
+    serial_kms = []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        per = 0.0
+        for s in sessions:
+            s.run()
+            per += s.stats[0].kernel_ms
+        serial_kms.append(per)
+    M.synchronize()
+    serial_ms = (time.perf_counter() - t0) * 1e3 / args.steps
+    same = True
+    for i, s in enumerate(sessions):
+        r = s.results()
+        same &= all(np.array_equal(r[k], batched[i][k])
+                    for k in ("dist", "parents", "comb_dist", "comb_parent"))
+    log(f"serial: {serial_ms:.3f} ms per pass (device {np.mean(serial_kms):.3f} ms); "
+        f"outputs identical: {same}")
+    line = {
+        "metric": "changes/s, 64 uneven update instances in one launch (vs 64 serial launches)",
+        "value": changes / (batched_ms / 1e3), "unit": "changes/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": batched_ms, "kernel_ms": float(np.mean(kms)),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "int64", "data": "synthetic (seeded road-like grids + batches)",
+        "config": {"workload": "64 road-like grids, n log-uniform [10k, 500k], K U{2..8}, "
+                               "w~U[1,100], batches log-uniform [1k, 50k] (40% inserts, safe), "
+                               "Pref random ints [1, 4]",
+                   "instances": len(sessions), "total_n": total_n, "changes": changes,
+                   "launch": choice, "graph_mode": mode},
+        "serial": {"value": changes / (serial_ms / 1e3), "ms_per_step": serial_ms,
+                   "kernel_ms": float(np.mean(serial_kms)), "delta_scale": 3.0},
+        "speedup_vs_serial": serial_ms / batched_ms,
+        "outputs_equal_serial": bool(same),
+    }
+    if tuned is not None:
+        line["autotune"] = {"seconds": tuned["seconds"], "trials": len(tuned["trials"])}
+    bu.close()
+    for s in sessions:
+        s.close()
     print(json.dumps(line), flush=True)
     return 0
 

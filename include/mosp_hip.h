@@ -167,6 +167,66 @@ int mosp_update_fused(const mosp_device_graph *g, int32_t k,
                       int32_t *d_comb_parent, mosp_sosp_stats *stats,
                       mosp_combine_stats *cstats, void *stream);
 
+/* ------------------------------------------------------------ batched --- */
+/* Many independent update instances (own graph, K, batch, source,
+ * preferences) in ONE persistent launch.  No counterpart in the reference,
+ * whose host loop runs one mospUpdate (src/mospUpdate.cu:57-225) at a time:
+ * each instance computes exactly what mosp_update_fused computes for it.
+ * The workgroups are split into groups, one per instance (sized to its
+ * estimated frontier, placed on one XCD when it fits); each group runs its
+ * instance with its own barrier, and a group given several instances runs
+ * them in turn. */
+typedef struct mosp_batch_instance {
+  const mosp_device_graph *graph;  /* updated graph in HBM */
+  int32_t k;                       /* objectives updated (<= graph's) */
+  const mosp_changes *changes;     /* k entries (device pointers inside) */
+  int32_t source;
+  const int64_t *delta;            /* k near-far widths (> 0) */
+  const int64_t *max_weight;       /* k */
+  const int32_t *pref;             /* n_pref ints (0 or k) */
+  int32_t n_pref;
+  const int64_t *d_dist_in;        /* k*n old trees (objective-major) */
+  const int32_t *d_parents_in;
+  int64_t *d_dist;                 /* k*n new trees */
+  int32_t *d_parents;
+  int64_t *d_comb_dist;            /* n combined-graph search */
+  int32_t *d_comb_parent;
+} mosp_batch_instance;
+
+typedef struct mosp_batch mosp_batch;
+
+/* Copies the descriptions (the device buffers stay the caller's) and
+ * reserves every instance's scratch.  NULL on error (mosp_last_error). */
+mosp_batch *mosp_batch_create(const mosp_batch_instance *instances,
+                              int32_t count);
+void mosp_batch_destroy(mosp_batch *b);
+/* Launch shape: workgroups (0 = one per CU), lanes per frontier vertex as
+ * 2^lane_bits (< 0 = keep), each instance's widths scaled by delta_scale
+ * (> 0; widths stay >= 1), xcd_aware != 0 places small groups on one XCD. */
+int mosp_batch_configure(mosp_batch *b, int32_t workgroups, int32_t lane_bits,
+                         double delta_scale, int32_t xcd_aware);
+/* Group sizing model: an instance of n vertices and k objectives is timed
+ * as (0.2 sqrt(n) + 10) * latency_items + k n vertex relaxations, divided
+ * over its workgroups for the second term; groups are sized so that every
+ * instance finishes at the same time.  latency_items (default 400) is the
+ * number of vertices a workgroup relaxes in one iteration's latency. */
+int mosp_batch_set_cost_model(mosp_batch *b, double latency_items);
+/* One pass over every instance, synchronous.  use_graph != 0 replays a HIP
+ * graph of the launch sequence (captured on first use and after
+ * mosp_batch_configure); *kernel_ms (optional) = device time of the pass. */
+int mosp_batch_run(mosp_batch *b, int32_t use_graph, float *kernel_ms);
+/* After a run: 0 with a message if an instance's old tree did not fit its
+ * graph (the check mosp_update_fused makes after every launch). */
+int mosp_batch_check(mosp_batch *b);
+/* Statistics of instance i from the last run: stats[k] for its k
+ * objectives, cstats for its combined step. */
+int mosp_batch_stats(mosp_batch *b, int32_t i, mosp_sosp_stats *stats,
+                     mosp_combine_stats *cstats);
+/* Schedule: group_size[i] = workgroups running instance i; graph_mode =
+ * 0 no graph yet, 1 captured cooperative launch, 2 captured plain launch. */
+int mosp_batch_schedule(const mosp_batch *b, int32_t *group_size,
+                        int32_t *graph_mode);
+
 /* ------------------------------------------------------ device graph ----- */
 
 /* Copy a host CSR (edge-major weights w[e*K + k], the Values.txt layout) to

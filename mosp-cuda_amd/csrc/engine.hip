@@ -17,13 +17,10 @@
 #include <vector>
 
 #include "engine.hpp"
+#include "engine_internal.hpp"
 #include "host_api.hpp"
 #include "mosp_hip.h"
 
-namespace mosp {
-const void *sosp_kernel_symbol();
-hipError_t launch_ctl_reset(Ctl *ctl, hipStream_t stream);
-}  // namespace mosp
 
 using namespace mosp;
 
@@ -31,7 +28,8 @@ using namespace mosp;
 // Errors
 // ---------------------------------------------------------------------------
 
-namespace {
+namespace mosp {
+namespace detail {
 
 thread_local std::string g_error;
 
@@ -40,20 +38,15 @@ int fail(const std::string &message) {
   return 0;
 }
 
-#define HIP_TRY(call)                                                      \
-  do {                                                                     \
-    hipError_t err_ = (call);                                              \
-    if (err_ != hipSuccess) {                                              \
-      return fail(std::string("HIP error: ") + hipGetErrorString(err_) +   \
-                  " at " + __FILE__ + ":" + std::to_string(__LINE__));     \
-    }                                                                      \
-  } while (0)
+void clear_error() { g_error.clear(); }
 
 hipStream_t as_stream(void *s) { return static_cast<hipStream_t>(s); }
 
 size_t align256(size_t bytes) { return (bytes + 255) & ~static_cast<size_t>(255); }
 
-}  // namespace
+}  // namespace detail
+}  // namespace mosp
+using namespace mosp::detail;
 
 extern "C" const char *mosp_last_error(void) { return g_error.c_str(); }
 
@@ -127,7 +120,8 @@ struct mosp_ws {
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
 };
 
-namespace {
+namespace mosp {
+namespace detail {
 
 void ws_release_slab(mosp_ws *ws) {
   (void)hipFree(ws->slab);
@@ -211,6 +205,27 @@ int ws_warm_up(mosp_ws *ws) {
   return run_launch(ws, args, 0, nullptr, nullptr);
 }
 
+size_t state_bytes(int n) {
+  const size_t count = static_cast<size_t>(std::max(n, 1));
+  return align256(count * sizeof(u64)) + 8 * align256(count * sizeof(int));
+}
+
+ObjState carve_state(char *&p, int n) {
+  const size_t count = static_cast<size_t>(std::max(n, 1));
+  const size_t wordBytes = align256(count * sizeof(u64));
+  const size_t intBytes = align256(count * sizeof(int));
+  ObjState st{};
+  st.packed = reinterpret_cast<u64 *>(p);
+  p += wordBytes;
+  int **ints[8] = {&st.stamp, &st.inFar, &st.flag, &st.ancestor, &st.near[0],
+                   &st.near[1], &st.far[0], &st.far[1]};
+  for (int i = 0; i < 8; ++i) {
+    *ints[i] = reinterpret_cast<int *>(p);
+    p += intBytes;
+  }
+  return st;
+}
+
 int ws_reserve(mosp_ws *ws, int capacity, int slots) {
   const bool fresh = ws->dCtl == nullptr;
   if (!ws_init(ws) || (fresh && !ws_warm_up(ws))) {
@@ -223,25 +238,15 @@ int ws_reserve(mosp_ws *ws, int capacity, int slots) {
   capacity = std::max(capacity, ws->capacity);
   slots = std::max(slots, ws->slots);
   ws_release_slab(ws);
-  const size_t n = static_cast<size_t>(std::max(capacity, 1));
-  const size_t wordBytes = align256(n * sizeof(u64));
-  const size_t intBytes = align256(n * sizeof(int));
-  const size_t perSlot = wordBytes + 9 * intBytes;
+  const int n = std::max(capacity, 1);
+  const size_t perSlot = state_bytes(n);
   HIP_TRY(hipMalloc(&ws->slab, perSlot * slots));
   // stamps, far flags and invalidation marks must start at zero
   HIP_TRY(hipMemset(ws->slab, 0, perSlot * slots));
   char *p = static_cast<char *>(ws->slab);
   ws->state.resize(slots);
   for (int s = 0; s < slots; ++s) {
-    ObjState &st = ws->state[s];
-    st.packed = reinterpret_cast<u64 *>(p);
-    p += wordBytes;
-    int **ints[9] = {&st.stamp, &st.inFar, &st.flag, &st.ancestor, &st.near[0],
-                     &st.near[1], &st.far[0], &st.far[1], &st.cand};
-    for (int i = 0; i < 9; ++i) {
-      *ints[i] = reinterpret_cast<int *>(p);
-      p += intBytes;
-    }
+    ws->state[s] = carve_state(p, n);
   }
   ws->capacity = capacity;
   ws->slots = slots;
@@ -263,13 +268,6 @@ int ws_reserve_combined(mosp_ws *ws, int n, int K) {
   return 1;
 }
 
-// Packing of (distance << b | parent) words for n vertices
-// (src/sospUpdateGpu.cu:127-143, 670-681).
-struct PackChoice {
-  int bits;
-  u64 noParent;
-  u64 bound;
-};
 
 int choose_packing(int n, long long maxWeight, PackChoice &pc) {
   const u64 outputMax = static_cast<u64>(DISTANCE_INF / 2 - 1);
@@ -402,7 +400,8 @@ ObjDesc make_desc(const mosp_csr &out, const mosp_csr *in,
   return d;
 }
 
-}  // namespace
+}  // namespace detail
+}  // namespace mosp
 
 extern "C" mosp_ws *mosp_ws_create(void) { return new mosp_ws(); }
 

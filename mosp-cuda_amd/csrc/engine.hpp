@@ -27,6 +27,9 @@ constexpr int MOSP_MAX_OBJ = 32;     // K <= 32 (csrGraph.cu:736-739)
 #define MOSP_BLOCK 1024
 #endif
 constexpr int BLOCK = MOSP_BLOCK;    // threads per workgroup (wave64 multiple)
+#ifndef MOSP_MIN_BLOCKS
+#define MOSP_MIN_BLOCKS 1            // co-resident workgroups per CU the code is built for
+#endif
 
 // Phases of one launch.
 enum : int {
@@ -51,7 +54,6 @@ struct ObjState {
   int *ancestor;   // pointer-jumping ancestors, n
   int *near[2];    // near frontiers (ping-pong), n each
   int *far[2];     // far piles (ping-pong), n each
-  int *cand;       // pull candidates, n
 };
 
 // One objective of a launch (device-resident descriptor).
@@ -79,7 +81,6 @@ struct ObjCtl {
   unsigned nearCount[3];   // near frontier sizes, slot = iteration % 3
   unsigned farCount[3];    // far pile sizes, slot = epoch % 3
   u64 farMin[3];           // min distance appended to the far pile of an epoch
-  unsigned candCount;
   unsigned invalidated;
   unsigned overflow;       // an input distance does not fit the packing
   unsigned iterations;
@@ -147,6 +148,14 @@ struct LaunchArgs {
   int *cPartials;          // one per workgroup (scan)
   const ObjDesc *cDesc;    // search on the combined CSR: outRow = row pointers,
                            // outCol / outW = edges, packing, scratch, outputs
+};
+
+// A group of workgroups of a batched launch and the instances it runs, in
+// order: LaunchArgs [first, last) of the launch's instance array.
+struct BatchGroup {
+  int size;   // workgroups in the group
+  int first;
+  int last;
 };
 
 }  // namespace mosp

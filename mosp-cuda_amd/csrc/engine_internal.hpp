@@ -1,0 +1,61 @@
+// engine_internal.hpp -- host-side helpers shared by the engine's sources
+// (engine.hip: single launches and the device graph; batch.hip: batched
+// launches).  Not part of the C ABI.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <string>
+
+#include "engine.hpp"
+#include "mosp_hip.h"
+
+namespace mosp {
+
+// Kernels and their launchers (sosp_kernel.hip).
+const void *sosp_kernel_symbol();
+const void *sosp_batched_symbol();
+hipError_t launch_ctl_reset(Ctl *ctl, hipStream_t stream);
+hipError_t launch_ctl_reset_many(Ctl *ctls, int count, hipStream_t stream);
+
+namespace detail {
+
+// Records the message for mosp_last_error() and returns 0.
+int fail(const std::string &message);
+void clear_error();
+hipStream_t as_stream(void *s);
+size_t align256(size_t bytes);
+
+#define HIP_TRY(call)                                                      \
+  do {                                                                     \
+    hipError_t err_ = (call);                                              \
+    if (err_ != hipSuccess) {                                              \
+      return ::mosp::detail::fail(std::string("HIP error: ") +             \
+                                  hipGetErrorString(err_) + " at " +       \
+                                  __FILE__ + ":" + std::to_string(__LINE__)); \
+    }                                                                      \
+  } while (0)
+
+// Packing of (distance << b | parent) words for n vertices
+// (src/sospUpdateGpu.cu:127-143, 670-681).
+struct PackChoice {
+  int bits;
+  u64 noParent;
+  u64 bound;
+};
+
+int choose_packing(int n, long long maxWeight, PackChoice &pc);
+// Pointer-jumping rounds for n vertices: ceil(log2 n) + 1.
+int jump_rounds(int n);
+void fill_stats(const Ctl &c, int k, const ObjDesc &d, float ms,
+                mosp_sosp_stats *stats);
+ObjDesc make_desc(const mosp_csr &out, const mosp_csr *in,
+                  const mosp_changes *ch, const PackChoice &pc, u64 delta,
+                  const ObjState &st);
+// Carves the scratch arrays of one objective for n vertices out of p
+// (advanced past them); state_bytes(n) is their size.
+size_t state_bytes(int n);
+ObjState carve_state(char *&p, int n);
+
+}  // namespace detail
+}  // namespace mosp

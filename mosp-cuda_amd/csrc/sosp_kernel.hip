@@ -94,6 +94,8 @@ __device__ __forceinline__ int *subprof_last() {
 struct GridBarrier {
   Ctl *ctl;
   unsigned epoch;
+  unsigned rank;  // workgroup rank within the group that synchronizes
+  unsigned size;  // workgroups in that group
 #ifdef MOSP_SUBPROFILE
   bool wasLast = false;  // this workgroup released the last barrier
 #endif
@@ -107,8 +109,8 @@ struct GridBarrier {
     // member access after one would be reloaded.
     const unsigned e = ++epoch;
     Ctl *const c = ctl;
-    const unsigned G = gridDim.x;
-    const unsigned g = blockIdx.x % BARRIER_GROUPS;
+    const unsigned G = size;
+    const unsigned g = rank % BARRIER_GROUPS;
     unsigned *const mail = c->mail;
     unsigned *const box = mail + g * MAIL_STRIDE;
     if (threadIdx.x < 64) {
@@ -184,40 +186,76 @@ __device__ __forceinline__ unsigned wave_reserve(unsigned *counter) {
   return base + rank;
 }
 
-// A list append staged in LDS: lanes reserve LDS slots; a workgroup's
-// entries are published by flush() with one global atomicAdd.  When the LDS
-// buffer is full the entry goes straight to the global list.
+// List appends staged in LDS.  The buffer is split into equal slices, one
+// per team of waves working on one objective (slice 0 = the whole buffer
+// when the whole workgroup is one team); lanes reserve slots in their slice
+// (one LDS atomic per wave) and a slice's entries are published with ONE
+// global atomicAdd.  A full slice sends entries straight to the global list.
+constexpr unsigned WAVES = BLOCK / 64;
+// HIP's second launch bound is the minimum number of waves per SIMD: 4 SIMDs
+// per CU share the workgroups' waves.
+constexpr unsigned WAVES_PER_SIMD = MOSP_MIN_BLOCKS * WAVES / 4;
+
 struct BlockQueue {
   int buf[QCAP];
-  unsigned count;
-  unsigned base;
+  unsigned count[WAVES];  // entries reserved per slice
+  unsigned base[WAVES];   // list offset of a slice being flushed
 };
 
-__device__ __forceinline__ void queue_push(BlockQueue &q, int v, int *list,
-                                           unsigned *listCount) {
-  const unsigned slot = wave_reserve(&q.count);
-  if (slot < QCAP) {
-    q.buf[slot] = v;
+__device__ __forceinline__ void slice_push(BlockQueue &q, unsigned slice, unsigned cap, int v,
+                                           int *list, unsigned *listCount) {
+  const unsigned slot = wave_reserve(&q.count[slice]);
+  if (slot < cap) {
+    q.buf[slice * cap + slot] = v;
   } else {
     list[wave_reserve(listCount)] = v;
   }
 }
 
-// Workgroup-uniform: publish the staged entries and empty the queue.
+__device__ __forceinline__ void queue_push(BlockQueue &q, int v, int *list,
+                                           unsigned *listCount) {
+  slice_push(q, 0, QCAP, v, list, listCount);
+}
+
+// Workgroup-uniform: publish the whole-buffer queue and empty it.
 __device__ void queue_flush(BlockQueue &q, int *list, unsigned *listCount) {
   __syncthreads();
-  const unsigned count = q.count < QCAP ? q.count : QCAP;
+  const unsigned count = q.count[0] < QCAP ? q.count[0] : QCAP;
   if (threadIdx.x == 0) {
-    q.base = count > 0 ? atomicAdd(listCount, count) : 0;
-    q.count = 0;
+    q.base[0] = count > 0 ? atomicAdd(listCount, count) : 0;
+    q.count[0] = 0;
   }
   __syncthreads();
   for (unsigned i = threadIdx.x; i < count; i += BLOCK) {
-    list[q.base + i] = q.buf[i];
+    list[q.base[0] + i] = q.buf[i];
   }
   __syncthreads();
 }
 
+// Workgroup-uniform: publish slices [0, teams) of `cap` entries to their
+// lists (one global atomicAdd per slice) and empty them.
+__device__ void flush_slices(BlockQueue &q, unsigned teams, unsigned cap,
+                             int *const *lists, unsigned *const *counts) {
+  __syncthreads();
+  if (threadIdx.x < teams) {
+    const unsigned s = threadIdx.x;
+    const unsigned n = q.count[s] < cap ? q.count[s] : cap;
+    q.base[s] = n > 0 ? atomicAdd(counts[s], n) : 0;
+    q.count[s] = n;
+  }
+  __syncthreads();
+  for (unsigned s = 0; s < teams; ++s) {
+    const unsigned n = q.count[s];
+    int *list = lists[s];
+    for (unsigned i = threadIdx.x; i < n; i += BLOCK) {
+      list[q.base[s] + i] = q.buf[s * cap + i];
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x < teams) {
+    q.count[threadIdx.x] = 0;
+  }
+}
 
 __device__ __forceinline__ u64 pack(u64 distance, int parent, int bits,
                                     u64 noParent) {
@@ -258,8 +296,10 @@ __device__ unsigned block_sum(unsigned value, unsigned *red) {
 }
 
 struct Threads {
-  unsigned id;     // global thread index
-  unsigned count;  // threads in the grid
+  unsigned id;      // thread index within the group of workgroups
+  unsigned count;   // threads in the group
+  unsigned block;   // workgroup rank within the group
+  unsigned blocks;  // workgroups in the group
 };
 
 // ---------------------------------------------------------------------------
@@ -277,7 +317,14 @@ struct Shared {
   int blockBegin[MOSP_MAX_OBJ];  // workgroups working on objective k
   int blockEnd[MOSP_MAX_OBJ];
   int any;
-  u64 farMin;                    // this workgroup's far appends
+  u64 farMin;                    // this workgroup's far appends (pull)
+  int mine[MOSP_MAX_OBJ];        // objectives this workgroup works on now
+  int nmine;
+  u64 sliceFarMin[WAVES];        // per team: min distance appended to far
+  int *sliceList[2][WAVES];      // per team: near / far target lists
+  unsigned *sliceCount[2][WAVES];
+  int sliceObj[WAVES];
+  int sliceFarSlot[WAVES];
   unsigned *src[MAIL_WORDS];     // counters the next grid barrier publishes
   unsigned snap[MAIL_WORDS];     // their values after it
   unsigned statIter[MOSP_MAX_OBJ];
@@ -341,13 +388,23 @@ __device__ void partition_blocks(Shared &sh, int nobj, unsigned G) {
 // farCount[0] / farMin[0]; `initThr` is the first threshold.
 // ---------------------------------------------------------------------------
 
+// A team: a contiguous run of waves of one workgroup working on one
+// objective in a near-far iteration, with its own LDS queue slices.  When a
+// workgroup serves several objectives (small groups, many objectives) its
+// waves split into teams that run them concurrently.
+struct Team {
+  unsigned slice;  // LDS queue slice of the team
+  unsigned tid;    // thread index within the team (teams start on waves)
+  unsigned size;   // threads in the team
+  unsigned cap;    // entries per slice
+  bool whole;      // the team is the whole workgroup: it may flush mid-pass
+};
+
 // Relax the out-edges of the frontier vertices of objective k: groups of
-// 2^laneBits lanes share a vertex and stride over its edges.  Workgroup-
-// uniform chunks, so the LDS queues can be flushed between chunks.
-__device__ void push_items(const ObjDesc &D, ObjCtl &c, Shared &sh, int k,
-                           unsigned it, int source, int gen, unsigned blockRank,
-                           unsigned blocks, int laneBits, Ctl *prof) {
-  (void)prof;
+// 2^laneBits lanes share a vertex and stride over its edges.
+__device__ void push_team(const ObjDesc &D, ObjCtl &c, Shared &sh, int k,
+                          unsigned it, int source, int gen, unsigned blockRank,
+                          unsigned blocks, int laneBits, const Team &tm) {
   const unsigned cnt = sh.cnt[k];
   const int e = sh.epoch[k];
   const int bits = D.parentBits;
@@ -368,15 +425,14 @@ __device__ void push_items(const ObjDesc &D, ObjCtl &c, Shared &sh, int k,
   unsigned *nextCnt = &c.nearCount[(it + 1) % 3];
   unsigned *farCnt = &c.farCount[e % 3];
   const unsigned lanes = 1u << laneBits;
-  const unsigned sub = threadIdx.x & (lanes - 1);
-  const unsigned perBlock = BLOCK >> laneBits;
+  const unsigned sub = tm.tid & (lanes - 1);
+  const unsigned perPass = tm.size >> laneBits;
   // Contiguous, even shares: a short frontier still spreads over every
   // workgroup of the objective (random loads cost per-CU address cycles).
   const unsigned lo = static_cast<unsigned>(static_cast<u64>(cnt) * blockRank / blocks);
   const unsigned hi = static_cast<unsigned>(static_cast<u64>(cnt) * (blockRank + 1) / blocks);
-  SUBPROF_MARK(tLoop);
-  for (unsigned base = lo; base < hi; base += perBlock) {
-    const unsigned i = base + (threadIdx.x >> laneBits);
+  for (unsigned base = lo; base < hi; base += perPass) {
+    const unsigned i = base + (tm.tid >> laneBits);
     if (i < hi) {
       const int u = in[i];
       // Independent loads, issued together.
@@ -405,35 +461,37 @@ __device__ void push_items(const ObjDesc &D, ObjCtl &c, Shared &sh, int k,
             if (nd < thr) {
               if (stampArr[w] != gen) {
                 stampArr[w] = gen;
-                queue_push(sh.near, w, outNear, nextCnt);
+                slice_push(sh.near, tm.slice, tm.cap, w, outNear, nextCnt);
               }
             } else if (inFar[w] == 0) {
               inFar[w] = 1;
-              queue_push(sh.far, w, farBuf, farCnt);
-              atomicMin(&sh.farMin, nd);
+              slice_push(sh.far, tm.slice, tm.cap, w, farBuf, farCnt);
+              atomicMin(&sh.sliceFarMin[tm.slice], nd);
             }
           }
         }
       }
     }
-    __syncthreads();
-    if (sh.near.count > QCAP / 2) queue_flush(sh.near, outNear, nextCnt);
-    if (sh.far.count > QCAP / 2) queue_flush(sh.far, farBuf, farCnt);
+    if (tm.whole) {
+      __syncthreads();
+      if (sh.near.count[0] > QCAP / 2) queue_flush(sh.near, outNear, nextCnt);
+      if (sh.far.count[0] > QCAP / 2) queue_flush(sh.far, farBuf, farCnt);
+    }
   }
-  SUBPROF_MARK(tFlush);
-  queue_flush(sh.near, outNear, nextCnt);
-  queue_flush(sh.far, farBuf, farCnt);
-  SUBPROF_MARK(tDone);
-  SUBPROF_ADD(3, tLoop, tFlush);
-  SUBPROF_ADD(4, tFlush, tDone);
-  SUBPROF_ADD(5, 0, 1);
+  if (tm.tid == 0) {
+    sh.sliceList[0][tm.slice] = outNear;
+    sh.sliceCount[0][tm.slice] = nextCnt;
+    sh.sliceList[1][tm.slice] = farBuf;
+    sh.sliceCount[1][tm.slice] = farCnt;
+    sh.sliceFarSlot[tm.slice] = e % 3;
+  }
 }
 
 // Raise the threshold past the far pile: move the entries below it to the
 // next near frontier, keep the rest in the other far buffer.
-__device__ void split_items(const ObjDesc &D, ObjCtl &c, Shared &sh, int k,
-                            unsigned it, int gen, unsigned blockRank,
-                            unsigned blocks) {
+__device__ void split_team(const ObjDesc &D, ObjCtl &c, Shared &sh, int k,
+                           unsigned it, int gen, unsigned blockRank,
+                           unsigned blocks, const Team &tm) {
   const unsigned cnt = sh.cnt[k];
   const int e = sh.epoch[k];
   const int bits = D.parentBits;
@@ -448,8 +506,8 @@ __device__ void split_items(const ObjDesc &D, ObjCtl &c, Shared &sh, int k,
   int *const inFar = D.s.inFar;
   const unsigned lo = static_cast<unsigned>(static_cast<u64>(cnt) * blockRank / blocks);
   const unsigned hi = static_cast<unsigned>(static_cast<u64>(cnt) * (blockRank + 1) / blocks);
-  for (unsigned base = lo; base < hi; base += BLOCK) {
-    const unsigned i = base + threadIdx.x;
+  for (unsigned base = lo; base < hi; base += tm.size) {
+    const unsigned i = base + tm.tid;
     if (i < hi) {
       const int v = in[i];
       const u64 word = packed[v];
@@ -458,19 +516,26 @@ __device__ void split_items(const ObjDesc &D, ObjCtl &c, Shared &sh, int k,
         inFar[v] = 0;
         if (stampArr[v] != gen) {  // duplicates of v in the pile: append once
           stampArr[v] = gen;
-          queue_push(sh.near, v, outNear, nextCnt);
+          slice_push(sh.near, tm.slice, tm.cap, v, outNear, nextCnt);
         }
       } else {
-        queue_push(sh.far, v, far2, far2Cnt);
-        atomicMin(&sh.farMin, d);
+        slice_push(sh.far, tm.slice, tm.cap, v, far2, far2Cnt);
+        atomicMin(&sh.sliceFarMin[tm.slice], d);
       }
     }
-    __syncthreads();
-    if (sh.near.count > QCAP - BLOCK) queue_flush(sh.near, outNear, nextCnt);
-    if (sh.far.count > QCAP - BLOCK) queue_flush(sh.far, far2, far2Cnt);
+    if (tm.whole) {
+      __syncthreads();
+      if (sh.near.count[0] > QCAP - BLOCK) queue_flush(sh.near, outNear, nextCnt);
+      if (sh.far.count[0] > QCAP - BLOCK) queue_flush(sh.far, far2, far2Cnt);
+    }
   }
-  queue_flush(sh.near, outNear, nextCnt);
-  queue_flush(sh.far, far2, far2Cnt);
+  if (tm.tid == 0) {
+    sh.sliceList[0][tm.slice] = outNear;
+    sh.sliceCount[0][tm.slice] = nextCnt;
+    sh.sliceList[1][tm.slice] = far2;
+    sh.sliceCount[1][tm.slice] = far2Cnt;
+    sh.sliceFarSlot[tm.slice] = (e + 1) % 3;
+  }
 }
 
 __device__ int near_far(const ObjDesc *objs, int nobj, ObjCtl *ctls,
@@ -487,10 +552,6 @@ __device__ int near_far(const ObjDesc *objs, int nobj, ObjCtl *ctls,
     sh.statIter[k] = 0;
     sh.statEpoch[k] = 0;
     sh.statPush[k] = 0;
-  }
-  if (threadIdx.x == 0) {
-    sh.near.count = 0;
-    sh.far.count = 0;
   }
   for (unsigned it = 0;; ++it) {
     SUBPROF_MARK(tTop);
@@ -539,12 +600,12 @@ __device__ int near_far(const ObjDesc *objs, int nobj, ObjCtl *ctls,
       break;
     }
     if (threadIdx.x == 0) {
-      partition_blocks(sh, nobj, gridDim.x);
+      partition_blocks(sh, nobj, t.blocks);
     }
     __syncthreads();
     SUBPROF_MARK(tSnap);
     const int gen = ++generation;
-    if (blockIdx.x == 0 && threadIdx.x < static_cast<unsigned>(nobj)) {
+    if (t.block == 0 && threadIdx.x < static_cast<unsigned>(nobj)) {
       // Slots that nobody reads or appends to in this iteration.
       const int k = threadIdx.x;
       ObjCtl &c = ctls[k];
@@ -559,30 +620,46 @@ __device__ int near_far(const ObjDesc *objs, int nobj, ObjCtl *ctls,
         sh.statPush[k] += sh.cnt[k];
       }
     }
-    for (int k = 0; k < nobj; ++k) {
-      const int b0 = sh.blockBegin[k], b1 = sh.blockEnd[k];
-      if (sh.mode[k] == MODE_DONE || static_cast<int>(blockIdx.x) < b0 ||
-          static_cast<int>(blockIdx.x) >= b1) {
-        continue;  // workgroup-uniform
+    // The objectives of this workgroup; with several, its waves split into
+    // teams (up to one per wave) that run them concurrently.
+    if (threadIdx.x == 0) {
+      int mine = 0;
+      for (int k = 0; k < nobj; ++k) {
+        if (sh.mode[k] != MODE_DONE && static_cast<int>(t.block) >= sh.blockBegin[k] &&
+            static_cast<int>(t.block) < sh.blockEnd[k]) {
+          sh.mine[mine++] = k;
+        }
       }
-      const ObjDesc &D = objs[k];
-      ObjCtl &c = ctls[k];
-      if (threadIdx.x == 0) {
-        sh.farMin = PACKED_INF;
+      sh.nmine = mine;
+    }
+    __syncthreads();
+    const int nmine = sh.nmine;
+    for (int first = 0; first < nmine; first += static_cast<int>(WAVES)) {
+      const unsigned teams = min(WAVES, static_cast<unsigned>(nmine - first));
+      const unsigned team = (threadIdx.x / 64) * teams / WAVES;
+      const unsigned w0 = (team * WAVES + teams - 1) / teams;
+      const unsigned w1 = ((team + 1) * WAVES + teams - 1) / teams;
+      const Team tm{team, threadIdx.x - 64 * w0, 64 * (w1 - w0), QCAP / teams, teams == 1};
+      const int k = sh.mine[first + static_cast<int>(team)];
+      if (threadIdx.x < teams) {
+        sh.sliceFarMin[threadIdx.x] = PACKED_INF;
       }
       __syncthreads();
-      const unsigned rank = blockIdx.x - b0, blocks = b1 - b0;
-      int slot;
+      const unsigned rank = t.block - static_cast<unsigned>(sh.blockBegin[k]);
+      const unsigned blocks = static_cast<unsigned>(sh.blockEnd[k] - sh.blockBegin[k]);
       if (sh.mode[k] == MODE_PUSH) {
-        push_items(D, c, sh, k, it, source, gen, rank, blocks, laneBits,
-                   grid.ctl);
-        slot = sh.epoch[k] % 3;
+        push_team(objs[k], ctls[k], sh, k, it, source, gen, rank, blocks, laneBits, tm);
       } else {
-        split_items(D, c, sh, k, it, gen, rank, blocks);
-        slot = (sh.epoch[k] + 1) % 3;
+        split_team(objs[k], ctls[k], sh, k, it, gen, rank, blocks, tm);
       }
-      if (threadIdx.x == 0 && sh.farMin != PACKED_INF) {
-        atomicMin(&c.farMin[slot], sh.farMin);
+      if (tm.tid == 0) {
+        sh.sliceObj[team] = k;
+      }
+      flush_slices(sh.near, teams, tm.cap, sh.sliceList[0], sh.sliceCount[0]);
+      flush_slices(sh.far, teams, tm.cap, sh.sliceList[1], sh.sliceCount[1]);
+      if (threadIdx.x < teams && sh.sliceFarMin[threadIdx.x] != PACKED_INF) {
+        atomicMin(&ctls[sh.sliceObj[threadIdx.x]].farMin[sh.sliceFarSlot[threadIdx.x]],
+                  sh.sliceFarMin[threadIdx.x]);
       }
     }
     // The barrier publishes the counters the next iteration starts from.
@@ -614,7 +691,7 @@ __device__ int near_far(const ObjDesc *objs, int nobj, ObjCtl *ctls,
       }
     }
   }
-  if (blockIdx.x == 0 && threadIdx.x < static_cast<unsigned>(nobj)) {
+  if (t.block == 0 && threadIdx.x < static_cast<unsigned>(nobj)) {
     const int k = threadIdx.x;
     atomicAdd(&ctls[k].iterations, sh.statIter[k]);
     atomicAdd(&ctls[k].epochs, sh.statEpoch[k]);
@@ -765,7 +842,7 @@ __device__ int update_step1(const LaunchArgs &a, GridBarrier &grid, Threads t,
         unsigned *outCount = &ctl->obj[k].jumpCount[(round + 1) % 3];
         const unsigned cnt = round == 0 ? static_cast<unsigned>(n) : sh.snap[k];
         const int *inList = D.s.near[round & 1];
-        for (unsigned base = blockIdx.x * BLOCK; base < cnt; base += t.count) {
+        for (unsigned base = t.block * BLOCK; base < cnt; base += t.count) {
           const unsigned i = base + threadIdx.x;
           if (i < cnt) {
             const int v = round == 0 ? static_cast<int>(i) : inList[i];
@@ -774,7 +851,7 @@ __device__ int update_step1(const LaunchArgs &a, GridBarrier &grid, Threads t,
             }
           }
           __syncthreads();
-          if (sh.near.count > QCAP - BLOCK) queue_flush(sh.near, outList, outCount);
+          if (sh.near.count[0] > QCAP - BLOCK) queue_flush(sh.near, outList, outCount);
         }
         queue_flush(sh.near, outList, outCount);
       }
@@ -832,7 +909,7 @@ __device__ int update_step1(const LaunchArgs &a, GridBarrier &grid, Threads t,
       sh.farMin = PACKED_INF;
     }
     __syncthreads();
-    for (unsigned base = blockIdx.x * BLOCK; base < static_cast<unsigned>(n);
+    for (unsigned base = t.block * BLOCK; base < static_cast<unsigned>(n);
          base += t.count) {
       const int v = static_cast<int>(base + threadIdx.x);
       if (v < n && v != a.source &&
@@ -862,7 +939,7 @@ __device__ int update_step1(const LaunchArgs &a, GridBarrier &grid, Threads t,
         }
       }
       __syncthreads();
-      if (sh.far.count > QCAP - BLOCK) queue_flush(sh.far, D.s.far[0], &c.farCount[0]);
+      if (sh.far.count[0] > QCAP - BLOCK) queue_flush(sh.far, D.s.far[0], &c.farCount[0]);
     }
     queue_flush(sh.far, D.s.far[0], &c.farCount[0]);
     if (threadIdx.x == 0 && sh.farMin != PACKED_INF) {
@@ -941,8 +1018,8 @@ __device__ void combined_build(const LaunchArgs &a, GridBarrier &grid,
   // Exclusive scan of the n + 1 degrees into row pointers: every workgroup
   // owns one contiguous chunk; chunk totals, then offsets, then local scans.
   const int total = n + 1;
-  const int chunk = (total + gridDim.x - 1) / gridDim.x;
-  const int lo = min(total, static_cast<int>(blockIdx.x) * chunk);
+  const int chunk = (total + static_cast<int>(t.blocks) - 1) / static_cast<int>(t.blocks);
+  const int lo = min(total, static_cast<int>(t.block) * chunk);
   const int hi = min(total, lo + chunk);
   const int per = (chunk + BLOCK - 1) / BLOCK;
   const int mlo = min(hi, lo + static_cast<int>(threadIdx.x) * per);
@@ -965,11 +1042,11 @@ __device__ void combined_build(const LaunchArgs &a, GridBarrier &grid,
   const int chunkTotal = sScan[BLOCK - 1];
   const int myExclusive = sScan[threadIdx.x] - mine;
   if (threadIdx.x == 0) {
-    a.cPartials[blockIdx.x] = chunkTotal;
+    a.cPartials[t.block] = chunkTotal;
   }
   grid.sync();
   int offset = 0;
-  for (unsigned b = threadIdx.x; b < blockIdx.x; b += BLOCK) {
+  for (unsigned b = threadIdx.x; b < t.block; b += BLOCK) {
     offset += a.cPartials[b];
   }
   __syncthreads();
@@ -1028,17 +1105,17 @@ __device__ void combined_build(const LaunchArgs &a, GridBarrier &grid,
 // The kernel
 // ---------------------------------------------------------------------------
 
-__global__ void __launch_bounds__(BLOCK) sosp_persistent(LaunchArgs a) {
-  __shared__ Shared sh;
-  GridBarrier grid{a.ctl, 0};
-  const Threads t{blockIdx.x * BLOCK + threadIdx.x, gridDim.x * BLOCK};
+// One update instance (graph, K objectives, combined step) on the group of
+// workgroups `t` describes; the group synchronizes through a.ctl.
+__device__ void run_instance(const LaunchArgs &a, const Threads t, Shared &sh) {
+  GridBarrier grid{a.ctl, 0, t.block, t.blocks};
   const int n = a.n;
   if (t.id == 0) {
     a.ctl->clkStart = wall_clock64();
   }
-  if (threadIdx.x == 0) {
-    sh.near.count = 0;
-    sh.far.count = 0;
+  if (threadIdx.x < WAVES) {
+    sh.near.count[threadIdx.x] = 0;
+    sh.far.count[threadIdx.x] = 0;
   }
 #ifdef MOSP_SUBPROFILE
   if (threadIdx.x < 16) subprof_lds()[threadIdx.x] = 0;
@@ -1134,7 +1211,45 @@ __global__ void __launch_bounds__(BLOCK) sosp_persistent(LaunchArgs a) {
   }
 }
 
-__global__ void ctl_reset(Ctl *ctl) {
+__global__ void __launch_bounds__(BLOCK, WAVES_PER_SIMD) sosp_persistent(LaunchArgs a) {
+  __shared__ Shared sh;
+  run_instance(a, Threads{blockIdx.x * BLOCK + threadIdx.x, gridDim.x * BLOCK, blockIdx.x, gridDim.x}, sh);
+}
+
+// Many independent instances in one launch: workgroup b belongs to group
+// groupOf[b] with rank rankOf[b]; a group runs its instances one after the
+// other (each instance synchronizes its group through its own Ctl).
+__global__ void __launch_bounds__(BLOCK, WAVES_PER_SIMD)
+    sosp_batched(const LaunchArgs *insts, const BatchGroup *groups,
+                 const int *groupOf, const int *rankOf) {
+  __shared__ Shared sh;
+  __shared__ LaunchArgs args;
+  const int g = groupOf[blockIdx.x];
+  if (g < 0) {
+    return;
+  }
+  const BatchGroup G = groups[g];
+  const unsigned rank = static_cast<unsigned>(rankOf[blockIdx.x]);
+  const Threads t{rank * BLOCK + threadIdx.x, static_cast<unsigned>(G.size) * BLOCK, rank,
+                  static_cast<unsigned>(G.size)};
+  for (int i = G.first; i < G.last; ++i) {
+    __syncthreads();
+    {
+      const unsigned *from = reinterpret_cast<const unsigned *>(&insts[i]);
+      unsigned *to = reinterpret_cast<unsigned *>(&args);
+      for (unsigned w = threadIdx.x; w < sizeof(LaunchArgs) / 4; w += BLOCK) {
+        to[w] = from[w];
+      }
+    }
+    __syncthreads();
+    run_instance(args, t, sh);
+  }
+}
+
+// Zero control block blockIdx.x of ctls (one workgroup each) and open the
+// far-pile minima.
+__global__ void ctl_reset(Ctl *ctls) {
+  Ctl *ctl = ctls + blockIdx.x;
   unsigned *words = reinterpret_cast<unsigned *>(ctl);
   const unsigned count = sizeof(Ctl) / sizeof(unsigned);
   for (unsigned i = threadIdx.x; i < count; i += blockDim.x) {
@@ -1156,9 +1271,17 @@ const void *sosp_kernel_symbol() {
   return reinterpret_cast<const void *>(&sosp_persistent);
 }
 
-hipError_t launch_ctl_reset(Ctl *ctl, hipStream_t stream) {
-  hipLaunchKernelGGL(ctl_reset, dim3(1), dim3(256), 0, stream, ctl);
+const void *sosp_batched_symbol() {
+  return reinterpret_cast<const void *>(&sosp_batched);
+}
+
+hipError_t launch_ctl_reset_many(Ctl *ctls, int count, hipStream_t stream) {
+  hipLaunchKernelGGL(ctl_reset, dim3(count), dim3(256), 0, stream, ctls);
   return hipGetLastError();
+}
+
+hipError_t launch_ctl_reset(Ctl *ctl, hipStream_t stream) {
+  return launch_ctl_reset_many(ctl, 1, stream);
 }
 
 }  // namespace mosp

@@ -58,6 +58,16 @@ class CombineStats(C.Structure):
                 ("combined_phase_ms", C.c_float)]
 
 
+class BatchInstance(C.Structure):
+    """mosp_batch_instance -- one update instance of a batched launch."""
+    _fields_ = [("graph", C.c_void_p), ("k", C.c_int32), ("changes", C.POINTER(Changes)),
+                ("source", C.c_int32), ("delta", C.POINTER(C.c_int64)),
+                ("max_weight", C.POINTER(C.c_int64)), ("pref", C.POINTER(C.c_int32)),
+                ("n_pref", C.c_int32), ("d_dist_in", C.c_void_p), ("d_parents_in", C.c_void_p),
+                ("d_dist", C.c_void_p), ("d_parents", C.c_void_p), ("d_comb_dist", C.c_void_p),
+                ("d_comb_parent", C.c_void_p)]
+
+
 EXPORTS = (
     "mosp_last_error", "mosp_ws_create", "mosp_ws_reserve", "mosp_ws_destroy",
     "mosp_ws_set_grid", "mosp_ws_grid", "mosp_ws_last_profile", "mosp_default_delta", "mosp_sosp_update",
@@ -67,7 +77,9 @@ EXPORTS = (
     "mosp_preference_scale", "mosp_combined_edge_weight", "mosp_choose_packing",
     "mosp_apply_batch", "mosp_update_host", "mosp_gen_road_grid", "mosp_gen_rmat",
     "mosp_gen_batch", "mosp_set_device", "mosp_device_count", "mosp_dev_alloc",
-    "mosp_dev_free", "mosp_memcpy", "mosp_device_synchronize",
+    "mosp_dev_free", "mosp_memcpy", "mosp_device_synchronize", "mosp_batch_create",
+    "mosp_batch_destroy", "mosp_batch_configure", "mosp_batch_run", "mosp_batch_check",
+    "mosp_batch_stats", "mosp_batch_schedule", "mosp_batch_set_cost_model",
 )
 
 _LIB: C.CDLL | None = None
@@ -145,6 +157,15 @@ def lib() -> C.CDLL:
     L.mosp_gen_batch.argtypes = [_I, _I, _i32p, _i32p, _i32p, _I, C.c_double, _I, _I, _I,
                                  C.c_uint64, _I, C.POINTER(C.c_int32), _i32p, _i32p, _i32p,
                                  C.POINTER(C.c_int32), _i32p, _i32p]
+    L.mosp_batch_create.restype = _VP
+    L.mosp_batch_create.argtypes = [C.POINTER(BatchInstance), _I]
+    L.mosp_batch_destroy.argtypes = [_VP]
+    L.mosp_batch_configure.argtypes = [_VP, _I, _I, C.c_double, _I]
+    L.mosp_batch_run.argtypes = [_VP, _I, C.POINTER(C.c_float)]
+    L.mosp_batch_check.argtypes = [_VP]
+    L.mosp_batch_stats.argtypes = [_VP, _I, C.POINTER(SospStats), C.POINTER(CombineStats)]
+    L.mosp_batch_schedule.argtypes = [_VP, C.POINTER(C.c_int32), C.POINTER(C.c_int32)]
+    L.mosp_batch_set_cost_model.argtypes = [_VP, C.c_double]
     _LIB = L
     return L
 
@@ -548,9 +569,12 @@ class Session:
         self.batch = batch
         self.maxw, wsum = objective_bounds(original, batch, self.K)
         m = max(original.m, 1)
-        self.delta = np.array([delta if delta > 0 else
-                               max(1, int(default_delta(m, self.n, int(wsum[k])) * delta_scale))
-                               for k in range(self.K)], np.int64)
+        # The reference's default width per objective (original m and weight
+        # sum, src/mospUpdate.cu:177-181); delta_scale multiplies it.
+        self.delta_base = np.array([delta if delta > 0 else
+                                    default_delta(m, self.n, int(wsum[k]))
+                                    for k in range(self.K)], np.int64)
+        self.set_delta_scale(1.0 if delta > 0 else delta_scale)
         self.dg = DeviceGraph(self.updated)
         self.ws = Workspace(self.n, self.K, grid)
         self.heads = _dev(_nz(_i32(batch.ins_to)), np.int32)
@@ -570,6 +594,11 @@ class Session:
         self.cpar = DeviceBuffer(4 * self.n)
         self.stats = (SospStats * self.K)()
         self.cstats = CombineStats()
+
+    def set_delta_scale(self, scale: float) -> None:
+        """Near-far widths = max(1, int(base width * scale)); outputs do not
+        depend on them."""
+        self.delta = np.maximum(1, (self.delta_base * float(scale)).astype(np.int64))
 
     def run(self, stream=None) -> None:
         s = C.c_void_p(stream) if stream is not None else None
