@@ -237,6 +237,28 @@ mosp_device_graph *mosp_device_graph_upload(int32_t n, int32_t k,
                                             const int32_t *col_ind,
                                             const int32_t *weights);
 void mosp_device_graph_free(mosp_device_graph *g);
+/* An empty graph, to be filled by mosp_device_graph_apply (no reference
+ * counterpart). */
+mosp_device_graph *mosp_device_graph_create(void);
+/* Apply a change batch to g in HBM, writing the updated graph to `out` (a
+ * different graph: empty, or a spare whose allocations are reused when large
+ * enough).  Same result, row order and weight-increase mask as the host
+ * applyChangeBatch (src/csrGraph.cu:727-894), which the reference runs on
+ * the host before every upload (src/mospUpdate.cu:94-99); the batch arrays are
+ * host memory (ins_w: n_ins * K, insertion-major); inc_mask (host, n_ins,
+ * may be NULL) receives the bits.  Synchronous on `stream`. */
+int mosp_device_graph_apply(const mosp_device_graph *g, int32_t n_ins,
+                            const int32_t *ins_from, const int32_t *ins_to,
+                            const int32_t *ins_w, int32_t n_del,
+                            const int32_t *del_from, const int32_t *del_to,
+                            mosp_device_graph *out, uint32_t *inc_mask,
+                            void *stream);
+/* Edge count of g. */
+int64_t mosp_device_graph_edges(const mosp_device_graph *g);
+/* Copy g's out-CSR to host arrays (row_ptr n+1, col_ind m, weights m*K
+ * edge-major: the Values.txt layout). */
+int mosp_device_graph_download(const mosp_device_graph *g, int32_t *row_ptr,
+                               int32_t *col_ind, int32_t *weights);
 /* Views of objective `obj`.  Replaces DeviceGraph::out/in
  * (src/deviceGraph.cu:67-85). */
 int mosp_device_graph_view(const mosp_device_graph *g, int32_t obj,

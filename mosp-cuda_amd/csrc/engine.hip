@@ -721,258 +721,15 @@ extern "C" int mosp_combined_sosp(const int32_t *d_parents, int32_t n,
 }
 
 // ---------------------------------------------------------------------------
-// Device graph (src/deviceGraph.cu:28-135) and path costs
+// MOSP path costs on the device graph (device_graph.hip)
 // ---------------------------------------------------------------------------
 
-struct mosp_device_graph {
-  int n = 0;
-  int m = 0;
-  int K = 0;
-  int *outRow = nullptr, *outCol = nullptr, *outW = nullptr;  // outW: K columns
-  int *inRow = nullptr, *inCol = nullptr, *inW = nullptr;
-};
 
 namespace {
-
-constexpr int TILE = 2048;  // scan tile: 256 threads x 8 items
-
-__global__ void split_weights_kernel(const int *edgeMajor, long long total,
-                                     int m, int K, int *columns) {
-  const long long i = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (i >= total) {
-    return;
-  }
-  const long long e = i / K;
-  const int k = static_cast<int>(i - e * K);
-  columns[static_cast<size_t>(k) * m + e] = edgeMajor[i];
-}
-
-__global__ void in_degree_kernel(const int *colInd, int m, int *degree) {
-  const int e = blockIdx.x * blockDim.x + threadIdx.x;
-  if (e < m) {
-    atomicAdd(&degree[colInd[e]], 1);
-  }
-}
-
-__global__ void tile_sums_kernel(const int *in, int count, int *sums) {
-  __shared__ int red[BLOCK];
-  const int base = blockIdx.x * TILE;
-  int s = 0;
-  for (int i = threadIdx.x; i < TILE; i += BLOCK) {
-    const int idx = base + i;
-    s += idx < count ? in[idx] : 0;
-  }
-  red[threadIdx.x] = s;
-  __syncthreads();
-  for (int w = BLOCK / 2; w > 0; w >>= 1) {
-    if (threadIdx.x < w) {
-      red[threadIdx.x] += red[threadIdx.x + w];
-    }
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) {
-    sums[blockIdx.x] = red[0];
-  }
-}
-
-// Exclusive scan of `sums` in place by one workgroup (any length).
-__global__ void scan_sums_kernel(int *sums, int count) {
-  __shared__ int buf[BLOCK];
-  __shared__ int carry;
-  if (threadIdx.x == 0) {
-    carry = 0;
-  }
-  __syncthreads();
-  for (int base = 0; base < count; base += BLOCK) {
-    const int idx = base + threadIdx.x;
-    const int v = idx < count ? sums[idx] : 0;
-    buf[threadIdx.x] = v;
-    __syncthreads();
-    for (int off = 1; off < BLOCK; off <<= 1) {
-      const int add = threadIdx.x >= off ? buf[threadIdx.x - off] : 0;
-      __syncthreads();
-      buf[threadIdx.x] += add;
-      __syncthreads();
-    }
-    if (idx < count) {
-      sums[idx] = carry + buf[threadIdx.x] - v;
-    }
-    __syncthreads();
-    if (threadIdx.x == BLOCK - 1) {
-      carry += buf[BLOCK - 1];
-    }
-    __syncthreads();
-  }
-}
-
-// out[i] = exclusive prefix of in[] (tile offsets from scan_sums_kernel).
-__global__ void tile_scan_kernel(const int *in, int count, const int *offsets,
-                                 int *out) {
-  __shared__ int buf[BLOCK];
-  constexpr int PER = TILE / BLOCK;
-  const int base = blockIdx.x * TILE + threadIdx.x * PER;
-  int vals[PER];
-  int mine = 0;
-#pragma unroll
-  for (int j = 0; j < PER; ++j) {
-    const int idx = base + j;
-    vals[j] = idx < count ? in[idx] : 0;
-    mine += vals[j];
-  }
-  buf[threadIdx.x] = mine;
-  __syncthreads();
-  for (int off = 1; off < BLOCK; off <<= 1) {
-    const int add = threadIdx.x >= off ? buf[threadIdx.x - off] : 0;
-    __syncthreads();
-    buf[threadIdx.x] += add;
-    __syncthreads();
-  }
-  int running = offsets[blockIdx.x] + buf[threadIdx.x] - mine;
-#pragma unroll
-  for (int j = 0; j < PER; ++j) {
-    const int idx = base + j;
-    if (idx < count) {
-      out[idx] = running;
-    }
-    running += vals[j];
-  }
-}
-
-__global__ void fill_reverse_kernel(int n, int m, int K, const int *rowPtr,
-                                    const int *colInd, const int *outW,
-                                    int *cursor, int *inCol, int *inW) {
-  const int u = blockIdx.x * blockDim.x + threadIdx.x;
-  if (u >= n) {
-    return;
-  }
-  for (int e = rowPtr[u]; e < rowPtr[u + 1]; ++e) {
-    const int pos = atomicAdd(&cursor[colInd[e]], 1);
-    inCol[pos] = u;
-    for (int k = 0; k < K; ++k) {
-      inW[static_cast<size_t>(k) * m + pos] = outW[static_cast<size_t>(k) * m + e];
-    }
-  }
-}
 
 int grid_for(long long work) {
   return static_cast<int>((std::max(work, 1LL) + BLOCK - 1) / BLOCK);
 }
-
-// Exclusive scan of count ints (count >= 1) with a temporary of tiles ints.
-int exclusive_scan(const int *in, int *out, int count, hipStream_t stream) {
-  const int tiles = (count + TILE - 1) / TILE;
-  int *sums = nullptr;
-  HIP_TRY(hipMallocAsync(reinterpret_cast<void **>(&sums), sizeof(int) * tiles, stream));
-  hipLaunchKernelGGL(tile_sums_kernel, dim3(tiles), dim3(BLOCK), 0, stream, in,
-                     count, sums);
-  hipLaunchKernelGGL(scan_sums_kernel, dim3(1), dim3(BLOCK), 0, stream, sums,
-                     tiles);
-  hipLaunchKernelGGL(tile_scan_kernel, dim3(tiles), dim3(BLOCK), 0, stream, in,
-                     count, sums, out);
-  HIP_TRY(hipGetLastError());
-  HIP_TRY(hipFreeAsync(sums, stream));
-  return 1;
-}
-
-void free_graph(mosp_device_graph *g) {
-  (void)hipFree(g->outRow);
-  (void)hipFree(g->outCol);
-  (void)hipFree(g->outW);
-  (void)hipFree(g->inRow);
-  (void)hipFree(g->inCol);
-  (void)hipFree(g->inW);
-  delete g;
-}
-
-int build_graph(mosp_device_graph *g, const int32_t *rowPtr,
-                const int32_t *colInd, const int32_t *weights) {
-  const int n = g->n, m = g->m, K = g->K;
-  const size_t mm = static_cast<size_t>(std::max(m, 1));
-  int *edgeMajor = nullptr, *cursor = nullptr;
-  HIP_TRY(hipMalloc(&g->outRow, sizeof(int) * (n + 1)));
-  HIP_TRY(hipMalloc(&g->outCol, sizeof(int) * mm));
-  HIP_TRY(hipMalloc(&g->outW, sizeof(int) * mm * K));
-  HIP_TRY(hipMalloc(&g->inRow, sizeof(int) * (n + 1)));
-  HIP_TRY(hipMalloc(&g->inCol, sizeof(int) * mm));
-  HIP_TRY(hipMalloc(&g->inW, sizeof(int) * mm * K));
-  HIP_TRY(hipMalloc(&edgeMajor, sizeof(int) * mm * K));
-  HIP_TRY(hipMalloc(&cursor, sizeof(int) * (n + 1)));
-  HIP_TRY(hipMemcpy(g->outRow, rowPtr, sizeof(int) * (n + 1), hipMemcpyHostToDevice));
-  if (m > 0) {
-    HIP_TRY(hipMemcpy(g->outCol, colInd, sizeof(int) * m, hipMemcpyHostToDevice));
-    HIP_TRY(hipMemcpy(edgeMajor, weights, sizeof(int) * static_cast<size_t>(m) * K,
-                      hipMemcpyHostToDevice));
-  }
-  const long long total = static_cast<long long>(m) * K;
-  hipLaunchKernelGGL(split_weights_kernel, dim3(grid_for(total)), dim3(BLOCK), 0,
-                     0, edgeMajor, total, m, K, g->outW);
-  HIP_TRY(hipMemset(cursor, 0, sizeof(int) * (n + 1)));
-  hipLaunchKernelGGL(in_degree_kernel, dim3(grid_for(m)), dim3(BLOCK), 0, 0,
-                     g->outCol, m, cursor);
-  if (!exclusive_scan(cursor, g->inRow, n + 1, 0)) {
-    return 0;
-  }
-  HIP_TRY(hipMemcpy(cursor, g->inRow, sizeof(int) * n, hipMemcpyDeviceToDevice));
-  hipLaunchKernelGGL(fill_reverse_kernel, dim3(grid_for(n)), dim3(BLOCK), 0, 0,
-                     n, m, K, g->outRow, g->outCol, g->outW, cursor, g->inCol,
-                     g->inW);
-  HIP_TRY(hipGetLastError());
-  HIP_TRY(hipDeviceSynchronize());
-  (void)hipFree(edgeMajor);
-  (void)hipFree(cursor);
-  return 1;
-}
-
-}  // namespace
-
-extern "C" mosp_device_graph *mosp_device_graph_upload(int32_t n, int32_t K,
-                                                       const int32_t *rowPtr,
-                                                       const int32_t *colInd,
-                                                       const int32_t *weights) {
-  g_error.clear();
-  if (n <= 0 || K <= 0 || K > MOSP_MAX_OBJ) {
-    fail("Error: invalid graph dimensions.");
-    return nullptr;
-  }
-  auto *g = new mosp_device_graph();
-  g->n = n;
-  g->m = rowPtr[n];
-  g->K = K;
-  if (!build_graph(g, rowPtr, colInd, weights)) {
-    const std::string keep = g_error;
-    free_graph(g);
-    g_error = keep.empty() ? "Error: could not allocate the graph on the GPU." : keep;
-    return nullptr;
-  }
-  return g;
-}
-
-extern "C" void mosp_device_graph_free(mosp_device_graph *g) {
-  if (g != nullptr) {
-    free_graph(g);
-  }
-}
-
-extern "C" int mosp_device_graph_view(const mosp_device_graph *g, int32_t obj,
-                                      mosp_csr *out, mosp_csr *in) {
-  if (obj < 0 || obj >= g->K) {
-    return fail("Error: objective out of range.");
-  }
-  const size_t off = static_cast<size_t>(obj) * g->m;
-  if (out != nullptr) {
-    *out = mosp_csr{g->n, g->m, g->outRow, g->outCol, g->outW + off};
-  }
-  if (in != nullptr) {
-    *in = mosp_csr{g->n, g->m, g->inRow, g->inCol, g->inW + off};
-  }
-  return 1;
-}
-
-extern "C" int32_t mosp_device_graph_objectives(const mosp_device_graph *g) {
-  return g->K;
-}
-
-namespace {
 
 // Path costs: every vertex starts with its tree edge's K weights (first (p,v)
 // edge of row p) and its parent as ancestor, then pointer jumping adds the
@@ -1056,7 +813,7 @@ extern "C" int mosp_path_costs_gpu(const mosp_device_graph *g,
   HIP_TRY(hipMemsetAsync(missing, 0, sizeof(int), s));
   long long *costA = reinterpret_cast<long long *>(d_costs);
   hipLaunchKernelGGL(path_cost_init, dim3(grid_for(n)), dim3(BLOCK), 0, s, n, K,
-                     g->m, g->outRow, g->outCol, g->outW, d_parent, source, ancA,
+                     static_cast<int>(g->stride), g->outRow, g->outCol, g->outW, d_parent, source, ancA,
                      costA, missing);
   const int rounds = jump_rounds(n);
   for (int r = 0; r < rounds; ++r) {

@@ -10,6 +10,21 @@
 #include "engine.hpp"
 #include "mosp_hip.h"
 
+// A graph in HBM (device_graph.hip): out-CSR + in-CSR, each with K weight
+// columns of `stride` (>= m) entries, objective k at w + k * stride.
+// Capacities let a spare graph be rebuilt in place by mosp_device_graph_apply.
+struct mosp_device_graph {
+  int n = 0;
+  int m = 0;
+  int K = 0;
+  long long stride = 0;    // weight column length (edge capacity)
+  int rowCap = 0;          // vertex capacity of the row arrays
+  int *outRow = nullptr, *outCol = nullptr, *outW = nullptr;
+  int *inRow = nullptr, *inCol = nullptr, *inW = nullptr;
+  char *scratch = nullptr;  // build / apply temporaries (grow-only)
+  size_t scratchBytes = 0;
+};
+
 namespace mosp {
 
 // Kernels and their launchers (sosp_kernel.hip).

@@ -80,6 +80,8 @@ EXPORTS = (
     "mosp_dev_free", "mosp_memcpy", "mosp_device_synchronize", "mosp_batch_create",
     "mosp_batch_destroy", "mosp_batch_configure", "mosp_batch_run", "mosp_batch_check",
     "mosp_batch_stats", "mosp_batch_schedule", "mosp_batch_set_cost_model",
+    "mosp_device_graph_create", "mosp_device_graph_apply", "mosp_device_graph_edges",
+    "mosp_device_graph_download",
 )
 
 _LIB: C.CDLL | None = None
@@ -132,6 +134,12 @@ def lib() -> C.CDLL:
     L.mosp_device_graph_upload.restype = _VP
     L.mosp_device_graph_upload.argtypes = [_I, _I, _i32p, _i32p, _i32p]
     L.mosp_device_graph_free.argtypes = [_VP]
+    L.mosp_device_graph_create.restype = _VP
+    L.mosp_device_graph_apply.argtypes = [_VP, _I, _i32p, _i32p, _i32p, _I, _i32p, _i32p, _VP,
+                                          _u32p, _VP]
+    L.mosp_device_graph_edges.argtypes = [_VP]
+    L.mosp_device_graph_edges.restype = _L
+    L.mosp_device_graph_download.argtypes = [_VP, _i32p, _i32p, _i32p]
     L.mosp_device_graph_view.argtypes = [_VP, _I, C.POINTER(Csr), C.POINTER(Csr)]
     L.mosp_device_graph_objectives.argtypes = [_VP]
     L.mosp_device_graph_objectives.restype = _I
@@ -459,12 +467,36 @@ class Workspace:
 class DeviceGraph:
     """mosp_device_graph: out-CSR + in-CSR with K weight columns in HBM."""
 
-    def __init__(self, g: HostGraph):
+    def __init__(self, g: HostGraph | None = None):
         _require_gpu()
+        if g is None:  # empty: the target of apply()
+            self.n = self.K = self.m = 0
+            self.handle = lib().mosp_device_graph_create()
+            return
         self.n, self.K, self.m = g.n, g.K, g.m
         self.handle = lib().mosp_device_graph_upload(g.n, g.K, g.row, _nz(g.col), _nz(g.w))
         if not self.handle:
             raise RuntimeError(f"device graph upload failed: {last_error()}")
+
+    def apply(self, b: "Batch", out: "DeviceGraph", stream=None) -> np.ndarray:
+        """Apply batch b in HBM (applyChangeBatch semantics), writing the
+        updated graph into `out`; returns the weight-increase mask."""
+        mask = np.zeros(max(b.n_ins, 1), np.uint32)
+        s = C.c_void_p(stream) if stream is not None else None
+        _check(lib().mosp_device_graph_apply(
+            self.handle, b.n_ins, _nz(_i32(b.ins_from)), _nz(_i32(b.ins_to)),
+            _nz(_i32(b.ins_w)), b.n_del, _nz(_i32(b.del_from)), _nz(_i32(b.del_to)),
+            out.handle, mask, s), "mosp_device_graph_apply")
+        out.n, out.K, out.m = self.n, self.K, int(lib().mosp_device_graph_edges(out.handle))
+        return mask[: b.n_ins].copy()
+
+    def download(self) -> HostGraph:
+        m = int(lib().mosp_device_graph_edges(self.handle))
+        row = np.zeros(self.n + 1, np.int32)
+        col = np.zeros(max(m, 1), np.int32)
+        w = np.zeros(max(m, 1) * self.K, np.int32)
+        _check(lib().mosp_device_graph_download(self.handle, row, col, w), "download")
+        return HostGraph(self.n, self.K, row, col[:m].copy(), w[: m * self.K].copy())
 
     def view(self, k: int) -> tuple[Csr, Csr]:
         out, inn = Csr(), Csr()
