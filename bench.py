@@ -95,9 +95,16 @@ def main() -> int:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--traffic", default=str(ROOT / "profiles" / "traffic_road1400_k3.json"),
                     help="measured HBM bytes per launch (rocprofv3 FETCH_SIZE/WRITE_SIZE)")
-    ap.add_argument("--mode", choices=("plain", "batched"), default="plain",
+    ap.add_argument("--mode", choices=("plain", "batched", "served"), default="plain",
                     help="plain: the 1.96M-vertex update (the headline line); batched: 64 "
-                         "uneven instances in one launch vs 64 serial launches")
+                         "uneven instances in one launch vs 64 serial launches; served: "
+                         "resident graphs, Poisson requests, continuous batching")
+    ap.add_argument("--requests", type=int, default=240)
+    ap.add_argument("--rate", type=float, default=60.0, help="served: requests per second")
+    ap.add_argument("--slots", type=int, default=4)
+    ap.add_argument("--coalesce", type=int, default=4)
+    ap.add_argument("--hbm-budget-gb", type=float, default=0.0,
+                    help="served: HBM budget (0 = resident state + room for every slot)")
     ap.add_argument("--instances", type=int, default=64)
     ap.add_argument("--tune-cache", default=str(ROOT / "profiles" / "autotune_batched.json"),
                     help="autotuned batched launch choices (read; --retune re-times)")
@@ -109,6 +116,8 @@ def main() -> int:
         return 2
     if args.mode == "batched":
         return bench_batched(args)
+    if args.mode == "served":
+        return bench_served(args)
 
     import mosp_amd as M
     from mosp_amd.roofline import alg_bytes
@@ -296,6 +305,76 @@ def bench_batched(args) -> int:
     bu.close()
     for s in sessions:
         s.close()
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+def bench_served(args) -> int:
+    """8 resident graphs (4 road-like 1400x1400 grids K=3, 4 R-MAT scale 20
+    K=4, weights U[1,100]); Poisson-arriving requests with batch sizes
+    log-uniform in [100, 100k] (50% insertions, 10% disconnecting deletions);
+    continuous batching over `slots` concurrent workers under an HBM budget.
+    Reports p50/p99 latency (arrival to completion) and changes/s."""
+    import mosp_amd as M
+    from mosp_amd import serve as S
+
+    M.set_device(0)
+    t = time.perf_counter()
+    originals, residents = [], []
+    for gid in range(8):
+        if gid < 4:
+            g = M.gen_road_grid(1400, 1400, 3, seed=100 + gid)
+            K, pref = 3, (1, 1, 1)
+        else:
+            g = M.gen_rmat(20, 16, 4, seed=200 + gid)
+            K, pref = 4, (1, 2, 3, 4)
+        d, p = M.initial_trees(g, K)
+        originals.append(g)
+        residents.append(S.ResidentGraph(gid, g, d, p, K=K, pref=pref))
+        log(f"graph {gid}: n={g.n} m={g.m} K={K} resident ({time.perf_counter() - t:.0f} s)")
+    budget = int(args.hbm_budget_gb * 2**30) if args.hbm_budget_gb > 0 else None
+    reqs = S.poisson_requests(residents, args.requests, args.rate, seed=11,
+                              originals=originals)
+    srv = S.Server(residents, slots=args.slots, max_coalesce=args.coalesce, hbm_budget=budget,
+                   delta_scale=args.delta_scale)
+    log(f"server: {args.slots} slots x {srv.slot_workgroups} workgroups, coalesce <= "
+        f"{args.coalesce}, HBM budget {srv.hbm_budget / 2**30:.2f} GiB (resident "
+        f"{srv.resident / 2**30:.2f} GiB, {srv.slab_bytes / 2**30:.2f} GiB scratch per slot)")
+    # Warm-up: one small request per graph (first use of spare graphs and
+    # scratch), outside the measured stream.
+    warm = []
+    for i in range(len(residents)):
+        b = S.poisson_requests([residents[i]], 1, 1.0, seed=600 + i, size_min=100,
+                               size_max=100, originals=[originals[i]])[0].batch
+        warm.append(S.Request(-1 - i, i, b))
+    srv.play(warm)
+    srv.close()
+    srv = S.Server(residents, slots=args.slots, max_coalesce=args.coalesce, hbm_budget=budget,
+                   delta_scale=args.delta_scale)
+    stats = srv.play(reqs, timeout=900)
+    srv.close()
+    lat = np.array([r.latency_ms for r in reqs])
+    log(f"served {stats.requests} requests, {stats.changes} changes in {stats.wall_s:.2f} s: "
+        f"p50 {stats.p50_ms:.1f} ms, p99 {stats.p99_ms:.1f} ms, mean coalesced "
+        f"{stats.mean_coalesced:.2f}, slot kernel ms {[round(x, 1) for x in stats.slot_kernel_ms]}")
+    line = {
+        "metric": "changes/s and p50/p99 latency, served: 8 resident graphs, Poisson requests",
+        "value": stats.changes_per_s, "unit": "changes/s",
+        "n_gpus": 1, "steps": stats.requests, "warmup": len(warm),
+        "ms_per_step": float(np.mean(lat)),
+        "p50_ms": stats.p50_ms, "p99_ms": stats.p99_ms, "max_ms": stats.max_ms,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "int64", "data": "synthetic (seeded road-like grids, R-MAT, Poisson requests)",
+        "config": {"workload": "4 road-like 1400x1400 grids K=3 + 4 R-MAT scale 20 K=4, "
+                               "w~U[1,100]; batches log-uniform [100, 100k], 50% inserts, "
+                               "10% disconnecting deletions",
+                   "rate_per_s": args.rate, "requests": stats.requests,
+                   "slots": args.slots, "slot_workgroups": srv.slot_workgroups,
+                   "max_coalesce": args.coalesce, "hbm_budget_bytes": srv.hbm_budget,
+                   "mean_coalesced": stats.mean_coalesced},
+    }
+    for r in residents:
+        r.close()
     print(json.dumps(line), flush=True)
     return 0
 

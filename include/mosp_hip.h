@@ -82,6 +82,8 @@ typedef struct mosp_device_graph mosp_device_graph;
  * 3 device->device. */
 int mosp_set_device(int32_t device);
 int32_t mosp_device_count(void);
+/* Compute units of the current device (0 on error). */
+int32_t mosp_device_cus(void);
 void *mosp_dev_alloc(uint64_t bytes);
 int mosp_dev_free(void *ptr);
 int mosp_memcpy(void *dst, const void *src, uint64_t bytes, int32_t kind);
@@ -200,6 +202,21 @@ typedef struct mosp_batch mosp_batch;
 mosp_batch *mosp_batch_create(const mosp_batch_instance *instances,
                               int32_t count);
 void mosp_batch_destroy(mosp_batch *b);
+/* A batch with room for max_instances instances of max_objectives
+ * objectives in all and slab_bytes of scratch (mosp_batch_instance_bytes per
+ * instance), to be loaded by mosp_batch_reset: a resident worker reuses it
+ * for every set of instances without allocating. */
+mosp_batch *mosp_batch_create_capacity(int32_t max_instances,
+                                       int32_t max_objectives,
+                                       int64_t slab_bytes);
+/* Replace the batch's instances (same rules as mosp_batch_create; fails if
+ * they exceed the capacity) and clear the scratch they use. */
+int mosp_batch_reset(mosp_batch *b, const mosp_batch_instance *instances,
+                     int32_t count);
+/* Scratch bytes one instance of n vertices and k objectives needs. */
+int64_t mosp_batch_instance_bytes(int32_t n, int32_t k);
+/* The batch's own HIP stream (non-blocking; its launches run there). */
+void *mosp_batch_stream(const mosp_batch *b);
 /* Launch shape: workgroups (0 = one per CU), lanes per frontier vertex as
  * 2^lane_bits (< 0 = keep), each instance's widths scaled by delta_scale
  * (> 0; widths stay >= 1), xcd_aware != 0 places small groups on one XCD. */
@@ -211,9 +228,12 @@ int mosp_batch_configure(mosp_batch *b, int32_t workgroups, int32_t lane_bits,
  * instance finishes at the same time.  latency_items (default 400) is the
  * number of vertices a workgroup relaxes in one iteration's latency. */
 int mosp_batch_set_cost_model(mosp_batch *b, double latency_items);
-/* One pass over every instance, synchronous.  use_graph != 0 replays a HIP
+/* One pass over every instance, synchronous.  use_graph 1 replays a HIP
  * graph of the launch sequence (captured on first use and after
- * mosp_batch_configure); *kernel_ms (optional) = device time of the pass. */
+ * mosp_batch_configure / mosp_batch_reset); 0 launches cooperatively; 2
+ * launches the same grid without the cooperative API (for callers that run
+ * several batches at once on disjoint shares of the CUs).  *kernel_ms
+ * (optional) = device time of the pass. */
 int mosp_batch_run(mosp_batch *b, int32_t use_graph, float *kernel_ms);
 /* After a run: 0 with a message if an instance's old tree did not fit its
  * graph (the check mosp_update_fused makes after every launch). */
@@ -255,6 +275,12 @@ int mosp_device_graph_apply(const mosp_device_graph *g, int32_t n_ins,
                             void *stream);
 /* Edge count of g. */
 int64_t mosp_device_graph_edges(const mosp_device_graph *g);
+/* Per-objective maximum and sum of g's weights (k entries each, either may
+ * be NULL), computed on the device: what mospUpdate takes from the original
+ * graph for the packing bound and the near-far width (src/mospUpdate.cu:
+ * 104-122, 177-181).  Synchronous on `stream`. */
+int mosp_device_graph_weight_stats(mosp_device_graph *g, int64_t *max_weight,
+                                   int64_t *weight_sum, void *stream);
 /* Copy g's out-CSR to host arrays (row_ptr n+1, col_ind m, weights m*K
  * edge-major: the Values.txt layout). */
 int mosp_device_graph_download(const mosp_device_graph *g, int32_t *row_ptr,

@@ -60,6 +60,9 @@ size_t combined_bytes(int n, int K, int partials) {
 
 struct mosp_batch {
   std::vector<Instance> inst;
+  int capInstances = 0;   // allocation capacities (mosp_batch_reset reuses them)
+  int capDescs = 0;
+  size_t capSlab = 0;
   int cus = 0;
   int maxBlocks = 0;  // co-resident workgroups of sosp_batched
   int xcds = 1;
@@ -170,7 +173,9 @@ int describe(Instance &I, const mosp_batch_instance &spec, int index) {
   return 1;
 }
 
-int init(mosp_batch *b, const mosp_batch_instance *spec, int count) {
+// Device attributes, and allocations for up to maxInstances instances, maxDescs
+// descriptors and slabBytes of scratch.
+int setup(mosp_batch *b, int maxInstances, int maxDescs, size_t slabBytes) {
   int device = 0, cooperative = 0, perCu = 0, clockKhz = 0;
   HIP_TRY(hipGetDevice(&device));
   HIP_TRY(hipDeviceGetAttribute(&cooperative, hipDeviceAttributeCooperativeLaunch, device));
@@ -193,34 +198,60 @@ int init(mosp_batch *b, const mosp_batch_instance *spec, int count) {
     b->xcds = std::max(1, std::atoi(env));
   }
   b->workgroups = std::min(b->cus, b->maxBlocks);
-  b->inst.resize(count);
-  size_t slabBytes = 0;
-  for (int i = 0; i < count; ++i) {
-    Instance &I = b->inst[i];
-    if (!describe(I, spec[i], i)) {
-      return 0;
-    }
-    I.slabOffset = slabBytes;
-    slabBytes += static_cast<size_t>(I.K) * state_bytes(I.n) +
-                 combined_bytes(I.n, I.K, b->maxBlocks);
-    I.descOffset = b->descCount;
-    b->descCount += I.K + 1;
-  }
-  HIP_TRY(hipMalloc(&b->slab, std::max<size_t>(slabBytes, 256)));
-  // stamps, far flags and invalidation marks must start at zero
-  HIP_TRY(hipMemset(b->slab, 0, std::max<size_t>(slabBytes, 256)));
-  HIP_TRY(hipMalloc(&b->dCtl, sizeof(Ctl) * count));
-  HIP_TRY(hipMalloc(&b->dGen, sizeof(int) * count));
-  HIP_TRY(hipMemset(b->dGen, 0, sizeof(int) * count));
-  HIP_TRY(hipMalloc(&b->dDesc, sizeof(ObjDesc) * b->descCount));
-  HIP_TRY(hipMalloc(&b->dArgs, sizeof(LaunchArgs) * count));
-  HIP_TRY(hipMalloc(&b->dGroups, sizeof(BatchGroup) * std::max(count, b->maxBlocks)));
+  maxInstances = std::max(maxInstances, 1);
+  maxDescs = std::max(maxDescs, 1);
+  b->capInstances = maxInstances;
+  b->capDescs = maxDescs;
+  b->capSlab = std::max<size_t>(slabBytes, 256);
+  HIP_TRY(hipMalloc(&b->slab, b->capSlab));
+  HIP_TRY(hipMalloc(&b->dCtl, sizeof(Ctl) * maxInstances));
+  HIP_TRY(hipMalloc(&b->dGen, sizeof(int) * maxInstances));
+  HIP_TRY(hipMalloc(&b->dDesc, sizeof(ObjDesc) * maxDescs));
+  HIP_TRY(hipMalloc(&b->dArgs, sizeof(LaunchArgs) * maxInstances));
+  HIP_TRY(hipMalloc(&b->dGroups, sizeof(BatchGroup) * std::max(maxInstances, b->maxBlocks)));
   HIP_TRY(hipMalloc(&b->dGroupOf, sizeof(int) * b->maxBlocks));
   HIP_TRY(hipMalloc(&b->dRankOf, sizeof(int) * b->maxBlocks));
   HIP_TRY(hipHostMalloc(&b->hCtl, sizeof(Ctl), hipHostMallocDefault));
   HIP_TRY(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
   HIP_TRY(hipEventCreate(&b->ev0));
   HIP_TRY(hipEventCreate(&b->ev1));
+  return 1;
+}
+
+// Slab bytes of one instance: K objective states + the combined workspace.
+size_t instance_bytes(int n, int K, int partials) {
+  return static_cast<size_t>(K) * state_bytes(n) + combined_bytes(n, K, partials);
+}
+
+// Describe `count` instances into b (within its capacity) and clear the
+// scratch they use (stamps, far flags and invalidation marks start at zero;
+// generations restart).
+int load(mosp_batch *b, const mosp_batch_instance *spec, int count) {
+  std::vector<Instance> inst(count);
+  size_t slabBytes = 0;
+  int descs = 0;
+  for (int i = 0; i < count; ++i) {
+    Instance &I = inst[i];
+    if (!describe(I, spec[i], i)) {
+      return 0;
+    }
+    I.slabOffset = slabBytes;
+    slabBytes += instance_bytes(I.n, I.K, b->maxBlocks);
+    I.descOffset = descs;
+    descs += I.K + 1;
+  }
+  if (count > b->capInstances || descs > b->capDescs || slabBytes > b->capSlab) {
+    return fail("Error: the batch needs " + std::to_string(slabBytes) + " bytes of scratch for " +
+                std::to_string(count) + " instances; this batch holds " +
+                std::to_string(b->capSlab) + " bytes for " + std::to_string(b->capInstances) +
+                ".");
+  }
+  b->inst = std::move(inst);
+  b->descCount = descs;
+  HIP_TRY(hipMemsetAsync(b->slab, 0, std::max<size_t>(slabBytes, 256), b->stream));
+  HIP_TRY(hipMemsetAsync(b->dGen, 0, sizeof(int) * count, b->stream));
+  HIP_TRY(hipStreamSynchronize(b->stream));
+  b->dirty = true;
   return 1;
 }
 
@@ -490,11 +521,72 @@ extern "C" mosp_batch *mosp_batch_create(const mosp_batch_instance *instances, i
     return nullptr;
   }
   std::unique_ptr<mosp_batch> b(new mosp_batch());
-  if (!init(b.get(), instances, count)) {
+  // Sizes first (they need the device's workgroup count), then allocations.
+  int descs = 0;
+  for (int i = 0; i < count; ++i) descs += std::max(instances[i].k, 0) + 1;
+  if (!setup(b.get(), count, descs, 256)) {
+    release(b.get());
+    return nullptr;
+  }
+  size_t slab = 0;
+  for (int i = 0; i < count; ++i) {
+    Instance probe;
+    if (!describe(probe, instances[i], i)) {
+      release(b.get());
+      return nullptr;
+    }
+    slab += instance_bytes(probe.n, probe.K, b->maxBlocks);
+  }
+  (void)hipFree(b->slab);
+  b->slab = nullptr;
+  b->capSlab = std::max<size_t>(slab, 256);
+  if (hipMalloc(&b->slab, b->capSlab) != hipSuccess || !load(b.get(), instances, count)) {
+    if (b->slab == nullptr) fail("Error: could not allocate the batch scratch.");
     release(b.get());
     return nullptr;
   }
   return b.release();
+}
+
+extern "C" mosp_batch *mosp_batch_create_capacity(int32_t max_instances, int32_t max_objectives,
+                                                  int64_t slab_bytes) {
+  clear_error();
+  if (max_instances <= 0 || max_objectives <= 0 || slab_bytes < 0) {
+    fail("Error: invalid batch capacity.");
+    return nullptr;
+  }
+  std::unique_ptr<mosp_batch> b(new mosp_batch());
+  if (!setup(b.get(), max_instances, max_objectives + max_instances,
+             static_cast<size_t>(slab_bytes))) {
+    release(b.get());
+    return nullptr;
+  }
+  return b.release();
+}
+
+extern "C" int mosp_batch_reset(mosp_batch *b, const mosp_batch_instance *instances,
+                                int32_t count) {
+  clear_error();
+  if (b == nullptr || instances == nullptr || count <= 0) {
+    return fail("Error: a batch needs at least one instance.");
+  }
+  drop_graph(b);
+  return load(b, instances, count);
+}
+
+extern "C" void *mosp_batch_stream(const mosp_batch *b) {
+  return b == nullptr ? nullptr : static_cast<void *>(b->stream);
+}
+
+extern "C" int64_t mosp_batch_instance_bytes(int32_t n, int32_t k) {
+  int device = 0, cus = 0, perCu = 0;
+  if (n <= 0 || k <= 0 || hipGetDevice(&device) != hipSuccess ||
+      hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess ||
+      hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCu, sosp_batched_symbol(), BLOCK, 0) !=
+          hipSuccess) {
+    return -1;
+  }
+  return static_cast<int64_t>(instance_bytes(n, k, std::max(perCu, 1) * cus));
 }
 
 extern "C" void mosp_batch_destroy(mosp_batch *b) {
@@ -538,17 +630,25 @@ extern "C" int mosp_batch_run(mosp_batch *b, int32_t use_graph, float *kernel_ms
   if (b == nullptr) return fail("Error: no batch.");
   if (b->dirty && !prepare(b)) return 0;
   const int count = static_cast<int>(b->inst.size());
-  if (use_graph) {
+  if (use_graph == 1) {
     if (b->exec == nullptr && !capture(b)) return 0;
     HIP_TRY(hipEventRecord(b->ev0, b->stream));
     HIP_TRY(hipGraphLaunch(b->exec, b->stream));
     HIP_TRY(hipEventRecord(b->ev1, b->stream));
   } else {
+    // use_graph 0: cooperative launch; 2: plain launch of the same
+    // occupancy-bounded grid (co-resident as long as the launches running
+    // together fit the device, e.g. slots of a server splitting the CUs).
     void *params[] = {&b->dArgs, &b->dGroups, &b->dGroupOf, &b->dRankOf};
     HIP_TRY(launch_ctl_reset_many(b->dCtl, count, b->stream));
     HIP_TRY(hipEventRecord(b->ev0, b->stream));
-    HIP_TRY(hipLaunchCooperativeKernel(sosp_batched_symbol(), dim3(b->workgroups), dim3(BLOCK),
-                                       params, 0, b->stream));
+    if (use_graph == 2) {
+      HIP_TRY(hipLaunchKernel(sosp_batched_symbol(), dim3(b->workgroups), dim3(BLOCK), params, 0,
+                              b->stream));
+    } else {
+      HIP_TRY(hipLaunchCooperativeKernel(sosp_batched_symbol(), dim3(b->workgroups),
+                                         dim3(BLOCK), params, 0, b->stream));
+    }
     HIP_TRY(hipEventRecord(b->ev1, b->stream));
   }
   HIP_TRY(hipStreamSynchronize(b->stream));

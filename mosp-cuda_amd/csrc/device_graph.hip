@@ -167,6 +167,40 @@ __global__ void zero_kernel(int *a, int count) {
   }
 }
 
+// Per-objective maximum and sum of the weights of column `k` (one grid-stride
+// pass per objective; workgroup reductions, then one atomic each).
+__global__ void weight_stats_kernel(const int *w, int m, unsigned long long *sum,
+                                    unsigned long long *maxv) {
+  __shared__ unsigned long long redSum[BLOCK / 64];
+  __shared__ unsigned long long redMax[BLOCK / 64];
+  unsigned long long s = 0, mx = 0;
+  for (long long e = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x; e < m;
+       e += static_cast<long long>(gridDim.x) * blockDim.x) {
+    const unsigned long long v = static_cast<unsigned long long>(w[e]);
+    s += v;
+    mx = v > mx ? v : mx;
+  }
+  for (int off = 32; off > 0; off >>= 1) {
+    s += __shfl_xor(s, off);
+    const unsigned long long o = __shfl_xor(mx, off);
+    mx = o > mx ? o : mx;
+  }
+  if (__lane_id() == 0) {
+    redSum[threadIdx.x / 64] = s;
+    redMax[threadIdx.x / 64] = mx;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned long long ts = 0, tm = 0;
+    for (unsigned i = 0; i < blockDim.x / 64; ++i) {
+      ts += redSum[i];
+      tm = redMax[i] > tm ? redMax[i] : tm;
+    }
+    atomicAdd(sum, ts);
+    atomicMax(maxv, tm);
+  }
+}
+
 // ------------------------------------------------------------------ apply --
 
 // Degrees, row tags and every original entry's slot (KEEP), before the
@@ -399,6 +433,8 @@ int scratch_reserve(mosp_device_graph *g, size_t bytes) {
 }
 
 void release_arrays(mosp_device_graph *g) {
+  (void)hipFree(g->stats);
+  g->stats = nullptr;
   (void)hipFree(g->scratch);
   g->scratch = nullptr;
   g->scratchBytes = 0;
@@ -530,6 +566,34 @@ extern "C" int mosp_device_graph_view(const mosp_device_graph *g, int32_t obj, m
 }
 
 extern "C" int32_t mosp_device_graph_objectives(const mosp_device_graph *g) { return g->K; }
+
+extern "C" int mosp_device_graph_weight_stats(mosp_device_graph *g, int64_t *maxWeight,
+                                              int64_t *weightSum, void *streamArg) {
+  clear_error();
+  if (g == nullptr || g->outRow == nullptr) return fail("Error: empty graph.");
+  hipStream_t stream = as_stream(streamArg);
+  const int K = g->K;
+  if (g->stats == nullptr) {
+    HIP_TRY(hipMalloc(&g->stats, sizeof(unsigned long long) * 2 * MOSP_MAX_OBJ));
+  }
+  hipLaunchKernelGGL(zero_kernel, dim3(1), dim3(BLOCK), 0, stream, reinterpret_cast<int *>(g->stats),
+                     4 * MOSP_MAX_OBJ);
+  const int blocks = std::max(1, std::min(1024, grid_for(g->m)));
+  for (int k = 0; k < K; ++k) {
+    hipLaunchKernelGGL(weight_stats_kernel, dim3(blocks), dim3(BLOCK), 0, stream,
+                       g->outW + static_cast<size_t>(k) * g->stride, g->m, g->stats + k,
+                       g->stats + MOSP_MAX_OBJ + k);
+  }
+  HIP_TRY(hipGetLastError());
+  unsigned long long host[2 * MOSP_MAX_OBJ];
+  HIP_TRY(hipMemcpyAsync(host, g->stats, sizeof(host), hipMemcpyDeviceToHost, stream));
+  HIP_TRY(hipStreamSynchronize(stream));
+  for (int k = 0; k < K; ++k) {
+    if (weightSum != nullptr) weightSum[k] = static_cast<int64_t>(host[k]);
+    if (maxWeight != nullptr) maxWeight[k] = static_cast<int64_t>(host[MOSP_MAX_OBJ + k]);
+  }
+  return 1;
+}
 
 extern "C" int64_t mosp_device_graph_edges(const mosp_device_graph *g) {
   return g == nullptr ? -1 : g->m;

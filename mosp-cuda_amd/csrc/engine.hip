@@ -55,6 +55,15 @@ extern "C" int mosp_set_device(int32_t device) {
   return 1;
 }
 
+extern "C" int32_t mosp_device_cus(void) {
+  int device = 0, cus = 0;
+  if (hipGetDevice(&device) != hipSuccess ||
+      hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess) {
+    return 0;
+  }
+  return cus;
+}
+
 extern "C" int32_t mosp_device_count(void) {
   int count = 0;
   if (hipGetDeviceCount(&count) != hipSuccess) {

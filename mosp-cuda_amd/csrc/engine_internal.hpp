@@ -23,6 +23,7 @@ struct mosp_device_graph {
   int *inRow = nullptr, *inCol = nullptr, *inW = nullptr;
   char *scratch = nullptr;  // build / apply temporaries (grow-only)
   size_t scratchBytes = 0;
+  unsigned long long *stats = nullptr;  // weight statistics (2 * K words)
 };
 
 namespace mosp {

@@ -81,7 +81,8 @@ EXPORTS = (
     "mosp_batch_destroy", "mosp_batch_configure", "mosp_batch_run", "mosp_batch_check",
     "mosp_batch_stats", "mosp_batch_schedule", "mosp_batch_set_cost_model",
     "mosp_device_graph_create", "mosp_device_graph_apply", "mosp_device_graph_edges",
-    "mosp_device_graph_download",
+    "mosp_device_graph_download", "mosp_device_graph_weight_stats", "mosp_batch_create_capacity",
+    "mosp_batch_reset", "mosp_batch_instance_bytes", "mosp_batch_stream", "mosp_device_cus",
 )
 
 _LIB: C.CDLL | None = None
@@ -102,6 +103,7 @@ def lib() -> C.CDLL:
     L = C.CDLL(str(LIB_PATH))
     L.mosp_set_device.argtypes = [_I]
     L.mosp_device_count.restype = _I
+    L.mosp_device_cus.restype = _I
     L.mosp_dev_alloc.restype = _VP
     L.mosp_dev_alloc.argtypes = [C.c_uint64]
     L.mosp_dev_free.argtypes = [_VP]
@@ -140,6 +142,7 @@ def lib() -> C.CDLL:
     L.mosp_device_graph_edges.argtypes = [_VP]
     L.mosp_device_graph_edges.restype = _L
     L.mosp_device_graph_download.argtypes = [_VP, _i32p, _i32p, _i32p]
+    L.mosp_device_graph_weight_stats.argtypes = [_VP, _i64p, _i64p, _VP]
     L.mosp_device_graph_view.argtypes = [_VP, _I, C.POINTER(Csr), C.POINTER(Csr)]
     L.mosp_device_graph_objectives.argtypes = [_VP]
     L.mosp_device_graph_objectives.restype = _I
@@ -174,6 +177,13 @@ def lib() -> C.CDLL:
     L.mosp_batch_stats.argtypes = [_VP, _I, C.POINTER(SospStats), C.POINTER(CombineStats)]
     L.mosp_batch_schedule.argtypes = [_VP, C.POINTER(C.c_int32), C.POINTER(C.c_int32)]
     L.mosp_batch_set_cost_model.argtypes = [_VP, C.c_double]
+    L.mosp_batch_create_capacity.restype = _VP
+    L.mosp_batch_create_capacity.argtypes = [_I, _I, _L]
+    L.mosp_batch_reset.argtypes = [_VP, C.POINTER(BatchInstance), _I]
+    L.mosp_batch_instance_bytes.restype = _L
+    L.mosp_batch_instance_bytes.argtypes = [_I, _I]
+    L.mosp_batch_stream.restype = _VP
+    L.mosp_batch_stream.argtypes = [_VP]
     _LIB = L
     return L
 
@@ -489,6 +499,15 @@ class DeviceGraph:
             out.handle, mask, s), "mosp_device_graph_apply")
         out.n, out.K, out.m = self.n, self.K, int(lib().mosp_device_graph_edges(out.handle))
         return mask[: b.n_ins].copy()
+
+    def weight_stats(self, stream=None) -> tuple[np.ndarray, np.ndarray]:
+        """(max weight, weight sum) per objective, computed on the device."""
+        mx = np.zeros(self.K, np.int64)
+        sm = np.zeros(self.K, np.int64)
+        _check(lib().mosp_device_graph_weight_stats(self.handle, mx, sm,
+                                                    C.c_void_p(stream) if stream else None),
+               "weight stats")
+        return mx, sm
 
     def download(self) -> HostGraph:
         m = int(lib().mosp_device_graph_edges(self.handle))

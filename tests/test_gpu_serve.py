@@ -1,0 +1,142 @@
+"""Served updates (mosp_amd.serve): resident graphs, continuous batching.
+
+Every request's result -- the K updated trees, combined distances and MOSP
+tree of its graph right after it -- must be the same when requests overlap
+on the GPU (several slots, several requests per launch) as when each runs
+alone, and equal the oracle's pipeline replayed request by request on the
+host (host apply + sequential SOSP update + combined graph + Dijkstra).
+Tolerance: none (integer outputs, bit-exact).
+"""
+from __future__ import annotations
+
+import sys
+
+import numpy as np
+import pytest
+
+import gen
+import oracle_lib as O
+from oracle_lib import i32
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def S():
+    from mosp_amd import serve
+    return serve
+
+
+def M():
+    import mosp_amd
+    return mosp_amd
+
+
+def small_graphs():
+    """Host graphs + K + Pref of a small served mix."""
+    Mm = M()
+    out = []
+    for (w, h, K, pref, seed) in [(25, 25, 3, (1, 1, 1), 31), (30, 20, 2, (1, 3), 32),
+                                  (18, 18, 4, (2, 1, 4, 3), 33)]:
+        g = gen.grid_graph(w, h, K, 100, 0.3, seed=seed)
+        out.append((Mm.HostGraph(g.n, g.K, i32(g.row), i32(g.col), i32(g.w)), K, pref))
+    r = Mm.gen_rmat(9, 8, 3, seed=34)
+    out.append((r, 3, (1, 2, 2)))
+    return out
+
+
+def residents(graphs):
+    res = []
+    for gid, (g, K, pref) in enumerate(graphs):
+        og = O.Graph(g.n, g.K, g.row, g.col, g.w)
+        d, p = gen.initial_trees(og, K)
+        res.append(S().ResidentGraph(gid, g, d, p, K=K, pref=pref))
+    return res
+
+
+def requests(graphs, count=36, seed=5):
+    res = residents(graphs)  # only for sizes; freed below
+    reqs = S().poisson_requests(res, count, rate=400.0, seed=seed, size_min=10, size_max=300,
+                                originals=[g for g, _, _ in graphs])
+    for r in res:
+        r.close()
+    return reqs
+
+
+def fresh(reqs):
+    return [S().Request(r.rid, r.gid, r.batch, arrival=r.arrival) for r in reqs]
+
+
+def same(a, b, tag):
+    for key in ("dist", "parents", "comb_dist", "comb_parent"):
+        assert np.array_equal(a[key], b[key]), f"{tag}: {key}"
+
+
+def test_mix_equals_alone_equals_oracle():
+    graphs = small_graphs()
+    reqs = requests(graphs)
+    # In the mix: 3 slots, up to 3 requests per launch, arrivals at 400/s.
+    mix_res = residents(graphs)
+    mix = S().Server(mix_res, slots=3, slot_workgroups=16, max_coalesce=3, max_batch=400,
+                     keep_results=True)
+    mix_reqs = fresh(reqs)
+    stats = mix.play(mix_reqs)
+    say(f"mix: {stats}")
+    assert max(r.coalesced for r in mix_reqs) >= 1
+    mix.close()
+    # Alone: one slot, one request per launch, submitted one after the other.
+    alone_res = residents(graphs)
+    alone = S().Server(alone_res, slots=1, slot_workgroups=64, max_coalesce=1, max_batch=400,
+                       keep_results=True)
+    alone_reqs = fresh(reqs)
+    alone.start()
+    for r in alone_reqs:
+        r.arrival = alone.now()
+        alone.submit(r)
+        while np.isnan(r.done):
+            with alone.cv:
+                alone.cv.wait(timeout=0.05)
+            if alone.errors:
+                raise alone.errors[0]
+    alone.drain()
+    alone.close()
+    for a, b in zip(mix_reqs, alone_reqs):
+        same(a.result, b.result, f"request {a.rid} (graph {a.gid}) mix vs alone")
+    # Oracle: replay every graph's requests in order on the host.
+    for gid, (g, K, pref) in enumerate(graphs):
+        og = O.Graph(g.n, g.K, g.row, g.col, g.w)
+        d, p = gen.initial_trees(og, K)
+        for r in [x for x in alone_reqs if x.gid == gid]:
+            b = r.batch
+            ob = O.Batch(g.K, b.ins_from, b.ins_to, b.ins_w, b.del_from, b.del_to)
+            want = O.mosp_update(og, ob, d, p, K, 0, list(pref))
+            assert want["ok"]
+            same(r.result, want, f"request {r.rid} (graph {gid}) vs oracle")
+            og, d, p = want["updated"], want["dist"], want["parents"]
+    for x in mix_res + alone_res:
+        x.close()
+
+
+def test_budget_admission_limits_coalescing():
+    graphs = small_graphs()[:3]
+    reqs = requests(graphs, count=16, seed=7)
+    res = residents(graphs)
+    foot = max(r.footprint() for r in res)
+    resident = sum(r.resident_bytes() for r in res)
+    # Room for exactly one update per slot, whatever max_coalesce says.
+    srv = S().Server(res, slots=2, max_coalesce=4, max_batch=400,
+                     hbm_budget=resident + 2 * foot + 1024)
+    run = fresh(reqs)
+    stats = srv.play(run)
+    say(f"budgeted: {stats}")
+    assert all(r.coalesced == 1 for r in run)
+    assert all(not np.isnan(r.done) for r in run)
+    srv.close()
+    with pytest.raises(RuntimeError):
+        S().Server(res, slots=2, hbm_budget=resident + foot // 2)
+    for r in res:
+        r.close()
