@@ -100,7 +100,13 @@ def main() -> int:
                          "uneven instances in one launch vs 64 serial launches; served: "
                          "resident graphs, Poisson requests, continuous batching")
     ap.add_argument("--requests", type=int, default=240)
-    ap.add_argument("--rate", type=float, default=60.0, help="served: requests per second")
+    ap.add_argument("--rate", type=float, default=0.0,
+                    help="served: Poisson requests per second for the latency pass (0: "
+                         "--load times the capacity measured by the saturation pass)")
+    ap.add_argument("--load", type=float, default=0.6,
+                    help="served: offered load of the latency pass as a fraction of capacity")
+    ap.add_argument("--serve-launch", type=int, default=2, choices=(0, 2),
+                    help="served: 0 cooperative launches, 2 plain launches (slots overlap)")
     ap.add_argument("--slots", type=int, default=4)
     ap.add_argument("--coalesce", type=int, default=4)
     ap.add_argument("--hbm-budget-gb", type=float, default=0.0,
@@ -333,15 +339,18 @@ def bench_served(args) -> int:
         residents.append(S.ResidentGraph(gid, g, d, p, K=K, pref=pref))
         log(f"graph {gid}: n={g.n} m={g.m} K={K} resident ({time.perf_counter() - t:.0f} s)")
     budget = int(args.hbm_budget_gb * 2**30) if args.hbm_budget_gb > 0 else None
-    reqs = S.poisson_requests(residents, args.requests, args.rate, seed=11,
-                              originals=originals)
-    srv = S.Server(residents, slots=args.slots, max_coalesce=args.coalesce, hbm_budget=budget,
-                   delta_scale=args.delta_scale)
+
+    def server():
+        return S.Server(residents, slots=args.slots, max_coalesce=args.coalesce,
+                        hbm_budget=budget, delta_scale=args.delta_scale, launch=args.serve_launch)
+
+    srv = server()
     log(f"server: {args.slots} slots x {srv.slot_workgroups} workgroups, coalesce <= "
-        f"{args.coalesce}, HBM budget {srv.hbm_budget / 2**30:.2f} GiB (resident "
-        f"{srv.resident / 2**30:.2f} GiB, {srv.slab_bytes / 2**30:.2f} GiB scratch per slot)")
+        f"{args.coalesce}, launch {args.serve_launch}, HBM budget {srv.hbm_budget / 2**30:.2f} "
+        f"GiB (resident {srv.resident / 2**30:.2f} GiB, {srv.slab_bytes / 2**30:.2f} GiB "
+        f"scratch per slot)")
     # Warm-up: one small request per graph (first use of spare graphs and
-    # scratch), outside the measured stream.
+    # scratch), outside the measured streams.
     warm = []
     for i in range(len(residents)):
         b = S.poisson_requests([residents[i]], 1, 1.0, seed=600 + i, size_min=100,
@@ -349,29 +358,52 @@ def bench_served(args) -> int:
         warm.append(S.Request(-1 - i, i, b))
     srv.play(warm)
     srv.close()
-    srv = S.Server(residents, slots=args.slots, max_coalesce=args.coalesce, hbm_budget=budget,
-                   delta_scale=args.delta_scale)
+
+    def report(tag, st):
+        ph = {k: round(v, 2) for k, v in st.phase_s.items()}
+        log(f"{tag}: {st.requests} requests, {st.changes} changes in {st.wall_s:.2f} s = "
+            f"{st.changes_per_s:.0f} changes/s; latency p50 {st.p50_ms:.1f} p99 {st.p99_ms:.1f} "
+            f"ms; service p50 {st.service_p50_ms:.1f} p99 {st.service_p99_ms:.1f} ms; "
+            f"{st.launches} launches, mean coalesced {st.mean_coalesced:.2f}; slot kernel ms "
+            f"{[round(x) for x in st.slot_kernel_ms]}; slot phase s {ph}")
+
+    # Saturation: every request queued at t=0 (same batches, same order per
+    # graph as the Poisson stream) -> sustainable changes/s.
+    sat_reqs = S.poisson_requests(residents, args.requests, 1.0, seed=11, originals=originals)
+    for r in sat_reqs:
+        r.arrival = 0.0
+    srv = server()
+    sat = srv.play(sat_reqs, timeout=900)
+    srv.close()
+    report("saturation", sat)
+    capacity = sat.requests / sat.wall_s
+    rate = args.rate if args.rate > 0 else args.load * capacity
+    # Latency: open-loop Poisson arrivals at `rate` (fresh seeds: new batches).
+    reqs = S.poisson_requests(residents, args.requests, rate, seed=12, originals=originals)
+    srv = server()
     stats = srv.play(reqs, timeout=900)
     srv.close()
-    lat = np.array([r.latency_ms for r in reqs])
-    log(f"served {stats.requests} requests, {stats.changes} changes in {stats.wall_s:.2f} s: "
-        f"p50 {stats.p50_ms:.1f} ms, p99 {stats.p99_ms:.1f} ms, mean coalesced "
-        f"{stats.mean_coalesced:.2f}, slot kernel ms {[round(x, 1) for x in stats.slot_kernel_ms]}")
+    report(f"poisson {rate:.1f}/s", stats)
     line = {
         "metric": "changes/s and p50/p99 latency, served: 8 resident graphs, Poisson requests",
-        "value": stats.changes_per_s, "unit": "changes/s",
+        "value": sat.changes_per_s, "unit": "changes/s",
         "n_gpus": 1, "steps": stats.requests, "warmup": len(warm),
-        "ms_per_step": float(np.mean(lat)),
+        "ms_per_step": float(sat.wall_s * 1e3 / sat.requests),
         "p50_ms": stats.p50_ms, "p99_ms": stats.p99_ms, "max_ms": stats.max_ms,
+        "service_p50_ms": stats.service_p50_ms, "service_p99_ms": stats.service_p99_ms,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "int64", "data": "synthetic (seeded road-like grids, R-MAT, Poisson requests)",
         "config": {"workload": "4 road-like 1400x1400 grids K=3 + 4 R-MAT scale 20 K=4, "
                                "w~U[1,100]; batches log-uniform [100, 100k], 50% inserts, "
                                "10% disconnecting deletions",
-                   "rate_per_s": args.rate, "requests": stats.requests,
-                   "slots": args.slots, "slot_workgroups": srv.slot_workgroups,
+                   "value_from": "saturation pass (all requests queued at t=0)",
+                   "latency_from": f"open-loop Poisson pass at {rate:.2f} requests/s",
+                   "capacity_requests_per_s": capacity, "rate_per_s": rate,
+                   "requests": stats.requests, "slots": args.slots,
+                   "slot_workgroups": srv.slot_workgroups, "launch": args.serve_launch,
                    "max_coalesce": args.coalesce, "hbm_budget_bytes": srv.hbm_budget,
-                   "mean_coalesced": stats.mean_coalesced},
+                   "mean_coalesced": stats.mean_coalesced,
+                   "poisson_changes_per_s": stats.changes_per_s},
     }
     for r in residents:
         r.close()

@@ -91,6 +91,7 @@ struct mosp_batch {
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   int graphMode = 0;
+  int hubs = 0;  // some instance's graph has a vertex of degree > HUB_DEGREE
   bool dirty = true;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   float lastMs = 0;
@@ -183,7 +184,7 @@ int setup(mosp_batch *b, int maxInstances, int maxDescs, size_t slabBytes) {
     return fail("Error: the GPU does not support cooperative launches.");
   }
   HIP_TRY(hipDeviceGetAttribute(&b->cus, hipDeviceAttributeMultiprocessorCount, device));
-  HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCu, sosp_batched_symbol(), BLOCK, 0));
+  HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCu, sosp_batched_symbol(true), BLOCK, 0));
   if (perCu < 1) {
     return fail("Error: the batched kernel does not fit on a CU.");
   }
@@ -248,6 +249,11 @@ int load(mosp_batch *b, const mosp_batch_instance *spec, int count) {
   }
   b->inst = std::move(inst);
   b->descCount = descs;
+  b->hubs = 0;
+  for (int i = 0; i < count; ++i) {
+    b->hubs |= spec[i].graph->maxDegree > HUB_DEGREE ? 1 : 0;
+  }
+  if (hubs_override() >= 0) b->hubs = hubs_override();
   HIP_TRY(hipMemsetAsync(b->slab, 0, std::max<size_t>(slabBytes, 256), b->stream));
   HIP_TRY(hipMemsetAsync(b->dGen, 0, sizeof(int) * count, b->stream));
   HIP_TRY(hipStreamSynchronize(b->stream));
@@ -490,9 +496,9 @@ int capture(mosp_batch *b) {
     HIP_TRY(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
     hipError_t err = launch_ctl_reset_many(b->dCtl, count, b->stream);
     if (err == hipSuccess) {
-      err = mode == 1 ? hipLaunchCooperativeKernel(sosp_batched_symbol(), dim3(b->workgroups),
+      err = mode == 1 ? hipLaunchCooperativeKernel(sosp_batched_symbol(b->hubs != 0), dim3(b->workgroups),
                                                    dim3(BLOCK), params, 0, b->stream)
-                      : hipLaunchKernel(sosp_batched_symbol(), dim3(b->workgroups), dim3(BLOCK),
+                      : hipLaunchKernel(sosp_batched_symbol(b->hubs != 0), dim3(b->workgroups), dim3(BLOCK),
                                         params, 0, b->stream);
     }
     hipGraph_t graph = nullptr;
@@ -582,7 +588,7 @@ extern "C" int64_t mosp_batch_instance_bytes(int32_t n, int32_t k) {
   int device = 0, cus = 0, perCu = 0;
   if (n <= 0 || k <= 0 || hipGetDevice(&device) != hipSuccess ||
       hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess ||
-      hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCu, sosp_batched_symbol(), BLOCK, 0) !=
+      hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCu, sosp_batched_symbol(true), BLOCK, 0) !=
           hipSuccess) {
     return -1;
   }
@@ -643,10 +649,10 @@ extern "C" int mosp_batch_run(mosp_batch *b, int32_t use_graph, float *kernel_ms
     HIP_TRY(launch_ctl_reset_many(b->dCtl, count, b->stream));
     HIP_TRY(hipEventRecord(b->ev0, b->stream));
     if (use_graph == 2) {
-      HIP_TRY(hipLaunchKernel(sosp_batched_symbol(), dim3(b->workgroups), dim3(BLOCK), params, 0,
+      HIP_TRY(hipLaunchKernel(sosp_batched_symbol(b->hubs != 0), dim3(b->workgroups), dim3(BLOCK), params, 0,
                               b->stream));
     } else {
-      HIP_TRY(hipLaunchCooperativeKernel(sosp_batched_symbol(), dim3(b->workgroups),
+      HIP_TRY(hipLaunchCooperativeKernel(sosp_batched_symbol(b->hubs != 0), dim3(b->workgroups),
                                          dim3(BLOCK), params, 0, b->stream));
     }
     HIP_TRY(hipEventRecord(b->ev1, b->stream));

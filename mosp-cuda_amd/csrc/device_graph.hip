@@ -11,12 +11,21 @@
 // order overwrite the first alive (u,v) -- original entries first, then the
 // row's appended ones -- or append; survivors keep their order and appended
 // edges follow; the weight-increase mask compares the final (u,v) weight with
-// the original first occurrence.  The host only groups the (small) batch by
-// row; one wave per changed row replays its changes with ballot searches, a
-// scan gives the new row pointers, and the rows are copied or compacted.
+// the original first occurrence.  Changes to different targets of a row never
+// interact, so the replay is split by (u,v) pair: of a pair with D deletions
+// and I insertions, the first D occurrences of v in the original row die, the
+// next one takes the last insertion's weights, and with none left the pair
+// appends one entry; a row's appended entries follow the order of each pair's
+// first insertion.  The host sorts the (small) batch into pairs; one wave per
+// pair finds its occurrences, one wave per changed row places the appended
+// entries, a scan gives the new row pointers, and rows are copied (edge-
+// parallel) or compacted.  No step is serial in a row's degree beyond one
+// wave's scan of it, so power-law hub rows cost microseconds, not a thread's
+// walk over tens of thousands of entries.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <limits>
 #include <numeric>
 #include <string>
 #include <vector>
@@ -144,19 +153,61 @@ __global__ void tile_scan_kernel(const int *in, int count, const int *offsets, i
   }
 }
 
-__global__ void fill_reverse_kernel(int n, long long stride, int K, const int *rowPtr,
-                                    const int *colInd, const int *outW, int *cursor, int *inCol,
-                                    int *inW) {
-  const int u = blockIdx.x * blockDim.x + threadIdx.x;
-  if (u >= n) {
+// Largest u in [lo, hi] with rowPtr[u] <= e: the row holding entry e when
+// rowPtr[lo] <= e < rowPtr[hi + 1].
+__device__ __forceinline__ int row_of(const int *rowPtr, int lo, int hi, long long e) {
+  while (lo < hi) {
+    const int mid = lo + (hi - lo + 1) / 2;
+    if (rowPtr[mid] <= e) {
+      lo = mid;
+    } else {
+      hi = mid - 1;
+    }
+  }
+  return lo;
+}
+
+constexpr int EDGE_BLOCK = 256;  // threads per workgroup of the edge-parallel kernels
+
+// Entry e of this thread and its row, for one thread per entry of a CSR with
+// m = rowPtr[n] entries (read on the device): two lanes find the rows of the
+// workgroup's first and last entries, every lane searches between them.
+// Returns false for lanes past the end (the whole workgroup when it starts
+// past m, before the barrier, uniformly).
+__device__ __forceinline__ bool entry_row(const int *rowPtr, int n, long long &e, int &u) {
+  __shared__ int span[2];
+  const long long m = rowPtr[n];
+  const long long first = static_cast<long long>(blockIdx.x) * EDGE_BLOCK;
+  if (first >= m) {
+    return false;
+  }
+  const long long last = std::min(first + EDGE_BLOCK, m) - 1;
+  if (threadIdx.x < 2) {
+    span[threadIdx.x] = row_of(rowPtr, 0, n - 1, threadIdx.x == 0 ? first : last);
+  }
+  __syncthreads();
+  e = first + threadIdx.x;
+  if (e > last) {
+    return false;
+  }
+  u = row_of(rowPtr, span[0], span[1], e);
+  return true;
+}
+
+// One thread per out-entry: place it in its target's in-row (order inside an
+// in-row follows the atomics; every consumer of the in-CSR is order-free).
+__global__ void __launch_bounds__(EDGE_BLOCK)
+fill_reverse_kernel(int n, long long stride, int K, const int *rowPtr, const int *colInd,
+                    const int *outW, int *cursor, int *inCol, int *inW) {
+  long long e;
+  int u;
+  if (!entry_row(rowPtr, n, e, u)) {
     return;
   }
-  for (int e = rowPtr[u]; e < rowPtr[u + 1]; ++e) {
-    const int pos = atomicAdd(&cursor[colInd[e]], 1);
-    inCol[pos] = u;
-    for (int k = 0; k < K; ++k) {
-      inW[k * stride + pos] = outW[k * stride + e];
-    }
+  const int pos = atomicAdd(&cursor[colInd[e]], 1);
+  inCol[pos] = u;
+  for (int k = 0; k < K; ++k) {
+    inW[k * stride + pos] = outW[k * stride + e];
   }
 }
 
@@ -164,6 +215,20 @@ __global__ void zero_kernel(int *a, int count) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < count) {
     a[i] = 0;
+  }
+}
+
+// Largest row length of rowPtr[0..n] folded into *result (atomicMax).
+__global__ void max_degree_kernel(const int *rowPtr, int n, unsigned *result) {
+  unsigned best = 0;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    best = max(best, static_cast<unsigned>(rowPtr[i + 1] - rowPtr[i]));
+  }
+  for (int offset = 32; offset > 0; offset >>= 1) {
+    best = max(best, static_cast<unsigned>(__shfl_xor(static_cast<int>(best), offset)));
+  }
+  if (__lane_id() == 0 && best > 0) {
+    atomicMax(result, best);
   }
 }
 
@@ -203,8 +268,8 @@ __global__ void weight_stats_kernel(const int *w, int m, unsigned long long *sum
 
 // ------------------------------------------------------------------ apply --
 
-// Degrees, row tags and every original entry's slot (KEEP), before the
-// row replay: the replay's only writer per row is then lane 0.
+// Degrees, row tags and every original entry's slot (KEEP) before the pair
+// pass, which then only writes the entries its pair kills or overwrites.
 __global__ void apply_init_kernel(int n, long long m, const int *oldRow, int *newDeg,
                                   int *rowTag, int *slot) {
   const long long i = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -219,111 +284,127 @@ __global__ void apply_init_kernel(int n, long long m, const int *oldRow, int *ne
   }
 }
 
-// First i in [begin, end) with pred(i), searched 64 at a time by one wave;
-// wave-uniform result, -1 if none.
-template <typename Pred>
-__device__ __forceinline__ int wave_find_first(int begin, int end, Pred pred) {
+__device__ __forceinline__ int wave_sum(int x) {
+  for (int off = 32; off > 0; off >>= 1) {
+    x += __shfl_xor(x, off);
+  }
+  return x;
+}
+
+// One wave per (u, v) pair: D = pairDel deletions and the insertions
+// insSorted[pairInsBeg .. +pairInsCnt) (file order; the last one is
+// pairLast).  Marks the first D occurrences of v in the original row DEAD and
+// the next one with pairLast; pairApp = 1 when no occurrence is left for an
+// insertion (the pair appends one entry).  The pair's insertions all get the
+// same weight-increase bits: the last insertion's weights against the first
+// original occurrence's (none when v was not in the row).  Pairs are distinct
+// (u, v), so no two waves write one entry.
+__global__ void apply_pairs_kernel(int P, int K, const int *pairU, const int *pairV,
+                                   const int *pairDel, const int *pairInsBeg,
+                                   const int *pairInsCnt, const int *pairLast,
+                                   const int *insSorted, const int *oldRow, const int *oldCol,
+                                   const int *oldW, long long oldStride, const int *insW,
+                                   int *slot, int *pairDead, int *pairApp, unsigned *mask) {
+  const int p = static_cast<int>(blockIdx.x);  // one wave per workgroup
+  if (p >= P) {
+    return;
+  }
   const int lane = static_cast<int>(__lane_id());
-  for (int base = begin; base < end; base += 64) {
-    const int i = base + lane;
-    const unsigned long long hits = __ballot(i < end && pred(i));
-    if (hits != 0) {
-      return base + __ffsll(static_cast<long long>(hits)) - 1;
+  const unsigned long long below = (1ull << lane) - 1ull;
+  const int u = pairU[p], v = pairV[p], D = pairDel[p], I = pairInsCnt[p], last = pairLast[p];
+  const int need = D + (I > 0 ? 1 : 0);
+  const int ob = oldRow[u], oe = oldRow[u + 1];
+  int found = 0, first = -1;
+  for (int base = ob; base < oe && found < need; base += 64) {
+    const int x = base + lane;
+    const bool hit = x < oe && oldCol[x] == v;
+    const unsigned long long bal = __ballot(hit);
+    if (hit) {
+      const int j = found + __popcll(bal & below);
+      if (j < D) {
+        slot[x] = DEAD;
+      } else if (j == D && I > 0) {
+        slot[x] = last;
+      }
+    }
+    if (first < 0 && bal != 0) {
+      first = base + __ffsll(static_cast<long long>(bal)) - 1;
+    }
+    found += __popcll(bal);
+  }
+  if (lane == 0) {
+    pairDead[p] = std::min(D, found);
+    pairApp[p] = (I > 0 && found <= D) ? 1 : 0;
+  }
+  if (I > 0) {
+    const bool up = lane < K && first >= 0 &&
+                    insW[static_cast<size_t>(last) * K + lane] > oldW[lane * oldStride + first];
+    const unsigned bits = static_cast<unsigned>(__ballot(up));
+    const int beg = pairInsBeg[p];
+    for (int t = lane; t < I; t += 64) {
+      mask[insSorted[beg + t]] = bits;
     }
   }
-  return -1;
 }
 
-// Lane 0's store, complete and visible before any lane's next load: an
-// agent-scope release + acquire (waits for the store, writes back and
-// invalidates the caches the next loads could hit).
-__device__ __forceinline__ void publish_lane0_store() {
-  __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "agent");
-}
-
-// One wave per changed row: replay the row's deletions, then its insertions,
-// in file order.  slot[e] of an original entry: KEEP (set by the init
-// kernel), DEAD or the insertion whose weights it takes; appended entries
-// live at appV/appSlot[insOff[r]..].  Only lane 0 writes.
-__global__ void apply_rows_kernel(int R, const int *rows, const int *oldRow, const int *oldCol,
-                                  const int *delOff, const int *delV, const int *insOff,
-                                  const int *insV, const int *insIdx, int *slot, int *appV,
-                                  int *appSlot, int *appCnt, int *newDeg, int *rowTag) {
+// One wave per changed row: its new degree, and each appending pair's index
+// among the row's appended entries (a row's pairs are ordered by their first
+// insertion in file order, pairs without insertions last).
+__global__ void apply_rows_kernel(int R, const int *rows, const int *rowPairs, const int *oldRow,
+                                  const int *pairDead, const int *pairApp, int *pairAppPos,
+                                  int *newDeg, int *rowTag) {
   const int r = static_cast<int>(blockIdx.x);  // one wave per workgroup
   if (r >= R) {
     return;
   }
   const int lane = static_cast<int>(__lane_id());
-  const int u = rows[r];
-  const int ob = oldRow[u], oe = oldRow[u + 1];
-  for (int j = delOff[r]; j < delOff[r + 1]; ++j) {
-    const int v = delV[j];
-    const int e = wave_find_first(ob, oe, [&](int x) { return oldCol[x] == v && slot[x] != DEAD; });
-    if (e >= 0 && lane == 0) {
-      slot[e] = DEAD;
+  const unsigned long long below = (1ull << lane) - 1ull;
+  const int u = rows[r], pe = rowPairs[r + 1];
+  int dead = 0, app = 0;
+  for (int base = rowPairs[r]; base < pe; base += 64) {
+    const int p = base + lane;
+    const bool in = p < pe;
+    const bool a = in && pairApp[p] != 0;
+    const unsigned long long bal = __ballot(a);
+    if (a) {
+      pairAppPos[p] = app + __popcll(bal & below);
     }
-    publish_lane0_store();
+    app += __popcll(bal);
+    dead += in ? pairDead[p] : 0;
   }
-  const int ab = insOff[r];
-  int count = 0;
-  for (int j = insOff[r]; j < insOff[r + 1]; ++j) {
-    const int v = insV[j], i = insIdx[j];
-    const int e = wave_find_first(ob, oe, [&](int x) { return oldCol[x] == v && slot[x] != DEAD; });
-    if (e >= 0) {
-      if (lane == 0) slot[e] = i;
-    } else {
-      const int a = wave_find_first(ab, ab + count, [&](int x) { return appV[x] == v; });
-      if (a >= 0) {
-        if (lane == 0) appSlot[a] = i;
-      } else {
-        if (lane == 0) {
-          appV[ab + count] = v;
-          appSlot[ab + count] = i;
-        }
-        ++count;
-      }
-    }
-    publish_lane0_store();
-  }
-  int alive = 0;
-  for (int base = ob; base < oe; base += 64) {
-    const int e = base + lane;
-    alive += __popcll(__ballot(e < oe && slot[e] != DEAD));
-  }
+  dead = wave_sum(dead);
   if (lane == 0) {
-    newDeg[u] = alive + count;
-    appCnt[r] = count;
+    newDeg[u] = oldRow[u + 1] - oldRow[u] - dead + app;
     rowTag[u] = r;
   }
 }
 
-// Unchanged rows: 16 lanes per row copy columns and the K weight columns.
-__global__ void apply_copy_kernel(int n, int K, const int *oldRow, const int *oldCol,
-                                  const int *oldW, long long oldStride, const int *rowTag,
-                                  const int *newRow, int *newCol, int *newW, long long newStride) {
-  const long long gid = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
-  const int u = static_cast<int>(gid / 16);
-  const int sub = static_cast<int>(gid % 16);
-  if (u >= n || rowTag[u] >= 0) {
+// Unchanged rows, one thread per original entry: column and K weights move to
+// the same offset in the row's new place.
+__global__ void __launch_bounds__(EDGE_BLOCK)
+apply_copy_kernel(int n, int K, const int *oldRow, const int *oldCol, const int *oldW,
+                  long long oldStride, const int *rowTag, const int *newRow, int *newCol,
+                  int *newW, long long newStride) {
+  long long e;
+  int u;
+  if (!entry_row(oldRow, n, e, u) || rowTag[u] >= 0) {
     return;
   }
-  const int ob = oldRow[u], deg = oldRow[u + 1] - ob, nb = newRow[u];
-  for (int x = sub; x < deg; x += 16) {
-    newCol[nb + x] = oldCol[ob + x];
-    for (int k = 0; k < K; ++k) {
-      newW[k * newStride + nb + x] = oldW[k * oldStride + ob + x];
-    }
+  const long long pos = newRow[u] + (e - oldRow[u]);
+  newCol[pos] = oldCol[e];
+  for (int k = 0; k < K; ++k) {
+    newW[k * newStride + pos] = oldW[k * oldStride + e];
   }
 }
 
 // Changed rows: one wave compacts the alive original entries in order, then
-// writes the appended ones.
-__global__ void apply_compact_kernel(int R, int K, const int *rows, const int *oldRow,
-                                     const int *oldCol, const int *oldW, long long oldStride,
-                                     const int *slot, const int *insOff, const int *appV,
-                                     const int *appSlot, const int *appCnt, const int *insW,
-                                     const int *newRow, int *newCol, int *newW,
-                                     long long newStride) {
+// writes the appended ones at their pairs' places.
+__global__ void apply_compact_kernel(int R, int K, const int *rows, const int *rowPairs,
+                                     const int *oldRow, const int *oldCol, const int *oldW,
+                                     long long oldStride, const int *slot, const int *pairV,
+                                     const int *pairLast, const int *pairApp,
+                                     const int *pairAppPos, const int *insW, const int *newRow,
+                                     int *newCol, int *newW, long long newStride) {
   const int r = static_cast<int>(blockIdx.x);  // one wave per workgroup
   if (r >= R) {
     return;
@@ -347,46 +428,23 @@ __global__ void apply_compact_kernel(int R, int K, const int *rows, const int *o
     }
     out += __popcll(alive);
   }
-  const int ab = insOff[r];
-  for (int a = lane; a < appCnt[r]; a += 64) {
-    const int pos = out + a;
-    const int s = appSlot[ab + a];
-    newCol[pos] = appV[ab + a];
-    for (int k = 0; k < K; ++k) {
-      newW[k * newStride + pos] = insW[static_cast<size_t>(s) * K + k];
-    }
-  }
-}
-
-// One wave per insertion: bit k set if the final (u,v) weight exceeds the
-// original first occurrence's.
-__global__ void apply_mask_kernel(int nIns, int K, const int *insFrom, const int *insTo,
-                                  const int *oldRow, const int *oldCol, const int *oldW,
-                                  long long oldStride, const int *newRow, const int *newCol,
-                                  const int *newW, long long newStride, unsigned *mask) {
-  const int i = static_cast<int>(blockIdx.x);  // one wave per workgroup
-  if (i >= nIns) {
-    return;
-  }
-  const int u = insFrom[i], v = insTo[i];
-  const int eo = wave_find_first(oldRow[u], oldRow[u + 1], [&](int x) { return oldCol[x] == v; });
-  const int en = wave_find_first(newRow[u], newRow[u + 1], [&](int x) { return newCol[x] == v; });
-  if (__lane_id() != 0) {
-    return;
-  }
-  unsigned bits = 0;
-  if (eo >= 0 && en >= 0) {
-    for (int k = 0; k < K; ++k) {
-      if (newW[k * newStride + en] > oldW[k * oldStride + eo]) {
-        bits |= 1u << k;
+  for (int p = rowPairs[r] + lane; p < rowPairs[r + 1]; p += 64) {
+    if (pairApp[p] != 0) {
+      const int pos = out + pairAppPos[p];
+      newCol[pos] = pairV[p];
+      for (int k = 0; k < K; ++k) {
+        newW[k * newStride + pos] = insW[static_cast<size_t>(pairLast[p]) * K + k];
       }
     }
   }
-  mask[i] = bits;
 }
 
 int grid_for(long long work) {
   return static_cast<int>((std::max(work, 1LL) + BLOCK - 1) / BLOCK);
+}
+
+int edge_grid(long long entries) {
+  return static_cast<int>((std::max(entries, 1LL) + EDGE_BLOCK - 1) / EDGE_BLOCK);
 }
 
 // Exclusive scan of count ints (count >= 1); `sums` holds scan_tiles(count).
@@ -494,10 +552,37 @@ int build_reverse(mosp_device_graph *g, char *at, long long mBound, hipStream_t 
   }
   hipLaunchKernelGGL(copy_ints_kernel, dim3(grid_for(n)), dim3(BLOCK), 0, stream, cursor,
                      g->inRow, n);
-  hipLaunchKernelGGL(fill_reverse_kernel, dim3(grid_for(n)), dim3(BLOCK), 0, stream, n,
-                     g->stride, K, g->outRow, g->outCol, g->outW, cursor, g->inCol, g->inW);
+  hipLaunchKernelGGL(fill_reverse_kernel, dim3(edge_grid(mBound)), dim3(EDGE_BLOCK), 0, stream,
+                     n, g->stride, K, g->outRow, g->outCol, g->outW, cursor, g->inCol, g->inW);
   HIP_TRY(hipGetLastError());
   return 1;
+}
+
+// Queues the largest out- or in-degree into *scratch (zeroed first).
+int queue_max_degree(const int *outRow, const int *inRow, int n, unsigned *scratch,
+                     hipStream_t stream) {
+  hipLaunchKernelGGL(zero_kernel, dim3(1), dim3(64), 0, stream, reinterpret_cast<int *>(scratch),
+                     1);
+  const int blocks = std::max(1, std::min(1024, grid_for(n)));
+  hipLaunchKernelGGL(max_degree_kernel, dim3(blocks), dim3(BLOCK), 0, stream, outRow, n, scratch);
+  if (inRow != nullptr) {
+    hipLaunchKernelGGL(max_degree_kernel, dim3(blocks), dim3(BLOCK), 0, stream, inRow, n,
+                       scratch);
+  }
+  HIP_TRY(hipGetLastError());
+  return 1;
+}
+
+// Statistics words of a graph: 2 * MOSP_MAX_OBJ weight words, then the degree.
+int ensure_stats(mosp_device_graph *g) {
+  if (g->stats == nullptr) {
+    HIP_TRY(hipMalloc(&g->stats, sizeof(unsigned long long) * (2 * MOSP_MAX_OBJ + 1)));
+  }
+  return 1;
+}
+
+unsigned *degree_word(mosp_device_graph *g) {
+  return reinterpret_cast<unsigned *>(g->stats + 2 * MOSP_MAX_OBJ);
 }
 
 }  // namespace
@@ -532,6 +617,10 @@ extern "C" mosp_device_graph *mosp_device_graph_upload(int32_t n, int32_t K, con
     hipLaunchKernelGGL(split_weights_kernel, dim3(grid_for(total)), dim3(BLOCK), 0, 0, edgeMajor,
                        total, g->stride, K, g->outW);
     if (!build_reverse(g, g->scratch + wBytes, m, 0)) return 0;
+    if (!ensure_stats(g) ||
+        !graph_max_degree(g->outRow, g->inRow, n, degree_word(g), 0, &g->maxDegree)) {
+      return 0;
+    }
     HIP_TRY(hipDeviceSynchronize());
     return 1;
   };
@@ -573,9 +662,7 @@ extern "C" int mosp_device_graph_weight_stats(mosp_device_graph *g, int64_t *max
   if (g == nullptr || g->outRow == nullptr) return fail("Error: empty graph.");
   hipStream_t stream = as_stream(streamArg);
   const int K = g->K;
-  if (g->stats == nullptr) {
-    HIP_TRY(hipMalloc(&g->stats, sizeof(unsigned long long) * 2 * MOSP_MAX_OBJ));
-  }
+  if (!ensure_stats(g)) return 0;
   hipLaunchKernelGGL(zero_kernel, dim3(1), dim3(BLOCK), 0, stream, reinterpret_cast<int *>(g->stats),
                      4 * MOSP_MAX_OBJ);
   const int blocks = std::max(1, std::min(1024, grid_for(g->m)));
@@ -645,103 +732,164 @@ extern "C" int mosp_device_graph_apply(const mosp_device_graph *g, int32_t nIns,
     }
   }
   hipStream_t stream = as_stream(streamArg);
-  // Group the changes by row, file order inside a row.
-  std::vector<int> dOrder(nDel), iOrder(nIns);
-  std::iota(dOrder.begin(), dOrder.end(), 0);
-  std::iota(iOrder.begin(), iOrder.end(), 0);
-  std::stable_sort(dOrder.begin(), dOrder.end(),
-                   [&](int a, int b) { return delFrom[a] < delFrom[b]; });
-  std::stable_sort(iOrder.begin(), iOrder.end(),
-                   [&](int a, int b) { return insFrom[a] < insFrom[b]; });
-  std::vector<int> rows, delOff{0}, insOff{0}, delV, insV, insIdx;
-  rows.reserve(nDel + nIns);
-  size_t a = 0, b = 0;
-  while (a < dOrder.size() || b < iOrder.size()) {
-    const int ua = a < dOrder.size() ? delFrom[dOrder[a]] : n;
-    const int ub = b < iOrder.size() ? insFrom[iOrder[b]] : n;
-    const int u = std::min(ua, ub);
-    while (a < dOrder.size() && delFrom[dOrder[a]] == u) delV.push_back(delTo[dOrder[a++]]);
-    while (b < iOrder.size() && insFrom[iOrder[b]] == u) {
-      insV.push_back(insTo[iOrder[b]]);
-      insIdx.push_back(iOrder[b++]);
+  // Changes sorted by (row, target, file position) and merged into (u, v)
+  // pairs; a row's pairs then ordered by their first insertion.
+  struct Key {
+    long long uv;
+    int idx;
+  };
+  auto sorted_keys = [](const int32_t *from, const int32_t *to, int count) {
+    std::vector<Key> keys(count);
+    for (int i = 0; i < count; ++i) {
+      keys[i] = {(static_cast<long long>(from[i]) << 32) | static_cast<unsigned>(to[i]), i};
     }
-    rows.push_back(u);
-    delOff.push_back(static_cast<int>(delV.size()));
-    insOff.push_back(static_cast<int>(insV.size()));
+    std::sort(keys.begin(), keys.end(), [](const Key &x, const Key &y) {
+      return x.uv != y.uv ? x.uv < y.uv : x.idx < y.idx;
+    });
+    return keys;
+  };
+  const std::vector<Key> dk = sorted_keys(delFrom, delTo, nDel);
+  const std::vector<Key> ik = sorted_keys(insFrom, insTo, nIns);
+  struct Pair {
+    int u, v, del, insBeg, insCnt, first, last;
+  };
+  std::vector<Pair> pairs;
+  pairs.reserve(static_cast<size_t>(nDel) + nIns);
+  constexpr long long NO_KEY = std::numeric_limits<long long>::max();
+  for (size_t a = 0, b = 0; a < dk.size() || b < ik.size();) {
+    const long long key = std::min(a < dk.size() ? dk[a].uv : NO_KEY,
+                                   b < ik.size() ? ik[b].uv : NO_KEY);
+    Pair pr{static_cast<int>(key >> 32), static_cast<int>(key & 0xffffffffLL), 0,
+            static_cast<int>(b), 0, std::numeric_limits<int>::max(), -1};
+    for (; a < dk.size() && dk[a].uv == key; ++a) ++pr.del;
+    for (; b < ik.size() && ik[b].uv == key; ++b) ++pr.insCnt;
+    if (pr.insCnt > 0) {
+      pr.first = ik[pr.insBeg].idx;
+      pr.last = ik[b - 1].idx;
+    }
+    pairs.push_back(pr);
   }
-  const int R = static_cast<int>(rows.size());
+  std::sort(pairs.begin(), pairs.end(), [](const Pair &x, const Pair &y) {
+    return x.u != y.u ? x.u < y.u : x.first != y.first ? x.first < y.first : x.v < y.v;
+  });
+  const int P = static_cast<int>(pairs.size());
+  // One host staging array -> one upload: rows, rowPairs, the six pair
+  // columns, the sorted insertion indices.
+  std::vector<int> rowList;
+  std::vector<int> rowStart;
+  for (int i = 0; i < P; ++i) {
+    if (i == 0 || pairs[i].u != pairs[i - 1].u) {
+      rowList.push_back(pairs[i].u);
+      rowStart.push_back(i);
+    }
+  }
+  const int R = static_cast<int>(rowList.size());
+  rowStart.push_back(P);
+  const size_t hostInts = static_cast<size_t>(R) + (R + 1) + 6 * static_cast<size_t>(P) + nIns;
+  std::vector<int> host(std::max<size_t>(hostInts, 1));
+  int *h = host.data();
+  std::copy(rowList.begin(), rowList.end(), h);
+  std::copy(rowStart.begin(), rowStart.end(), h + R);
+  int *hp = h + R + (R + 1);
+  for (int i = 0; i < P; ++i) {
+    hp[i] = pairs[i].u;
+    hp[P + i] = pairs[i].v;
+    hp[2 * P + i] = pairs[i].del;
+    hp[3 * P + i] = pairs[i].insBeg;
+    hp[4 * P + i] = pairs[i].insCnt;
+    hp[5 * P + i] = pairs[i].last;
+  }
+  for (int i = 0; i < nIns; ++i) {
+    hp[6 * P + i] = ik[i].idx;
+  }
   const long long mOld = g->m;
   if (!reserve(out, n, mOld + nIns, K)) return 0;
   // Temporaries in the destination graph's scratch (no allocation once it
   // is large enough).
   const size_t W = static_cast<size_t>(nIns) * K;
-  const size_t bytes = Carve::bytes<int>(R) * 2 + Carve::bytes<int>(R + 1) * 2 +
-                       Carve::bytes<int>(delV.size()) + Carve::bytes<int>(insV.size()) * 2 +
-                       Carve::bytes<int>(W) + Carve::bytes<int>(nIns) * 4 +
+  const size_t bytes = Carve::bytes<int>(hostInts) + Carve::bytes<int>(W) +
+                       Carve::bytes<int>(P) * 3 +
                        Carve::bytes<int>(static_cast<size_t>(std::max(mOld, 1LL))) +
                        Carve::bytes<int>(n) + Carve::bytes<int>(n + 1) +
                        Carve::bytes<int>(scan_tiles(n + 1)) + Carve::bytes<unsigned>(nIns) +
                        reverse_bytes(n);
   if (!scratch_reserve(out, bytes)) return 0;
   Carve c{out->scratch};
-  int *dRows = c.take<int>(R), *dAppCnt = c.take<int>(R);
-  int *dDelOff = c.take<int>(R + 1), *dInsOff = c.take<int>(R + 1);
-  int *dDelV = c.take<int>(delV.size()), *dInsV = c.take<int>(insV.size());
-  int *dInsIdx = c.take<int>(insIdx.size()), *dInsW = c.take<int>(W);
-  int *dInsFrom = c.take<int>(nIns), *dInsTo = c.take<int>(nIns);
-  int *appV = c.take<int>(nIns), *appSlot = c.take<int>(nIns);
+  int *dHost = c.take<int>(hostInts), *dInsW = c.take<int>(W);
+  int *pairDead = c.take<int>(P), *pairApp = c.take<int>(P), *pairAppPos = c.take<int>(P);
   int *slot = c.take<int>(static_cast<size_t>(std::max(mOld, 1LL)));
   int *rowTag = c.take<int>(n), *newDeg = c.take<int>(n + 1);
   int *sums = c.take<int>(scan_tiles(n + 1));
   unsigned *dMask = c.take<unsigned>(nIns);
   char *reverseAt = c.p;
-  auto up = [&](int *dst, const int *src, size_t count) -> int {
-    if (count > 0) {
-      HIP_TRY(hipMemcpyAsync(dst, src, sizeof(int) * count, hipMemcpyHostToDevice, stream));
-    }
-    return 1;
-  };
-  if (!up(dRows, rows.data(), R) || !up(dDelOff, delOff.data(), R + 1) ||
-      !up(dInsOff, insOff.data(), R + 1) || !up(dDelV, delV.data(), delV.size()) ||
-      !up(dInsV, insV.data(), insV.size()) || !up(dInsIdx, insIdx.data(), insIdx.size()) ||
-      !up(dInsW, insW, W) || !up(dInsFrom, insFrom, nIns) || !up(dInsTo, insTo, nIns)) {
-    return 0;
+  const int *dRows = dHost, *dRowPairs = dHost + R;
+  const int *pU = dHost + R + (R + 1), *pV = pU + P, *pDel = pU + 2 * P, *pInsBeg = pU + 3 * P;
+  const int *pInsCnt = pU + 4 * P, *pLast = pU + 5 * P, *insSorted = pU + 6 * P;
+  if (hostInts > 0) {
+    HIP_TRY(hipMemcpyAsync(dHost, host.data(), sizeof(int) * hostInts, hipMemcpyHostToDevice,
+                           stream));
+  }
+  if (W > 0) {
+    HIP_TRY(hipMemcpyAsync(dInsW, insW, sizeof(int) * W, hipMemcpyHostToDevice, stream));
   }
   hipLaunchKernelGGL(apply_init_kernel, dim3(grid_for(std::max<long long>(n + 1, mOld))),
                      dim3(BLOCK), 0, stream, n, mOld, g->outRow, newDeg, rowTag, slot);
-  if (R > 0) {
-    hipLaunchKernelGGL(apply_rows_kernel, dim3(R), dim3(64), 0, stream, R,
-                       dRows, g->outRow, g->outCol, dDelOff, dDelV, dInsOff, dInsV, dInsIdx,
-                       slot, appV, appSlot, dAppCnt, newDeg, rowTag);
+  if (P > 0) {
+    hipLaunchKernelGGL(apply_pairs_kernel, dim3(P), dim3(64), 0, stream, P, K, pU, pV, pDel,
+                       pInsBeg, pInsCnt, pLast, insSorted, g->outRow, g->outCol, g->outW,
+                       g->stride, dInsW, slot, pairDead, pairApp, dMask);
+    hipLaunchKernelGGL(apply_rows_kernel, dim3(R), dim3(64), 0, stream, R, dRows, dRowPairs,
+                       g->outRow, pairDead, pairApp, pairAppPos, newDeg, rowTag);
   }
   if (!exclusive_scan(newDeg, out->outRow, n + 1, sums, stream)) return 0;
   out->n = n;
   out->K = K;
-  hipLaunchKernelGGL(apply_copy_kernel, dim3(grid_for(static_cast<long long>(n) * 16)),
-                     dim3(BLOCK), 0, stream, n, K, g->outRow, g->outCol, g->outW, g->stride,
-                     rowTag, out->outRow, out->outCol, out->outW, out->stride);
+  hipLaunchKernelGGL(apply_copy_kernel, dim3(edge_grid(mOld)), dim3(EDGE_BLOCK), 0, stream, n, K,
+                     g->outRow, g->outCol, g->outW, g->stride, rowTag, out->outRow, out->outCol,
+                     out->outW, out->stride);
   if (R > 0) {
-    hipLaunchKernelGGL(apply_compact_kernel, dim3(R), dim3(64), 0, stream, R,
-                       K, dRows, g->outRow, g->outCol, g->outW, g->stride, slot, dInsOff, appV,
-                       appSlot, dAppCnt, dInsW, out->outRow, out->outCol, out->outW,
+    hipLaunchKernelGGL(apply_compact_kernel, dim3(R), dim3(64), 0, stream, R, K, dRows,
+                       dRowPairs, g->outRow, g->outCol, g->outW, g->stride, slot, pV, pLast,
+                       pairApp, pairAppPos, dInsW, out->outRow, out->outCol, out->outW,
                        out->stride);
-  }
-  if (nIns > 0) {
-    hipLaunchKernelGGL(apply_mask_kernel, dim3(nIns), dim3(64), 0, stream, nIns, K, dInsFrom, dInsTo, g->outRow, g->outCol,
-                       g->outW, g->stride, out->outRow, out->outCol, out->outW, out->stride,
-                       dMask);
   }
   HIP_TRY(hipGetLastError());
   if (!build_reverse(out, reverseAt, mOld + nIns, stream)) return 0;
+  if (!ensure_stats(out) ||
+      !queue_max_degree(out->outRow, out->inRow, n, degree_word(out), stream)) {
+    return 0;
+  }
   // Results to the host only after every kernel of the rebuild.
   int mNew = 0;
+  unsigned maxDegree = 0;
   HIP_TRY(hipMemcpyAsync(&mNew, out->outRow + n, sizeof(int), hipMemcpyDeviceToHost, stream));
+  HIP_TRY(hipMemcpyAsync(&maxDegree, degree_word(out), sizeof(unsigned), hipMemcpyDeviceToHost,
+                         stream));
   if (nIns > 0 && incMask != nullptr) {
     HIP_TRY(hipMemcpyAsync(incMask, dMask, sizeof(unsigned) * nIns, hipMemcpyDeviceToHost,
                            stream));
   }
   HIP_TRY(hipStreamSynchronize(stream));
   out->m = mNew;
+  out->maxDegree = static_cast<int>(maxDegree);
   HIP_TRY(hipStreamSynchronize(stream));
   return 1;
 }
+
+namespace mosp {
+namespace detail {
+
+int graph_max_degree(const int *outRow, const int *inRow, int n, unsigned *scratch,
+                     hipStream_t stream, int *result) {
+  unsigned host = 0;
+  if (n > 0) {
+    if (!queue_max_degree(outRow, inRow, n, scratch, stream)) return 0;
+    HIP_TRY(hipMemcpyAsync(&host, scratch, sizeof(unsigned), hipMemcpyDeviceToHost, stream));
+    HIP_TRY(hipStreamSynchronize(stream));
+  }
+  *result = static_cast<int>(host);
+  return 1;
+}
+
+}  // namespace detail
+}  // namespace mosp

@@ -126,6 +126,7 @@ struct mosp_ws {
   int gridBlocks = 0;       // workgroups per launch
   int gridOverride = 0;
   int laneBits = 2;         // 4 lanes per frontier vertex
+  unsigned *degreeWord = nullptr;  // max-degree reduction of caller CSRs
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
 };
 
@@ -164,7 +165,7 @@ int ws_init(mosp_ws *ws) {
   HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount,
                                 device));
   HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(
-      &perCu, reinterpret_cast<const void *>(sosp_kernel_symbol()), BLOCK, 0));
+      &perCu, sosp_kernel_symbol(true), BLOCK, 0));
   if (perCu < 1) {
     return fail("Error: the persistent kernel does not fit on a CU.");
   }
@@ -196,6 +197,7 @@ int ws_init(mosp_ws *ws) {
   HIP_TRY(hipHostMalloc(&ws->hCtl, sizeof(Ctl), hipHostMallocDefault));
   HIP_TRY(hipMalloc(&ws->genDev, sizeof(int)));
   HIP_TRY(hipMemset(ws->genDev, 0, sizeof(int)));
+  HIP_TRY(hipMalloc(&ws->degreeWord, sizeof(unsigned)));
   HIP_TRY(hipMalloc(&ws->cPartials, sizeof(int) * ws->maxBlocks));
   HIP_TRY(hipEventCreate(&ws->ev0));
   HIP_TRY(hipEventCreate(&ws->ev1));
@@ -340,7 +342,8 @@ int run_launch(mosp_ws *ws, LaunchArgs &args, int descCount,
   HIP_TRY(launch_ctl_reset(ws->dCtl, stream));
   void *params[] = {&args};
   HIP_TRY(hipEventRecord(ws->ev0, stream));
-  HIP_TRY(hipLaunchCooperativeKernel(sosp_kernel_symbol(),
+  const int forced = hubs_override();
+  HIP_TRY(hipLaunchCooperativeKernel(sosp_kernel_symbol(forced >= 0 ? forced != 0 : args.hubs != 0),
                                      dim3(ws->gridBlocks), dim3(BLOCK), params,
                                      0, stream));
   HIP_TRY(hipEventRecord(ws->ev1, stream));
@@ -409,6 +412,36 @@ ObjDesc make_desc(const mosp_csr &out, const mosp_csr *in,
   return d;
 }
 
+int hubs_override() {
+  const char *env = std::getenv("MOSP_HUBS");
+  return env == nullptr || *env == '\0' ? -1 : (std::atoi(env) != 0 ? 1 : 0);
+}
+
+// Whether some vertex of the caller's CSRs (out and in row arrays of K
+// objectives; views of one graph share them) has more than HUB_DEGREE edges.
+int caller_hubs(mosp_ws *ws, const mosp_csr *out, const mosp_csr *in, int K, hipStream_t stream,
+                int &hubs) {
+  hubs = 0;
+  std::vector<const int *> seen;
+  for (int k = 0; k < K && !hubs; ++k) {
+    for (const int *row : {out[k].row_ptr, in != nullptr ? in[k].row_ptr : nullptr}) {
+      if (row == nullptr || std::find(seen.begin(), seen.end(), row) != seen.end()) {
+        continue;
+      }
+      seen.push_back(row);
+      int degree = 0;
+      if (!graph_max_degree(row, nullptr, out[k].n, ws->degreeWord, stream, &degree)) {
+        return 0;
+      }
+      if (degree > HUB_DEGREE) {
+        hubs = 1;
+        break;
+      }
+    }
+  }
+  return 1;
+}
+
 }  // namespace detail
 }  // namespace mosp
 
@@ -429,6 +462,7 @@ extern "C" void mosp_ws_destroy(mosp_ws *ws) {
   (void)hipFree(ws->dCtl);
   (void)hipHostFree(ws->hCtl);
   (void)hipFree(ws->genDev);
+  (void)hipFree(ws->degreeWord);
   (void)hipFree(ws->cPartials);
   if (ws->ev0) (void)hipEventDestroy(ws->ev0);
   if (ws->ev1) (void)hipEventDestroy(ws->ev1);
@@ -544,6 +578,9 @@ extern "C" int mosp_sosp_update_multi(
   args.phases = PH_UPDATE;
   args.maxRounds = jump_rounds(n);
   args.anyFallback = anyFallback;
+  if (!caller_hubs(ws, out, in, K, as_stream(stream), args.hubs)) {
+    return 0;
+  }
   float ms = 0;
   if (!run_launch(ws, args, K, as_stream(stream), &ms)) {
     return 0;
@@ -603,6 +640,9 @@ extern "C" int mosp_sosp_from_scratch(const mosp_csr *out, int32_t source,
   args.source = source;
   args.phases = PH_SCRATCH;
   args.maxRounds = jump_rounds(n);
+  if (!caller_hubs(ws, out, nullptr, 1, as_stream(stream), args.hubs)) {
+    return 0;
+  }
   float ms = 0;
   if (!run_launch(ws, args, 1, as_stream(stream), &ms)) {
     return 0;
@@ -721,6 +761,9 @@ extern "C" int mosp_combined_sosp(const int32_t *d_parents, int32_t n,
   args.source = source;
   args.phases = PH_COMBINED;
   args.maxRounds = jump_rounds(n);
+  // The combined graph is built inside the launch: its degrees are unknown
+  // here, so the variant that handles any degree runs.
+  args.hubs = 1;
   float ms = 0;
   if (!run_launch(ws, args, 1, as_stream(stream), &ms)) {
     return 0;
@@ -916,6 +959,8 @@ extern "C" int mosp_update_fused(const mosp_device_graph *g, int32_t K,
   args.phases = PH_UPDATE | PH_COMBINED;
   args.maxRounds = jump_rounds(n);
   args.anyFallback = anyFallback;
+  // A combined-graph vertex has at most its out-degree in the graph.
+  args.hubs = g->maxDegree > HUB_DEGREE ? 1 : 0;
   float ms = 0;
   if (!run_launch(ws, args, K + 1, as_stream(stream), &ms)) {
     return 0;

@@ -30,6 +30,9 @@ constexpr int BLOCK = MOSP_BLOCK;    // threads per workgroup (wave64 multiple)
 #ifndef MOSP_MIN_BLOCKS
 #define MOSP_MIN_BLOCKS 1            // co-resident workgroups per CU the code is built for
 #endif
+// Graphs with a vertex of more than HUB_DEGREE out- or in-edges run the
+// kernel variant that hands such vertices to whole waves.
+constexpr int HUB_DEGREE = 32;
 
 // Phases of one launch.
 enum : int {
@@ -133,6 +136,7 @@ struct LaunchArgs {
   int maxRounds;           // pointer-jumping rounds: ceil(log2 n) + 1
   int anyFallback;         // some objective of objs uses distance-only words
   int laneBits;            // 2^laneBits lanes share a frontier vertex
+  int hubs;                // host only: launch the HUB_DEGREE-aware variant
   int *genDev;             // device-resident stamp generation (persists)
   Ctl *ctl;
   // Combined graph (PH_COMBINED): parents of cK trees, objective-major.

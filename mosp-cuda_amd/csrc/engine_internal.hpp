@@ -23,18 +23,28 @@ struct mosp_device_graph {
   int *inRow = nullptr, *inCol = nullptr, *inW = nullptr;
   char *scratch = nullptr;  // build / apply temporaries (grow-only)
   size_t scratchBytes = 0;
-  unsigned long long *stats = nullptr;  // weight statistics (2 * K words)
+  unsigned long long *stats = nullptr;  // weight statistics (2 * MOSP_MAX_OBJ words) + degree
+  int maxDegree = 0;       // largest out- or in-degree
 };
 
 namespace mosp {
 
 // Kernels and their launchers (sosp_kernel.hip).
-const void *sosp_kernel_symbol();
-const void *sosp_batched_symbol();
+// hubs: the variant for graphs with vertices of more than HUB_DEGREE edges.
+const void *sosp_kernel_symbol(bool hubs);
+const void *sosp_batched_symbol(bool hubs);
 hipError_t launch_ctl_reset(Ctl *ctl, hipStream_t stream);
 hipError_t launch_ctl_reset_many(Ctl *ctls, int count, hipStream_t stream);
 
 namespace detail {
+
+// Largest out- or in-degree (inRow may be null) of a CSR with n rows, via
+// one device word `scratch`; synchronous on `stream` (device_graph.hip).
+int graph_max_degree(const int *outRow, const int *inRow, int n, unsigned *scratch,
+                     hipStream_t stream, int *result);
+
+// MOSP_HUBS=0|1 forces the kernel variant (tests, tuning); -1 when unset.
+int hubs_override();
 
 // Records the message for mosp_last_error() and returns 0.
 int fail(const std::string &message);

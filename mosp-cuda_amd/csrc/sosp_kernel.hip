@@ -186,6 +186,44 @@ __device__ __forceinline__ unsigned wave_reserve(unsigned *counter) {
   return base + rank;
 }
 
+// Lanes that own a vertex with more edges than this hand it to their whole
+// wave: on power-law graphs a hub's tens of thousands of edges would
+// otherwise be one lane's (or one lane group's) serial chain of dependent
+// loads and atomics, which every other workgroup waits for at the next grid
+// barrier.
+constexpr int HEAVY_GROUP = 64;   // lane groups (push): more than one wave pass
+constexpr unsigned HEAVY_LIST = 128;  // per-wave LDS list of heavy push vertices
+constexpr int HEAVY_THREAD = HUB_DEGREE;  // one lane per vertex (pull, parent recovery)
+
+__device__ __forceinline__ u64 shfl_u64(u64 x, int src) {
+  const unsigned lo = __shfl(static_cast<unsigned>(x), src);
+  const unsigned hi = __shfl(static_cast<unsigned>(x >> 32), src);
+  return (static_cast<u64>(hi) << 32) | lo;
+}
+
+__device__ __forceinline__ u64 wave_min_u64(u64 x) {
+#pragma unroll
+  for (int offset = 32; offset > 0; offset >>= 1) {
+    const unsigned lo = __shfl_xor(static_cast<unsigned>(x), offset);
+    const unsigned hi = __shfl_xor(static_cast<unsigned>(x >> 32), offset);
+    const u64 y = (static_cast<u64>(hi) << 32) | lo;
+    x = y < x ? y : x;
+  }
+  return x;
+}
+
+// Calls f(src) for every lane src of the wave with `mine` set, in lane order;
+// wave-uniform control flow.
+template <typename F>
+__device__ __forceinline__ void wave_each(bool mine, F &&f) {
+  u64 bits = __ballot(mine);
+  while (bits != 0) {
+    const int src = __ffsll(static_cast<long long>(bits)) - 1;
+    bits &= bits - 1;
+    f(src);
+  }
+}
+
 // List appends staged in LDS.  The buffer is split into equal slices, one
 // per team of waves working on one objective (slice 0 = the whole buffer
 // when the whole workgroup is one team); lanes reserve slots in their slice
@@ -332,6 +370,7 @@ struct Shared {
   u64 statPush[MOSP_MAX_OBJ];
   unsigned red[BLOCK / 64];
   int scan[BLOCK];
+  int heavy[WAVES][HEAVY_LIST];  // push: per-wave list of heavy vertices
   BlockQueue near;
   BlockQueue far;
 };
@@ -402,6 +441,7 @@ struct Team {
 
 // Relax the out-edges of the frontier vertices of objective k: groups of
 // 2^laneBits lanes share a vertex and stride over its edges.
+template <bool HUBS>
 __device__ void push_team(const ObjDesc &D, ObjCtl &c, Shared &sh, int k,
                           unsigned it, int source, int gen, unsigned blockRank,
                           unsigned blocks, int laneBits, const Team &tm) {
@@ -427,55 +467,129 @@ __device__ void push_team(const ObjDesc &D, ObjCtl &c, Shared &sh, int k,
   const unsigned lanes = 1u << laneBits;
   const unsigned sub = tm.tid & (lanes - 1);
   const unsigned perPass = tm.size >> laneBits;
+  const int lane = static_cast<int>(__lane_id());
+  auto relax = [&](int u, u64 du, int x) {
+    const int w = outCol[x];
+    if (w == source) {
+      return;  // the source is never relaxed
+    }
+    const u64 nd = du + static_cast<u64>(outW[x]);
+    if (nd > maxDist) {
+      return;  // never shortest; would not fit the packed word
+    }
+    const u64 cand = pack(nd, u, bits, noParent);
+    const u64 prev = atomicMin(&packed[w], cand);
+    if (nd < dist_of(prev, bits)) {
+      // Claims without atomics: two workgroups may both append w (a
+      // duplicate entry only repeats work), but a vertex improved in this
+      // iteration is never missed: the stamp only ever holds this
+      // iteration's generation if w was already appended in it, and inFar is
+      // only set during push iterations of this epoch.
+      if (nd < thr) {
+        if (stampArr[w] != gen) {
+          stampArr[w] = gen;
+          slice_push(sh.near, tm.slice, tm.cap, w, outNear, nextCnt);
+        }
+      } else if (inFar[w] == 0) {
+        inFar[w] = 1;
+        slice_push(sh.far, tm.slice, tm.cap, w, farBuf, farCnt);
+        atomicMin(&sh.sliceFarMin[tm.slice], nd);
+      }
+    }
+  };
   // Contiguous, even shares: a short frontier still spreads over every
   // workgroup of the objective (random loads cost per-CU address cycles).
   const unsigned lo = static_cast<unsigned>(static_cast<u64>(cnt) * blockRank / blocks);
   const unsigned hi = static_cast<unsigned>(static_cast<u64>(cnt) * (blockRank + 1) / blocks);
-  for (unsigned base = lo; base < hi; base += perPass) {
-    const unsigned i = base + (tm.tid >> laneBits);
-    if (i < hi) {
-      const int u = in[i];
-      // Independent loads, issued together.
-      const u64 word = packed[u];
-      const int begin = outRow[u];
-      const int end = outRow[u + 1];
-      if (word != PACKED_INF) {
-        const u64 du = dist_of(word, bits);
-        for (int x = begin + static_cast<int>(sub); x < end; x += lanes) {
-          const int w = outCol[x];
-          if (w == source) {
-            continue;  // the source is never relaxed
-          }
-          const u64 nd = du + static_cast<u64>(outW[x]);
-          if (nd > maxDist) {
-            continue;  // never shortest; would not fit the packed word
-          }
-          const u64 cand = pack(nd, u, bits, noParent);
-          const u64 prev = atomicMin(&packed[w], cand);
-          if (nd < dist_of(prev, bits)) {
-            // Claims without atomics: two workgroups may both append w (a
-            // duplicate entry only repeats work), but a vertex improved in
-            // this iteration is never missed: the stamp only ever holds this
-            // iteration's generation if w was already appended in it, and
-            // inFar is only set during push iterations of this epoch.
-            if (nd < thr) {
-              if (stampArr[w] != gen) {
-                stampArr[w] = gen;
-                slice_push(sh.near, tm.slice, tm.cap, w, outNear, nextCnt);
-              }
-            } else if (inFar[w] == 0) {
-              inFar[w] = 1;
-              slice_push(sh.far, tm.slice, tm.cap, w, farBuf, farCnt);
-              atomicMin(&sh.sliceFarMin[tm.slice], nd);
-            }
+  if constexpr (!HUBS) {
+    for (unsigned base = lo; base < hi; base += perPass) {
+      const unsigned i = base + (tm.tid >> laneBits);
+      if (i < hi) {
+        const int u = in[i];
+        // Independent loads, issued together.
+        const u64 word = packed[u];
+        const int begin = outRow[u];
+        const int end = outRow[u + 1];
+        if (word != PACKED_INF) {
+          const u64 du = dist_of(word, bits);
+          for (int x = begin + static_cast<int>(sub); x < end; x += static_cast<int>(lanes)) {
+            relax(u, du, x);
           }
         }
       }
+      if (tm.whole) {
+        __syncthreads();
+        if (sh.near.count[0] > QCAP / 2) queue_flush(sh.near, outNear, nextCnt);
+        if (sh.far.count[0] > QCAP / 2) queue_flush(sh.far, farBuf, farCnt);
+      }
     }
-    if (tm.whole) {
-      __syncthreads();
-      if (sh.near.count[0] > QCAP / 2) queue_flush(sh.near, outNear, nextCnt);
-      if (sh.far.count[0] > QCAP / 2) queue_flush(sh.far, farBuf, farCnt);
+  } else {
+    // Heavy vertices wait in this wave's LDS list; the whole wave relaxes
+    // them one by one when the list could overflow and after the last pass
+    // (their distance is re-read then: a newer one is as valid a source).  One
+    // relax site serves both: each lane walks (vertex, first edge, stride, end)
+    // -- its group's light vertex, then the list's vertices in turn.
+    int *const heavyList = sh.heavy[threadIdx.x / 64];
+    unsigned heavyCount = 0;  // wave-uniform
+    const unsigned long long below = (1ull << lane) - 1ull;
+    for (unsigned base = lo;; base += perPass) {
+      const bool more = base < hi;  // team-uniform
+      int cu = -1, cx = 0, cend = 0, cstep = static_cast<int>(lanes);
+      u64 cdu = 0;
+      if (more) {
+        const unsigned i = base + (tm.tid >> laneBits);
+        bool heavy = false;
+        if (i < hi) {
+          const int u = in[i];
+          // Independent loads, issued together.
+          const u64 word = packed[u];
+          const int begin = outRow[u];
+          const int end = outRow[u + 1];
+          if (word != PACKED_INF) {
+            heavy = end - begin > HEAVY_GROUP;
+            cu = u;
+            if (!heavy) {
+              cdu = dist_of(word, bits);
+              cx = begin + static_cast<int>(sub);
+              cend = end;
+            }
+          }
+        }
+        const bool lead = heavy && sub == 0;
+        const unsigned long long hv = __ballot(lead);
+        if (lead) {
+          heavyList[heavyCount + __popcll(hv & below)] = cu;
+        }
+        heavyCount += __popcll(hv);
+      }
+      const bool drainNow = !more || heavyCount > HEAVY_LIST - 64;
+      for (unsigned j = 0;;) {
+        for (; cx < cend; cx += cstep) {
+          relax(cu, cdu, cx);
+        }
+        if (!drainNow || j >= heavyCount) {
+          break;
+        }
+        __builtin_amdgcn_wave_barrier();
+        cu = heavyList[j++];
+        const u64 word = packed[cu];
+        cx = outRow[cu] + lane;
+        cend = word != PACKED_INF ? outRow[cu + 1] : 0;
+        cdu = dist_of(word, bits);
+        cstep = 64;
+      }
+      if (drainNow) {
+        heavyCount = 0;
+        __builtin_amdgcn_wave_barrier();
+      }
+      if (!more) {
+        break;
+      }
+      if (tm.whole) {
+        __syncthreads();
+        if (sh.near.count[0] > QCAP / 2) queue_flush(sh.near, outNear, nextCnt);
+        if (sh.far.count[0] > QCAP / 2) queue_flush(sh.far, farBuf, farCnt);
+      }
     }
   }
   if (tm.tid == 0) {
@@ -538,6 +652,7 @@ __device__ void split_team(const ObjDesc &D, ObjCtl &c, Shared &sh, int k,
   }
 }
 
+template <bool HUBS>
 __device__ int near_far(const ObjDesc *objs, int nobj, ObjCtl *ctls,
                         int source, int generation, u64 initThr,
                         const u64 *deltaOverride, int laneBits,
@@ -648,7 +763,7 @@ __device__ int near_far(const ObjDesc *objs, int nobj, ObjCtl *ctls,
       const unsigned rank = t.block - static_cast<unsigned>(sh.blockBegin[k]);
       const unsigned blocks = static_cast<unsigned>(sh.blockEnd[k] - sh.blockBegin[k]);
       if (sh.mode[k] == MODE_PUSH) {
-        push_team(objs[k], ctls[k], sh, k, it, source, gen, rank, blocks, laneBits, tm);
+        push_team<HUBS>(objs[k], ctls[k], sh, k, it, source, gen, rank, blocks, laneBits, tm);
       } else {
         split_team(objs[k], ctls[k], sh, k, it, gen, rank, blocks, tm);
       }
@@ -728,19 +843,54 @@ __device__ void unpack(const ObjDesc &D, int n, int source, Threads t) {
 }
 
 // Lowest-id parent over tight out-edges (src/sospUpdateGpu.cu:490-511).
+template <bool HUBS>
 __device__ void recover_parents(const ObjDesc &D, int n, int source,
                                 Threads t) {
-  for (int u = t.id; u < n; u += t.count) {
-    const u64 du = D.s.packed[u];
-    if (du == PACKED_INF) {
-      continue;
+  auto scan = [&](int u, u64 du, int x) {
+    const int w = D.outCol[x];
+    if (w != source && du + static_cast<u64>(D.outW[x]) == D.s.packed[w]) {
+      atomicMin(&D.parentOut[w], u);
     }
-    for (int x = D.outRow[u]; x < D.outRow[u + 1]; ++x) {
-      const int w = D.outCol[x];
-      if (w != source && du + static_cast<u64>(D.outW[x]) == D.s.packed[w]) {
-        atomicMin(&D.parentOut[w], u);
+  };
+  if constexpr (!HUBS) {
+    for (int u = t.id; u < n; u += t.count) {
+      const u64 du = D.s.packed[u];
+      if (du == PACKED_INF) {
+        continue;
+      }
+      for (int x = D.outRow[u]; x < D.outRow[u + 1]; ++x) {
+        scan(u, du, x);
       }
     }
+    return;
+  }
+  const int lane = static_cast<int>(__lane_id());
+  // Whole waves step together (t.id - lane is wave-aligned, t.count a
+  // multiple of 64), so heavy vertices can be handed to the wave.
+  for (int base = static_cast<int>(t.id) - lane; base < n; base += static_cast<int>(t.count)) {
+    const int u = base + lane;
+    int begin = 0, end = 0;
+    u64 du = PACKED_INF;
+    if (u < n) {
+      du = D.s.packed[u];
+      if (du != PACKED_INF) {
+        begin = D.outRow[u];
+        end = D.outRow[u + 1];
+      }
+    }
+    const bool heavy = end - begin > HEAVY_THREAD;
+    if (!heavy) {
+      for (int x = begin; x < end; ++x) {
+        scan(u, du, x);
+      }
+    }
+    wave_each(heavy, [&](int src) {
+      const int hu = __shfl(u, src), hb = __shfl(begin, src), he = __shfl(end, src);
+      const u64 hdu = shfl_u64(du, src);
+      for (int x = hb + lane; x < he; x += 64) {
+        scan(hu, hdu, x);
+      }
+    });
   }
 }
 
@@ -780,6 +930,7 @@ __device__ __forceinline__ bool jump(int *anc, int *flag, int v) {
   return next >= 0;
 }
 
+template <bool HUBS>
 __device__ int update_step1(const LaunchArgs &a, GridBarrier &grid, Threads t,
                             int generation, Shared &sh) {
   const int n = a.n;
@@ -912,30 +1063,59 @@ __device__ int update_step1(const LaunchArgs &a, GridBarrier &grid, Threads t,
     for (unsigned base = t.block * BLOCK; base < static_cast<unsigned>(n);
          base += t.count) {
       const int v = static_cast<int>(base + threadIdx.x);
-      if (v < n && v != a.source &&
-          ((D.nChanged > 0 && D.s.flag[v]) || D.s.stamp[v] == genInv)) {
-        const u64 current = D.s.packed[v];
-        u64 best = current;
-        for (int x = D.inRow[v]; x < D.inRow[v + 1]; ++x) {
-          const int u = D.inCol[x];
-          const u64 word = D.s.packed[u];
-          if (word == PACKED_INF) {
-            continue;
-          }
-          const u64 nd = dist_of(word, bits) + static_cast<u64>(D.inW[x]);
-          if (nd <= D.maxDistance) {  // larger: never shortest, may not fit
-            const u64 cand = pack(nd, u, bits, D.noParent);
+      const bool want = v < n && v != a.source &&
+                        ((D.nChanged > 0 && D.s.flag[v]) || D.s.stamp[v] == genInv);
+      u64 current = PACKED_INF;
+      int begin = 0, end = 0;
+      if (want) {
+        current = D.s.packed[v];
+        begin = D.inRow[v];
+        end = D.inRow[v + 1];
+      }
+      auto candidate = [&](int x) -> u64 {
+        const int u = D.inCol[x];
+        const u64 word = D.s.packed[u];
+        if (word == PACKED_INF) {
+          return PACKED_INF;
+        }
+        const u64 nd = dist_of(word, bits) + static_cast<u64>(D.inW[x]);
+        // Larger than maxDistance: never shortest, may not fit the word.
+        return nd <= D.maxDistance ? pack(nd, u, bits, D.noParent) : PACKED_INF;
+      };
+      u64 best = current;
+      if constexpr (!HUBS) {
+        for (int x = begin; x < end; ++x) {
+          const u64 cand = candidate(x);
+          best = cand < best ? cand : best;
+        }
+      } else {
+        const bool heavy = end - begin > HEAVY_THREAD;
+        if (!heavy) {
+          for (int x = begin; x < end; ++x) {
+            const u64 cand = candidate(x);
             best = cand < best ? cand : best;
           }
         }
-        if (best < current) {
-          const u64 prev = atomicMin(&D.s.packed[v], best);
-          const u64 bd = dist_of(best, bits);
-          if (bd < dist_of(prev, bits)) {
-            D.s.inFar[v] = 1;
-            queue_push(sh.far, v, D.s.far[0], &c.farCount[0]);
-            atomicMin(&sh.farMin, bd);
+        wave_each(heavy, [&](int src) {
+          const int hb = __shfl(begin, src), he = __shfl(end, src);
+          u64 part = PACKED_INF;
+          for (int x = hb + static_cast<int>(__lane_id()); x < he; x += 64) {
+            const u64 cand = candidate(x);
+            part = cand < part ? cand : part;
           }
+          part = wave_min_u64(part);
+          if (static_cast<int>(__lane_id()) == src) {
+            best = part < best ? part : best;
+          }
+        });
+      }
+      if (want && best < current) {
+        const u64 prev = atomicMin(&D.s.packed[v], best);
+        const u64 bd = dist_of(best, bits);
+        if (bd < dist_of(prev, bits)) {
+          D.s.inFar[v] = 1;
+          queue_push(sh.far, v, D.s.far[0], &c.farCount[0]);
+          atomicMin(&sh.farMin, bd);
         }
       }
       __syncthreads();
@@ -1107,6 +1287,7 @@ __device__ void combined_build(const LaunchArgs &a, GridBarrier &grid,
 
 // One update instance (graph, K objectives, combined step) on the group of
 // workgroups `t` describes; the group synchronizes through a.ctl.
+template <bool HUBS>
 __device__ void run_instance(const LaunchArgs &a, const Threads t, Shared &sh) {
   GridBarrier grid{a.ctl, 0, t.block, t.blocks};
   const int n = a.n;
@@ -1140,8 +1321,8 @@ __device__ void run_instance(const LaunchArgs &a, const Threads t, Shared &sh) {
   __syncthreads();
 
   if (a.phases & PH_UPDATE) {
-    generation = update_step1(a, grid, t, generation, sh);
-    generation = near_far(a.objs, a.nobj, a.ctl->obj, a.source, generation, 0,
+    generation = update_step1<HUBS>(a, grid, t, generation, sh);
+    generation = near_far<HUBS>(a.objs, a.nobj, a.ctl->obj, a.source, generation, 0,
                           nullptr, a.laneBits, grid, t, sh, 8);
     stamp(a.ctl, PHASE_NEARFAR, t.id);
     for (int k = 0; k < a.nobj; ++k) {
@@ -1151,7 +1332,7 @@ __device__ void run_instance(const LaunchArgs &a, const Threads t, Shared &sh) {
       grid.sync();
       for (int k = 0; k < a.nobj; ++k) {
         if (a.objs[k].parentBits == 0) {
-          recover_parents(a.objs[k], n, a.source, t);
+          recover_parents<HUBS>(a.objs[k], n, a.source, t);
         }
       }
     }
@@ -1165,12 +1346,12 @@ __device__ void run_instance(const LaunchArgs &a, const Threads t, Shared &sh) {
     const ObjDesc &D = a.objs[0];
     scratch_init(D, a.ctl->obj[0], n, a.source, t);
     grid.sync();
-    generation = near_far(a.objs, 1, a.ctl->obj, a.source, generation,
+    generation = near_far<HUBS>(a.objs, 1, a.ctl->obj, a.source, generation,
                           D.delta, nullptr, a.laneBits, grid, t, sh, 8);
     unpack(D, n, a.source, t);
     if (D.parentBits == 0) {
       grid.sync();
-      recover_parents(D, n, a.source, t);
+      recover_parents<HUBS>(D, n, a.source, t);
     }
   }
 
@@ -1188,13 +1369,13 @@ __device__ void run_instance(const LaunchArgs &a, const Threads t, Shared &sh) {
     grid.sync(sh.src, 2, sh.snap);
     stamp(a.ctl, PHASE_COMB_FILL, t.id);
     const u64 delta = static_cast<u64>(sh.snap[0]) | (static_cast<u64>(sh.snap[1]) << 32);
-    generation = near_far(a.cDesc, 1, &cc, a.source, generation, delta, &delta,
+    generation = near_far<HUBS>(a.cDesc, 1, &cc, a.source, generation, delta, &delta,
                           a.laneBits, grid, t, sh, 12);
     stamp(a.ctl, PHASE_COMB_SEARCH, t.id);
     unpack(C, n, a.source, t);
     if (C.parentBits == 0) {
       grid.sync();
-      recover_parents(C, n, a.source, t);
+      recover_parents<HUBS>(C, n, a.source, t);
     }
   }
 
@@ -1211,14 +1392,20 @@ __device__ void run_instance(const LaunchArgs &a, const Threads t, Shared &sh) {
   }
 }
 
+// HUBS: the variant with wave-cooperative handling of high-degree vertices
+// (chosen by the host when some vertex has more than HUB_DEGREE edges); the
+// other variant keeps the lighter code, and fewer registers, of graphs
+// without them.  Both compute the same results.
+template <bool HUBS>
 __global__ void __launch_bounds__(BLOCK, WAVES_PER_SIMD) sosp_persistent(LaunchArgs a) {
   __shared__ Shared sh;
-  run_instance(a, Threads{blockIdx.x * BLOCK + threadIdx.x, gridDim.x * BLOCK, blockIdx.x, gridDim.x}, sh);
+  run_instance<HUBS>(a, Threads{blockIdx.x * BLOCK + threadIdx.x, gridDim.x * BLOCK, blockIdx.x, gridDim.x}, sh);
 }
 
 // Many independent instances in one launch: workgroup b belongs to group
 // groupOf[b] with rank rankOf[b]; a group runs its instances one after the
 // other (each instance synchronizes its group through its own Ctl).
+template <bool HUBS>
 __global__ void __launch_bounds__(BLOCK, WAVES_PER_SIMD)
     sosp_batched(const LaunchArgs *insts, const BatchGroup *groups,
                  const int *groupOf, const int *rankOf) {
@@ -1242,7 +1429,7 @@ __global__ void __launch_bounds__(BLOCK, WAVES_PER_SIMD)
       }
     }
     __syncthreads();
-    run_instance(args, t, sh);
+    run_instance<HUBS>(args, t, sh);
   }
 }
 
@@ -1267,12 +1454,14 @@ __global__ void ctl_reset(Ctl *ctls) {
 }  // namespace
 
 // Host-visible handles of the kernels (used by engine.hip).
-const void *sosp_kernel_symbol() {
-  return reinterpret_cast<const void *>(&sosp_persistent);
+const void *sosp_kernel_symbol(bool hubs) {
+  return hubs ? reinterpret_cast<const void *>(&sosp_persistent<true>)
+              : reinterpret_cast<const void *>(&sosp_persistent<false>);
 }
 
-const void *sosp_batched_symbol() {
-  return reinterpret_cast<const void *>(&sosp_batched);
+const void *sosp_batched_symbol(bool hubs) {
+  return hubs ? reinterpret_cast<const void *>(&sosp_batched<true>)
+              : reinterpret_cast<const void *>(&sosp_batched<false>);
 }
 
 hipError_t launch_ctl_reset_many(Ctl *ctls, int count, hipStream_t stream) {

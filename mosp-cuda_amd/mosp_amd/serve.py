@@ -127,11 +127,16 @@ class Slot:
         self.delta_scale = delta_scale
         self.launch = launch
         self.kernel_ms = 0.0
+        # Wall seconds per phase: host preparation + graph application,
+        # the batched launch (reset, run, check), statistics after it.
+        self.phase_s = {"apply": 0.0, "launch": 0.0, "post": 0.0}
+        self.launches = 0
 
     def run(self, reqs: list[Request], graphs: list[ResidentGraph], keep_results: bool) -> None:
         keep = []
         specs = (BatchInstance * len(reqs))()
         offset = 0
+        t0 = time.perf_counter()
         for i, r in enumerate(reqs):
             rg = graphs[r.gid]
             cur, spare = rg.graphs[rg.cur], rg.graphs[1 - rg.cur]
@@ -171,11 +176,14 @@ class Slot:
                                      rg.source, dl, mw, pr, len(rg.pref), rg.dist[rg.cur].ptr,
                                      rg.par[rg.cur].ptr, rg.dist[nxt].ptr, rg.par[nxt].ptr,
                                      rg.cdist.ptr, rg.cpar.ptr)
+        t1 = time.perf_counter()
         _check(lib().mosp_batch_reset(self.handle, specs, len(reqs)), "mosp_batch_reset")
         ms = C.c_float(0)
         _check(lib().mosp_batch_run(self.handle, self.launch, C.byref(ms)), "mosp_batch_run")
         _check(lib().mosp_batch_check(self.handle), "mosp_batch_check")
         self.kernel_ms += ms.value
+        self.launches += 1
+        t2 = time.perf_counter()
         for r in reqs:
             rg = graphs[r.gid]
             rg.cur = 1 - rg.cur
@@ -187,6 +195,10 @@ class Slot:
             if keep_results:
                 r.result = rg.trees()
         del keep
+        t3 = time.perf_counter()
+        self.phase_s["apply"] += t1 - t0
+        self.phase_s["launch"] += t2 - t1
+        self.phase_s["post"] += t3 - t2
 
     def close(self) -> None:
         if self.handle:
@@ -205,7 +217,11 @@ class ServeStats:
     max_ms: float
     changes_per_s: float
     mean_coalesced: float
+    service_p50_ms: float = math.nan   # admission to completion
+    service_p99_ms: float = math.nan
     slot_kernel_ms: list = field(default_factory=list)
+    phase_s: dict = field(default_factory=dict)   # summed over slots
+    launches: int = 0
 
 
 class Server:
@@ -335,17 +351,24 @@ class Server:
                 time.sleep(wait)
             self.submit(r)
         self.drain(timeout)
-        return summarize(requests, [s.kernel_ms for s in self.slots])
+        return summarize(requests, self.slots)
 
 
-def summarize(requests: list[Request], slot_kernel_ms=()) -> ServeStats:
+def summarize(requests: list[Request], slots=()) -> ServeStats:
     lat = np.array([r.latency_ms for r in requests])
+    service = np.array([(r.done - r.admitted) * 1e3 for r in requests])
     wall = max(r.done for r in requests) - min(r.arrival for r in requests)
     changes = sum(r.changes for r in requests)
+    phases = {}
+    for s in slots:
+        for k, v in s.phase_s.items():
+            phases[k] = phases.get(k, 0.0) + v
     return ServeStats(len(requests), changes, wall, float(np.percentile(lat, 50)),
                       float(np.percentile(lat, 99)), float(lat.mean()), float(lat.max()),
                       changes / wall if wall > 0 else float("nan"),
-                      float(np.mean([r.coalesced for r in requests])), list(slot_kernel_ms))
+                      float(np.mean([r.coalesced for r in requests])),
+                      float(np.percentile(service, 50)), float(np.percentile(service, 99)),
+                      [s.kernel_ms for s in slots], phases, sum(s.launches for s in slots))
 
 
 def poisson_requests(graphs: list[ResidentGraph], count: int, rate: float, seed: int,

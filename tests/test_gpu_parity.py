@@ -248,6 +248,58 @@ def test_edge_cases():
     check_pipeline(g, b, tag="insert-into-source")
 
 
+# -------------------------------------------------- power-law (R-MAT) graphs --
+
+def rmat(scale, edge_factor, K, seed, wmax=100):
+    mg = M().gen_rmat(scale, edge_factor, K, seed=seed, wmax=wmax)
+    return O.Graph(mg.n, K, i32(mg.row), i32(mg.col), i32(mg.w))
+
+
+def max_degree(g: O.Graph) -> int:
+    return int(max(np.diff(g.row).max(), np.bincount(g.col, minlength=g.n).max()))
+
+
+@pytest.mark.parametrize("kind", gen.BATCH_KINDS)
+def test_rmat_hub_vertices(kind):
+    """Power-law graphs: vertices with hundreds to thousands of edges are
+    relaxed, pulled and scanned for parents by whole waves in the kernel
+    variant the engine picks for graphs with vertices of more than 32 edges;
+    results equal the oracle's for every batch kind."""
+    i = gen.BATCH_KINDS.index(kind)
+    g = rmat(12, 16, 3, seed=41 + i)
+    assert max_degree(g) > 32
+    b = gen.make_batch(kind, g, 800, 500 + i)
+    check_pipeline(g, b, pref=[1, 2, 3], tag=f"rmat/{kind}")
+
+
+def test_kernel_variants_agree(monkeypatch):
+    """Both kernel variants (forced with MOSP_HUBS) give identical outputs on
+    a hub graph and on a grid."""
+    for tag, g in (("rmat", rmat(11, 16, 2, seed=77)), ("grid", gen.grid_graph(50, 40, 2, 80,
+                                                                               0.2, 78))):
+        b = gen.uniform_batch(g, 600, 50, 79)
+        d0, p0 = gen.initial_trees(g, 2)
+        runs = []
+        for forced in ("0", "1"):
+            monkeypatch.setenv("MOSP_HUBS", forced)
+            say(f"variants {tag} MOSP_HUBS={forced}")
+            runs.append(M().update_host(mgraph(g), mbatch(b), d0, p0, pref=[1, 2], fused=True))
+        for key in ("dist", "parents", "comb_dist", "comb_parent"):
+            assert np.array_equal(runs[0][key], runs[1][key]), f"{tag}: {key}"
+
+
+def test_rmat_large_weights_fallback():
+    """R-MAT scale 17, weights up to 2^31 - 1: (n - 1) * maxW does not fit next
+    to an 18-bit parent id, so the distance-only words and the lowest-id
+    parent recovery run -- over hub vertices (the power-law stress workload
+    in miniature)."""
+    g = rmat(17, 16, 2, seed=91, wmax=2**31 - 1)
+    assert max_degree(g) > 32
+    b = gen.uniform_batch(g, 4000, 50, 92, wmax=2**31 - 1)
+    got, _ = check_pipeline(g, b, pref=[1, 2], tag="rmat-large-weights")
+    assert all(s["packed_parents"] == 0 for s in got["stats"])
+
+
 # ------------------------------------------------------ benchmark workloads --
 
 def test_config1_small_grid_against_cpu_path():
