@@ -95,10 +95,12 @@ def main() -> int:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--traffic", default=str(ROOT / "profiles" / "traffic_road1400_k3.json"),
                     help="measured HBM bytes per launch (rocprofv3 FETCH_SIZE/WRITE_SIZE)")
-    ap.add_argument("--mode", choices=("plain", "batched", "served"), default="plain",
-                    help="plain: the 1.96M-vertex update (the headline line); batched: 64 "
-                         "uneven instances in one launch vs 64 serial launches; served: "
-                         "resident graphs, Poisson requests, continuous batching")
+    ap.add_argument("--mode", choices=("plain", "rmat", "batched", "served"), default="plain",
+                    help="plain: the 1.96M-vertex update (the headline line); rmat: the "
+                         "power-law stress update (R-MAT scale 22, K=5, weights up to 2^24); "
+                         "batched: 64 uneven instances in one launch vs 64 serial launches; "
+                         "served: resident graphs, Poisson requests, continuous batching")
+    ap.add_argument("--scale", type=int, default=22, help="rmat: R-MAT scale")
     ap.add_argument("--requests", type=int, default=240)
     ap.add_argument("--rate", type=float, default=0.0,
                     help="served: Poisson requests per second for the latency pass (0: "
@@ -122,6 +124,8 @@ def main() -> int:
         return 2
     if args.mode == "batched":
         return bench_batched(args)
+    if args.mode == "rmat":
+        return bench_rmat(args)
     if args.mode == "served":
         return bench_served(args)
 
@@ -197,6 +201,76 @@ def main() -> int:
     if not args.no_cpu_baseline:
         log("cpu baseline: oracle CPU path on one core (about 20 s) ...")
         line["cpu_baseline"] = cpu_baseline(g, b, d0, p0, res)
+    sess.close()
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+def bench_rmat(args) -> int:
+    """Power-law stress: R-MAT scale 22 (a=0.57, b=c=0.19, edge factor 16,
+    self-loops and duplicates removed), K=5 weights U[1, 2^24] -- (n - 1) *
+    maxW does not fit next to a 23-bit parent id, so every objective runs the
+    distance-only words and the lowest-id parent recovery -- 100,000 changes
+    (50% insertions, deletions drawn from all arcs, so some disconnect
+    vertices), Pref=(1,2,3,4,5).  Same step and timing as the plain mode."""
+    import mosp_amd as M
+    from mosp_amd.roofline import alg_bytes
+
+    M.set_device(0)
+    K, pref = 5, (1, 2, 3, 4, 5)
+    t = time.perf_counter()
+    g = M.gen_rmat(args.scale, 16, K, seed=31, wmin=1, wmax=2**24)
+    b = M.gen_batch(g, 100_000, 50.0, seed=32, safe=False, wmax=2**24)
+    deg = np.diff(g.row)
+    log(f"workload: R-MAT scale {args.scale}: n={g.n} m={g.m} max out-degree {deg.max()} "
+        f"K={K} inserts={b.n_ins} deletes={b.n_del} ({time.perf_counter() - t:.1f} s)")
+    d0, p0 = M.initial_trees(g, K)
+    log(f"initial trees ({time.perf_counter() - t:.1f} s)")
+    sess = M.Session(g, b, d0, p0, pref=pref, grid=args.grid, delta_scale=args.delta_scale)
+    for _ in range(args.warmup):
+        sess.run()
+    M.synchronize()
+    kernel_ms, sosp_ms, comb_ms = [], [], []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        sess.run()
+        kernel_ms.append(sess.stats[0].kernel_ms)
+        sosp_ms.append(sess.cstats.sosp_phase_ms)
+        comb_ms.append(sess.cstats.combined_phase_ms)
+    M.synchronize()
+    ms_per_step = (time.perf_counter() - t0) * 1e3 / args.steps
+    res = sess.results()
+    changes = b.n_ins + b.n_del
+    log(f"{ms_per_step:.3f} ms/step; kernel {np.mean(kernel_ms):.3f} ms (K updates "
+        f"{np.mean(sosp_ms):.3f} ms, combined {np.mean(comb_ms):.3f} ms); stats {res['stats']}")
+    log(f"phase profile (last step, device clock): {sess.ws.last_profile()}")
+    changed = [M.changed_lists(b, sess.mask, k) for k in range(K)]
+    in_row = np.zeros(g.n + 1, np.int64)
+    np.add.at(in_row, sess.updated.col.astype(np.int64) + 1, 1)
+    in_row = np.cumsum(in_row)
+    ab = alg_bytes(g.n, sess.updated.row, in_row, 0, p0, d0, res["dist"], changed, b.ins_to,
+                   res["combined_edges"])
+    k_ms = float(np.mean(kernel_ms))
+    achieved = ab / (k_ms * 1e-3) / 1e9
+    line = {
+        "metric": "changes/s, power-law stress update (R-MAT scale 22, K=5, distance-only "
+                  "fallback)",
+        "value": changes / (ms_per_step / 1e3), "unit": "changes/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": ms_per_step, "gpu_compute_ms": ms_per_step, "kernel_ms": k_ms,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "int64", "data": "synthetic (seeded R-MAT + batch, GPU-computed canonical "
+                                  "initial trees)",
+        "config": {"workload": f"R-MAT scale {args.scale} (n={g.n}, m={g.m}), K=5 "
+                               f"w~U[1,2^24], {changes} changes (50% inserts, unsafe "
+                               "deletions), Pref=(1,2,3,4,5), source 0",
+                   "n": g.n, "m": g.m, "K": K, "batch": changes,
+                   "packed_parents": [s["packed_parents"] for s in res["stats"]],
+                   "max_out_degree": int(deg.max())},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                     "frac": achieved / PEAK_HBM_GBS, "traffic": None, "alg_bytes": ab,
+                     "kernel": "sosp_persistent (hub variant, one launch per update)"},
+    }
     sess.close()
     print(json.dumps(line), flush=True)
     return 0

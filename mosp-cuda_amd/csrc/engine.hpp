@@ -89,6 +89,7 @@ struct ObjCtl {
   unsigned iterations;
   unsigned epochs;
   unsigned jumpCount[3];   // pointer jumping: still-jumping list, slot = round % 3
+  unsigned pullHeavy;      // pull candidates left for whole workgroups (hub variant)
   u64 pushes;
 };
 

@@ -193,7 +193,9 @@ __device__ __forceinline__ unsigned wave_reserve(unsigned *counter) {
 // barrier.
 constexpr int HEAVY_GROUP = 64;   // lane groups (push): more than one wave pass
 constexpr unsigned HEAVY_LIST = 128;  // per-wave LDS list of heavy push vertices
-constexpr int HEAVY_THREAD = HUB_DEGREE;  // one lane per vertex (pull, parent recovery)
+constexpr int HEAVY_THREAD = HUB_DEGREE;  // one lane per vertex (parent recovery)
+constexpr int PULL_WAVE = 8;       // pull: more in-edges -> the candidate's wave
+constexpr int PULL_BLOCK = 1024;   // pull: more in-edges -> a whole workgroup, after the sweep
 
 __device__ __forceinline__ u64 shfl_u64(u64 x, int src) {
   const unsigned lo = __shfl(static_cast<unsigned>(x), src);
@@ -222,6 +224,20 @@ __device__ __forceinline__ void wave_each(bool mine, F &&f) {
     bits &= bits - 1;
     f(src);
   }
+}
+
+// Largest u in [lo, hi] with rowPtr[u] <= e: the row holding edge e when
+// rowPtr[lo] <= e < rowPtr[hi + 1].
+__device__ __forceinline__ int row_of(const int *rowPtr, int lo, int hi, long long e) {
+  while (lo < hi) {
+    const int mid = lo + (hi - lo + 1) / 2;
+    if (rowPtr[mid] <= e) {
+      lo = mid;
+    } else {
+      hi = mid - 1;
+    }
+  }
+  return lo;
 }
 
 // List appends staged in LDS.  The buffer is split into equal slices, one
@@ -333,6 +349,22 @@ __device__ unsigned block_sum(unsigned value, unsigned *red) {
   return total;
 }
 
+// Workgroup minimum of one value per thread (uses `red`, WAVES entries).
+__device__ u64 block_min_u64(u64 value, u64 *red) {
+  value = wave_min_u64(value);
+  __syncthreads();
+  if (__lane_id() == 0) {
+    red[threadIdx.x / 64] = value;
+  }
+  __syncthreads();
+  u64 best = PACKED_INF;
+  for (unsigned w = 0; w < WAVES; ++w) {
+    best = red[w] < best ? red[w] : best;
+  }
+  __syncthreads();
+  return best;
+}
+
 struct Threads {
   unsigned id;      // thread index within the group of workgroups
   unsigned count;   // threads in the group
@@ -371,6 +403,8 @@ struct Shared {
   unsigned red[BLOCK / 64];
   int scan[BLOCK];
   int heavy[WAVES][HEAVY_LIST];  // push: per-wave list of heavy vertices
+  int span[2];                   // edge-parallel passes: rows of the first/last edge
+  u64 redMin[WAVES];             // workgroup minimum
   BlockQueue near;
   BlockQueue far;
 };
@@ -845,7 +879,7 @@ __device__ void unpack(const ObjDesc &D, int n, int source, Threads t) {
 // Lowest-id parent over tight out-edges (src/sospUpdateGpu.cu:490-511).
 template <bool HUBS>
 __device__ void recover_parents(const ObjDesc &D, int n, int source,
-                                Threads t) {
+                                Threads t, Shared &sh) {
   auto scan = [&](int u, u64 du, int x) {
     const int w = D.outCol[x];
     if (w != source && du + static_cast<u64>(D.outW[x]) == D.s.packed[w]) {
@@ -864,33 +898,28 @@ __device__ void recover_parents(const ObjDesc &D, int n, int source,
     }
     return;
   }
-  const int lane = static_cast<int>(__lane_id());
-  // Whole waves step together (t.id - lane is wave-aligned, t.count a
-  // multiple of 64), so heavy vertices can be handed to the wave.
-  for (int base = static_cast<int>(t.id) - lane; base < n; base += static_cast<int>(t.count)) {
-    const int u = base + lane;
-    int begin = 0, end = 0;
-    u64 du = PACKED_INF;
-    if (u < n) {
-      du = D.s.packed[u];
+  // Edge-parallel: every workgroup takes an even, contiguous range of the m
+  // edges, BLOCK at a time; each lane finds its edge's row by binary search
+  // between the rows of the pass's first and last edges.  Balanced whatever
+  // the degrees (power-law generators put the hubs side by side).
+  const int m = D.outRow[n];
+  const long long lo = static_cast<long long>(m) * t.block / t.blocks;
+  const long long hi = static_cast<long long>(m) * (t.block + 1) / t.blocks;
+  for (long long base = lo; base < hi; base += BLOCK) {
+    const long long last = (base + BLOCK < hi ? base + BLOCK : hi) - 1;
+    __syncthreads();
+    if (threadIdx.x < 2) {
+      sh.span[threadIdx.x] = row_of(D.outRow, 0, n - 1, threadIdx.x == 0 ? base : last);
+    }
+    __syncthreads();
+    const long long e = base + threadIdx.x;
+    if (e <= last) {
+      const int u = row_of(D.outRow, sh.span[0], sh.span[1], e);
+      const u64 du = D.s.packed[u];
       if (du != PACKED_INF) {
-        begin = D.outRow[u];
-        end = D.outRow[u + 1];
+        scan(u, du, static_cast<int>(e));
       }
     }
-    const bool heavy = end - begin > HEAVY_THREAD;
-    if (!heavy) {
-      for (int x = begin; x < end; ++x) {
-        scan(u, du, x);
-      }
-    }
-    wave_each(heavy, [&](int src) {
-      const int hu = __shfl(u, src), hb = __shfl(begin, src), he = __shfl(end, src);
-      const u64 hdu = shfl_u64(du, src);
-      for (int x = hb + lane; x < he; x += 64) {
-        scan(hu, hdu, x);
-      }
-    });
   }
 }
 
@@ -1051,7 +1080,22 @@ __device__ int update_step1(const LaunchArgs &a, GridBarrier &grid, Threads t,
 
   // Pull: every invalidated vertex and insert head takes the best (distance,
   // parent) over its in-neighbours; improved ones start in the far pile of
-  // epoch 0 (the first near-far iteration splits it at min + delta).
+  // epoch 0 (the first near-far iteration splits it at min + delta).  In the
+  // hub variant, candidates with more than PULL_BLOCK in-edges are listed
+  // (near[0]) and pulled afterwards by one whole workgroup each, round robin
+  // over the grid: the heaviest ones would otherwise hold their wave -- and
+  // the whole grid at the next barrier -- for milliseconds.
+  auto pulled = [&](const ObjDesc &D, ObjCtl &c, int v, u64 best, u64 current) {
+    if (best < current) {
+      const u64 prev = atomicMin(&D.s.packed[v], best);
+      const u64 bd = dist_of(best, D.parentBits);
+      if (bd < dist_of(prev, D.parentBits)) {
+        D.s.inFar[v] = 1;
+        queue_push(sh.far, v, D.s.far[0], &c.farCount[0]);
+        atomicMin(&sh.farMin, bd);
+      }
+    }
+  };
   for (int k = 0; k < a.nobj; ++k) {
     const ObjDesc &D = a.objs[k];
     ObjCtl &c = ctl->obj[k];
@@ -1060,11 +1104,21 @@ __device__ int update_step1(const LaunchArgs &a, GridBarrier &grid, Threads t,
       sh.farMin = PACKED_INF;
     }
     __syncthreads();
+    auto candidate = [&](int x) -> u64 {
+      const int u = D.inCol[x];
+      const u64 word = D.s.packed[u];
+      if (word == PACKED_INF) {
+        return PACKED_INF;
+      }
+      const u64 nd = dist_of(word, bits) + static_cast<u64>(D.inW[x]);
+      // Larger than maxDistance: never shortest, may not fit the word.
+      return nd <= D.maxDistance ? pack(nd, u, bits, D.noParent) : PACKED_INF;
+    };
     for (unsigned base = t.block * BLOCK; base < static_cast<unsigned>(n);
          base += t.count) {
       const int v = static_cast<int>(base + threadIdx.x);
-      const bool want = v < n && v != a.source &&
-                        ((D.nChanged > 0 && D.s.flag[v]) || D.s.stamp[v] == genInv);
+      bool want = v < n && v != a.source &&
+                  ((D.nChanged > 0 && D.s.flag[v]) || D.s.stamp[v] == genInv);
       u64 current = PACKED_INF;
       int begin = 0, end = 0;
       if (want) {
@@ -1072,16 +1126,6 @@ __device__ int update_step1(const LaunchArgs &a, GridBarrier &grid, Threads t,
         begin = D.inRow[v];
         end = D.inRow[v + 1];
       }
-      auto candidate = [&](int x) -> u64 {
-        const int u = D.inCol[x];
-        const u64 word = D.s.packed[u];
-        if (word == PACKED_INF) {
-          return PACKED_INF;
-        }
-        const u64 nd = dist_of(word, bits) + static_cast<u64>(D.inW[x]);
-        // Larger than maxDistance: never shortest, may not fit the word.
-        return nd <= D.maxDistance ? pack(nd, u, bits, D.noParent) : PACKED_INF;
-      };
       u64 best = current;
       if constexpr (!HUBS) {
         for (int x = begin; x < end; ++x) {
@@ -1089,7 +1133,12 @@ __device__ int update_step1(const LaunchArgs &a, GridBarrier &grid, Threads t,
           best = cand < best ? cand : best;
         }
       } else {
-        const bool heavy = end - begin > HEAVY_THREAD;
+        if (want && end - begin > PULL_BLOCK) {
+          queue_push(sh.near, v, D.s.near[0], &c.pullHeavy);
+          want = false;
+          end = begin;
+        }
+        const bool heavy = end - begin > PULL_WAVE;
         if (!heavy) {
           for (int x = begin; x < end; ++x) {
             const u64 cand = candidate(x);
@@ -1109,21 +1158,61 @@ __device__ int update_step1(const LaunchArgs &a, GridBarrier &grid, Threads t,
           }
         });
       }
-      if (want && best < current) {
-        const u64 prev = atomicMin(&D.s.packed[v], best);
-        const u64 bd = dist_of(best, bits);
-        if (bd < dist_of(prev, bits)) {
-          D.s.inFar[v] = 1;
-          queue_push(sh.far, v, D.s.far[0], &c.farCount[0]);
-          atomicMin(&sh.farMin, bd);
-        }
+      if (want) {
+        pulled(D, c, v, best, current);
       }
       __syncthreads();
       if (sh.far.count[0] > QCAP - BLOCK) queue_flush(sh.far, D.s.far[0], &c.farCount[0]);
+      if constexpr (HUBS) {
+        if (sh.near.count[0] > QCAP - BLOCK) queue_flush(sh.near, D.s.near[0], &c.pullHeavy);
+      }
     }
     queue_flush(sh.far, D.s.far[0], &c.farCount[0]);
+    if constexpr (HUBS) {
+      queue_flush(sh.near, D.s.near[0], &c.pullHeavy);
+    }
     if (threadIdx.x == 0 && sh.farMin != PACKED_INF) {
       atomicMin(&c.farMin[0], sh.farMin);
+    }
+  }
+  if constexpr (HUBS) {
+    if (threadIdx.x < static_cast<unsigned>(a.nobj)) {
+      sh.src[threadIdx.x] = &ctl->obj[threadIdx.x].pullHeavy;
+    }
+    grid.sync(sh.src, a.nobj, sh.snap);
+    for (int k = 0; k < a.nobj; ++k) {
+      const ObjDesc &D = a.objs[k];
+      ObjCtl &c = ctl->obj[k];
+      const int bits = D.parentBits;
+      if (threadIdx.x == 0) {
+        sh.farMin = PACKED_INF;
+      }
+      __syncthreads();
+      const unsigned heavyCount = sh.snap[k];
+      for (unsigned i = t.block; i < heavyCount; i += t.blocks) {
+        const int v = D.s.near[0][i];
+        const u64 current = D.s.packed[v];
+        u64 part = PACKED_INF;
+        for (int x = D.inRow[v] + static_cast<int>(threadIdx.x); x < D.inRow[v + 1]; x += BLOCK) {
+          const int u = D.inCol[x];
+          const u64 word = D.s.packed[u];
+          if (word != PACKED_INF) {
+            const u64 nd = dist_of(word, bits) + static_cast<u64>(D.inW[x]);
+            if (nd <= D.maxDistance) {
+              const u64 cand = pack(nd, u, bits, D.noParent);
+              part = cand < part ? cand : part;
+            }
+          }
+        }
+        const u64 best = block_min_u64(part, sh.redMin);
+        if (threadIdx.x == 0) {
+          pulled(D, c, v, best < current ? best : current, current);
+        }
+      }
+      queue_flush(sh.far, D.s.far[0], &c.farCount[0]);
+      if (threadIdx.x == 0 && sh.farMin != PACKED_INF) {
+        atomicMin(&c.farMin[0], sh.farMin);
+      }
     }
   }
   grid.sync();
@@ -1332,7 +1421,7 @@ __device__ void run_instance(const LaunchArgs &a, const Threads t, Shared &sh) {
       grid.sync();
       for (int k = 0; k < a.nobj; ++k) {
         if (a.objs[k].parentBits == 0) {
-          recover_parents<HUBS>(a.objs[k], n, a.source, t);
+          recover_parents<HUBS>(a.objs[k], n, a.source, t, sh);
         }
       }
     }
@@ -1351,7 +1440,7 @@ __device__ void run_instance(const LaunchArgs &a, const Threads t, Shared &sh) {
     unpack(D, n, a.source, t);
     if (D.parentBits == 0) {
       grid.sync();
-      recover_parents<HUBS>(D, n, a.source, t);
+      recover_parents<HUBS>(D, n, a.source, t, sh);
     }
   }
 
@@ -1375,7 +1464,7 @@ __device__ void run_instance(const LaunchArgs &a, const Threads t, Shared &sh) {
     unpack(C, n, a.source, t);
     if (C.parentBits == 0) {
       grid.sync();
-      recover_parents<HUBS>(C, n, a.source, t);
+      recover_parents<HUBS>(C, n, a.source, t, sh);
     }
   }
 

@@ -302,7 +302,7 @@ def test_rmat_large_weights_fallback():
 
 # ------------------------------------------------------ benchmark workloads --
 
-def test_config1_small_grid_against_cpu_path():
+def test_small_grid_30x30_against_cpu_path():
     """30x30 grid, K=3, 100 changes (50% inserts), Pref (1,1,1)."""
     mg = M().gen_road_grid(30, 30, 3, seed=1, drop=0.0, diag=0.0)
     assert mg.n == 900 and mg.m == 3480
@@ -310,11 +310,11 @@ def test_config1_small_grid_against_cpu_path():
     assert mb.n_ins == 50 and mb.n_del == 50
     g = O.Graph(mg.n, 3, mg.row, mg.col, mg.w)
     b = Batch(3, mb.ins_from, mb.ins_to, mb.ins_w, mb.del_from, mb.del_to)
-    check_pipeline(g, b, pref=[1, 1, 1], tag="config1")
-    check_pipeline(g, b, pref=[1, 1, 1], fused=False, tag="config1-serial")
+    check_pipeline(g, b, pref=[1, 1, 1], tag="grid30x30")
+    check_pipeline(g, b, pref=[1, 1, 1], fused=False, tag="grid30x30-serial")
 
 
-def test_config2_road_grid_full_size():
+def test_road_grid_1400x1400_full_size():
     """1400x1400 road-like grid (1.96M vertices), K=3, 50k safe changes:
     every tree equals canonical Dijkstra on the updated graph, the MOSP tree
     equals Dijkstra on the host combined graph, and the GPU initial trees
@@ -322,7 +322,7 @@ def test_config2_road_grid_full_size():
     mg = M().gen_road_grid(1400, 1400, 3, seed=7)
     mb = M().gen_batch(mg, 50_000, 50.0, seed=8, safe=True)
     g = O.Graph(mg.n, 3, mg.row, mg.col, mg.w)
-    say(f"config2 n={g.n} m={g.m} ins={mb.n_ins} del={mb.n_del}")
+    say(f"road1400 n={g.n} m={g.m} ins={mb.n_ins} del={mb.n_del}")
     d0, p0 = M().initial_trees(mg, 3)
     for k in range(3):
         rd, rp = O.dijkstra(g, k, 0)
@@ -336,7 +336,41 @@ def test_config2_road_grid_full_size():
     comb = O.combined_graph(r["parents"], g.n, 3, 0, [])
     cd, cp = O.dijkstra(comb, 0, 0)
     assert np.array_equal(r["comb_dist"], cd) and np.array_equal(r["comb_parent"], cp)
-    say(f"config2 gpu_compute_ms={r['timings_ms']['gpu_compute']:.3f} stats={r['stats'][0]}")
+    say(f"road1400 gpu_compute_ms={r['timings_ms']['gpu_compute']:.3f} stats={r['stats'][0]}")
+
+
+def test_rmat_scale22_k5_full_size():
+    """R-MAT scale 22 (4.19M vertices, ~60M arcs after removing self-loops
+    and duplicates), K=5 weights U[1, 2^24] -- the distance-only words and
+    the lowest-id parent recovery on every objective -- 100,000 changes with
+    deletions drawn from all arcs (some disconnect vertices), Pref (1..5):
+    every tree equals canonical Dijkstra on the updated graph, the MOSP tree
+    equals Dijkstra on the host combined graph, and the GPU initial tree of
+    objective 0 equals Dijkstra on the original graph."""
+    K, pref = 5, [1, 2, 3, 4, 5]
+    mg = M().gen_rmat(22, 16, K, seed=31, wmax=2**24)
+    mb = M().gen_batch(mg, 100_000, 50.0, seed=32, safe=False, wmax=2**24)
+    g = O.Graph(mg.n, K, mg.row, mg.col, mg.w)
+    say(f"rmat22 n={g.n} m={g.m} ins={mb.n_ins} del={mb.n_del}")
+    d0, p0 = M().initial_trees(mg, K)
+    rd, rp = O.dijkstra(g, 0, 0)
+    assert np.array_equal(d0[0], rd) and np.array_equal(p0[0], rp), "initial tree 0"
+    r = M().update_host(mg, mb, d0, p0, pref=pref, fused=True)
+    assert all(s["packed_parents"] == 0 for s in r["stats"])
+    del g, mg
+    upd = O.Graph(r["updated"].n, K, r["updated"].row, r["updated"].col, r["updated"].w)
+    unreachable = 0
+    for k in range(K):
+        say(f"rmat22 objective {k}")
+        rd, rp = O.dijkstra(upd, k, 0)
+        assert np.array_equal(r["dist"][k], rd), f"objective {k} distances"
+        assert np.array_equal(r["parents"][k], rp), f"objective {k} parents"
+        unreachable = max(unreachable, int((rd >= INF // 2).sum()))
+    assert unreachable > 0, "the batch disconnects some vertices"
+    comb = O.combined_graph(r["parents"], upd.n, K, 0, pref)
+    cd, cp = O.dijkstra(comb, 0, 0)
+    assert np.array_equal(r["comb_dist"], cd) and np.array_equal(r["comb_parent"], cp)
+    say(f"rmat22 gpu_compute_ms={r['timings_ms']['gpu_compute']:.3f} stats={r['stats'][0]}")
 
 
 def test_session_repeats_out_of_place():
