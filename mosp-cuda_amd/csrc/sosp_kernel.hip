@@ -512,19 +512,24 @@ __device__ void push_team(const ObjDesc &D, ObjCtl &c, Shared &sh, int k,
       return;  // never shortest; would not fit the packed word
     }
     const u64 cand = pack(nd, u, bits, noParent);
+    // The claim word (stamp for the near list, far mark for the far pile) is
+    // read before the atomic so both round trips overlap.  Claims need no
+    // atomics: two lanes may both append w (a duplicate entry only repeats
+    // work), but an improvement is never missed: the stamp only ever holds
+    // this iteration's generation once w is appended in it (the entry reads
+    // w's word when processed, after this iteration's atomics), and inFar is
+    // only set during push iterations of this epoch.  Reading the claim word
+    // early can only see an older value, i.e. add a duplicate.
+    const bool near = nd < thr;
+    const int claim = near ? stampArr[w] : inFar[w];
     const u64 prev = atomicMin(&packed[w], cand);
     if (nd < dist_of(prev, bits)) {
-      // Claims without atomics: two workgroups may both append w (a
-      // duplicate entry only repeats work), but a vertex improved in this
-      // iteration is never missed: the stamp only ever holds this
-      // iteration's generation if w was already appended in it, and inFar is
-      // only set during push iterations of this epoch.
-      if (nd < thr) {
-        if (stampArr[w] != gen) {
+      if (near) {
+        if (claim != gen) {
           stampArr[w] = gen;
           slice_push(sh.near, tm.slice, tm.cap, w, outNear, nextCnt);
         }
-      } else if (inFar[w] == 0) {
+      } else if (claim == 0) {
         inFar[w] = 1;
         slice_push(sh.far, tm.slice, tm.cap, w, farBuf, farCnt);
         atomicMin(&sh.sliceFarMin[tm.slice], nd);
