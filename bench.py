@@ -92,6 +92,8 @@ def main() -> int:
     ap.add_argument("--grid", type=int, default=0, help="workgroups of the persistent launch")
     ap.add_argument("--delta-scale", type=float, default=3.0,
                     help="multiply the default near-far width (tuning)")
+    ap.add_argument("--comb-delta-scale", type=int, default=10,
+                    help="multiply the combined search's default near-far width (tuning)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--traffic", default=str(ROOT / "profiles" / "traffic_road1400_k3.json"),
                     help="measured HBM bytes per launch (rocprofv3 FETCH_SIZE/WRITE_SIZE)")
@@ -110,6 +112,8 @@ def main() -> int:
     ap.add_argument("--serve-launch", type=int, default=2, choices=(0, 2),
                     help="served: 0 cooperative launches, 2 plain launches (slots overlap)")
     ap.add_argument("--slots", type=int, default=4)
+    ap.add_argument("--cu-policy", choices=("static", "greedy"), default="static",
+                    help="served: fixed CU share per slot, or launches take the idle CUs")
     ap.add_argument("--coalesce", type=int, default=4)
     ap.add_argument("--hbm-budget-gb", type=float, default=0.0,
                     help="served: HBM budget (0 = resident state + room for every slot)")
@@ -139,8 +143,9 @@ def main() -> int:
     log(f"workload: n={g.n} m={g.m} K=3 inserts={b.n_ins} deletes={b.n_del} "
         f"({time.perf_counter() - t:.1f} s to generate)")
     d0, p0 = M.initial_trees(g, 3)
-    sess = M.Session(g, b, d0, p0, grid=args.grid, delta_scale=args.delta_scale)
-    log(f"resident in HBM; persistent grid = {sess.ws.grid} workgroups x 256")
+    sess = M.Session(g, b, d0, p0, grid=args.grid, delta_scale=args.delta_scale,
+                     comb_delta_scale=args.comb_delta_scale)
+    log(f"resident in HBM; persistent grid = {sess.ws.grid} workgroups x 1024")
 
     for _ in range(args.warmup):
         sess.run()
@@ -193,7 +198,9 @@ def main() -> int:
                                f"(n={g.n}, m={g.m}), K=3 w~U[1,100], {changes} safe changes "
                                "(50% inserts), Pref=(1,1,1), source 0",
                    "n": g.n, "m": g.m, "K": 3, "batch": changes,
-                   "grid_workgroups": sess.ws.grid},
+                   "grid_workgroups": sess.ws.grid,
+                   "delta": [int(x) for x in sess.delta],
+                   "combined_delta": int(sess.cstats.delta)},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS,
                      "unit": "GB/s", "frac": achieved / PEAK_HBM_GBS, "traffic": traffic,
                      "alg_bytes": ab, "kernel": "sosp_persistent (one launch per update)"},
@@ -226,7 +233,8 @@ def bench_rmat(args) -> int:
         f"K={K} inserts={b.n_ins} deletes={b.n_del} ({time.perf_counter() - t:.1f} s)")
     d0, p0 = M.initial_trees(g, K)
     log(f"initial trees ({time.perf_counter() - t:.1f} s)")
-    sess = M.Session(g, b, d0, p0, pref=pref, grid=args.grid, delta_scale=args.delta_scale)
+    sess = M.Session(g, b, d0, p0, pref=pref, grid=args.grid, delta_scale=args.delta_scale,
+                     comb_delta_scale=args.comb_delta_scale)
     for _ in range(args.warmup):
         sess.run()
     M.synchronize()
@@ -416,7 +424,8 @@ def bench_served(args) -> int:
 
     def server():
         return S.Server(residents, slots=args.slots, max_coalesce=args.coalesce,
-                        hbm_budget=budget, delta_scale=args.delta_scale, launch=args.serve_launch)
+                        hbm_budget=budget, delta_scale=args.delta_scale, launch=args.serve_launch,
+                        cu_policy=args.cu_policy)
 
     srv = server()
     log(f"server: {args.slots} slots x {srv.slot_workgroups} workgroups, coalesce <= "
@@ -475,6 +484,7 @@ def bench_served(args) -> int:
                    "capacity_requests_per_s": capacity, "rate_per_s": rate,
                    "requests": stats.requests, "slots": args.slots,
                    "slot_workgroups": srv.slot_workgroups, "launch": args.serve_launch,
+                   "cu_policy": args.cu_policy,
                    "max_coalesce": args.coalesce, "hbm_budget_bytes": srv.hbm_budget,
                    "mean_coalesced": stats.mean_coalesced,
                    "poisson_changes_per_s": stats.changes_per_s},

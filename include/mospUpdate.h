@@ -20,6 +20,9 @@ struct MospOptions {
   // true: the K updates and Steps 2-3 run in ONE persistent launch (the
   // objectives advance together); false: K launches + 1, as the reference.
   bool fused = true;
+  // Step 3's near-far width = the default rule times this (>= 1).  Tuning
+  // only (no reference counterpart); outputs do not depend on it.
+  int combinedDeltaScale = 1;
 };
 
 struct MospTimings {

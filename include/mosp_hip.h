@@ -104,6 +104,12 @@ void mosp_ws_destroy(mosp_ws *ws);
 /* Co-resident workgroups of the persistent launch (0 = occupancy maximum). */
 int mosp_ws_set_grid(mosp_ws *ws, int32_t workgroups);
 int32_t mosp_ws_grid(const mosp_ws *ws);
+/* Near-far width of the combined search in the fused launch when none is
+ * given: the reference's rule (src/sospUpdateGpu.cu:607-616, evaluated on the
+ * device over the combined graph) times `scale` (>= 1; default 1).  A tuning
+ * knob with no reference counterpart: outputs do not depend on it;
+ * mosp_combine_stats.delta reports the width used. */
+int mosp_ws_set_combined_delta_scale(mosp_ws *ws, int32_t scale);
 
 /* Device-clock profile of the last launch on this workspace (no reference
  * counterpart; the reference's --timing stages are host-side): ms[i] = time

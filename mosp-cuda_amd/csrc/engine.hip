@@ -127,6 +127,7 @@ struct mosp_ws {
   int gridOverride = 0;
   int laneBits = 2;         // 4 lanes per frontier vertex
   unsigned *degreeWord = nullptr;  // max-degree reduction of caller CSRs
+  int combDeltaScale = 1;   // combined search: default width times this
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
 };
 
@@ -335,6 +336,7 @@ int run_launch(mosp_ws *ws, LaunchArgs &args, int descCount,
   args.genDev = ws->genDev;
   args.laneBits = ws->laneBits;
   args.cPartials = ws->cPartials;
+  args.cDeltaScale = ws->combDeltaScale;
   if (descCount > 0) {
     HIP_TRY(hipMemcpyAsync(ws->dDesc, ws->hDesc, sizeof(ObjDesc) * descCount,
                            hipMemcpyHostToDevice, stream));
@@ -479,6 +481,15 @@ extern "C" int mosp_ws_set_grid(mosp_ws *ws, int32_t workgroups) {
 }
 
 extern "C" int32_t mosp_ws_grid(const mosp_ws *ws) { return ws->gridBlocks; }
+
+extern "C" int mosp_ws_set_combined_delta_scale(mosp_ws *ws, int32_t scale) {
+  clear_error();
+  if (ws == nullptr || scale < 1) {
+    return fail("Error: the combined width scale must be >= 1.");
+  }
+  ws->combDeltaScale = scale;
+  return 1;
+}
 
 extern "C" int mosp_ws_last_profile(const mosp_ws *ws, double *ms, int32_t nPhases,
                                     int32_t *barriers) {

@@ -145,7 +145,8 @@ struct LaunchArgs {
   int cK;
   int cBase;               // L * (K + 1)
   int cTerms[MOSP_MAX_OBJ];// L / Pref_k
-  long long cDelta;        // > 0: use it; else the default rule on device
+  long long cDelta;        // > 0: use it; else the default rule on device ...
+  int cDeltaScale;         // ... times this factor (>= 1)
   int *cDegree;            // n + 1 (counts, then fill cursors)
   int *cRowPtr;            // n + 1 (written here, read through cDesc->outRow)
   int *cColInd;            // K * n

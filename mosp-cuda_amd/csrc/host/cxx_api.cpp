@@ -393,6 +393,10 @@ bool mospUpdate(const CsrGraph &original, ChangeBatch &batch,
       std::cerr << "Error: could not copy the MOSP inputs to the GPU.\n";
       return false;
     }
+    if (!mosp_ws_set_combined_delta_scale(ws.handle, std::max(1, options.combinedDeltaScale))) {
+      std::cerr << mosp_last_error() << "\n";
+      return false;
+    }
   }
   tm.upload = ms_since(t0);
 

@@ -1357,7 +1357,8 @@ __device__ void combined_build(const LaunchArgs &a, GridBarrier &grid,
     }
   }
   // Near-far width for Step 3: the caller's, or 32 * average weight /
-  // average out-degree (src/sospUpdateGpu.cu:607-616), in double as there.
+  // average out-degree (src/sospUpdateGpu.cu:607-616), in double as there,
+  // times the launch's scale (a tuning factor; outputs do not depend on it).
   if (t.id == 0) {
     long long delta = a.cDelta;
     if (delta <= 0) {
@@ -1368,7 +1369,7 @@ __device__ void combined_build(const LaunchArgs &a, GridBarrier &grid,
         const double avgW = static_cast<double>(ws) / static_cast<double>(m);
         const double avgDeg = static_cast<double>(m) / static_cast<double>(n);
         const long long d = static_cast<long long>(32.0 * avgW / avgDeg);
-        delta = d > 1 ? d : 1;
+        delta = (d > 1 ? d : 1) * (a.cDeltaScale > 1 ? a.cDeltaScale : 1);
       }
     }
     store_relaxed(&ctl->combDelta, static_cast<u64>(delta));

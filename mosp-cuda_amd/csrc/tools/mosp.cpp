@@ -43,6 +43,7 @@ struct Options {
   std::vector<int> pref;
   int K = 0, source = 0;
   long long delta = 0;
+  int combDeltaScale = 1;
   bool canonicalize = false, writeOutput = true, validate = false, quiet = false;
   bool serial = false;
 };
@@ -52,7 +53,7 @@ void usage() {
                "            [-k K] [--source s] [--pref p1,..,pK] [--delta D]\n"
                "            [--cache file] [--canonicalize] [--out dir]\n"
                "            [--no-output] [--validate] [--timing file.csv]\n"
-               "            [--quiet] [--serial]\n";
+               "            [--quiet] [--serial] [--comb-delta-scale S]\n";
 }
 
 bool parse(int argc, char **argv, Options &o) {
@@ -92,6 +93,9 @@ bool parse(int argc, char **argv, Options &o) {
     } else if (a == "--delta") {
       if (!integer(1, LLONG_MAX)) return false;
       o.delta = x;
+    } else if (a == "--comb-delta-scale") {
+      if (!integer(1, INT_MAX)) return false;
+      o.combDeltaScale = static_cast<int>(x);
     } else if (a == "--pref") {
       if (!next(v)) return false;
       std::istringstream list(v);
@@ -304,6 +308,7 @@ int main(int argc, char **argv) {
   options.preferences = opt.pref;
   options.delta = opt.delta;
   options.fused = !opt.serial;
+  options.combinedDeltaScale = opt.combDeltaScale;
   CsrGraph updated;
   MospResult result;
   if (!mospUpdate(original, batch, distances, parents, options, updated, result)) return 1;

@@ -70,7 +70,7 @@ class BatchInstance(C.Structure):
 
 EXPORTS = (
     "mosp_last_error", "mosp_ws_create", "mosp_ws_reserve", "mosp_ws_destroy",
-    "mosp_ws_set_grid", "mosp_ws_grid", "mosp_ws_last_profile", "mosp_default_delta", "mosp_sosp_update",
+    "mosp_ws_set_grid", "mosp_ws_grid", "mosp_ws_set_combined_delta_scale", "mosp_ws_last_profile", "mosp_default_delta", "mosp_sosp_update",
     "mosp_sosp_update_multi", "mosp_sosp_from_scratch", "mosp_combined_sosp",
     "mosp_update_fused", "mosp_device_graph_upload", "mosp_device_graph_free",
     "mosp_device_graph_view", "mosp_device_graph_objectives", "mosp_path_costs_gpu",
@@ -113,6 +113,7 @@ def lib() -> C.CDLL:
     L.mosp_ws_reserve.argtypes = [_VP, _I, _I]
     L.mosp_ws_destroy.argtypes = [_VP]
     L.mosp_ws_set_grid.argtypes = [_VP, _I]
+    L.mosp_ws_set_combined_delta_scale.argtypes = [_VP, _I]
     L.mosp_ws_grid.argtypes = [_VP]
     L.mosp_ws_grid.restype = _I
     L.mosp_ws_last_profile.argtypes = [_VP, _f64p, _I, C.POINTER(C.c_int32)]
@@ -446,6 +447,11 @@ class Workspace:
     def set_grid(self, workgroups: int) -> None:
         lib().mosp_ws_set_grid(self.handle, workgroups)
 
+    def set_combined_delta_scale(self, scale: int) -> None:
+        """Combined search width = the reference's rule x scale (>= 1)."""
+        _check(lib().mosp_ws_set_combined_delta_scale(self.handle, int(scale)),
+               "combined width scale")
+
     @property
     def grid(self) -> int:
         return lib().mosp_ws_grid(self.handle)
@@ -613,7 +619,8 @@ class Session:
 
     def __init__(self, original: HostGraph, batch: Batch, init_dist: np.ndarray,
                  init_parents: np.ndarray, K: int = 0, source: int = 0, pref=(),
-                 delta: int = 0, grid: int = 0, delta_scale: float = 1.0):
+                 delta: int = 0, grid: int = 0, delta_scale: float = 1.0,
+                 comb_delta_scale: int = 1):
         self.K = min(K or original.K, original.K)
         self.n, self.source, self.pref = original.n, source, _i32(pref)
         self.updated, self.mask = apply_batch(original, batch)
@@ -628,6 +635,8 @@ class Session:
         self.set_delta_scale(1.0 if delta > 0 else delta_scale)
         self.dg = DeviceGraph(self.updated)
         self.ws = Workspace(self.n, self.K, grid)
+        if comb_delta_scale > 1:
+            self.ws.set_combined_delta_scale(comb_delta_scale)
         self.heads = _dev(_nz(_i32(batch.ins_to)), np.int32)
         self._keep = []
         changes = (Changes * self.K)()

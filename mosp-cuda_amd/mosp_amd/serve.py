@@ -16,9 +16,15 @@ requests (a change batch against one graph) as they arrive:
   device arena for change lists.  A free slot takes the oldest waiting
   requests of idle graphs, as many as fit its scratch (the HBM budget is
   split between the resident state and the slots' scratch) and
-  `max_coalesce`, and runs them in ONE batched launch on its share of the
-  CUs.  Slots run concurrently, so requests that arrive while others are on
-  the GPU start as soon as a slot and the memory are free.
+  `max_coalesce`, and runs them in ONE batched launch.  Slots run
+  concurrently, so requests that arrive while others are on the GPU start as
+  soon as a slot, the memory and CUs are free.
+* **CUs** (`cu_policy`): "static" gives every slot a fixed 1/slots of the
+  CUs; "greedy" lets a launch take the idle CUs, split evenly over the
+  launches that could start at that moment (at least `min_workgroups`, else
+  the requests wait), so a lone request runs on the whole GPU.  Either way
+  the workgroups of concurrent launches sum to at most one per CU, which
+  keeps every persistent workgroup co-resident.
 * **Per request**: the batch is applied to the graph in HBM
   (`mosp_device_graph_apply`, into the spare graph), the changed lists are
   built from the returned weight-increase mask, the update (K objectives +
@@ -121,6 +127,7 @@ class Slot:
         if not self.handle:
             raise RuntimeError(f"slot {index}: {last_error()}")
         _check(lib().mosp_batch_configure(self.handle, workgroups, -1, 1.0, 1), "configure")
+        self.workgroups = workgroups
         self.stream = lib().mosp_batch_stream(self.handle)
         self.arena = DeviceBuffer(4 * max(arena_ints, 1))
         self.arena_ints = arena_ints
@@ -131,8 +138,13 @@ class Slot:
         # the batched launch (reset, run, check), statistics after it.
         self.phase_s = {"apply": 0.0, "launch": 0.0, "post": 0.0}
         self.launches = 0
+        self.busy = False
 
-    def run(self, reqs: list[Request], graphs: list[ResidentGraph], keep_results: bool) -> None:
+    def run(self, reqs: list[Request], graphs: list[ResidentGraph], keep_results: bool,
+            workgroups: int = 0) -> None:
+        if workgroups and workgroups != self.workgroups:
+            _check(lib().mosp_batch_configure(self.handle, workgroups, -1, 1.0, 1), "configure")
+            self.workgroups = workgroups
         keep = []
         specs = (BatchInstance * len(reqs))()
         offset = 0
@@ -229,10 +241,17 @@ class Server:
 
     def __init__(self, graphs: list[ResidentGraph], slots: int = 4, slot_workgroups: int = 0,
                  hbm_budget: int | None = None, max_coalesce: int = 4, max_batch: int = 100_000,
-                 delta_scale: float = 1.0, launch: int = 2, keep_results: bool = False):
+                 delta_scale: float = 1.0, launch: int = 2, keep_results: bool = False,
+                 cu_policy: str = "static", min_workgroups: int = 0):
+        if cu_policy not in ("static", "greedy"):
+            raise ValueError(f"unknown CU policy {cu_policy!r}")
         self.graphs = graphs
         cus = int(lib().mosp_device_cus()) or 256
+        self.cus = cus
+        self.cu_policy = cu_policy
         self.slot_workgroups = slot_workgroups or max(1, cus // slots)
+        self.min_workgroups = min_workgroups or max(1, cus // (2 * slots))
+        self.free_cus = cus
         foot = max(g.footprint() for g in graphs)
         resident = sum(g.resident_bytes() for g in graphs)
         if hbm_budget is None:
@@ -274,9 +293,19 @@ class Server:
             self.pending += 1
             self.cv.notify_all()
 
-    def _admit(self) -> list[Request]:
-        """Oldest waiting requests of idle graphs that fit one slot."""
+    def _admit(self) -> tuple[list[Request], int]:
+        """Oldest waiting requests of idle graphs that fit one slot, and the
+        workgroups their launch gets (0 = the slot's fixed share)."""
         heads = [g.queue[0] for g in self.graphs if g.queue and not g.busy]
+        if not heads:
+            return [], 0
+        workgroups = 0
+        if self.cu_policy == "greedy":
+            idle_slots = sum(1 for s in self.slots if not s.busy)
+            launches = max(1, min(idle_slots, -(-len(heads) // self.max_coalesce)))
+            workgroups = self.free_cus // launches
+            if workgroups < self.min_workgroups:
+                return [], 0
         heads.sort(key=lambda r: (r.arrival, r.rid))
         chosen, used = [], 0
         for r in heads:
@@ -289,24 +318,26 @@ class Server:
             g = self.graphs[r.gid]
             g.queue.popleft()
             g.busy = True
-        return chosen
+        self.free_cus -= workgroups
+        return chosen, workgroups
 
     def _worker(self, slot: Slot) -> None:
         while True:
             with self.cv:
                 while not self.stop:
-                    reqs = self._admit()
+                    reqs, workgroups = self._admit()
                     if reqs:
                         break
                     self.cv.wait()
                 else:
                     return
+                slot.busy = True
                 t = self.now()
                 for r in reqs:
                     r.admitted = t
                     r.slot = slot.index
             try:
-                slot.run(reqs, self.graphs, self.keep_results)
+                slot.run(reqs, self.graphs, self.keep_results, workgroups)
             except BaseException as exc:  # surfaced by drain()
                 with self.cv:
                     self.errors.append(exc)
@@ -319,6 +350,8 @@ class Server:
                     r.done = t
                     self.graphs[r.gid].busy = False
                 self.pending -= len(reqs)
+                self.free_cus += workgroups
+                slot.busy = False
                 self.cv.notify_all()
 
     def drain(self, timeout: float = 600.0) -> None:

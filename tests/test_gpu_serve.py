@@ -88,6 +88,13 @@ def test_mix_equals_alone_equals_oracle():
     say(f"mix: {stats}")
     assert max(r.coalesced for r in mix_reqs) >= 1
     mix.close()
+    # Greedy CU grants: launches take the idle CUs (a lone request gets them all).
+    greedy_res = residents(graphs)
+    greedy = S().Server(greedy_res, slots=3, max_coalesce=2, max_batch=400, keep_results=True,
+                        cu_policy="greedy")
+    greedy_reqs = fresh(reqs)
+    say(f"greedy: {greedy.play(greedy_reqs)}")
+    greedy.close()
     # Alone: one slot, one request per launch, submitted one after the other.
     alone_res = residents(graphs)
     alone = S().Server(alone_res, slots=1, slot_workgroups=64, max_coalesce=1, max_batch=400,
@@ -104,8 +111,9 @@ def test_mix_equals_alone_equals_oracle():
                 raise alone.errors[0]
     alone.drain()
     alone.close()
-    for a, b in zip(mix_reqs, alone_reqs):
+    for a, g, b in zip(mix_reqs, greedy_reqs, alone_reqs):
         same(a.result, b.result, f"request {a.rid} (graph {a.gid}) mix vs alone")
+        same(g.result, b.result, f"request {g.rid} (graph {g.gid}) greedy vs alone")
     # Oracle: replay every graph's requests in order on the host.
     for gid, (g, K, pref) in enumerate(graphs):
         og = O.Graph(g.n, g.K, g.row, g.col, g.w)
@@ -117,7 +125,7 @@ def test_mix_equals_alone_equals_oracle():
             assert want["ok"]
             same(r.result, want, f"request {r.rid} (graph {gid}) vs oracle")
             og, d, p = want["updated"], want["dist"], want["parents"]
-    for x in mix_res + alone_res:
+    for x in mix_res + greedy_res + alone_res:
         x.close()
 
 
