@@ -109,10 +109,8 @@ def main() -> int:
                          "--load times the capacity measured by the saturation pass)")
     ap.add_argument("--load", type=float, default=0.6,
                     help="served: offered load of the latency pass as a fraction of capacity")
-    ap.add_argument("--serve-launch", type=int, default=2, choices=(0, 2),
-                    help="served: 0 cooperative launches, 2 plain launches (slots overlap)")
     ap.add_argument("--slots", type=int, default=4)
-    ap.add_argument("--cu-policy", choices=("static", "greedy"), default="static",
+    ap.add_argument("--cu-policy", choices=("static", "greedy"), default="greedy",
                     help="served: fixed CU share per slot, or launches take the idle CUs")
     ap.add_argument("--coalesce", type=int, default=4)
     ap.add_argument("--hbm-budget-gb", type=float, default=0.0,
@@ -399,16 +397,22 @@ def bench_batched(args) -> int:
 
 def bench_served(args) -> int:
     """8 resident graphs (4 road-like 1400x1400 grids K=3, 4 R-MAT scale 20
-    K=4, weights U[1,100]); Poisson-arriving requests with batch sizes
-    log-uniform in [100, 100k] (50% insertions, 10% disconnecting deletions);
-    continuous batching over `slots` concurrent workers under an HBM budget.
-    Reports p50/p99 latency (arrival to completion) and changes/s."""
+    K=4, weights U[1,100]); requests with batch sizes log-uniform in [100,
+    100k] (50% insertions, 10% disconnecting deletions), served by the
+    library's native server (continuous batching over `slots` concurrent
+    launches under an HBM budget).  Two passes on the same resident graphs:
+    saturation (every request queued at t=0) gives changes/s; open-loop
+    Poisson arrivals at `--rate` (or `--load` x the measured capacity) give
+    p50/p99 latency (submission to completion)."""
     import mosp_amd as M
     from mosp_amd import serve as S
 
     M.set_device(0)
     t = time.perf_counter()
-    originals, residents = [], []
+    budget = int(args.hbm_budget_gb * 2**30) if args.hbm_budget_gb > 0 else None
+    srv = S.Server(slots=args.slots, max_coalesce=args.coalesce, max_batch=100_000,
+                   cu_policy=args.cu_policy, delta_scale=args.delta_scale, hbm_budget=budget)
+    originals = []
     for gid in range(8):
         if gid < 4:
             g = M.gen_road_grid(1400, 1400, 3, seed=100 + gid)
@@ -418,29 +422,19 @@ def bench_served(args) -> int:
             K, pref = 4, (1, 2, 3, 4)
         d, p = M.initial_trees(g, K)
         originals.append(g)
-        residents.append(S.ResidentGraph(gid, g, d, p, K=K, pref=pref))
+        srv.add_graph(g, d, p, K=K, pref=pref)
         log(f"graph {gid}: n={g.n} m={g.m} K={K} resident ({time.perf_counter() - t:.0f} s)")
-    budget = int(args.hbm_budget_gb * 2**30) if args.hbm_budget_gb > 0 else None
-
-    def server():
-        return S.Server(residents, slots=args.slots, max_coalesce=args.coalesce,
-                        hbm_budget=budget, delta_scale=args.delta_scale, launch=args.serve_launch,
-                        cu_policy=args.cu_policy)
-
-    srv = server()
-    log(f"server: {args.slots} slots x {srv.slot_workgroups} workgroups, coalesce <= "
-        f"{args.coalesce}, launch {args.serve_launch}, HBM budget {srv.hbm_budget / 2**30:.2f} "
-        f"GiB (resident {srv.resident / 2**30:.2f} GiB, {srv.slab_bytes / 2**30:.2f} GiB "
-        f"scratch per slot)")
+    srv.start()
+    st, _ = srv.counters()
+    log(f"server: {args.slots} slots, {st.cus} CUs, policy {args.cu_policy}, coalesce <= "
+        f"{args.coalesce}, HBM budget {st.hbm_budget / 2**30:.2f} GiB (resident "
+        f"{st.resident_bytes / 2**30:.2f} GiB, {st.slot_bytes / 2**30:.2f} GiB scratch per slot)")
     # Warm-up: one small request per graph (first use of spare graphs and
-    # scratch), outside the measured streams.
-    warm = []
-    for i in range(len(residents)):
-        b = S.poisson_requests([residents[i]], 1, 1.0, seed=600 + i, size_min=100,
-                               size_max=100, originals=[originals[i]])[0].batch
-        warm.append(S.Request(-1 - i, i, b))
+    # scratch), outside the measured passes.
+    warm = [S.Request(-1 - i, i, S.poisson_requests([originals[i]], 1, 1.0, seed=600 + i,
+                                                     size_min=100, size_max=100)[0].batch)
+            for i in range(len(originals))]
     srv.play(warm)
-    srv.close()
 
     def report(tag, st):
         ph = {k: round(v, 2) for k, v in st.phase_s.items()}
@@ -450,22 +444,17 @@ def bench_served(args) -> int:
             f"{st.launches} launches, mean coalesced {st.mean_coalesced:.2f}; slot kernel ms "
             f"{[round(x) for x in st.slot_kernel_ms]}; slot phase s {ph}")
 
-    # Saturation: every request queued at t=0 (same batches, same order per
-    # graph as the Poisson stream) -> sustainable changes/s.
-    sat_reqs = S.poisson_requests(residents, args.requests, 1.0, seed=11, originals=originals)
+    # Saturation: every request queued at t=0 -> sustainable changes/s.
+    sat_reqs = S.poisson_requests(originals, args.requests, 1.0, seed=11)
     for r in sat_reqs:
         r.arrival = 0.0
-    srv = server()
     sat = srv.play(sat_reqs, timeout=900)
-    srv.close()
     report("saturation", sat)
     capacity = sat.requests / sat.wall_s
     rate = args.rate if args.rate > 0 else args.load * capacity
     # Latency: open-loop Poisson arrivals at `rate` (fresh seeds: new batches).
-    reqs = S.poisson_requests(residents, args.requests, rate, seed=12, originals=originals)
-    srv = server()
+    reqs = S.poisson_requests(originals, args.requests, rate, seed=12)
     stats = srv.play(reqs, timeout=900)
-    srv.close()
     report(f"poisson {rate:.1f}/s", stats)
     line = {
         "metric": "changes/s and p50/p99 latency, served: 8 resident graphs, Poisson requests",
@@ -482,15 +471,13 @@ def bench_served(args) -> int:
                    "value_from": "saturation pass (all requests queued at t=0)",
                    "latency_from": f"open-loop Poisson pass at {rate:.2f} requests/s",
                    "capacity_requests_per_s": capacity, "rate_per_s": rate,
-                   "requests": stats.requests, "slots": args.slots,
-                   "slot_workgroups": srv.slot_workgroups, "launch": args.serve_launch,
-                   "cu_policy": args.cu_policy,
-                   "max_coalesce": args.coalesce, "hbm_budget_bytes": srv.hbm_budget,
+                   "requests": stats.requests, "slots": args.slots, "cu_policy": args.cu_policy,
+                   "max_coalesce": args.coalesce, "hbm_budget_bytes": st.hbm_budget,
+                   "resident_bytes": st.resident_bytes,
                    "mean_coalesced": stats.mean_coalesced,
                    "poisson_changes_per_s": stats.changes_per_s},
     }
-    for r in residents:
-        r.close()
+    srv.close()
     print(json.dumps(line), flush=True)
     return 0
 

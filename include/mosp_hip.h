@@ -253,6 +253,82 @@ int mosp_batch_stats(mosp_batch *b, int32_t i, mosp_sosp_stats *stats,
 int mosp_batch_schedule(const mosp_batch *b, int32_t *group_size,
                         int32_t *graph_mode);
 
+/* ------------------------------------------------------------ served ---- */
+/* Served updates: graphs stay resident in HBM and change batches arrive as
+ * requests.  No counterpart in the reference, which runs one mospUpdate per
+ * process (src/mosp.cu:236-412); each request computes exactly what
+ * mospUpdate computes on its graph's current state (apply + K updates +
+ * combined graph + its search; src/mospUpdate.cu:57-225), so a result never
+ * depends on the other requests.  Requests on one graph run in order; slots
+ * (worker threads, each with a reusable batched workspace) run launches of
+ * up to max_coalesce requests concurrently under an HBM budget. */
+typedef struct mosp_server_config {
+  int32_t slots;           /* concurrent launches (worker threads), >= 1 */
+  int32_t max_coalesce;    /* requests (of distinct graphs) per launch, >= 1 */
+  int32_t max_batch;       /* most changes per request (sizes the change arenas) */
+  int32_t cu_policy;       /* 0: each slot has CUs/slots workgroups; 1: a launch takes the idle CUs */
+  int32_t min_workgroups;  /* policy 1: smallest grant (0 = CUs / (2 slots)) */
+  int32_t keep_results;    /* != 0: snapshot each request's trees (tests) */
+  double delta_scale;      /* objective near-far widths = default rule x this (> 0) */
+  int64_t hbm_budget;      /* bytes for resident state + slot scratch (0 = room for
+                              max_coalesce updates per slot) */
+} mosp_server_config;
+
+typedef struct mosp_request_info {
+  int32_t graph;
+  int32_t slot;            /* slot that ran it (-1: not yet admitted) */
+  int32_t coalesced;       /* requests in its launch */
+  int32_t status;          /* 0 waiting or running, 1 done, -1 failed */
+  int64_t changes;
+  double arrival, admitted, done;  /* seconds on the server clock (-1: not yet) */
+} mosp_request_info;
+
+typedef struct mosp_server_counters {
+  int32_t slots, cus;
+  int64_t launches;
+  double kernel_ms;        /* summed over slots (HIP events on their streams) */
+  double apply_s, launch_s, post_s;  /* slot wall time: preparation + apply,
+                                        batched launch, statistics */
+  int64_t hbm_budget, resident_bytes, slot_bytes;
+} mosp_server_counters;
+
+typedef struct mosp_server mosp_server;
+
+mosp_server *mosp_server_create(const mosp_server_config *config);
+/* Make a graph resident (host CSR, edge-major weights, kg objectives) with
+ * its k current trees (k*n objective-major, host) and preferences (0 or k
+ * values); returns its id, or -1.  Before mosp_server_start. */
+int32_t mosp_server_add_graph(mosp_server *s, int32_t n, int32_t kg, const int32_t *row_ptr,
+                              const int32_t *col_ind, const int32_t *weights, int32_t k,
+                              const int64_t *dist, const int32_t *parents,
+                              const int32_t *pref, int32_t n_pref, int32_t source);
+/* Split the budget, create the slots, start the workers and the clock. */
+int mosp_server_start(mosp_server *s);
+/* Queue a change batch (host arrays, copied; ins_w n_ins*kg insertion-major)
+ * for graph `gid`; returns the request id, or -1. */
+int64_t mosp_server_submit(mosp_server *s, int32_t gid, int32_t n_ins, const int32_t *ins_from,
+                           const int32_t *ins_to, const int32_t *ins_w, int32_t n_del,
+                           const int32_t *del_from, const int32_t *del_to);
+int mosp_server_wait(mosp_server *s, int64_t request, double timeout_s);
+/* Wait until every submitted request is done (0 on timeout or failure). */
+int mosp_server_drain(mosp_server *s, double timeout_s);
+int mosp_server_request(mosp_server *s, int64_t request, mosp_request_info *info);
+/* The graph's K trees (k*n), combined distances and MOSP tree (n) right
+ * after the request (keep_results servers). */
+int mosp_server_request_trees(mosp_server *s, int64_t request, int64_t *dist, int32_t *parents,
+                              int64_t *comb_dist, int32_t *comb_parent);
+/* Forget a finished request's record. */
+int mosp_server_release(mosp_server *s, int64_t request);
+/* Totals since the start or the last reset; slot_kernel_ms (n_slots
+ * entries, may be NULL) receives each slot's device time. */
+int mosp_server_stats(mosp_server *s, mosp_server_counters *stats, double *slot_kernel_ms,
+                      int32_t n_slots);
+int mosp_server_reset_stats(mosp_server *s);
+/* Seconds since mosp_server_start. */
+double mosp_server_clock(const mosp_server *s);
+/* Stop the workers (after the running launches) and free everything. */
+void mosp_server_destroy(mosp_server *s);
+
 /* ------------------------------------------------------ device graph ----- */
 
 /* Copy a host CSR (edge-major weights w[e*K + k], the Values.txt layout) to

@@ -83,6 +83,10 @@ EXPORTS = (
     "mosp_device_graph_create", "mosp_device_graph_apply", "mosp_device_graph_edges",
     "mosp_device_graph_download", "mosp_device_graph_weight_stats", "mosp_batch_create_capacity",
     "mosp_batch_reset", "mosp_batch_instance_bytes", "mosp_batch_stream", "mosp_device_cus",
+    "mosp_server_create", "mosp_server_add_graph", "mosp_server_start", "mosp_server_submit",
+    "mosp_server_wait", "mosp_server_drain", "mosp_server_request", "mosp_server_request_trees",
+    "mosp_server_release", "mosp_server_stats", "mosp_server_reset_stats", "mosp_server_clock",
+    "mosp_server_destroy",
 )
 
 _LIB: C.CDLL | None = None
@@ -185,6 +189,24 @@ def lib() -> C.CDLL:
     L.mosp_batch_instance_bytes.argtypes = [_I, _I]
     L.mosp_batch_stream.restype = _VP
     L.mosp_batch_stream.argtypes = [_VP]
+    L.mosp_server_create.restype = _VP
+    L.mosp_server_create.argtypes = [_VP]
+    L.mosp_server_add_graph.restype = _I
+    L.mosp_server_add_graph.argtypes = [_VP, _I, _I, _i32p, _i32p, _i32p, _I, _i64p, _i32p,
+                                        _i32p, _I, _I]
+    L.mosp_server_start.argtypes = [_VP]
+    L.mosp_server_submit.restype = _L
+    L.mosp_server_submit.argtypes = [_VP, _I, _I, _i32p, _i32p, _i32p, _I, _i32p, _i32p]
+    L.mosp_server_wait.argtypes = [_VP, _L, C.c_double]
+    L.mosp_server_drain.argtypes = [_VP, C.c_double]
+    L.mosp_server_request.argtypes = [_VP, _L, _VP]
+    L.mosp_server_request_trees.argtypes = [_VP, _L, _i64p, _i32p, _i64p, _i32p]
+    L.mosp_server_release.argtypes = [_VP, _L]
+    L.mosp_server_stats.argtypes = [_VP, _VP, _VP, _I]
+    L.mosp_server_reset_stats.argtypes = [_VP]
+    L.mosp_server_clock.restype = C.c_double
+    L.mosp_server_clock.argtypes = [_VP]
+    L.mosp_server_destroy.argtypes = [_VP]
     _LIB = L
     return L
 

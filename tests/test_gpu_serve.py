@@ -49,22 +49,20 @@ def small_graphs():
     return out
 
 
-def residents(graphs):
-    res = []
-    for gid, (g, K, pref) in enumerate(graphs):
+def make_server(graphs, **kw):
+    """A started server holding `graphs` with their canonical initial trees."""
+    srv = S().Server(**kw)
+    for g, K, pref in graphs:
         og = O.Graph(g.n, g.K, g.row, g.col, g.w)
         d, p = gen.initial_trees(og, K)
-        res.append(S().ResidentGraph(gid, g, d, p, K=K, pref=pref))
-    return res
+        srv.add_graph(g, d, p, K=K, pref=pref)
+    srv.start()
+    return srv
 
 
 def requests(graphs, count=36, seed=5):
-    res = residents(graphs)  # only for sizes; freed below
-    reqs = S().poisson_requests(res, count, rate=400.0, seed=seed, size_min=10, size_max=300,
-                                originals=[g for g, _, _ in graphs])
-    for r in res:
-        r.close()
-    return reqs
+    return S().poisson_requests([g for g, _, _ in graphs], count, rate=400.0, seed=seed,
+                                size_min=10, size_max=300)
 
 
 def fresh(reqs):
@@ -79,37 +77,29 @@ def same(a, b, tag):
 def test_mix_equals_alone_equals_oracle():
     graphs = small_graphs()
     reqs = requests(graphs)
-    # In the mix: 3 slots, up to 3 requests per launch, arrivals at 400/s.
-    mix_res = residents(graphs)
-    mix = S().Server(mix_res, slots=3, slot_workgroups=16, max_coalesce=3, max_batch=400,
-                     keep_results=True)
+    # In the mix: 3 slots with fixed CU shares, up to 3 requests per launch,
+    # every request queued at once (so launches coalesce).
+    mix = make_server(graphs, slots=3, max_coalesce=3, max_batch=400, keep_results=True)
     mix_reqs = fresh(reqs)
-    stats = mix.play(mix_reqs)
-    say(f"mix: {stats}")
-    assert max(r.coalesced for r in mix_reqs) >= 1
+    for r in mix_reqs:
+        r.arrival = 0.0
+    say(f"mix: {mix.play(mix_reqs)}")
     mix.close()
-    # Greedy CU grants: launches take the idle CUs (a lone request gets them all).
-    greedy_res = residents(graphs)
-    greedy = S().Server(greedy_res, slots=3, max_coalesce=2, max_batch=400, keep_results=True,
-                        cu_policy="greedy")
+    assert max(r.coalesced for r in mix_reqs) > 1, "no launch was coalesced"
+    # Greedy CU grants, Poisson arrivals at 400/s.
+    # (Launches take the idle CUs; a lone request gets them all.)
+    greedy = make_server(graphs, slots=3, max_coalesce=2, max_batch=400, keep_results=True,
+                         cu_policy="greedy")
     greedy_reqs = fresh(reqs)
     say(f"greedy: {greedy.play(greedy_reqs)}")
     greedy.close()
-    # Alone: one slot, one request per launch, submitted one after the other.
-    alone_res = residents(graphs)
-    alone = S().Server(alone_res, slots=1, slot_workgroups=64, max_coalesce=1, max_batch=400,
-                       keep_results=True)
+    # Alone: one slot, one request per launch, each submitted after the last is done.
+    alone = make_server(graphs, slots=1, max_coalesce=1, max_batch=400, keep_results=True)
     alone_reqs = fresh(reqs)
-    alone.start()
     for r in alone_reqs:
-        r.arrival = alone.now()
         alone.submit(r)
-        while np.isnan(r.done):
-            with alone.cv:
-                alone.cv.wait(timeout=0.05)
-            if alone.errors:
-                raise alone.errors[0]
-    alone.drain()
+        alone.wait(r)
+        assert r.coalesced == 1
     alone.close()
     for a, g, b in zip(mix_reqs, greedy_reqs, alone_reqs):
         same(a.result, b.result, f"request {a.rid} (graph {a.gid}) mix vs alone")
@@ -125,26 +115,42 @@ def test_mix_equals_alone_equals_oracle():
             assert want["ok"]
             same(r.result, want, f"request {r.rid} (graph {gid}) vs oracle")
             og, d, p = want["updated"], want["dist"], want["parents"]
-    for x in mix_res + greedy_res + alone_res:
-        x.close()
 
 
 def test_budget_admission_limits_coalescing():
     graphs = small_graphs()[:3]
     reqs = requests(graphs, count=16, seed=7)
-    res = residents(graphs)
-    foot = max(r.footprint() for r in res)
-    resident = sum(r.resident_bytes() for r in res)
+    probe = make_server(graphs, slots=2, max_coalesce=4, max_batch=400)
+    resident = probe.counters()[0].resident_bytes
+    probe.close()
+    foot = max(S().footprint(g.n, K) for g, K, _ in graphs)
     # Room for exactly one update per slot, whatever max_coalesce says.
-    srv = S().Server(res, slots=2, max_coalesce=4, max_batch=400,
-                     hbm_budget=resident + 2 * foot + 1024)
+    srv = make_server(graphs, slots=2, max_coalesce=4, max_batch=400,
+                      hbm_budget=resident + 2 * foot + 1024)
     run = fresh(reqs)
-    stats = srv.play(run)
-    say(f"budgeted: {stats}")
+    say(f"budgeted: {srv.play(run)}")
     assert all(r.coalesced == 1 for r in run)
     assert all(not np.isnan(r.done) for r in run)
     srv.close()
+    small = S().Server(slots=2, hbm_budget=resident + foot // 2)
+    for g, K, pref in graphs:
+        og = O.Graph(g.n, g.K, g.row, g.col, g.w)
+        d, p = gen.initial_trees(og, K)
+        small.add_graph(g, d, p, K=K, pref=pref)
     with pytest.raises(RuntimeError):
-        S().Server(res, slots=2, hbm_budget=resident + foot // 2)
-    for r in res:
-        r.close()
+        small.start()
+    small.close()
+
+
+def test_request_errors_are_reported():
+    graphs = small_graphs()[:1]
+    srv = make_server(graphs, slots=1, max_batch=50)
+    big = requests(graphs, count=1, seed=9)[0]
+    z = np.zeros(0, np.int32)
+    with pytest.raises(RuntimeError):  # more changes than max_batch
+        srv.submit(S().Request(0, 0, M().Batch(3, np.zeros(60, np.int32),
+                                                 np.ones(60, np.int32),
+                                                 np.ones(180, np.int32), z, z)))
+    with pytest.raises(RuntimeError):  # unknown graph
+        srv.submit(S().Request(1, 5, big.batch))
+    srv.close()
