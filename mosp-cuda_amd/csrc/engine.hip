@@ -128,6 +128,8 @@ struct mosp_ws {
   int laneBits = 2;         // 4 lanes per frontier vertex
   unsigned *degreeWord = nullptr;  // max-degree reduction of caller CSRs
   int combDeltaScale = 1;   // combined search: default width times this
+  int hopsUpdate = 1;       // local relaxation rounds per near-far iteration
+  int hopsSearch = 2;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
 };
 
@@ -191,6 +193,7 @@ int ws_init(mosp_ws *ws) {
   if (const char *env = std::getenv("MOSP_LANE_BITS")) {
     ws->laneBits = std::max(0, std::min(5, std::atoi(env)));
   }
+  local_hops_override(ws->hopsUpdate, ws->hopsSearch);
   HIP_TRY(hipMalloc(&ws->dDesc, sizeof(ObjDesc) * (MOSP_MAX_OBJ + 1)));
   HIP_TRY(hipHostMalloc(&ws->hDesc, sizeof(ObjDesc) * (MOSP_MAX_OBJ + 1),
                         hipHostMallocDefault));
@@ -335,6 +338,8 @@ int run_launch(mosp_ws *ws, LaunchArgs &args, int descCount,
   args.ctl = ws->dCtl;
   args.genDev = ws->genDev;
   args.laneBits = ws->laneBits;
+  args.hopsUpdate = ws->hopsUpdate;
+  args.hopsSearch = ws->hopsSearch;
   args.cPartials = ws->cPartials;
   args.cDeltaScale = ws->combDeltaScale;
   if (descCount > 0) {
@@ -412,6 +417,15 @@ ObjDesc make_desc(const mosp_csr &out, const mosp_csr *in,
   d.delta = delta;
   d.s = st;
   return d;
+}
+
+void local_hops_override(int &update, int &search) {
+  const char *env = std::getenv("MOSP_LOCAL_HOPS");
+  int u = 0, c = 0;
+  if (env != nullptr && std::sscanf(env, "%d,%d", &u, &c) == 2) {
+    update = std::max(0, std::min(8, u));
+    search = std::max(0, std::min(8, c));
+  }
 }
 
 int hubs_override() {

@@ -137,6 +137,9 @@ struct LaunchArgs {
   int maxRounds;           // pointer-jumping rounds: ceil(log2 n) + 1
   int anyFallback;         // some objective of objs uses distance-only words
   int laneBits;            // 2^laneBits lanes share a frontier vertex
+  int hopsUpdate;          // local relaxation rounds per near-far iteration: K updates
+  int hopsSearch;          // ... from-scratch and combined searches (single-update
+                           // kernel without hub handling only)
   int hubs;                // host only: launch the HUB_DEGREE-aware variant
   int *genDev;             // device-resident stamp generation (persists)
   Ctl *ctl;

@@ -45,6 +45,9 @@ int graph_max_degree(const int *outRow, const int *inRow, int n, unsigned *scrat
 
 // MOSP_HUBS=0|1 forces the kernel variant (tests, tuning); -1 when unset.
 int hubs_override();
+// MOSP_LOCAL_HOPS=u,s overrides the local relaxation rounds per near-far
+// iteration of the K updates (u) and of the searches (s) (tuning).
+void local_hops_override(int &update, int &search);
 
 // Records the message for mosp_last_error() and returns 0.
 int fail(const std::string &message);

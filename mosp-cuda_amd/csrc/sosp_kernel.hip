@@ -475,10 +475,10 @@ struct Team {
 
 // Relax the out-edges of the frontier vertices of objective k: groups of
 // 2^laneBits lanes share a vertex and stride over its edges.
-template <bool HUBS>
+template <bool HUBS, bool LOCAL>
 __device__ void push_team(const ObjDesc &D, ObjCtl &c, Shared &sh, int k,
                           unsigned it, int source, int gen, unsigned blockRank,
-                          unsigned blocks, int laneBits, const Team &tm) {
+                          unsigned blocks, int laneBits, int localHops, const Team &tm) {
   const unsigned cnt = sh.cnt[k];
   const int e = sh.epoch[k];
   const int bits = D.parentBits;
@@ -631,6 +631,43 @@ __device__ void push_team(const ObjDesc &D, ObjCtl &c, Shared &sh, int k,
       }
     }
   }
+  // Local hops: relax, within this iteration, the near vertices this
+  // workgroup has just listed (still in its LDS queue), then those these
+  // list, localHops rounds.  They stay listed for the next iteration as well
+  // (relaxing a vertex again is always valid), so this only brings
+  // improvements forward: up to localHops + 1 hops advance per grid barrier,
+  // which halves the iterations.  No flush inside the rounds (the entries
+  // being read stay in place; a full queue spills to the global list).
+  // Heavy vertices are left to the next iteration's balanced handling.
+  // Compiled into the single-update kernel only: in the batched kernel the
+  // extra loop pushes register use past the budget and the spills cost more
+  // than the hops save.
+  if (LOCAL && tm.whole) {
+    unsigned from = 0;
+    for (int hop = 0; hop < localHops; ++hop) {
+      __syncthreads();
+      const unsigned to = sh.near.count[0] < QCAP ? sh.near.count[0] : QCAP;
+      if (from >= to) {
+        break;
+      }
+      for (unsigned base = from; base < to; base += perPass) {
+        const unsigned i = base + (tm.tid >> laneBits);
+        if (i < to) {
+          const int u = sh.near.buf[i];
+          const u64 word = packed[u];
+          const int begin = outRow[u];
+          const int end = outRow[u + 1];
+          if (word != PACKED_INF && (!HUBS || end - begin <= HEAVY_GROUP)) {
+            const u64 du = dist_of(word, bits);
+            for (int x = begin + static_cast<int>(sub); x < end; x += static_cast<int>(lanes)) {
+              relax(u, du, x);
+            }
+          }
+        }
+      }
+      from = to;
+    }
+  }
   if (tm.tid == 0) {
     sh.sliceList[0][tm.slice] = outNear;
     sh.sliceCount[0][tm.slice] = nextCnt;
@@ -691,10 +728,10 @@ __device__ void split_team(const ObjDesc &D, ObjCtl &c, Shared &sh, int k,
   }
 }
 
-template <bool HUBS>
+template <bool HUBS, bool LOCAL>
 __device__ int near_far(const ObjDesc *objs, int nobj, ObjCtl *ctls,
                         int source, int generation, u64 initThr,
-                        const u64 *deltaOverride, int laneBits,
+                        const u64 *deltaOverride, int laneBits, int localHops,
                         GridBarrier &grid, Threads t, Shared &sh,
                         int profBase) {
   (void)profBase;
@@ -802,7 +839,8 @@ __device__ int near_far(const ObjDesc *objs, int nobj, ObjCtl *ctls,
       const unsigned rank = t.block - static_cast<unsigned>(sh.blockBegin[k]);
       const unsigned blocks = static_cast<unsigned>(sh.blockEnd[k] - sh.blockBegin[k]);
       if (sh.mode[k] == MODE_PUSH) {
-        push_team<HUBS>(objs[k], ctls[k], sh, k, it, source, gen, rank, blocks, laneBits, tm);
+        push_team<HUBS, LOCAL>(objs[k], ctls[k], sh, k, it, source, gen, rank, blocks,
+                               laneBits, localHops, tm);
       } else {
         split_team(objs[k], ctls[k], sh, k, it, gen, rank, blocks, tm);
       }
@@ -1382,7 +1420,7 @@ __device__ void combined_build(const LaunchArgs &a, GridBarrier &grid,
 
 // One update instance (graph, K objectives, combined step) on the group of
 // workgroups `t` describes; the group synchronizes through a.ctl.
-template <bool HUBS>
+template <bool HUBS, bool LOCAL>
 __device__ void run_instance(const LaunchArgs &a, const Threads t, Shared &sh) {
   GridBarrier grid{a.ctl, 0, t.block, t.blocks};
   const int n = a.n;
@@ -1417,8 +1455,8 @@ __device__ void run_instance(const LaunchArgs &a, const Threads t, Shared &sh) {
 
   if (a.phases & PH_UPDATE) {
     generation = update_step1<HUBS>(a, grid, t, generation, sh);
-    generation = near_far<HUBS>(a.objs, a.nobj, a.ctl->obj, a.source, generation, 0,
-                          nullptr, a.laneBits, grid, t, sh, 8);
+    generation = near_far<HUBS, LOCAL>(a.objs, a.nobj, a.ctl->obj, a.source, generation, 0,
+                                nullptr, a.laneBits, a.hopsUpdate, grid, t, sh, 8);
     stamp(a.ctl, PHASE_NEARFAR, t.id);
     for (int k = 0; k < a.nobj; ++k) {
       unpack(a.objs[k], n, a.source, t);
@@ -1441,8 +1479,8 @@ __device__ void run_instance(const LaunchArgs &a, const Threads t, Shared &sh) {
     const ObjDesc &D = a.objs[0];
     scratch_init(D, a.ctl->obj[0], n, a.source, t);
     grid.sync();
-    generation = near_far<HUBS>(a.objs, 1, a.ctl->obj, a.source, generation,
-                          D.delta, nullptr, a.laneBits, grid, t, sh, 8);
+    generation = near_far<HUBS, LOCAL>(a.objs, 1, a.ctl->obj, a.source, generation,
+                                D.delta, nullptr, a.laneBits, a.hopsSearch, grid, t, sh, 8);
     unpack(D, n, a.source, t);
     if (D.parentBits == 0) {
       grid.sync();
@@ -1464,8 +1502,8 @@ __device__ void run_instance(const LaunchArgs &a, const Threads t, Shared &sh) {
     grid.sync(sh.src, 2, sh.snap);
     stamp(a.ctl, PHASE_COMB_FILL, t.id);
     const u64 delta = static_cast<u64>(sh.snap[0]) | (static_cast<u64>(sh.snap[1]) << 32);
-    generation = near_far<HUBS>(a.cDesc, 1, &cc, a.source, generation, delta, &delta,
-                          a.laneBits, grid, t, sh, 12);
+    generation = near_far<HUBS, LOCAL>(a.cDesc, 1, &cc, a.source, generation, delta, &delta,
+                                a.laneBits, a.hopsSearch, grid, t, sh, 12);
     stamp(a.ctl, PHASE_COMB_SEARCH, t.id);
     unpack(C, n, a.source, t);
     if (C.parentBits == 0) {
@@ -1494,7 +1532,7 @@ __device__ void run_instance(const LaunchArgs &a, const Threads t, Shared &sh) {
 template <bool HUBS>
 __global__ void __launch_bounds__(BLOCK, WAVES_PER_SIMD) sosp_persistent(LaunchArgs a) {
   __shared__ Shared sh;
-  run_instance<HUBS>(a, Threads{blockIdx.x * BLOCK + threadIdx.x, gridDim.x * BLOCK, blockIdx.x, gridDim.x}, sh);
+  run_instance<HUBS, !HUBS>(a, Threads{blockIdx.x * BLOCK + threadIdx.x, gridDim.x * BLOCK, blockIdx.x, gridDim.x}, sh);
 }
 
 // Many independent instances in one launch: workgroup b belongs to group
@@ -1524,7 +1562,7 @@ __global__ void __launch_bounds__(BLOCK, WAVES_PER_SIMD)
       }
     }
     __syncthreads();
-    run_instance<HUBS>(args, t, sh);
+    run_instance<HUBS, false>(args, t, sh);
   }
 }
 
