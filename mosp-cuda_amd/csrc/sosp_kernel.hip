@@ -1540,10 +1540,9 @@ __global__ void __launch_bounds__(BLOCK, WAVES_PER_SIMD) sosp_persistent(LaunchA
 // other (each instance synchronizes its group through its own Ctl).
 template <bool HUBS>
 __global__ void __launch_bounds__(BLOCK, WAVES_PER_SIMD)
-    sosp_batched(const LaunchArgs *insts, const BatchGroup *groups,
+    sosp_batched(const LaunchArgs *__restrict__ insts, const BatchGroup *groups,
                  const int *groupOf, const int *rankOf) {
   __shared__ Shared sh;
-  __shared__ LaunchArgs args;
   const int g = groupOf[blockIdx.x];
   if (g < 0) {
     return;
@@ -1554,15 +1553,9 @@ __global__ void __launch_bounds__(BLOCK, WAVES_PER_SIMD)
                   static_cast<unsigned>(G.size)};
   for (int i = G.first; i < G.last; ++i) {
     __syncthreads();
-    {
-      const unsigned *from = reinterpret_cast<const unsigned *>(&insts[i]);
-      unsigned *to = reinterpret_cast<unsigned *>(&args);
-      for (unsigned w = threadIdx.x; w < sizeof(LaunchArgs) / 4; w += BLOCK) {
-        to[w] = from[w];
-      }
-    }
-    __syncthreads();
-    run_instance<HUBS, false>(args, t, sh);
+    // Read in place (uniform, read-only: scalar loads); an LDS copy would
+    // put every field in vector registers and the kernel would spill.
+    run_instance<HUBS, false>(insts[i], t, sh);
   }
 }
 
