@@ -194,6 +194,14 @@ __device__ __forceinline__ unsigned wave_reserve(unsigned *counter) {
 constexpr int HEAVY_GROUP = 64;   // lane groups (push): more than one wave pass
 constexpr unsigned HEAVY_LIST = 128;  // per-wave LDS list of heavy push vertices
 constexpr int HEAVY_THREAD = HUB_DEGREE;  // one lane per vertex (parent recovery)
+// Combined-graph count in the hub variant: increments of a parent are first
+// summed in a per-workgroup LDS table (open addressing, AGG_PROBES probes;
+// misses go straight to the global counter), then added once per parent:
+// a hub is the parent of ~10^5 children, whose same-address atomics would
+// serialize.
+constexpr int AGG_BITS = 10;
+constexpr int AGG_SLOTS = 1 << AGG_BITS;
+constexpr int AGG_PROBES = 8;
 constexpr int PULL_WAVE = 8;       // pull: more in-edges -> the candidate's wave
 constexpr int PULL_BLOCK = 1024;   // pull: more in-edges -> a whole workgroup, after the sweep
 
@@ -404,6 +412,8 @@ struct Shared {
   int scan[BLOCK];
   int heavy[WAVES][HEAVY_LIST];  // push: per-wave list of heavy vertices
   int span[2];                   // edge-parallel passes: rows of the first/last edge
+  int aggKey[AGG_SLOTS];         // combined count (hub variant): parent ids ...
+  int aggCount[AGG_SLOTS];       // ... and their increments in this workgroup
   u64 redMin[WAVES];             // workgroup minimum
   BlockQueue near;
   BlockQueue far;
@@ -919,6 +929,73 @@ __device__ void unpack(const ObjDesc &D, int n, int source, Threads t) {
   }
 }
 
+// Row of edge e of this pass: binary search over the pass's row pointers,
+// staged in LDS when the pass spans at most BLOCK rows (global otherwise).
+__device__ __forceinline__ int pass_row(const int *rowPtr, const int *cache, bool cached, int r0,
+                                        int r1, long long e) {
+  if (!cached) {
+    return row_of(rowPtr, r0, r1, e);
+  }
+  int lo = 0, hi = r1 - r0;
+  while (lo < hi) {
+    const int mid = lo + (hi - lo + 1) / 2;
+    if (cache[mid] <= e) {
+      lo = mid;
+    } else {
+      hi = mid - 1;
+    }
+  }
+  return r0 + lo;
+}
+
+// Lowest-id parent over tight out-edges for `count` distance-only
+// objectives sharing one out-CSR (src/sospUpdateGpu.cu:490-511), edge-
+// parallel: every workgroup takes an even, contiguous range of the m edges,
+// BLOCK at a time, and each lane finds its edge's row by binary search
+// between the rows of the pass's first and last edges (their row pointers
+// staged in LDS).  Balanced whatever the degrees (power-law generators put
+// the hubs side by side); one sweep serves every objective (row search,
+// column read).
+__device__ void recover_edges(const ObjDesc *objs, int count, int n, int source, Threads t,
+                              Shared &sh) {
+  const ObjDesc &D0 = objs[0];
+  const int *const outRow = D0.outRow;
+  const int *const outCol = D0.outCol;
+  const int m = outRow[n];
+  const long long lo = static_cast<long long>(m) * t.block / t.blocks;
+  const long long hi = static_cast<long long>(m) * (t.block + 1) / t.blocks;
+  for (long long base = lo; base < hi; base += BLOCK) {
+    const long long last = (base + BLOCK < hi ? base + BLOCK : hi) - 1;
+    __syncthreads();
+    if (threadIdx.x < 2) {
+      sh.span[threadIdx.x] = row_of(outRow, 0, n - 1, threadIdx.x == 0 ? base : last);
+    }
+    __syncthreads();
+    const int r0 = sh.span[0], r1 = sh.span[1];
+    const bool cached = r1 - r0 < static_cast<int>(BLOCK);
+    if (cached) {
+      for (int i = static_cast<int>(threadIdx.x); i <= r1 - r0; i += BLOCK) {
+        sh.scan[i] = outRow[r0 + i];
+      }
+    }
+    __syncthreads();
+    const long long e = base + threadIdx.x;
+    if (e <= last) {
+      const int u = pass_row(outRow, sh.scan, cached, r0, r1, e);
+      const int w = outCol[e];
+      if (w != source) {
+        for (int k = 0; k < count; ++k) {
+          const ObjDesc &D = objs[k];
+          const u64 du = D.s.packed[u];
+          if (du != PACKED_INF && du + static_cast<u64>(D.outW[e]) == D.s.packed[w]) {
+            atomicMin(&D.parentOut[w], u);
+          }
+        }
+      }
+    }
+  }
+}
+
 // Lowest-id parent over tight out-edges (src/sospUpdateGpu.cu:490-511).
 template <bool HUBS>
 __device__ void recover_parents(const ObjDesc &D, int n, int source,
@@ -941,29 +1018,7 @@ __device__ void recover_parents(const ObjDesc &D, int n, int source,
     }
     return;
   }
-  // Edge-parallel: every workgroup takes an even, contiguous range of the m
-  // edges, BLOCK at a time; each lane finds its edge's row by binary search
-  // between the rows of the pass's first and last edges.  Balanced whatever
-  // the degrees (power-law generators put the hubs side by side).
-  const int m = D.outRow[n];
-  const long long lo = static_cast<long long>(m) * t.block / t.blocks;
-  const long long hi = static_cast<long long>(m) * (t.block + 1) / t.blocks;
-  for (long long base = lo; base < hi; base += BLOCK) {
-    const long long last = (base + BLOCK < hi ? base + BLOCK : hi) - 1;
-    __syncthreads();
-    if (threadIdx.x < 2) {
-      sh.span[threadIdx.x] = row_of(D.outRow, 0, n - 1, threadIdx.x == 0 ? base : last);
-    }
-    __syncthreads();
-    const long long e = base + threadIdx.x;
-    if (e <= last) {
-      const int u = row_of(D.outRow, sh.span[0], sh.span[1], e);
-      const u64 du = D.s.packed[u];
-      if (du != PACKED_INF) {
-        scan(u, du, static_cast<int>(e));
-      }
-    }
-  }
+  recover_edges(&D, 1, n, source, t, sh);
 }
 
 // Source-only start of a from-scratch search (src/sospUpdateGpu.cu:222-231).
@@ -1267,6 +1322,26 @@ __device__ int update_step1(const LaunchArgs &a, GridBarrier &grid, Threads t,
 // Combined graph of cK trees (src/combinedGraphGpu.cu:55-121) + its CSR.
 // ---------------------------------------------------------------------------
 
+__device__ __forceinline__ void agg_add(int *keys, int *counts, int *global, int p) {
+  unsigned h = (static_cast<unsigned>(p) * 2654435761u) >> (32 - AGG_BITS);
+  for (int probe = 0; probe < AGG_PROBES; ++probe) {
+    const int key = keys[h];  // a slot's key only ever goes from -1 to a parent
+    if (key == p) {
+      atomicAdd(&counts[h], 1);
+      return;
+    }
+    if (key == -1) {
+      const int prev = atomicCAS(&keys[h], -1, p);
+      if (prev == -1 || prev == p) {
+        atomicAdd(&counts[h], 1);
+        return;
+      }
+    }
+    h = (h + 1) & (AGG_SLOTS - 1);
+  }
+  atomicAdd(&global[p], 1);
+}
+
 // Parent k of v if it is the first occurrence of its value among the K
 // parents; its weight is base - sum of terms over the trees containing it.
 __device__ __forceinline__ bool combined_edge(const LaunchArgs &a, int v, int k,
@@ -1291,8 +1366,10 @@ __device__ __forceinline__ bool combined_edge(const LaunchArgs &a, int v, int k,
   return true;
 }
 
+template <bool HUBS>
 __device__ void combined_build(const LaunchArgs &a, GridBarrier &grid,
-                               Threads t, int *sScan) {
+                               Threads t, Shared &sh) {
+  int *sScan = sh.scan;
   const int n = a.n;
   Ctl *ctl = a.ctl;
   // Clear the degree counters (n + 1).
@@ -1301,6 +1378,13 @@ __device__ void combined_build(const LaunchArgs &a, GridBarrier &grid,
   }
   grid.sync();
   // Count: out-degree of every parent; edge count and weight sum.
+  if constexpr (HUBS) {
+    for (int i = threadIdx.x; i < AGG_SLOTS; i += BLOCK) {
+      sh.aggKey[i] = -1;
+      sh.aggCount[i] = 0;
+    }
+    __syncthreads();
+  }
   u64 edges = 0, wsum = 0;
   for (int v = t.id; v < n; v += t.count) {
     if (v == a.source) {
@@ -1309,9 +1393,21 @@ __device__ void combined_build(const LaunchArgs &a, GridBarrier &grid,
     for (int k = 0; k < a.cK; ++k) {
       int p, weight;
       if (combined_edge(a, v, k, p, weight)) {
-        atomicAdd(&a.cDegree[p], 1);
+        if constexpr (HUBS) {
+          agg_add(sh.aggKey, sh.aggCount, a.cDegree, p);
+        } else {
+          atomicAdd(&a.cDegree[p], 1);
+        }
         edges += 1;
         wsum += static_cast<u64>(weight);
+      }
+    }
+  }
+  if constexpr (HUBS) {
+    __syncthreads();
+    for (int i = threadIdx.x; i < AGG_SLOTS; i += BLOCK) {
+      if (sh.aggCount[i] > 0) {
+        atomicAdd(&a.cDegree[sh.aggKey[i]], sh.aggCount[i]);
       }
     }
   }
@@ -1463,11 +1559,25 @@ __device__ void run_instance(const LaunchArgs &a, const Threads t, Shared &sh) {
     }
     if (a.anyFallback) {
       grid.sync();
+      // One edge sweep for every distance-only objective when they are all
+      // views of one out-CSR (the fused update's case), else one each.
+      bool fused = HUBS;
       for (int k = 0; k < a.nobj; ++k) {
-        if (a.objs[k].parentBits == 0) {
-          recover_parents<HUBS>(a.objs[k], n, a.source, t, sh);
+        fused = fused && a.objs[k].parentBits == 0 && a.objs[k].outRow == a.objs[0].outRow &&
+                a.objs[k].outCol == a.objs[0].outCol;
+      }
+      if (fused) {
+        recover_edges(a.objs, a.nobj, n, a.source, t, sh);
+      } else {
+        for (int k = 0; k < a.nobj; ++k) {
+          if (a.objs[k].parentBits == 0) {
+            recover_parents<HUBS>(a.objs[k], n, a.source, t, sh);
+          }
         }
       }
+    }
+    if (a.phases & PH_COMBINED) {
+      grid.sync();  // the K new trees are complete (and the stamp below is the grid's)
     }
     if (t.id == 0) {
       a.ctl->clkSospEnd = wall_clock64();
@@ -1489,12 +1599,9 @@ __device__ void run_instance(const LaunchArgs &a, const Threads t, Shared &sh) {
   }
 
   if (a.phases & PH_COMBINED) {
-    if (a.phases & PH_UPDATE) {
-      grid.sync();  // the K new trees are complete
-    }
     const ObjDesc &C = *a.cDesc;
     ObjCtl &cc = a.ctl->obj[MOSP_MAX_OBJ];
-    combined_build(a, grid, t, sh.scan);
+    combined_build<HUBS>(a, grid, t, sh);
     scratch_init(C, cc, n, a.source, t);
     if (threadIdx.x < 2) {
       sh.src[threadIdx.x] = reinterpret_cast<unsigned *>(&a.ctl->combDelta) + threadIdx.x;
