@@ -466,6 +466,7 @@ int prepare(mosp_batch *b) {
     a.cColInd = cColInd;
     a.cWeights = cWeights;
     a.cPartials = cPartials;
+    a.stage = reinterpret_cast<u64 *>(cColInd);  // cColInd, cWeights: 8 * K * n bytes in a row
     a.cDesc = b->dDesc + I.descOffset + K;
     b->hArgs[b->argsIndex[i]] = a;
   }
@@ -669,10 +670,8 @@ extern "C" int mosp_batch_check(mosp_batch *b) {
   for (size_t i = 0; i < b->inst.size(); ++i) {
     HIP_TRY(hipMemcpy(b->hCtl, b->dCtl + i, CTL_READBACK, hipMemcpyDeviceToHost));
     for (int k = 0; k < b->inst[i].K; ++k) {
-      if (b->hCtl->obj[k].overflow) {
-        return fail("Error: instance " + std::to_string(i) +
-                    ": an input distance is negative or larger than (n - 1) * maxWeight; "
-                    "the initial tree does not belong to this graph.");
+      if (!input_tree_ok(b->hCtl->obj[k].overflow, "instance " + std::to_string(i) + ": ")) {
+        return 0;
       }
     }
   }

@@ -147,8 +147,7 @@ void ws_release_slab(mosp_ws *ws) {
 void ws_release_combined(mosp_ws *ws) {
   (void)hipFree(ws->cDegree);
   (void)hipFree(ws->cRowPtr);
-  (void)hipFree(ws->cColInd);
-  (void)hipFree(ws->cWeights);
+  (void)hipFree(ws->cColInd);  // cWeights lies in the same allocation
   ws->cDegree = ws->cRowPtr = ws->cColInd = ws->cWeights = nullptr;
   ws->cCapacity = 0;
   ws->cK = 0;
@@ -205,6 +204,19 @@ int ws_init(mosp_ws *ws) {
   HIP_TRY(hipMalloc(&ws->cPartials, sizeof(int) * ws->maxBlocks));
   HIP_TRY(hipEventCreate(&ws->ev0));
   HIP_TRY(hipEventCreate(&ws->ev1));
+  return 1;
+}
+
+// Reports the input-tree checks of a launch's pack pass (ObjCtl::overflow).
+int input_tree_ok(unsigned overflow, const std::string &where) {
+  if (overflow & 2u) {
+    return fail("Error: " + where + "an input parent is not a vertex id in [-1, n); the "
+                "initial tree does not belong to this graph.");
+  }
+  if (overflow & 1u) {
+    return fail("Error: " + where + "an input distance is negative or larger than "
+                "(n - 1) * maxWeight; the initial tree does not belong to this graph.");
+  }
   return 1;
 }
 
@@ -276,8 +288,10 @@ int ws_reserve_combined(mosp_ws *ws, int n, int K) {
   const size_t nn = static_cast<size_t>(std::max(n, 1));
   HIP_TRY(hipMalloc(&ws->cDegree, (nn + 1) * sizeof(int)));
   HIP_TRY(hipMalloc(&ws->cRowPtr, (nn + 1) * sizeof(int)));
-  HIP_TRY(hipMalloc(&ws->cColInd, nn * K * sizeof(int)));
-  HIP_TRY(hipMalloc(&ws->cWeights, nn * K * sizeof(int)));
+  // Columns and weights in one allocation: together they also hold the K
+  // staged words per vertex of the fused parent recovery (LaunchArgs::stage).
+  HIP_TRY(hipMalloc(&ws->cColInd, 2 * nn * K * sizeof(int)));
+  ws->cWeights = ws->cColInd + nn * K;
   ws->cCapacity = n;
   ws->cK = K;
   return 1;
@@ -383,10 +397,8 @@ int run_launch(mosp_ws *ws, LaunchArgs &args, int descCount,
     *kernelMs = ms;
   }
   for (int k = 0; k < args.nobj; ++k) {
-    if (ws->hCtl->obj[k].overflow) {
-      return fail("Error: an input distance is negative or larger than "
-                  "(n - 1) * maxWeight; the initial tree does not belong to "
-                  "this graph.");
+    if (!input_tree_ok(ws->hCtl->obj[k].overflow, "")) {
+      return 0;
     }
   }
   return 1;
@@ -977,6 +989,7 @@ extern "C" int mosp_update_fused(const mosp_device_graph *g, int32_t K,
                       d_comb_parent, K, cstats, cpc)) {
     return 0;
   }
+  args.stage = reinterpret_cast<u64 *>(ws->cColInd);  // 8 * K * n bytes, free until Step 3
   args.objs = ws->dDesc;
   args.nobj = K;
   args.n = n;

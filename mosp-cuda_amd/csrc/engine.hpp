@@ -85,7 +85,8 @@ struct ObjCtl {
   unsigned farCount[3];    // far pile sizes, slot = epoch % 3
   u64 farMin[3];           // min distance appended to the far pile of an epoch
   unsigned invalidated;
-  unsigned overflow;       // an input distance does not fit the packing
+  unsigned overflow;       // bit 0: an input distance does not fit the packing;
+                           // bit 1: an input parent is not in [-1, n)
   unsigned iterations;
   unsigned epochs;
   unsigned jumpCount[3];   // pointer jumping: still-jumping list, slot = round % 3
@@ -155,6 +156,9 @@ struct LaunchArgs {
   int *cColInd;            // K * n
   int *cWeights;           // K * n
   int *cPartials;          // one per workgroup (scan)
+  u64 *stage;              // nobj * n words (may alias cColInd..cWeights, which are
+                           // written only after it is read): the fused parent
+                           // recovery's per-vertex records; nullptr: none
   const ObjDesc *cDesc;    // search on the combined CSR: outRow = row pointers,
                            // outCol / outW = edges, packing, scratch, outputs
 };

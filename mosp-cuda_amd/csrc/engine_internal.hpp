@@ -51,6 +51,9 @@ void local_hops_override(int &update, int &search);
 
 // Records the message for mosp_last_error() and returns 0.
 int fail(const std::string &message);
+// 0 (with the error recorded, prefixed by `where`) when a launch's pack pass
+// found an input parent or distance that cannot belong to the graph.
+int input_tree_ok(unsigned overflow, const std::string &where);
 void clear_error();
 hipStream_t as_stream(void *s);
 size_t align256(size_t bytes);

@@ -202,6 +202,7 @@ constexpr int HEAVY_THREAD = HUB_DEGREE;  // one lane per vertex (parent recover
 constexpr int AGG_BITS = 10;
 constexpr int AGG_SLOTS = 1 << AGG_BITS;
 constexpr int AGG_PROBES = 8;
+constexpr int STAGE_MAX = 8;    // objectives of a staged parent recovery
 constexpr int PULL_WAVE = 8;       // pull: more in-edges -> the candidate's wave
 constexpr int PULL_BLOCK = 1024;   // pull: more in-edges -> a whole workgroup, after the sweep
 
@@ -929,71 +930,130 @@ __device__ void unpack(const ObjDesc &D, int n, int source, Threads t) {
   }
 }
 
-// Row of edge e of this pass: binary search over the pass's row pointers,
-// staged in LDS when the pass spans at most BLOCK rows (global otherwise).
-__device__ __forceinline__ int pass_row(const int *rowPtr, const int *cache, bool cached, int r0,
-                                        int r1, long long e) {
-  if (!cached) {
-    return row_of(rowPtr, r0, r1, e);
+// Edge-parallel sweep of this workgroup's even, contiguous share of the m
+// = rowPtr[n] edges, BLOCK edges a pass: f(u, e) for edge e of row u.
+// Balanced whatever the degrees (power-law generators put the hubs side by
+// side).  Rows come from a window of BLOCK row pointers staged in LDS: a lane
+// binary-searches its edge there, and the window is reloaded (one coalesced
+// load, two barriers) only when the passes have used up its edges, so a pass
+// costs one round of edge loads.  Only the share's first row is searched in
+// global memory.
+template <typename F>
+__device__ void sweep_edges(const int *rowPtr, int n, Threads t, Shared &sh, F &&f) {
+  const long long m = rowPtr[n];
+  const long long lo = m * t.block / t.blocks;
+  const long long hi = m * (t.block + 1) / t.blocks;
+  if (lo >= hi) {
+    return;
   }
-  int lo = 0, hi = r1 - r0;
-  while (lo < hi) {
-    const int mid = lo + (hi - lo + 1) / 2;
-    if (cache[mid] <= e) {
-      lo = mid;
-    } else {
-      hi = mid - 1;
+  int next = row_of(rowPtr, 0, n - 1, lo);  // rowPtr[next] <= base: the next window
+  int first = -1, rows = 0;                 // current window: rows [first, first + rows)
+  long long end = 0;                        // ... and their edges [sh.scan[0], end)
+  for (long long base = lo; base < hi;) {
+    if (next != first) {
+      __syncthreads();  // the last window's searches are done
+      first = next;
+      rows = n - first < static_cast<int>(BLOCK) ? n - first : static_cast<int>(BLOCK);
+      for (int i = static_cast<int>(threadIdx.x); i < rows; i += BLOCK) {
+        sh.scan[i] = rowPtr[first + i];
+      }
+      end = rowPtr[first + rows];
+      __syncthreads();
     }
+    long long stop = base + BLOCK < hi ? base + BLOCK : hi;
+    stop = stop < end ? stop : end;
+    const long long e = base + threadIdx.x;
+    if (e < stop) {
+      int a = 0, b = rows - 1;  // the last window row starting at or before e
+      while (a < b) {
+        const int mid = a + (b - a + 1) / 2;
+        if (sh.scan[mid] <= e) {
+          a = mid;
+        } else {
+          b = mid - 1;
+        }
+      }
+      f(first + a, e);
+    }
+    if (stop == end) {
+      next = first + rows;  // rowPtr[first + rows] == end == the next base
+    }
+    base = stop;
   }
-  return r0 + lo;
 }
 
 // Lowest-id parent over tight out-edges for `count` distance-only
 // objectives sharing one out-CSR (src/sospUpdateGpu.cu:490-511), edge-
-// parallel: every workgroup takes an even, contiguous range of the m edges,
-// BLOCK at a time, and each lane finds its edge's row by binary search
-// between the rows of the pass's first and last edges (their row pointers
-// staged in LDS).  Balanced whatever the degrees (power-law generators put
-// the hubs side by side); one sweep serves every objective (row search,
+// parallel (sweep_edges); one sweep serves every objective (row search,
 // column read).
 __device__ void recover_edges(const ObjDesc *objs, int count, int n, int source, Threads t,
                               Shared &sh) {
-  const ObjDesc &D0 = objs[0];
-  const int *const outRow = D0.outRow;
-  const int *const outCol = D0.outCol;
-  const int m = outRow[n];
-  const long long lo = static_cast<long long>(m) * t.block / t.blocks;
-  const long long hi = static_cast<long long>(m) * (t.block + 1) / t.blocks;
-  for (long long base = lo; base < hi; base += BLOCK) {
-    const long long last = (base + BLOCK < hi ? base + BLOCK : hi) - 1;
-    __syncthreads();
-    if (threadIdx.x < 2) {
-      sh.span[threadIdx.x] = row_of(outRow, 0, n - 1, threadIdx.x == 0 ? base : last);
+  const int *const outCol = objs[0].outCol;
+  sweep_edges(objs[0].outRow, n, t, sh, [&](int u, long long e) {
+    const int w = outCol[e];
+    if (w == source) {
+      return;
     }
-    __syncthreads();
-    const int r0 = sh.span[0], r1 = sh.span[1];
-    const bool cached = r1 - r0 < static_cast<int>(BLOCK);
-    if (cached) {
-      for (int i = static_cast<int>(threadIdx.x); i <= r1 - r0; i += BLOCK) {
-        sh.scan[i] = outRow[r0 + i];
+    for (int k = 0; k < count; ++k) {
+      const ObjDesc &D = objs[k];
+      const u64 du = D.s.packed[u];
+      if (du != PACKED_INF && du + static_cast<u64>(D.outW[e]) == D.s.packed[w]) {
+        atomicMin(&D.parentOut[w], u);
       }
     }
-    __syncthreads();
-    const long long e = base + threadIdx.x;
-    if (e <= last) {
-      const int u = pass_row(outRow, sh.scan, cached, r0, r1, e);
-      const int w = outCol[e];
-      if (w != source) {
-        for (int k = 0; k < count; ++k) {
-          const ObjDesc &D = objs[k];
-          const u64 du = D.s.packed[u];
-          if (du != PACKED_INF && du + static_cast<u64>(D.outW[e]) == D.s.packed[w]) {
-            atomicMin(&D.parentOut[w], u);
-          }
-        }
+  });
+}
+
+// unpack() of `count` distance-only objectives in one pass that also writes
+// each vertex's K words side by side (stage[v * count + k]) for
+// recover_staged: one record per vertex instead of K arrays.
+__device__ void unpack_staged(const ObjDesc *objs, int count, int n, int source, Threads t,
+                              u64 *stage) {
+  for (int v = t.id; v < n; v += t.count) {
+    for (int k = 0; k < count; ++k) {
+      const ObjDesc &D = objs[k];
+      if (D.nChanged > 0 && D.s.flag != nullptr) {
+        D.s.flag[v] = 0;
       }
+      const u64 word = D.s.packed[v];
+      stage[static_cast<size_t>(v) * count + k] = word;
+      D.distOut[v] = word == PACKED_INF ? DISTANCE_INF : static_cast<int64_t>(word);
+      D.parentOut[v] = word == PACKED_INF || v == source ? -1 : INT_MAX;
     }
   }
+}
+
+// recover_edges over the staged records: a tight-edge test reads the head's
+// K distances from one record (one or two cache lines) instead of K arrays
+// (K lines); the sweep and the lowest-id rule are recover_edges'
+// (src/sospUpdateGpu.cu:490-511).  count <= STAGE_MAX.
+__device__ void recover_staged(const ObjDesc *objs, int count, int n, int source, Threads t,
+                               Shared &sh, const u64 *stage) {
+  const int *const outCol = objs[0].outCol;
+  sweep_edges(objs[0].outRow, n, t, sh, [&](int u, long long e) {
+    const int w = outCol[e];
+    if (w == source) {
+      return;
+    }
+    const u64 *const tail = stage + static_cast<size_t>(u) * count;
+    const u64 *const head = stage + static_cast<size_t>(w) * count;
+    u64 du[STAGE_MAX], dw[STAGE_MAX];
+    int wt[STAGE_MAX];
+#pragma unroll
+    for (int k = 0; k < STAGE_MAX; ++k) {
+      if (k < count) {
+        du[k] = tail[k];
+        dw[k] = head[k];
+        wt[k] = objs[k].outW[e];
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < STAGE_MAX; ++k) {
+      if (k < count && du[k] != PACKED_INF && du[k] + static_cast<u64>(wt[k]) == dw[k]) {
+        atomicMin(&objs[k].parentOut[w], u);
+      }
+    }
+  });
 }
 
 // Lowest-id parent over tight out-edges (src/sospUpdateGpu.cu:490-511).
@@ -1070,12 +1130,16 @@ __device__ int update_step1(const LaunchArgs &a, GridBarrier &grid, Threads t,
     unsigned overflow = 0;
     for (int v = t.id; v < n; v += t.count) {
       const int64_t d = D.distIn[v];
-      const int p = D.parentIn[v];
+      int p = D.parentIn[v];
+      if (p < -1 || p >= n) {
+        overflow |= 2;  // not a vertex: reported, and never used as an index
+        p = -1;
+      }
       u64 word;
       if (d >= DISTANCE_INF / 2) {
         word = PACKED_INF;
       } else if (d < 0 || static_cast<u64>(d) > D.maxDistance) {
-        overflow = 1;
+        overflow |= 1;
         word = PACKED_INF;
       } else {
         word = pack(static_cast<u64>(d), p, D.parentBits, D.noParent);
@@ -1086,7 +1150,7 @@ __device__ int update_step1(const LaunchArgs &a, GridBarrier &grid, Threads t,
       }
     }
     if (overflow) {
-      atomicOr(&ctl->obj[k].overflow, 1u);
+      atomicOr(&ctl->obj[k].overflow, overflow);
     }
     for (int i = t.id; i < D.nChanged; i += t.count) {
       const int v = D.chTo[i];
@@ -1554,19 +1618,26 @@ __device__ void run_instance(const LaunchArgs &a, const Threads t, Shared &sh) {
     generation = near_far<HUBS, LOCAL>(a.objs, a.nobj, a.ctl->obj, a.source, generation, 0,
                                 nullptr, a.laneBits, a.hopsUpdate, grid, t, sh, 8);
     stamp(a.ctl, PHASE_NEARFAR, t.id);
+    // One edge sweep for every distance-only objective when they are all
+    // views of one out-CSR (the fused update's case), else one each.
+    bool fused = HUBS && a.anyFallback;
     for (int k = 0; k < a.nobj; ++k) {
-      unpack(a.objs[k], n, a.source, t);
+      fused = fused && a.objs[k].parentBits == 0 && a.objs[k].outRow == a.objs[0].outRow &&
+              a.objs[k].outCol == a.objs[0].outCol;
+    }
+    u64 *const stage = fused && a.nobj <= STAGE_MAX ? a.stage : nullptr;
+    if (stage != nullptr) {
+      unpack_staged(a.objs, a.nobj, n, a.source, t, stage);
+    } else {
+      for (int k = 0; k < a.nobj; ++k) {
+        unpack(a.objs[k], n, a.source, t);
+      }
     }
     if (a.anyFallback) {
       grid.sync();
-      // One edge sweep for every distance-only objective when they are all
-      // views of one out-CSR (the fused update's case), else one each.
-      bool fused = HUBS;
-      for (int k = 0; k < a.nobj; ++k) {
-        fused = fused && a.objs[k].parentBits == 0 && a.objs[k].outRow == a.objs[0].outRow &&
-                a.objs[k].outCol == a.objs[0].outCol;
-      }
-      if (fused) {
+      if (stage != nullptr) {
+        recover_staged(a.objs, a.nobj, n, a.source, t, sh, stage);
+      } else if (fused) {
         recover_edges(a.objs, a.nobj, n, a.source, t, sh);
       } else {
         for (int k = 0; k < a.nobj; ++k) {

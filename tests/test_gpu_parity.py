@@ -248,6 +248,26 @@ def test_edge_cases():
     check_pipeline(g, b, tag="insert-into-source")
 
 
+def test_invalid_input_trees_are_reported():
+    """A parent outside [-1, n) or a negative distance in the old trees is an
+    error (the kernel reads such a parent as 'none', never as an index); a
+    valid update afterwards is unaffected."""
+    g = gen.grid_graph(12, 12, 2, 50, 0.1, 5)
+    b = gen.make_batch("uniform-unsafe", g, 40, 6)
+    d0, p0 = gen.initial_trees(g, 2)
+    for k, v, bad in ((0, 7, g.n + 7), (1, 9, -3), (1, 11, 2**31 - 1)):
+        p = p0.copy()
+        p[k, v] = bad
+        say(f"case bad parent obj{k} v={v} value={bad}")
+        with pytest.raises(RuntimeError, match="not a vertex id"):
+            M().update_host(mgraph(g), mbatch(b), d0, p, K=2)
+    d = d0.copy()
+    d[0, 5] = -5
+    with pytest.raises(RuntimeError, match="input distance"):
+        M().update_host(mgraph(g), mbatch(b), d, p0, K=2)
+    check_pipeline(g, b, tag="after invalid inputs")
+
+
 # -------------------------------------------------- power-law (R-MAT) graphs --
 
 def rmat(scale, edge_factor, K, seed, wmax=100):
