@@ -37,6 +37,29 @@ namespace {
 
 enum : int { MODE_DONE = 0, MODE_PUSH = 1, MODE_SPLIT = 2 };
 
+// Launch arguments and objective descriptors are written by the host before
+// a launch and only read by the kernels, so they are read through the
+// constant address space: uniform reads become scalar loads (results in
+// SGPRs), and stores and atomics elsewhere cannot force them to be read
+// again.  Through generic pointers every field would be a vector load
+// repeated after each atomic, on the dependent path of every edge.
+using Args = const __attribute__((address_space(4))) LaunchArgs;
+using Desc = const __attribute__((address_space(4))) ObjDesc;
+
+__device__ __forceinline__ Desc *desc(const ObjDesc *p) {
+  return (Desc *)p;
+}
+
+// The same pointer, known to be uniform (SGPRs): arguments of calls arrive in
+// VGPRs, and loads through them would be per-lane vector loads.
+template <typename T>
+__device__ __forceinline__ T *uniform(T *p) {
+  const unsigned long long bits = reinterpret_cast<unsigned long long>(p);
+  const unsigned lo = __builtin_amdgcn_readfirstlane(static_cast<unsigned>(bits));
+  const unsigned hi = __builtin_amdgcn_readfirstlane(static_cast<unsigned>(bits >> 32));
+  return reinterpret_cast<T *>((static_cast<unsigned long long>(hi) << 32) | lo);
+}
+
 constexpr unsigned QCAP = 4096;  // LDS queue entries per list per workgroup
 
 // ---------------------------------------------------------------------------
@@ -487,7 +510,7 @@ struct Team {
 // Relax the out-edges of the frontier vertices of objective k: groups of
 // 2^laneBits lanes share a vertex and stride over its edges.
 template <bool HUBS, bool LOCAL>
-__device__ void push_team(const ObjDesc &D, ObjCtl &c, Shared &sh, int k,
+__device__ void push_team(Desc &D, ObjCtl &c, Shared &sh, int k,
                           unsigned it, int source, int gen, unsigned blockRank,
                           unsigned blocks, int laneBits, int localHops, const Team &tm) {
   const unsigned cnt = sh.cnt[k];
@@ -690,7 +713,7 @@ __device__ void push_team(const ObjDesc &D, ObjCtl &c, Shared &sh, int k,
 
 // Raise the threshold past the far pile: move the entries below it to the
 // next near frontier, keep the rest in the other far buffer.
-__device__ void split_team(const ObjDesc &D, ObjCtl &c, Shared &sh, int k,
+__device__ void split_team(Desc &D, ObjCtl &c, Shared &sh, int k,
                            unsigned it, int gen, unsigned blockRank,
                            unsigned blocks, const Team &tm) {
   const unsigned cnt = sh.cnt[k];
@@ -740,11 +763,13 @@ __device__ void split_team(const ObjDesc &D, ObjCtl &c, Shared &sh, int k,
 }
 
 template <bool HUBS, bool LOCAL>
-__device__ int near_far(const ObjDesc *objs, int nobj, ObjCtl *ctls,
+__device__ int near_far(Desc *objsArg, int nobj, ObjCtl *ctlsArg,
                         int source, int generation, u64 initThr,
                         const u64 *deltaOverride, int laneBits, int localHops,
                         GridBarrier &grid, Threads t, Shared &sh,
                         int profBase) {
+  Desc *const objs = uniform(objsArg);
+  ObjCtl *const ctls = uniform(ctlsArg);
   (void)profBase;
   if (threadIdx.x < static_cast<unsigned>(nobj)) {
     const int k = threadIdx.x;
@@ -905,7 +930,7 @@ __device__ int near_far(const ObjDesc *objs, int nobj, ObjCtl *ctls,
 
 // Words -> (distance, parent); distance-only words get INT_MAX parents for
 // the recovery pass (src/sospUpdateGpu.cu:478-497).
-__device__ void unpack(const ObjDesc &D, int n, int source, Threads t) {
+__device__ void unpack(Desc &D, int n, int source, Threads t) {
   const int bits = D.parentBits;
   const bool clearFlags = D.nChanged > 0 && D.s.flag != nullptr;
   for (int v = t.id; v < n; v += t.count) {
@@ -986,7 +1011,7 @@ __device__ void sweep_edges(const int *rowPtr, int n, Threads t, Shared &sh, F &
 // objectives sharing one out-CSR (src/sospUpdateGpu.cu:490-511), edge-
 // parallel (sweep_edges); one sweep serves every objective (row search,
 // column read).
-__device__ void recover_edges(const ObjDesc *objs, int count, int n, int source, Threads t,
+__device__ void recover_edges(Desc *objs, int count, int n, int source, Threads t,
                               Shared &sh) {
   const int *const outCol = objs[0].outCol;
   sweep_edges(objs[0].outRow, n, t, sh, [&](int u, long long e) {
@@ -995,7 +1020,7 @@ __device__ void recover_edges(const ObjDesc *objs, int count, int n, int source,
       return;
     }
     for (int k = 0; k < count; ++k) {
-      const ObjDesc &D = objs[k];
+      Desc &D = objs[k];
       const u64 du = D.s.packed[u];
       if (du != PACKED_INF && du + static_cast<u64>(D.outW[e]) == D.s.packed[w]) {
         atomicMin(&D.parentOut[w], u);
@@ -1007,11 +1032,11 @@ __device__ void recover_edges(const ObjDesc *objs, int count, int n, int source,
 // unpack() of `count` distance-only objectives in one pass that also writes
 // each vertex's K words side by side (stage[v * count + k]) for
 // recover_staged: one record per vertex instead of K arrays.
-__device__ void unpack_staged(const ObjDesc *objs, int count, int n, int source, Threads t,
+__device__ void unpack_staged(Desc *objs, int count, int n, int source, Threads t,
                               u64 *stage) {
   for (int v = t.id; v < n; v += t.count) {
     for (int k = 0; k < count; ++k) {
-      const ObjDesc &D = objs[k];
+      Desc &D = objs[k];
       if (D.nChanged > 0 && D.s.flag != nullptr) {
         D.s.flag[v] = 0;
       }
@@ -1026,39 +1051,54 @@ __device__ void unpack_staged(const ObjDesc *objs, int count, int n, int source,
 // recover_edges over the staged records: a tight-edge test reads the head's
 // K distances from one record (one or two cache lines) instead of K arrays
 // (K lines); the sweep and the lowest-id rule are recover_edges'
-// (src/sospUpdateGpu.cu:490-511).  count <= STAGE_MAX.
-__device__ void recover_staged(const ObjDesc *objs, int count, int n, int source, Threads t,
-                               Shared &sh, const u64 *stage) {
+// (src/sospUpdateGpu.cu:490-511).  N objectives, known at compile time, so
+// every load of an edge is issued before the first test waits.
+template <int N>
+__device__ void recover_staged_n(Desc *objs, int n, int source, Threads t, Shared &sh,
+                                 const u64 *stage) {
   const int *const outCol = objs[0].outCol;
   sweep_edges(objs[0].outRow, n, t, sh, [&](int u, long long e) {
     const int w = outCol[e];
     if (w == source) {
       return;
     }
-    const u64 *const tail = stage + static_cast<size_t>(u) * count;
-    const u64 *const head = stage + static_cast<size_t>(w) * count;
-    u64 du[STAGE_MAX], dw[STAGE_MAX];
-    int wt[STAGE_MAX];
+    const u64 *const tail = stage + static_cast<size_t>(u) * N;
+    const u64 *const head = stage + static_cast<size_t>(w) * N;
+    u64 du[N], dw[N];
+    int wt[N];
 #pragma unroll
-    for (int k = 0; k < STAGE_MAX; ++k) {
-      if (k < count) {
-        du[k] = tail[k];
-        dw[k] = head[k];
-        wt[k] = objs[k].outW[e];
-      }
+    for (int k = 0; k < N; ++k) {
+      du[k] = tail[k];
+      dw[k] = head[k];
+      wt[k] = objs[k].outW[e];
     }
 #pragma unroll
-    for (int k = 0; k < STAGE_MAX; ++k) {
-      if (k < count && du[k] != PACKED_INF && du[k] + static_cast<u64>(wt[k]) == dw[k]) {
+    for (int k = 0; k < N; ++k) {
+      if (du[k] != PACKED_INF && du[k] + static_cast<u64>(wt[k]) == dw[k]) {
         atomicMin(&objs[k].parentOut[w], u);
       }
     }
   });
 }
 
+// count <= STAGE_MAX.
+__device__ void recover_staged(Desc *objs, int count, int n, int source, Threads t,
+                               Shared &sh, const u64 *stage) {
+  switch (count) {
+    case 1: recover_staged_n<1>(objs, n, source, t, sh, stage); break;
+    case 2: recover_staged_n<2>(objs, n, source, t, sh, stage); break;
+    case 3: recover_staged_n<3>(objs, n, source, t, sh, stage); break;
+    case 4: recover_staged_n<4>(objs, n, source, t, sh, stage); break;
+    case 5: recover_staged_n<5>(objs, n, source, t, sh, stage); break;
+    case 6: recover_staged_n<6>(objs, n, source, t, sh, stage); break;
+    case 7: recover_staged_n<7>(objs, n, source, t, sh, stage); break;
+    default: recover_staged_n<STAGE_MAX>(objs, n, source, t, sh, stage); break;
+  }
+}
+
 // Lowest-id parent over tight out-edges (src/sospUpdateGpu.cu:490-511).
 template <bool HUBS>
-__device__ void recover_parents(const ObjDesc &D, int n, int source,
+__device__ void recover_parents(Desc &D, int n, int source,
                                 Threads t, Shared &sh) {
   auto scan = [&](int u, u64 du, int x) {
     const int w = D.outCol[x];
@@ -1082,7 +1122,7 @@ __device__ void recover_parents(const ObjDesc &D, int n, int source,
 }
 
 // Source-only start of a from-scratch search (src/sospUpdateGpu.cu:222-231).
-__device__ void scratch_init(const ObjDesc &D, ObjCtl &c, int n, int source,
+__device__ void scratch_init(Desc &D, ObjCtl &c, int n, int source,
                              Threads t) {
   for (int v = t.id; v < n; v += t.count) {
     D.s.packed[v] = v == source ? pack(0, -1, D.parentBits, D.noParent)
@@ -1118,14 +1158,14 @@ __device__ __forceinline__ bool jump(int *anc, int *flag, int v) {
 }
 
 template <bool HUBS>
-__device__ int update_step1(const LaunchArgs &a, GridBarrier &grid, Threads t,
+__device__ int update_step1(Args &a, GridBarrier &grid, Threads t,
                             int generation, Shared &sh) {
   const int n = a.n;
   Ctl *ctl = a.ctl;
   // Pack the old trees; ancestors; roots of changed tree edges.
   bool anyChanged = false;
   for (int k = 0; k < a.nobj; ++k) {
-    const ObjDesc &D = a.objs[k];
+    Desc &D = desc(a.objs)[k];
     anyChanged = anyChanged || D.nChanged > 0;
     unsigned overflow = 0;
     for (int v = t.id; v < n; v += t.count) {
@@ -1174,7 +1214,7 @@ __device__ int update_step1(const LaunchArgs &a, GridBarrier &grid, Threads t,
         ctl->rounds = round + 1;
       }
       for (int k = 0; k < a.nobj; ++k) {
-        const ObjDesc &D = a.objs[k];
+        Desc &D = desc(a.objs)[k];
         if (D.nChanged == 0) {
           continue;
         }
@@ -1216,7 +1256,7 @@ __device__ int update_step1(const LaunchArgs &a, GridBarrier &grid, Threads t,
   // set until the unpack sweep clears them.
   const int genInv = ++generation;
   for (int k = 0; k < a.nobj; ++k) {
-    const ObjDesc &D = a.objs[k];
+    Desc &D = desc(a.objs)[k];
     unsigned count = 0;
     if (D.nChanged > 0) {
       for (int v = t.id; v < n; v += t.count) {
@@ -1247,7 +1287,7 @@ __device__ int update_step1(const LaunchArgs &a, GridBarrier &grid, Threads t,
   // (near[0]) and pulled afterwards by one whole workgroup each, round robin
   // over the grid: the heaviest ones would otherwise hold their wave -- and
   // the whole grid at the next barrier -- for milliseconds.
-  auto pulled = [&](const ObjDesc &D, ObjCtl &c, int v, u64 best, u64 current) {
+  auto pulled = [&](Desc &D, ObjCtl &c, int v, u64 best, u64 current) {
     if (best < current) {
       const u64 prev = atomicMin(&D.s.packed[v], best);
       const u64 bd = dist_of(best, D.parentBits);
@@ -1259,7 +1299,7 @@ __device__ int update_step1(const LaunchArgs &a, GridBarrier &grid, Threads t,
     }
   };
   for (int k = 0; k < a.nobj; ++k) {
-    const ObjDesc &D = a.objs[k];
+    Desc &D = desc(a.objs)[k];
     ObjCtl &c = ctl->obj[k];
     const int bits = D.parentBits;
     if (threadIdx.x == 0) {
@@ -1343,7 +1383,7 @@ __device__ int update_step1(const LaunchArgs &a, GridBarrier &grid, Threads t,
     }
     grid.sync(sh.src, a.nobj, sh.snap);
     for (int k = 0; k < a.nobj; ++k) {
-      const ObjDesc &D = a.objs[k];
+      Desc &D = desc(a.objs)[k];
       ObjCtl &c = ctl->obj[k];
       const int bits = D.parentBits;
       if (threadIdx.x == 0) {
@@ -1408,7 +1448,7 @@ __device__ __forceinline__ void agg_add(int *keys, int *counts, int *global, int
 
 // Parent k of v if it is the first occurrence of its value among the K
 // parents; its weight is base - sum of terms over the trees containing it.
-__device__ __forceinline__ bool combined_edge(const LaunchArgs &a, int v, int k,
+__device__ __forceinline__ bool combined_edge(Args &a, int v, int k,
                                               int &p, int &weight) {
   const int n = a.n;
   const int *par = a.cParents;
@@ -1431,8 +1471,9 @@ __device__ __forceinline__ bool combined_edge(const LaunchArgs &a, int v, int k,
 }
 
 template <bool HUBS>
-__device__ void combined_build(const LaunchArgs &a, GridBarrier &grid,
+__device__ void combined_build(Args &args, GridBarrier &grid,
                                Threads t, Shared &sh) {
+  Args &a = *uniform(&args);
   int *sScan = sh.scan;
   const int n = a.n;
   Ctl *ctl = a.ctl;
@@ -1581,7 +1622,8 @@ __device__ void combined_build(const LaunchArgs &a, GridBarrier &grid,
 // One update instance (graph, K objectives, combined step) on the group of
 // workgroups `t` describes; the group synchronizes through a.ctl.
 template <bool HUBS, bool LOCAL>
-__device__ void run_instance(const LaunchArgs &a, const Threads t, Shared &sh) {
+__device__ void run_instance(Args &args, const Threads t, Shared &sh) {
+  Args &a = *uniform(&args);
   GridBarrier grid{a.ctl, 0, t.block, t.blocks};
   const int n = a.n;
   if (t.id == 0) {
@@ -1600,12 +1642,12 @@ __device__ void run_instance(const LaunchArgs &a, const Threads t, Shared &sh) {
   if (generation > (1 << 30)) {
     for (int k = 0; k < a.nobj; ++k) {
       for (int v = t.id; v < n; v += t.count) {
-        a.objs[k].s.stamp[v] = 0;
+        desc(a.objs)[k].s.stamp[v] = 0;
       }
     }
     if (a.cDesc != nullptr) {
       for (int v = t.id; v < n; v += t.count) {
-        a.cDesc->s.stamp[v] = 0;
+        desc(a.cDesc)->s.stamp[v] = 0;
       }
     }
     generation = 0;
@@ -1615,34 +1657,34 @@ __device__ void run_instance(const LaunchArgs &a, const Threads t, Shared &sh) {
 
   if (a.phases & PH_UPDATE) {
     generation = update_step1<HUBS>(a, grid, t, generation, sh);
-    generation = near_far<HUBS, LOCAL>(a.objs, a.nobj, a.ctl->obj, a.source, generation, 0,
+    generation = near_far<HUBS, LOCAL>(desc(a.objs), a.nobj, a.ctl->obj, a.source, generation, 0,
                                 nullptr, a.laneBits, a.hopsUpdate, grid, t, sh, 8);
     stamp(a.ctl, PHASE_NEARFAR, t.id);
     // One edge sweep for every distance-only objective when they are all
     // views of one out-CSR (the fused update's case), else one each.
     bool fused = HUBS && a.anyFallback;
     for (int k = 0; k < a.nobj; ++k) {
-      fused = fused && a.objs[k].parentBits == 0 && a.objs[k].outRow == a.objs[0].outRow &&
-              a.objs[k].outCol == a.objs[0].outCol;
+      fused = fused && desc(a.objs)[k].parentBits == 0 && desc(a.objs)[k].outRow == desc(a.objs)[0].outRow &&
+              desc(a.objs)[k].outCol == desc(a.objs)[0].outCol;
     }
     u64 *const stage = fused && a.nobj <= STAGE_MAX ? a.stage : nullptr;
     if (stage != nullptr) {
-      unpack_staged(a.objs, a.nobj, n, a.source, t, stage);
+      unpack_staged(desc(a.objs), a.nobj, n, a.source, t, stage);
     } else {
       for (int k = 0; k < a.nobj; ++k) {
-        unpack(a.objs[k], n, a.source, t);
+        unpack(desc(a.objs)[k], n, a.source, t);
       }
     }
     if (a.anyFallback) {
       grid.sync();
       if (stage != nullptr) {
-        recover_staged(a.objs, a.nobj, n, a.source, t, sh, stage);
+        recover_staged(desc(a.objs), a.nobj, n, a.source, t, sh, stage);
       } else if (fused) {
-        recover_edges(a.objs, a.nobj, n, a.source, t, sh);
+        recover_edges(desc(a.objs), a.nobj, n, a.source, t, sh);
       } else {
         for (int k = 0; k < a.nobj; ++k) {
-          if (a.objs[k].parentBits == 0) {
-            recover_parents<HUBS>(a.objs[k], n, a.source, t, sh);
+          if (desc(a.objs)[k].parentBits == 0) {
+            recover_parents<HUBS>(desc(a.objs)[k], n, a.source, t, sh);
           }
         }
       }
@@ -1657,10 +1699,10 @@ __device__ void run_instance(const LaunchArgs &a, const Threads t, Shared &sh) {
   }
 
   if (a.phases & PH_SCRATCH) {
-    const ObjDesc &D = a.objs[0];
+    Desc &D = desc(a.objs)[0];
     scratch_init(D, a.ctl->obj[0], n, a.source, t);
     grid.sync();
-    generation = near_far<HUBS, LOCAL>(a.objs, 1, a.ctl->obj, a.source, generation,
+    generation = near_far<HUBS, LOCAL>(desc(a.objs), 1, a.ctl->obj, a.source, generation,
                                 D.delta, nullptr, a.laneBits, a.hopsSearch, grid, t, sh, 8);
     unpack(D, n, a.source, t);
     if (D.parentBits == 0) {
@@ -1670,7 +1712,7 @@ __device__ void run_instance(const LaunchArgs &a, const Threads t, Shared &sh) {
   }
 
   if (a.phases & PH_COMBINED) {
-    const ObjDesc &C = *a.cDesc;
+    Desc &C = *desc(a.cDesc);
     ObjCtl &cc = a.ctl->obj[MOSP_MAX_OBJ];
     combined_build<HUBS>(a, grid, t, sh);
     scratch_init(C, cc, n, a.source, t);
@@ -1680,7 +1722,7 @@ __device__ void run_instance(const LaunchArgs &a, const Threads t, Shared &sh) {
     grid.sync(sh.src, 2, sh.snap);
     stamp(a.ctl, PHASE_COMB_FILL, t.id);
     const u64 delta = static_cast<u64>(sh.snap[0]) | (static_cast<u64>(sh.snap[1]) << 32);
-    generation = near_far<HUBS, LOCAL>(a.cDesc, 1, &cc, a.source, generation, delta, &delta,
+    generation = near_far<HUBS, LOCAL>(desc(a.cDesc), 1, &cc, a.source, generation, delta, &delta,
                                 a.laneBits, a.hopsSearch, grid, t, sh, 12);
     stamp(a.ctl, PHASE_COMB_SEARCH, t.id);
     unpack(C, n, a.source, t);
@@ -1710,7 +1752,10 @@ __device__ void run_instance(const LaunchArgs &a, const Threads t, Shared &sh) {
 template <bool HUBS>
 __global__ void __launch_bounds__(BLOCK, WAVES_PER_SIMD) sosp_persistent(LaunchArgs a) {
   __shared__ Shared sh;
-  run_instance<HUBS, !HUBS>(a, Threads{blockIdx.x * BLOCK + threadIdx.x, gridDim.x * BLOCK, blockIdx.x, gridDim.x}, sh);
+  (void)a;  // read in place: the first kernel argument starts the kernarg segment
+  Args &args = *(Args *)__builtin_amdgcn_kernarg_segment_ptr();
+  run_instance<HUBS, !HUBS>(args, Threads{blockIdx.x * BLOCK + threadIdx.x, gridDim.x * BLOCK,
+                                          blockIdx.x, gridDim.x}, sh);
 }
 
 // Many independent instances in one launch: workgroup b belongs to group
@@ -1733,7 +1778,7 @@ __global__ void __launch_bounds__(BLOCK, WAVES_PER_SIMD)
     __syncthreads();
     // Read in place (uniform, read-only: scalar loads); an LDS copy would
     // put every field in vector registers and the kernel would spill.
-    run_instance<HUBS, false>(insts[i], t, sh);
+    run_instance<HUBS, false>(*(Args *)(insts + i), t, sh);
   }
 }
 
