@@ -60,6 +60,18 @@ __device__ __forceinline__ T *uniform(T *p) {
   return reinterpret_cast<T *>((static_cast<unsigned long long>(hi) << 32) | lo);
 }
 
+// uniform() of a pointer into device (global) memory, rebuilt in the global
+// address space: atomics through it stay global atomics (through a rebuilt
+// generic pointer they would be flat atomics with aperture checks).
+template <typename T>
+__device__ __forceinline__ T *uniform_global(T *p) {
+  using G = __attribute__((address_space(1))) T;
+  const unsigned long long bits = reinterpret_cast<unsigned long long>(p);
+  const unsigned lo = __builtin_amdgcn_readfirstlane(static_cast<unsigned>(bits));
+  const unsigned hi = __builtin_amdgcn_readfirstlane(static_cast<unsigned>(bits >> 32));
+  return (T *)reinterpret_cast<G *>((static_cast<unsigned long long>(hi) << 32) | lo);
+}
+
 constexpr unsigned QCAP = 4096;  // LDS queue entries per list per workgroup
 
 // ---------------------------------------------------------------------------
@@ -288,13 +300,16 @@ struct BlockQueue {
   unsigned base[WAVES];   // list offset of a slice being flushed
 };
 
+// The two stores name their address spaces (LDS queue, global list):
+// otherwise the compiler merges them into one flat store through a pointer
+// that may be either.
 __device__ __forceinline__ void slice_push(BlockQueue &q, unsigned slice, unsigned cap, int v,
                                            int *list, unsigned *listCount) {
   const unsigned slot = wave_reserve(&q.count[slice]);
   if (slot < cap) {
-    q.buf[slice * cap + slot] = v;
+    *(__attribute__((address_space(3))) int *)(q.buf + slice * cap + slot) = v;
   } else {
-    list[wave_reserve(listCount)] = v;
+    *(__attribute__((address_space(1))) int *)(list + wave_reserve(listCount)) = v;
   }
 }
 
@@ -519,17 +534,18 @@ __device__ void push_team(Desc &D, ObjCtl &c, Shared &sh, int k,
   const u64 noParent = D.noParent;
   const u64 thr = sh.thr[k];
   const u64 maxDist = D.maxDistance;
-  // Locals: D lives in global memory, and any store or atomic below could
-  // alias it, so field reads inside the loop would be reissued each time.
-  u64 *const packed = D.s.packed;
-  int *const stampArr = D.s.stamp;
-  int *const inFar = D.s.inFar;
-  const int *const outRow = D.outRow;
-  const int *const outCol = D.outCol;
-  const int *const outW = D.outW;
-  const int *in = D.s.near[it & 1];
-  int *outNear = D.s.near[(it + 1) & 1];
-  int *farBuf = D.s.far[e & 1];
+  // Locals, read once; a team is whole waves, so they are wave-uniform and
+  // live in SGPRs (k, hence D, comes from LDS: the compiler cannot tell).
+  // All are device memory: global, not flat, accesses.
+  u64 *const packed = uniform_global(D.s.packed);
+  int *const stampArr = uniform_global(D.s.stamp);
+  int *const inFar = uniform_global(D.s.inFar);
+  const int *const outRow = uniform_global(D.outRow);
+  const int *const outCol = uniform_global(D.outCol);
+  const int *const outW = uniform_global(D.outW);
+  const int *in = uniform_global(D.s.near[it & 1]);
+  int *outNear = uniform_global(D.s.near[(it + 1) & 1]);
+  int *farBuf = uniform_global(D.s.far[e & 1]);
   unsigned *nextCnt = &c.nearCount[(it + 1) % 3];
   unsigned *farCnt = &c.farCount[e % 3];
   const unsigned lanes = 1u << laneBits;
@@ -769,7 +785,7 @@ __device__ int near_far(Desc *objsArg, int nobj, ObjCtl *ctlsArg,
                         GridBarrier &grid, Threads t, Shared &sh,
                         int profBase) {
   Desc *const objs = uniform(objsArg);
-  ObjCtl *const ctls = uniform(ctlsArg);
+  ObjCtl *const ctls = uniform_global(ctlsArg);
   (void)profBase;
   if (threadIdx.x < static_cast<unsigned>(nobj)) {
     const int k = threadIdx.x;
