@@ -944,15 +944,23 @@ __device__ int near_far(Desc *objsArg, int nobj, ObjCtl *ctlsArg,
   return generation;
 }
 
+// The root marks of Step 1 back to zero for the next update: they are set
+// only at heads of changed edges.
+__device__ void clear_roots(Desc &D, Threads t) {
+  if (D.s.flag == nullptr) {
+    return;
+  }
+  for (int i = t.id; i < D.nChanged; i += t.count) {
+    D.s.flag[D.chTo[i]] = 0;
+  }
+}
+
 // Words -> (distance, parent); distance-only words get INT_MAX parents for
 // the recovery pass (src/sospUpdateGpu.cu:478-497).
 __device__ void unpack(Desc &D, int n, int source, Threads t) {
   const int bits = D.parentBits;
-  const bool clearFlags = D.nChanged > 0 && D.s.flag != nullptr;
+  clear_roots(D, t);
   for (int v = t.id; v < n; v += t.count) {
-    if (clearFlags) {
-      D.s.flag[v] = 0;  // invalidation marks are left clean for the next update
-    }
     const u64 word = D.s.packed[v];
     if (bits > 0) {
       if (word == PACKED_INF) {
@@ -1050,12 +1058,12 @@ __device__ void recover_edges(Desc *objs, int count, int n, int source, Threads 
 // recover_staged: one record per vertex instead of K arrays.
 __device__ void unpack_staged(Desc *objs, int count, int n, int source, Threads t,
                               u64 *stage) {
+  for (int k = 0; k < count; ++k) {
+    clear_roots(objs[k], t);
+  }
   for (int v = t.id; v < n; v += t.count) {
     for (int k = 0; k < count; ++k) {
       Desc &D = objs[k];
-      if (D.nChanged > 0 && D.s.flag != nullptr) {
-        D.s.flag[v] = 0;
-      }
       const u64 word = D.s.packed[v];
       stage[static_cast<size_t>(v) * count + k] = word;
       D.distOut[v] = word == PACKED_INF ? DISTANCE_INF : static_cast<int64_t>(word);
@@ -1154,23 +1162,47 @@ __device__ void scratch_init(Desc &D, ObjCtl &c, int n, int source,
 // Step 1 of the update for every objective (src/sospUpdateGpu.cu:233-344)
 // ---------------------------------------------------------------------------
 
-// One pointer-jumping step for vertex v.  Invariant for every x: flag[x] == 1
-// implies a root among x and its ancestors; flag[x] == 0 implies no root on
-// the tree path from x up to (excluding) ancestor[x].  Only v's thread writes
-// (ancestor[v], flag[v]) and every value another thread can observe keeps the
-// invariant, so the update is in place.  Returns true if v still jumps.
-__device__ __forceinline__ bool jump(int *anc, int *flag, int v) {
-  const int x = anc[v];
-  if (x < 0 || flag[v]) {
+// Pointer jumping over one word per vertex: bit 31 = "a root is among this
+// vertex and its ancestors" (JUMP_FLAG), bits 0-30 = ancestor + 1 (0: none).
+// Invariant for every x: a flagged word implies a root among x and its
+// ancestors; an unflagged one, no root on the tree path from x up to
+// (excluding) its ancestor.  Only v's thread writes word[v] and every word
+// another thread can observe keeps the invariant, so the update is in place;
+// one 32-bit word carries both fields, so a read sees a consistent pair.
+constexpr int JUMP_FLAG = INT_MIN;
+
+// Round 0, every vertex: roots[] marks the roots (heads of changed tree
+// edges), which get their flag here.  Returns true if v still jumps.
+__device__ __forceinline__ bool jump_first(int *word, const int *roots, int v) {
+  const int w = word[v];
+  if (roots[v]) {
+    word[v] = w | JUMP_FLAG;
     return false;
   }
-  if (flag[x]) {
-    flag[v] = 1;
+  if (w == 0) {
     return false;
   }
-  const int next = anc[x];
-  anc[v] = next;
-  return next >= 0;
+  const int x = w - 1;
+  const int wx = word[x];
+  if (roots[x] || (wx & JUMP_FLAG)) {
+    word[v] = w | JUMP_FLAG;
+    return false;
+  }
+  word[v] = wx;
+  return wx != 0;
+}
+
+// Later rounds, vertices still jumping (unflagged, with an ancestor): every
+// root carries its flag by now, so one read of the ancestor's word decides.
+__device__ __forceinline__ bool jump_next(int *word, int v) {
+  const int w = word[v];
+  const int wx = word[w - 1];
+  if (wx & JUMP_FLAG) {
+    word[v] = w | JUMP_FLAG;
+    return false;
+  }
+  word[v] = wx;
+  return wx != 0;
 }
 
 template <bool HUBS>
@@ -1202,7 +1234,7 @@ __device__ int update_step1(Args &a, GridBarrier &grid, Threads t,
       }
       D.s.packed[v] = word;
       if (D.nChanged > 0) {
-        D.s.ancestor[v] = p;
+        D.s.ancestor[v] = p + 1;  // jump word: ancestor + 1, no flag
       }
     }
     if (overflow) {
@@ -1244,7 +1276,7 @@ __device__ int update_step1(Args &a, GridBarrier &grid, Threads t,
           const unsigned i = base + threadIdx.x;
           if (i < cnt) {
             const int v = round == 0 ? static_cast<int>(i) : inList[i];
-            if (jump(anc, flag, v)) {
+            if (round == 0 ? jump_first(anc, flag, v) : jump_next(anc, v)) {
               queue_push(sh.near, v, outList, outCount);
             }
           }
@@ -1268,15 +1300,15 @@ __device__ int update_step1(Args &a, GridBarrier &grid, Threads t,
   }
   stamp(ctl, PHASE_JUMP, t.id);
 
-  // Invalidate flagged vertices (INF); mark insert heads (stamp).  Flags stay
-  // set until the unpack sweep clears them.
+  // Invalidate flagged vertices (INF); mark insert heads (stamp).  The jump
+  // words keep the flags until the next update's pack pass rewrites them.
   const int genInv = ++generation;
   for (int k = 0; k < a.nobj; ++k) {
     Desc &D = desc(a.objs)[k];
     unsigned count = 0;
     if (D.nChanged > 0) {
       for (int v = t.id; v < n; v += t.count) {
-        if (D.s.flag[v]) {
+        if (D.s.ancestor[v] & JUMP_FLAG) {
           D.s.packed[v] = PACKED_INF;
           ++count;
         }
@@ -1336,7 +1368,7 @@ __device__ int update_step1(Args &a, GridBarrier &grid, Threads t,
          base += t.count) {
       const int v = static_cast<int>(base + threadIdx.x);
       bool want = v < n && v != a.source &&
-                  ((D.nChanged > 0 && D.s.flag[v]) || D.s.stamp[v] == genInv);
+                  ((D.nChanged > 0 && (D.s.ancestor[v] & JUMP_FLAG)) || D.s.stamp[v] == genInv);
       u64 current = PACKED_INF;
       int begin = 0, end = 0;
       if (want) {
