@@ -258,6 +258,11 @@ def bench_rmat(args) -> int:
                    res["combined_edges"])
     k_ms = float(np.mean(kernel_ms))
     achieved = ab / (k_ms * 1e-3) / 1e9
+    # Measured HBM bytes per launch (rocprofv3 FETCH_SIZE / WRITE_SIZE, scale 22).
+    traffic = None
+    tp = ROOT / "profiles" / "traffic_rmat22_k5.json"
+    if args.scale == 22 and tp.exists():
+        traffic = json.loads(tp.read_text()).get("bytes_per_launch")
     line = {
         "metric": "changes/s, power-law stress update (R-MAT scale 22, K=5, distance-only "
                   "fallback)",
@@ -274,7 +279,7 @@ def bench_rmat(args) -> int:
                    "packed_parents": [s["packed_parents"] for s in res["stats"]],
                    "max_out_degree": int(deg.max())},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
-                     "frac": achieved / PEAK_HBM_GBS, "traffic": None, "alg_bytes": ab,
+                     "frac": achieved / PEAK_HBM_GBS, "traffic": traffic, "alg_bytes": ab,
                      "kernel": "sosp_persistent (hub variant, one launch per update)"},
     }
     sess.close()

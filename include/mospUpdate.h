@@ -23,6 +23,9 @@ struct MospOptions {
   // Step 3's near-far width = the default rule times this (>= 1).  Tuning
   // only (no reference counterpart); outputs do not depend on it.
   int combinedDeltaScale = 1;
+  // With delta <= 0: each objective's width = max(1, trunc(default rule *
+  // deltaScale)).  Tuning only (no reference counterpart), as above.
+  double deltaScale = 1.0;
 };
 
 struct MospTimings {

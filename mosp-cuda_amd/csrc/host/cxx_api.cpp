@@ -404,9 +404,12 @@ bool mospUpdate(const CsrGraph &original, ChangeBatch &batch,
   std::vector<long long> delta(K);
   std::vector<mosp_changes> changes(K);
   for (int k = 0; k < K; ++k) {
-    delta[k] = options.delta > 0 ? options.delta
-                                 : defaultDelta(m > 0 ? static_cast<long long>(m) : edges, n,
-                                                weightSum[k]);
+    const long long rule =
+        defaultDelta(m > 0 ? static_cast<long long>(m) : edges, n, weightSum[k]);
+    delta[k] = options.delta > 0
+                   ? options.delta
+                   : std::max(1LL, static_cast<long long>(static_cast<double>(rule) *
+                                                          std::max(options.deltaScale, 0.0)));
     changes[k] = mosp_changes{dFrom[k].data(), dTo[k].data(), static_cast<int32_t>(chFrom[k].size()),
                               dHeads.data(), batch.numberOfInserts()};
   }

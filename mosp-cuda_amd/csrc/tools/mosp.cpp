@@ -43,6 +43,7 @@ struct Options {
   std::vector<int> pref;
   int K = 0, source = 0;
   long long delta = 0;
+  double deltaScale = 1.0;
   int combDeltaScale = 1;
   bool canonicalize = false, writeOutput = true, validate = false, quiet = false;
   bool serial = false;
@@ -53,7 +54,7 @@ void usage() {
                "            [-k K] [--source s] [--pref p1,..,pK] [--delta D]\n"
                "            [--cache file] [--canonicalize] [--out dir]\n"
                "            [--no-output] [--validate] [--timing file.csv]\n"
-               "            [--quiet] [--serial] [--comb-delta-scale S]\n";
+               "            [--quiet] [--serial] [--comb-delta-scale S] [--delta-scale F]\n";
 }
 
 bool parse(int argc, char **argv, Options &o) {
@@ -93,6 +94,17 @@ bool parse(int argc, char **argv, Options &o) {
     } else if (a == "--delta") {
       if (!integer(1, LLONG_MAX)) return false;
       o.delta = x;
+    } else if (a == "--delta-scale") {
+      if (!next(v)) return false;
+      try {
+        o.deltaScale = std::stod(v);
+      } catch (...) {
+        o.deltaScale = -1;
+      }
+      if (!(o.deltaScale > 0)) {
+        std::cerr << "--delta-scale needs a positive number, got '" << v << "'\n";
+        return false;
+      }
     } else if (a == "--comb-delta-scale") {
       if (!integer(1, INT_MAX)) return false;
       o.combDeltaScale = static_cast<int>(x);
@@ -307,6 +319,7 @@ int main(int argc, char **argv) {
   options.numberOfObjectives = K;
   options.preferences = opt.pref;
   options.delta = opt.delta;
+  options.deltaScale = opt.deltaScale;
   options.fused = !opt.serial;
   options.combinedDeltaScale = opt.combDeltaScale;
   CsrGraph updated;
