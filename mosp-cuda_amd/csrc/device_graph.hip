@@ -410,38 +410,82 @@ apply_copy_kernel(int n, int K, const int *oldRow, const int *oldCol, const int 
   }
 }
 
-// Changed rows: one wave compacts the alive original entries in order, then
-// writes the appended ones at their pairs' places.
-__global__ void apply_compact_kernel(int R, int K, const int *rows, const int *rowPairs,
-                                     const int *oldRow, const int *oldCol, const int *oldW,
-                                     long long oldStride, const int *slot, const int *pairV,
-                                     const int *pairLast, const int *pairApp,
-                                     const int *pairAppPos, const int *insW, const int *newRow,
-                                     int *newCol, int *newW, long long newStride) {
-  const int r = static_cast<int>(blockIdx.x);  // one wave per workgroup
+// Changed rows: one workgroup per row compacts the alive original entries in
+// order, COMPACT_PASS entries a pass (each lane holds COMPACT_PER_THREAD,
+// loaded together; per-wave alive counts in LDS give every entry its place),
+// then writes the appended ones at their pairs' places.  A power-law hub row
+// of tens of thousands of entries is a few dozen passes, not hundreds of one
+// wave's dependent ones.
+constexpr int COMPACT_THREADS = 256;
+constexpr int COMPACT_PER_THREAD = 4;
+constexpr int COMPACT_WAVES = COMPACT_THREADS / 64;
+constexpr int COMPACT_PASS = COMPACT_THREADS * COMPACT_PER_THREAD;
+
+__global__ void __launch_bounds__(COMPACT_THREADS)
+apply_compact_kernel(int R, int K, const int *__restrict__ rows,
+                     const int *__restrict__ rowPairs, const int *__restrict__ oldRow,
+                     const int *__restrict__ oldCol, const int *__restrict__ oldW,
+                     long long oldStride, const int *__restrict__ slot,
+                     const int *__restrict__ pairV, const int *__restrict__ pairLast,
+                     const int *__restrict__ pairApp, const int *__restrict__ pairAppPos,
+                     const int *__restrict__ insW, const int *__restrict__ newRow,
+                     int *__restrict__ newCol, int *__restrict__ newW, long long newStride) {
+  // Alive counts of a pass, (entry group j, wave) in entry order; two
+  // buffers, so a pass's counts are not overwritten while a slower wave
+  // still reads the previous pass's.
+  __shared__ int counts[2][COMPACT_PER_THREAD * COMPACT_WAVES];
+  const int r = static_cast<int>(blockIdx.x);
   if (r >= R) {
     return;
   }
   const int lane = static_cast<int>(__lane_id());
+  const int wave = static_cast<int>(threadIdx.x) / 64;
   const int u = rows[r];
   const int ob = oldRow[u], oe = oldRow[u + 1];
   int out = newRow[u];
   const unsigned long long below = (1ull << lane) - 1ull;
-  for (int base = ob; base < oe; base += 64) {
-    const int e = base + lane;
-    const int s = e < oe ? slot[e] : DEAD;
-    const unsigned long long alive = __ballot(s != DEAD);
-    if (s != DEAD) {
-      const int pos = out + __popcll(alive & below);
-      newCol[pos] = oldCol[e];
-      for (int k = 0; k < K; ++k) {
-        newW[k * newStride + pos] =
-            s == KEEP ? oldW[k * oldStride + e] : insW[static_cast<size_t>(s) * K + k];
+  int buf = 0;
+  for (int base = ob; base < oe; base += COMPACT_PASS, buf ^= 1) {
+    int s[COMPACT_PER_THREAD], col[COMPACT_PER_THREAD], rank[COMPACT_PER_THREAD];
+    for (int j = 0; j < COMPACT_PER_THREAD; ++j) {
+      const int e = base + j * COMPACT_THREADS + static_cast<int>(threadIdx.x);
+      s[j] = e < oe ? slot[e] : DEAD;
+      col[j] = e < oe ? oldCol[e] : 0;
+    }
+    for (int j = 0; j < COMPACT_PER_THREAD; ++j) {
+      const unsigned long long alive = __ballot(s[j] != DEAD);
+      rank[j] = __popcll(alive & below);
+      if (lane == 0) {
+        counts[buf][j * COMPACT_WAVES + wave] = __popcll(alive);
       }
     }
-    out += __popcll(alive);
+    __syncthreads();
+    int before[COMPACT_PER_THREAD];
+    int total = 0;
+    for (int i = 0; i < COMPACT_PER_THREAD * COMPACT_WAVES; ++i) {
+      for (int j = 0; j < COMPACT_PER_THREAD; ++j) {
+        if (i == j * COMPACT_WAVES + wave) {
+          before[j] = total;
+        }
+      }
+      total += counts[buf][i];
+    }
+    for (int j = 0; j < COMPACT_PER_THREAD; ++j) {
+      if (s[j] == DEAD) {
+        continue;
+      }
+      const int e = base + j * COMPACT_THREADS + static_cast<int>(threadIdx.x);
+      const int pos = out + before[j] + rank[j];
+      newCol[pos] = col[j];
+      for (int k = 0; k < K; ++k) {
+        newW[k * newStride + pos] =
+            s[j] == KEEP ? oldW[k * oldStride + e] : insW[static_cast<size_t>(s[j]) * K + k];
+      }
+    }
+    out += total;
   }
-  for (int p = rowPairs[r] + lane; p < rowPairs[r + 1]; p += 64) {
+  for (int p = rowPairs[r] + static_cast<int>(threadIdx.x); p < rowPairs[r + 1];
+       p += COMPACT_THREADS) {
     if (pairApp[p] != 0) {
       const int pos = out + pairAppPos[p];
       newCol[pos] = pairV[p];
@@ -861,9 +905,9 @@ extern "C" int mosp_device_graph_apply(const mosp_device_graph *g, int32_t nIns,
                      g->outRow, g->outCol, g->outW, g->stride, rowTag, out->outRow, out->outCol,
                      out->outW, out->stride);
   if (R > 0) {
-    hipLaunchKernelGGL(apply_compact_kernel, dim3(R), dim3(64), 0, stream, R, K, dRows,
-                       dRowPairs, g->outRow, g->outCol, g->outW, g->stride, slot, pV, pLast,
-                       pairApp, pairAppPos, dInsW, out->outRow, out->outCol, out->outW,
+    hipLaunchKernelGGL(apply_compact_kernel, dim3(R), dim3(COMPACT_THREADS), 0, stream, R, K,
+                       dRows, dRowPairs, g->outRow, g->outCol, g->outW, g->stride, slot, pV,
+                       pLast, pairApp, pairAppPos, dInsW, out->outRow, out->outCol, out->outW,
                        out->stride);
   }
   HIP_TRY(hipGetLastError());

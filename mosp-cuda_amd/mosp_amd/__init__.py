@@ -550,6 +550,20 @@ class DeviceGraph:
         _check(lib().mosp_device_graph_view(self.handle, k, C.byref(out), C.byref(inn)), "view")
         return out, inn
 
+    def download_reverse(self, k: int) -> tuple[np.ndarray, np.ndarray, np.ndarray]:
+        """The in-CSR built on the device: (row pointers, sources, objective k's
+        weights); the order inside an in-row is unspecified."""
+        _, inn = self.view(k)
+        m = int(lib().mosp_device_graph_edges(self.handle))
+        row = np.zeros(self.n + 1, np.int32)
+        col = np.zeros(max(m, 1), np.int32)
+        w = np.zeros(max(m, 1), np.int32)
+        _check(lib().mosp_memcpy(row.ctypes.data, inn.row_ptr, row.nbytes, 2), "copy in-rows")
+        if m > 0:
+            _check(lib().mosp_memcpy(col.ctypes.data, inn.col_ind, 4 * m, 2), "copy in-columns")
+            _check(lib().mosp_memcpy(w.ctypes.data, inn.weights, 4 * m, 2), "copy in-weights")
+        return row, col[:m], w[:m]
+
     def close(self) -> None:
         if self.handle:
             lib().mosp_device_graph_free(self.handle)

@@ -99,6 +99,59 @@ def test_grid_and_rmat_like_hubs():
     check_apply(hub_graph, b, "hubs")
 
 
+def check_reverse(dg, g, tag):
+    """The device in-CSR of every objective against the host transpose:
+    identical in-row pointers, and each in-row holds the same (source,
+    weight) entries (their order inside a row is unspecified)."""
+    deg = np.bincount(g.col, minlength=g.n)
+    want_row = np.concatenate([[0], np.cumsum(deg)]).astype(np.int64)
+    src = np.repeat(np.arange(g.n), np.diff(g.row))
+    w = np.asarray(g.w).reshape(-1, g.K) if g.m else np.zeros((0, g.K), np.int64)
+    for k in range(g.K):
+        row, col, wk = dg.download_reverse(k)
+        assert np.array_equal(row.astype(np.int64), want_row), f"{tag} obj{k} in-row pointers"
+        tgt = np.repeat(np.arange(g.n), np.diff(row))
+        got = np.lexsort((wk, col, tgt))
+        want = np.lexsort((w[:, k], src, g.col))
+        assert np.array_equal(col[got], src[want]), f"{tag} obj{k} in-row sources"
+        assert np.array_equal(wk[got], w[want, k]), f"{tag} obj{k} in-row weights"
+
+
+def test_reverse_csr_matches_host_transpose():
+    """The device in-CSR (in-degree atomics, scan, scatter) at upload and
+    after an apply: in-hubs with thousands of in-edges, a graph whose rows
+    are nearly all empty, and the R-MAT generator's hubs."""
+    rng = np.random.default_rng(21)
+    n, K = 5000, 2
+    us, vs = [], []
+    for u in range(n):                      # every vertex points at the two in-hubs
+        us += [u, u]
+        vs += [0 if u != 0 else 1, 7 if u != 7 else 8]
+    for u in rng.choice(n, 300, replace=False):   # a few scattered rows, most rows small
+        for v in rng.choice(n, 5, replace=False):
+            if v != u and v not in (0, 7):
+                us.append(int(u))
+                vs.append(int(v))
+    ws = rng.integers(1, 1000, size=(len(us), K))
+    in_hubs = gen._csr(n, K, us, vs, ws)
+    sparse_rows = gen._csr(20000, 1, [19990, 5, 5, 19999], [3, 19990, 4, 0],
+                           rng.integers(1, 9, size=(4, 1)))
+    mg = M().gen_rmat(14, 16, 3, seed=17, wmax=100)
+    rmat = O.Graph(mg.n, 3, i32(mg.row), i32(mg.col), i32(mg.w))
+    for tag, g in (("in-hubs", in_hubs), ("empty rows", sparse_rows), ("rmat14", rmat)):
+        say(f"case reverse {tag} n={g.n} m={g.m}")
+        dg = M().DeviceGraph(mgraph(g))
+        check_reverse(dg, g, tag + " upload")
+        b = gen.uniform_batch(g, min(400, max(g.m // 2, 2)), 50.0, seed=22)
+        spare = M().DeviceGraph()
+        dg.apply(mbatch(b), spare)
+        updated = spare.download()
+        check_reverse(spare, O.Graph(updated.n, updated.K, i32(updated.row), i32(updated.col),
+                                     i32(updated.w)), tag + " apply")
+        spare.close()
+        dg.close()
+
+
 def test_duplicates_repeats_and_cancelling_changes():
     K = 2
     # duplicate arcs 0->1 (x3) and 2->3 (x2)
