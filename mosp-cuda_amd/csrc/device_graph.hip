@@ -54,14 +54,6 @@ __global__ void split_weights_kernel(const int *edgeMajor, long long total, long
   columns[k * stride + e] = edgeMajor[i];
 }
 
-// In-degrees of the m = *edges arcs (edge count read on the device, so a
-// graph rebuilt on the device needs no host round trip).
-__global__ void in_degree_kernel(const int *colInd, const int *edges, int *degree) {
-  const int e = blockIdx.x * blockDim.x + threadIdx.x;
-  if (e < *edges) {
-    atomicAdd(&degree[colInd[e]], 1);
-  }
-}
 
 __global__ void copy_ints_kernel(int *dst, const int *src, int count) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -194,6 +186,18 @@ __device__ __forceinline__ bool entry_row(const int *rowPtr, int n, long long &e
   return true;
 }
 
+// Reverse CSR, direct: one atomic per out-entry on its target's in-degree,
+// then on its cursor.  Cheapest when no target is a hub (the atomics do not
+// contend); graphs with hubs use the bucketed transpose below.
+// In-degrees of the m = *edges arcs (edge count read on the device, so a
+// graph rebuilt on the device needs no host round trip).
+__global__ void in_degree_kernel(const int *colInd, const int *edges, int *degree) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e < *edges) {
+    atomicAdd(&degree[colInd[e]], 1);
+  }
+}
+
 // One thread per out-entry: place it in its target's in-row (order inside an
 // in-row follows the atomics; every consumer of the in-CSR is order-free).
 __global__ void __launch_bounds__(EDGE_BLOCK)
@@ -208,6 +212,256 @@ fill_reverse_kernel(int n, long long stride, int K, const int *rowPtr, const int
   inCol[pos] = u;
   for (int k = 0; k < K; ++k) {
     inW[k * stride + pos] = outW[k * stride + e];
+  }
+}
+
+// Reverse CSR by a bucketed transpose.  Targets fall in buckets of 2^bits
+// consecutive ids.  (1) Every chunk of out-entries counts its entries per
+// bucket in LDS; (2) a scan of the bucket-major (bucket, chunk) counts gives
+// every chunk its places in a record array sorted by bucket; (3) the chunk
+// scatters (target, source, weights) records there, a bucket's entries of a
+// chunk in one run; (4) in-degrees and (5) in-row places are then taken per
+// tile of bucket-sorted records with one LDS counter per target of the
+// tile's (at most TR_WINDOW) targets, so a target costs one global atomic per
+// tile instead of one per entry -- power-law hubs no longer serialise the
+// memory-side atomics, and a target's entries of a tile land in one run of
+// its in-row.  A tile whose records span more targets (sparse buckets) uses
+// one global atomic per record.  The order inside an in-row is unspecified;
+// every consumer of the in-CSR is order-free.
+constexpr int TR_THREADS = 256;
+constexpr int TR_MAX_BUCKETS = 8192;
+constexpr int TR_MIN_BITS = 11;
+constexpr int TR_WINDOW = 4096;         // targets one tile can count in LDS
+constexpr int TR_TILE = 16384;          // records per tile of passes (4) and (5)
+constexpr int TR_ROW_WINDOW = 2048;     // row pointers staged per reload in (3)
+constexpr int TR_UNROLL = 8;            // entries / records per lane and pass, loads issued together
+
+// (1) counts[b * chunks + c] = entries of chunk c whose target is in bucket b.
+__global__ void __launch_bounds__(TR_THREADS)
+tr_count_kernel(const int *rowPtr, int n, const int *colInd, int bits, int buckets,
+                long long chunk, int chunks, int *counts) {
+  extern __shared__ int hist[];  // buckets counters
+  const long long m = rowPtr[n];
+  const int c = static_cast<int>(blockIdx.x);
+  for (int b = static_cast<int>(threadIdx.x); b < buckets; b += TR_THREADS) {
+    hist[b] = 0;
+  }
+  __syncthreads();
+  const long long c0 = c * chunk;
+  const long long c1 = std::min(c0 + chunk, m);
+  for (long long e = c0 + threadIdx.x; e < c1; e += TR_THREADS) {
+    atomicAdd(&hist[colInd[e] >> bits], 1);
+  }
+  __syncthreads();
+  for (int b = static_cast<int>(threadIdx.x); b < buckets; b += TR_THREADS) {
+    counts[static_cast<long long>(b) * chunks + c] = hist[b];
+  }
+}
+
+// (3) Records of chunk c at places[b * chunks + c] onward, per bucket b.
+// Rows come from a window of row pointers staged in LDS, reloaded when the
+// passes have used up its entries; a lane takes TR_UNROLL entries a pass,
+// every load issued before the first use.
+__global__ void __launch_bounds__(TR_THREADS)
+tr_scatter_kernel(const int *rowPtr, int n, const int *colInd, const int *outW,
+                  long long stride, int K, int bits, int buckets, long long chunk, int chunks,
+                  const int *places, long long recCap, int *recT, int *recU, int *recW) {
+  extern __shared__ int lds[];  // buckets next-places, then TR_ROW_WINDOW row pointers
+  int *const next = lds;
+  int *const rows = lds + buckets;
+  const long long m = rowPtr[n];
+  const int c = static_cast<int>(blockIdx.x);
+  const long long c0 = c * chunk;
+  if (c0 >= m) {
+    return;
+  }
+  const long long c1 = std::min(c0 + chunk, m);
+  for (int b = static_cast<int>(threadIdx.x); b < buckets; b += TR_THREADS) {
+    next[b] = places[static_cast<long long>(b) * chunks + c];
+  }
+  int want = row_of(rowPtr, 0, n - 1, c0);  // rowPtr[want] <= base: the next window
+  int first = -1, count = 0;
+  long long end = 0;
+  for (long long base = c0; base < c1;) {
+    if (want != first) {
+      __syncthreads();  // (first pass: the bucket places above are written)
+      first = want;
+      count = n - first < TR_ROW_WINDOW ? n - first : TR_ROW_WINDOW;
+      for (int i = static_cast<int>(threadIdx.x); i < count; i += TR_THREADS) {
+        rows[i] = rowPtr[first + i];
+      }
+      end = rowPtr[first + count];
+      __syncthreads();
+    }
+    long long stop = std::min(base + TR_THREADS * TR_UNROLL, c1);
+    stop = std::min(stop, end);
+    int t[TR_UNROLL], u[TR_UNROLL], at[TR_UNROLL];
+    for (int j = 0; j < TR_UNROLL; ++j) {
+      const long long e = base + j * TR_THREADS + threadIdx.x;
+      t[j] = e < stop ? colInd[e] : -1;
+    }
+    for (int j = 0; j < TR_UNROLL; ++j) {
+      const long long e = base + j * TR_THREADS + threadIdx.x;
+      if (t[j] < 0) {
+        continue;
+      }
+      int a = 0, b = count - 1;
+      while (a < b) {
+        const int mid = a + (b - a + 1) / 2;
+        if (rows[mid] <= e) {
+          a = mid;
+        } else {
+          b = mid - 1;
+        }
+      }
+      u[j] = first + a;
+      at[j] = atomicAdd(&next[t[j] >> bits], 1);
+      recT[at[j]] = t[j];
+      recU[at[j]] = u[j];
+    }
+    for (int k = 0; k < K; ++k) {
+      int w[TR_UNROLL];
+      for (int j = 0; j < TR_UNROLL; ++j) {
+        const long long e = base + j * TR_THREADS + threadIdx.x;
+        w[j] = t[j] >= 0 ? outW[k * stride + e] : 0;
+      }
+      for (int j = 0; j < TR_UNROLL; ++j) {
+        if (t[j] >= 0) {
+          recW[k * recCap + at[j]] = w[j];
+        }
+      }
+    }
+    if (stop == end) {
+      want = first + count;  // rowPtr[first + count] == end == the next base
+    }
+    base = stop;
+  }
+}
+
+// The buckets of a tile of bucket-sorted records: those of its first and
+// last records; its targets lie in [lo, lo + width) (tr_tile_targets).
+__device__ __forceinline__ void tr_tile_span(const int *recT, long long i0, long long i1,
+                                             int bits, int *span) {
+  if (threadIdx.x == 0) {
+    span[0] = recT[i0] >> bits;
+    span[1] = recT[i1 - 1] >> bits;
+  }
+}
+
+__device__ __forceinline__ void tr_tile_targets(const int *span, int bits, int &lo,
+                                                long long &width) {
+  lo = span[0] << bits;
+  width = static_cast<long long>(span[1] - span[0] + 1) << bits;
+}
+
+// (4) In-degrees (degree[] zeroed; m read from rowPtr[n]).
+__global__ void __launch_bounds__(TR_THREADS)
+tr_degree_kernel(const int *rowPtr, int n, const int *recT, int bits, int *degree) {
+  __shared__ int counts[TR_WINDOW];
+  __shared__ int span[2];
+  const long long m = rowPtr[n];
+  const long long i0 = static_cast<long long>(blockIdx.x) * TR_TILE;
+  if (i0 >= m) {
+    return;
+  }
+  const long long i1 = std::min(i0 + TR_TILE, m);
+  tr_tile_span(recT, i0, i1, bits, span);
+  for (int j = static_cast<int>(threadIdx.x); j < TR_WINDOW; j += TR_THREADS) {
+    counts[j] = 0;
+  }
+  __syncthreads();
+  int lo;
+  long long width;
+  tr_tile_targets(span, bits, lo, width);
+  const bool local = width <= TR_WINDOW;
+  for (long long i = i0 + threadIdx.x; i < i1; i += TR_THREADS * TR_UNROLL) {
+    int t[TR_UNROLL];
+    for (int j = 0; j < TR_UNROLL; ++j) {
+      const long long r = i + j * TR_THREADS;
+      t[j] = r < i1 ? recT[r] : -1;
+    }
+    for (int j = 0; j < TR_UNROLL; ++j) {
+      if (t[j] < 0) {
+        continue;
+      }
+      if (local) {
+        atomicAdd(&counts[t[j] - lo], 1);
+      } else {
+        atomicAdd(&degree[t[j]], 1);
+      }
+    }
+  }
+  __syncthreads();
+  if (local) {
+    for (int j = static_cast<int>(threadIdx.x); j < width; j += TR_THREADS) {
+      if (counts[j] > 0) {
+        atomicAdd(&degree[lo + j], counts[j]);
+      }
+    }
+  }
+}
+
+// (5) In-row places: a tile reserves each target's run with one atomic on
+// the target's cursor (the in-row starts, advanced by reservations), then
+// hands out the run's places with LDS atomics.
+__global__ void __launch_bounds__(TR_THREADS)
+tr_place_kernel(const int *rowPtr, int n, int K, const int *recT, const int *recU,
+                const int *recW, long long recCap, int bits, int *cursor, int *inCol, int *inW,
+                long long stride) {
+  __shared__ int slots[TR_WINDOW];  // counts, then each target's next place
+  __shared__ int span[2];
+  const long long m = rowPtr[n];
+  const long long i0 = static_cast<long long>(blockIdx.x) * TR_TILE;
+  if (i0 >= m) {
+    return;
+  }
+  const long long i1 = std::min(i0 + TR_TILE, m);
+  tr_tile_span(recT, i0, i1, bits, span);
+  for (int j = static_cast<int>(threadIdx.x); j < TR_WINDOW; j += TR_THREADS) {
+    slots[j] = 0;
+  }
+  __syncthreads();
+  int lo;
+  long long width;
+  tr_tile_targets(span, bits, lo, width);
+  const bool local = width <= TR_WINDOW;
+  if (local) {
+    for (long long i = i0 + threadIdx.x; i < i1; i += TR_THREADS) {
+      atomicAdd(&slots[recT[i] - lo], 1);
+    }
+    __syncthreads();
+    for (int j = static_cast<int>(threadIdx.x); j < width; j += TR_THREADS) {
+      if (slots[j] > 0) {
+        slots[j] = atomicAdd(&cursor[lo + j], slots[j]);
+      }
+    }
+    __syncthreads();
+  }
+  for (long long i = i0 + threadIdx.x; i < i1; i += TR_THREADS * TR_UNROLL) {
+    int t[TR_UNROLL], u[TR_UNROLL], pos[TR_UNROLL];
+    for (int j = 0; j < TR_UNROLL; ++j) {
+      const long long r = i + j * TR_THREADS;
+      t[j] = r < i1 ? recT[r] : -1;
+      u[j] = r < i1 ? recU[r] : 0;
+    }
+    for (int j = 0; j < TR_UNROLL; ++j) {
+      if (t[j] >= 0) {
+        pos[j] = local ? atomicAdd(&slots[t[j] - lo], 1) : atomicAdd(&cursor[t[j]], 1);
+        inCol[pos[j]] = u[j];
+      }
+    }
+    for (int k = 0; k < K; ++k) {
+      int w[TR_UNROLL];
+      for (int j = 0; j < TR_UNROLL; ++j) {
+        const long long r = i + j * TR_THREADS;
+        w[j] = t[j] >= 0 ? recW[k * recCap + r] : 0;
+      }
+      for (int j = 0; j < TR_UNROLL; ++j) {
+        if (t[j] >= 0) {
+          inW[k * stride + pos[j]] = w[j];
+        }
+      }
+    }
   }
 }
 
@@ -588,15 +842,49 @@ int reserve(mosp_device_graph *g, int n, long long m, int K) {
   return 1;
 }
 
-size_t reverse_bytes(int n) {
-  return Carve::bytes<int>(n + 1) + Carve::bytes<int>(scan_tiles(n + 1));
+// Bucket size, chunking and record capacity of the transpose of a graph with
+// n vertices and at most mBound entries.
+struct TransposePlan {
+  int bits;          // bucket of target t = t >> bits
+  int buckets;
+  long long chunk;   // out-entries per chunk of passes (1) and (3)
+  int chunks;
+  long long recCap;  // records (>= mBound)
+};
+
+TransposePlan transpose_plan(int n, long long mBound) {
+  TransposePlan p{};
+  p.bits = TR_MIN_BITS;
+  while (((static_cast<long long>(n) + (1LL << p.bits) - 1) >> p.bits) > TR_MAX_BUCKETS) {
+    ++p.bits;
+  }
+  p.buckets = static_cast<int>((static_cast<long long>(n) + (1LL << p.bits) - 1) >> p.bits);
+  // ~2048 chunks: enough workgroups to fill the GPU, few (bucket, chunk) counts.
+  long long chunk = std::max(4096LL, std::min(32768LL, (mBound + 2047) / 2048));
+  chunk = std::max(chunk, (mBound + (1LL << 20) - 1) >> 20);  // <= 2^20 chunks
+  p.chunk = (chunk + TR_THREADS - 1) / TR_THREADS * TR_THREADS;
+  p.chunks = static_cast<int>(std::max(1LL, (mBound + p.chunk - 1) / p.chunk));
+  p.recCap = std::max(mBound, 1LL);
+  return p;
 }
 
-// In-CSR (rows, columns, K weight columns) from the out-CSR, on `stream`;
-// temporaries at `at` (reverse_bytes(n)); mBound >= the edge count, which is
-// read from outRow[n] on the device.  Only kernels on one stream: each sees
-// its predecessors' writes.
-int build_reverse(mosp_device_graph *g, char *at, long long mBound, hipStream_t stream) {
+// Scratch bytes of build_reverse (hubs: the bucketed transpose, else the
+// direct build).
+size_t reverse_bytes(int n, long long mBound, int K, bool hubs) {
+  const size_t direct = Carve::bytes<int>(n + 1) + Carve::bytes<int>(scan_tiles(n + 1));
+  if (!hubs) {
+    return direct;
+  }
+  const TransposePlan p = transpose_plan(n, mBound);
+  const long long cells = static_cast<long long>(p.buckets) * p.chunks;
+  return Carve::bytes<int>(n + 1) * 2 + Carve::bytes<int>(scan_tiles(n + 1)) +
+         Carve::bytes<int>(cells) * 2 +
+         Carve::bytes<int>(scan_tiles(static_cast<int>(cells))) +
+         Carve::bytes<int>(p.recCap) * (2 + static_cast<size_t>(K));
+}
+
+// In-CSR by one atomic per entry (no hubs).
+int build_reverse_direct(mosp_device_graph *g, char *at, long long mBound, hipStream_t stream) {
   const int n = g->n, K = g->K;
   Carve c{at};
   int *cursor = c.take<int>(n + 1);
@@ -611,6 +899,54 @@ int build_reverse(mosp_device_graph *g, char *at, long long mBound, hipStream_t 
                      g->inRow, n);
   hipLaunchKernelGGL(fill_reverse_kernel, dim3(edge_grid(mBound)), dim3(EDGE_BLOCK), 0, stream,
                      n, g->stride, K, g->outRow, g->outCol, g->outW, cursor, g->inCol, g->inW);
+  HIP_TRY(hipGetLastError());
+  return 1;
+}
+
+// In-CSR (rows, columns, K weight columns) from the out-CSR, on `stream`;
+// temporaries at `at` (reverse_bytes(n, mBound, K, hubs)); mBound >= the
+// edge count, which is read from outRow[n] on the device.  Only kernels on
+// one stream: each sees its predecessors' writes.  `hubs`: some vertex has
+// more than HUB_DEGREE edges, so one atomic per entry would contend.
+int build_reverse(mosp_device_graph *g, char *at, long long mBound, bool hubs,
+                  hipStream_t stream) {
+  if (!hubs) {
+    return build_reverse_direct(g, at, mBound, stream);
+  }
+  const int n = g->n, K = g->K;
+  const TransposePlan p = transpose_plan(n, mBound);
+  const long long cells = static_cast<long long>(p.buckets) * p.chunks;
+  Carve c{at};
+  int *degree = c.take<int>(n + 1);
+  int *cursor = c.take<int>(n + 1);
+  int *sums = c.take<int>(scan_tiles(n + 1));
+  int *counts = c.take<int>(cells);
+  int *places = c.take<int>(cells);
+  int *cellSums = c.take<int>(scan_tiles(static_cast<int>(cells)));
+  int *recT = c.take<int>(p.recCap);
+  int *recU = c.take<int>(p.recCap);
+  int *recW = c.take<int>(p.recCap * std::max(K, 1));
+  const int tiles = static_cast<int>((p.recCap + TR_TILE - 1) / TR_TILE);
+  const size_t countLds = sizeof(int) * p.buckets;
+  const size_t scatterLds = sizeof(int) * (p.buckets + TR_ROW_WINDOW);
+  hipLaunchKernelGGL(tr_count_kernel, dim3(p.chunks), dim3(TR_THREADS), countLds, stream,
+                     g->outRow, n, g->outCol, p.bits, p.buckets, p.chunk, p.chunks, counts);
+  if (!exclusive_scan(counts, places, static_cast<int>(cells), cellSums, stream)) {
+    return 0;
+  }
+  hipLaunchKernelGGL(tr_scatter_kernel, dim3(p.chunks), dim3(TR_THREADS), scatterLds, stream,
+                     g->outRow, n, g->outCol, g->outW, g->stride, K, p.bits, p.buckets, p.chunk,
+                     p.chunks, places, p.recCap, recT, recU, recW);
+  hipLaunchKernelGGL(zero_kernel, dim3(grid_for(n + 1)), dim3(BLOCK), 0, stream, degree, n + 1);
+  hipLaunchKernelGGL(tr_degree_kernel, dim3(tiles), dim3(TR_THREADS), 0, stream, g->outRow, n,
+                     recT, p.bits, degree);
+  if (!exclusive_scan(degree, g->inRow, n + 1, sums, stream)) {
+    return 0;
+  }
+  hipLaunchKernelGGL(copy_ints_kernel, dim3(grid_for(n)), dim3(BLOCK), 0, stream, cursor,
+                     g->inRow, n);
+  hipLaunchKernelGGL(tr_place_kernel, dim3(tiles), dim3(TR_THREADS), 0, stream, g->outRow, n, K,
+                     recT, recU, recW, p.recCap, p.bits, cursor, g->inCol, g->inW, g->stride);
   HIP_TRY(hipGetLastError());
   return 1;
 }
@@ -662,7 +998,12 @@ extern "C" mosp_device_graph *mosp_device_graph_upload(int32_t n, int32_t K, con
     g->m = m;
     g->K = K;
     const size_t wBytes = Carve::bytes<int>(static_cast<size_t>(m) * K);
-    if (!scratch_reserve(g, wBytes + reverse_bytes(n))) return 0;
+    int maxOut = 0;
+    for (int v = 0; v < n; ++v) {
+      maxOut = std::max(maxOut, rowPtr[v + 1] - rowPtr[v]);
+    }
+    const bool hubs = maxOut > HUB_DEGREE;
+    if (!scratch_reserve(g, wBytes + reverse_bytes(n, m, K, hubs))) return 0;
     int *edgeMajor = reinterpret_cast<int *>(g->scratch);
     HIP_TRY(hipMemcpy(g->outRow, rowPtr, sizeof(int) * (n + 1), hipMemcpyHostToDevice));
     if (m > 0) {
@@ -673,7 +1014,7 @@ extern "C" mosp_device_graph *mosp_device_graph_upload(int32_t n, int32_t K, con
     const long long total = static_cast<long long>(m) * K;
     hipLaunchKernelGGL(split_weights_kernel, dim3(grid_for(total)), dim3(BLOCK), 0, 0, edgeMajor,
                        total, g->stride, K, g->outW);
-    if (!build_reverse(g, g->scratch + wBytes, m, 0)) return 0;
+    if (!build_reverse(g, g->scratch + wBytes, m, hubs, 0)) return 0;
     if (!ensure_stats(g) ||
         !graph_max_degree(g->outRow, g->inRow, n, degree_word(g), 0, &g->maxDegree)) {
       return 0;
@@ -860,6 +1201,9 @@ extern "C" int mosp_device_graph_apply(const mosp_device_graph *g, int32_t nIns,
     hp[6 * P + i] = ik[i].idx;
   }
   const long long mOld = g->m;
+  // The reverse build's method follows the graph's degrees before the batch
+  // (a batch changes them little).
+  const bool hubs = g->maxDegree > HUB_DEGREE;
   if (!reserve(out, n, mOld + nIns, K)) return 0;
   // Temporaries in the destination graph's scratch (no allocation once it
   // is large enough).
@@ -869,7 +1213,7 @@ extern "C" int mosp_device_graph_apply(const mosp_device_graph *g, int32_t nIns,
                        Carve::bytes<int>(static_cast<size_t>(std::max(mOld, 1LL))) +
                        Carve::bytes<int>(n) + Carve::bytes<int>(n + 1) +
                        Carve::bytes<int>(scan_tiles(n + 1)) + Carve::bytes<unsigned>(nIns) +
-                       reverse_bytes(n);
+                       reverse_bytes(n, mOld + nIns, K, hubs);
   if (!scratch_reserve(out, bytes)) return 0;
   Carve c{out->scratch};
   int *dHost = c.take<int>(hostInts), *dInsW = c.take<int>(W);
@@ -911,7 +1255,7 @@ extern "C" int mosp_device_graph_apply(const mosp_device_graph *g, int32_t nIns,
                        out->stride);
   }
   HIP_TRY(hipGetLastError());
-  if (!build_reverse(out, reverseAt, mOld + nIns, stream)) return 0;
+  if (!build_reverse(out, reverseAt, mOld + nIns, hubs, stream)) return 0;
   if (!ensure_stats(out) ||
       !queue_max_degree(out->outRow, out->inRow, n, degree_word(out), stream)) {
     return 0;

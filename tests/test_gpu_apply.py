@@ -118,9 +118,11 @@ def check_reverse(dg, g, tag):
 
 
 def test_reverse_csr_matches_host_transpose():
-    """The device in-CSR (in-degree atomics, scan, scatter) at upload and
-    after an apply: in-hubs with thousands of in-edges, a graph whose rows
-    are nearly all empty, and the R-MAT generator's hubs."""
+    """The device in-CSR at upload and after an apply, by both methods (one
+    atomic per entry when no vertex has more than 32 edges, else the
+    bucketed transpose): in-hubs with thousands of in-edges (direct at
+    upload, transpose after it), a graph whose rows are nearly all empty,
+    and the R-MAT generator's hubs."""
     rng = np.random.default_rng(21)
     n, K = 5000, 2
     us, vs = [], []
@@ -134,8 +136,10 @@ def test_reverse_csr_matches_host_transpose():
                 vs.append(int(v))
     ws = rng.integers(1, 1000, size=(len(us), K))
     in_hubs = gen._csr(n, K, us, vs, ws)
-    sparse_rows = gen._csr(20000, 1, [19990, 5, 5, 19999], [3, 19990, 4, 0],
-                           rng.integers(1, 9, size=(4, 1)))
+    # nearly all rows empty, one row of 40 edges (so the transpose runs)
+    su = [19990, 5, 5, 19999] + [12000] * 40
+    sv = [3, 19990, 4, 0] + list(range(100, 140))
+    sparse_rows = gen._csr(20000, 1, su, sv, rng.integers(1, 9, size=(len(su), 1)))
     mg = M().gen_rmat(14, 16, 3, seed=17, wmax=100)
     rmat = O.Graph(mg.n, 3, i32(mg.row), i32(mg.col), i32(mg.w))
     for tag, g in (("in-hubs", in_hubs), ("empty rows", sparse_rows), ("rmat14", rmat)):
