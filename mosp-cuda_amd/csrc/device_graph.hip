@@ -159,31 +159,77 @@ __device__ __forceinline__ int row_of(const int *rowPtr, int lo, int hi, long lo
   return lo;
 }
 
-constexpr int EDGE_BLOCK = 256;  // threads per workgroup of the edge-parallel kernels
+// Edge-parallel kernels: a workgroup of SWEEP_THREADS lanes takes a
+// contiguous chunk of CSR entries, SWEEP_PASS at a time (SWEEP_UNROLL per
+// lane, their loads issued together).
+constexpr int SWEEP_THREADS = 256;
+constexpr int SWEEP_UNROLL = 8;
+constexpr int SWEEP_PASS = SWEEP_THREADS * SWEEP_UNROLL;
+constexpr int ROW_WINDOW = 2048;  // row pointers staged in LDS at a time
 
-// Entry e of this thread and its row, for one thread per entry of a CSR with
-// m = rowPtr[n] entries (read on the device): two lanes find the rows of the
-// workgroup's first and last entries, every lane searches between them.
-// Returns false for lanes past the end (the whole workgroup when it starts
-// past m, before the barrier, uniformly).
-__device__ __forceinline__ bool entry_row(const int *rowPtr, int n, long long &e, int &u) {
-  __shared__ int span[2];
-  const long long m = rowPtr[n];
-  const long long first = static_cast<long long>(blockIdx.x) * EDGE_BLOCK;
-  if (first >= m) {
-    return false;
+// Rows of a workgroup's entries [c0, c1): row pointers are staged in LDS,
+// ROW_WINDOW at a time, and reloaded only when the passes have used them up
+// (so a pass costs no global search); each entry's row is a binary search
+// there.  pass() and its barriers are workgroup-uniform.
+struct RowWindow {
+  int *rows;  // LDS, ROW_WINDOW ints
+  const int *rowPtr;
+  int n;
+  int want;   // rowPtr[want] <= the next pass's first entry
+  int first = -1, count = 0;
+  long long end = 0;  // the window's entries are [rows[0], end)
+
+  __device__ RowWindow(const int *rp, int nRows, long long c0, int *lds)
+      : rows(lds), rowPtr(rp), n(nRows), want(row_of(rp, 0, nRows - 1, c0)) {}
+
+  // Stages the window holding entry `base` if needed; returns the end of
+  // the pass starting there: min(base + span, c1, window end).
+  __device__ long long pass(long long base, long long span, long long c1) {
+    if (want != first) {
+      __syncthreads();  // the last window's searches are done
+      first = want;
+      count = n - first < ROW_WINDOW ? n - first : ROW_WINDOW;
+      for (int i = static_cast<int>(threadIdx.x); i < count; i += static_cast<int>(blockDim.x)) {
+        rows[i] = rowPtr[first + i];
+      }
+      end = rowPtr[first + count];
+      __syncthreads();
+    }
+    long long stop = std::min(base + span, c1);
+    stop = std::min(stop, end);
+    if (stop == end) {
+      want = first + count;  // rowPtr[first + count] == end == the next base
+    }
+    return stop;
   }
-  const long long last = std::min(first + EDGE_BLOCK, m) - 1;
-  if (threadIdx.x < 2) {
-    span[threadIdx.x] = row_of(rowPtr, 0, n - 1, threadIdx.x == 0 ? first : last);
+
+  // Index in the window of the row holding entry e (staged entries only).
+  __device__ int slot(long long e) const {
+    int a = 0, b = count - 1;
+    while (a < b) {
+      const int mid = a + (b - a + 1) / 2;
+      if (rows[mid] <= e) {
+        a = mid;
+      } else {
+        b = mid - 1;
+      }
+    }
+    return a;
   }
-  __syncthreads();
-  e = first + threadIdx.x;
-  if (e > last) {
-    return false;
-  }
-  u = row_of(rowPtr, span[0], span[1], e);
-  return true;
+};
+
+// Entries per chunk of an edge-parallel kernel over at most mBound entries:
+// ~2048 chunks (enough workgroups, few per-chunk counts), 4096-32768
+// entries each, at most 2^20 chunks.
+long long sweep_chunk(long long mBound) {
+  long long chunk = std::max(4096LL, std::min(32768LL, (mBound + 2047) / 2048));
+  chunk = std::max(chunk, (mBound + (1LL << 20) - 1) >> 20);
+  return (chunk + SWEEP_PASS - 1) / SWEEP_PASS * SWEEP_PASS;
+}
+
+int sweep_grid(long long mBound) {
+  const long long chunk = sweep_chunk(mBound);
+  return static_cast<int>(std::max(1LL, (mBound + chunk - 1) / chunk));
 }
 
 // Reverse CSR, direct: one atomic per out-entry on its target's in-degree,
@@ -198,20 +244,47 @@ __global__ void in_degree_kernel(const int *colInd, const int *edges, int *degre
   }
 }
 
-// One thread per out-entry: place it in its target's in-row (order inside an
-// in-row follows the atomics; every consumer of the in-CSR is order-free).
-__global__ void __launch_bounds__(EDGE_BLOCK)
+// Places every out-entry in its target's in-row (cursor: the in-rows'
+// starts, advanced by one atomic per entry; the order inside an in-row
+// follows the atomics, and every consumer of the in-CSR is order-free).
+__global__ void __launch_bounds__(SWEEP_THREADS)
 fill_reverse_kernel(int n, long long stride, int K, const int *rowPtr, const int *colInd,
-                    const int *outW, int *cursor, int *inCol, int *inW) {
-  long long e;
-  int u;
-  if (!entry_row(rowPtr, n, e, u)) {
+                    const int *outW, long long chunk, int *cursor, int *inCol, int *inW) {
+  __shared__ int rows[ROW_WINDOW];
+  const long long m = rowPtr[n];
+  const long long c0 = static_cast<long long>(blockIdx.x) * chunk;
+  if (c0 >= m) {
     return;
   }
-  const int pos = atomicAdd(&cursor[colInd[e]], 1);
-  inCol[pos] = u;
-  for (int k = 0; k < K; ++k) {
-    inW[k * stride + pos] = outW[k * stride + e];
+  const long long c1 = std::min(c0 + chunk, m);
+  RowWindow win(rowPtr, n, c0, rows);
+  for (long long base = c0; base < c1;) {
+    const long long stop = win.pass(base, SWEEP_PASS, c1);
+    int t[SWEEP_UNROLL], pos[SWEEP_UNROLL];
+    for (int j = 0; j < SWEEP_UNROLL; ++j) {
+      const long long e = base + j * SWEEP_THREADS + threadIdx.x;
+      t[j] = e < stop ? colInd[e] : -1;
+    }
+    for (int j = 0; j < SWEEP_UNROLL; ++j) {
+      if (t[j] >= 0) {
+        const long long e = base + j * SWEEP_THREADS + threadIdx.x;
+        pos[j] = atomicAdd(&cursor[t[j]], 1);
+        inCol[pos[j]] = win.first + win.slot(e);
+      }
+    }
+    for (int k = 0; k < K; ++k) {
+      int w[SWEEP_UNROLL];
+      for (int j = 0; j < SWEEP_UNROLL; ++j) {
+        const long long e = base + j * SWEEP_THREADS + threadIdx.x;
+        w[j] = t[j] >= 0 ? outW[k * stride + e] : 0;
+      }
+      for (int j = 0; j < SWEEP_UNROLL; ++j) {
+        if (t[j] >= 0) {
+          inW[k * stride + pos[j]] = w[j];
+        }
+      }
+    }
+    base = stop;
   }
 }
 
@@ -228,45 +301,40 @@ fill_reverse_kernel(int n, long long stride, int K, const int *rowPtr, const int
 // its in-row.  A tile whose records span more targets (sparse buckets) uses
 // one global atomic per record.  The order inside an in-row is unspecified;
 // every consumer of the in-CSR is order-free.
-constexpr int TR_THREADS = 256;
 constexpr int TR_MAX_BUCKETS = 8192;
 constexpr int TR_MIN_BITS = 11;
 constexpr int TR_WINDOW = 4096;         // targets one tile can count in LDS
 constexpr int TR_TILE = 16384;          // records per tile of passes (4) and (5)
-constexpr int TR_ROW_WINDOW = 2048;     // row pointers staged per reload in (3)
-constexpr int TR_UNROLL = 8;            // entries / records per lane and pass, loads issued together
 
 // (1) counts[b * chunks + c] = entries of chunk c whose target is in bucket b.
-__global__ void __launch_bounds__(TR_THREADS)
+__global__ void __launch_bounds__(SWEEP_THREADS)
 tr_count_kernel(const int *rowPtr, int n, const int *colInd, int bits, int buckets,
                 long long chunk, int chunks, int *counts) {
   extern __shared__ int hist[];  // buckets counters
   const long long m = rowPtr[n];
   const int c = static_cast<int>(blockIdx.x);
-  for (int b = static_cast<int>(threadIdx.x); b < buckets; b += TR_THREADS) {
+  for (int b = static_cast<int>(threadIdx.x); b < buckets; b += SWEEP_THREADS) {
     hist[b] = 0;
   }
   __syncthreads();
   const long long c0 = c * chunk;
   const long long c1 = std::min(c0 + chunk, m);
-  for (long long e = c0 + threadIdx.x; e < c1; e += TR_THREADS) {
+  for (long long e = c0 + threadIdx.x; e < c1; e += SWEEP_THREADS) {
     atomicAdd(&hist[colInd[e] >> bits], 1);
   }
   __syncthreads();
-  for (int b = static_cast<int>(threadIdx.x); b < buckets; b += TR_THREADS) {
+  for (int b = static_cast<int>(threadIdx.x); b < buckets; b += SWEEP_THREADS) {
     counts[static_cast<long long>(b) * chunks + c] = hist[b];
   }
 }
 
-// (3) Records of chunk c at places[b * chunks + c] onward, per bucket b.
-// Rows come from a window of row pointers staged in LDS, reloaded when the
-// passes have used up its entries; a lane takes TR_UNROLL entries a pass,
-// every load issued before the first use.
-__global__ void __launch_bounds__(TR_THREADS)
+// (3) Records of chunk c at places[b * chunks + c] onward, per bucket b
+// (rows from a RowWindow).
+__global__ void __launch_bounds__(SWEEP_THREADS)
 tr_scatter_kernel(const int *rowPtr, int n, const int *colInd, const int *outW,
                   long long stride, int K, int bits, int buckets, long long chunk, int chunks,
                   const int *places, long long recCap, int *recT, int *recU, int *recW) {
-  extern __shared__ int lds[];  // buckets next-places, then TR_ROW_WINDOW row pointers
+  extern __shared__ int lds[];  // buckets next-places, then ROW_WINDOW row pointers
   int *const next = lds;
   int *const rows = lds + buckets;
   const long long m = rowPtr[n];
@@ -276,63 +344,37 @@ tr_scatter_kernel(const int *rowPtr, int n, const int *colInd, const int *outW,
     return;
   }
   const long long c1 = std::min(c0 + chunk, m);
-  for (int b = static_cast<int>(threadIdx.x); b < buckets; b += TR_THREADS) {
+  for (int b = static_cast<int>(threadIdx.x); b < buckets; b += SWEEP_THREADS) {
     next[b] = places[static_cast<long long>(b) * chunks + c];
   }
-  int want = row_of(rowPtr, 0, n - 1, c0);  // rowPtr[want] <= base: the next window
-  int first = -1, count = 0;
-  long long end = 0;
+  RowWindow win(rowPtr, n, c0, rows);  // its first pass's barrier also orders next[]
   for (long long base = c0; base < c1;) {
-    if (want != first) {
-      __syncthreads();  // (first pass: the bucket places above are written)
-      first = want;
-      count = n - first < TR_ROW_WINDOW ? n - first : TR_ROW_WINDOW;
-      for (int i = static_cast<int>(threadIdx.x); i < count; i += TR_THREADS) {
-        rows[i] = rowPtr[first + i];
-      }
-      end = rowPtr[first + count];
-      __syncthreads();
-    }
-    long long stop = std::min(base + TR_THREADS * TR_UNROLL, c1);
-    stop = std::min(stop, end);
-    int t[TR_UNROLL], u[TR_UNROLL], at[TR_UNROLL];
-    for (int j = 0; j < TR_UNROLL; ++j) {
-      const long long e = base + j * TR_THREADS + threadIdx.x;
+    const long long stop = win.pass(base, SWEEP_PASS, c1);
+    int t[SWEEP_UNROLL], at[SWEEP_UNROLL];
+    for (int j = 0; j < SWEEP_UNROLL; ++j) {
+      const long long e = base + j * SWEEP_THREADS + threadIdx.x;
       t[j] = e < stop ? colInd[e] : -1;
     }
-    for (int j = 0; j < TR_UNROLL; ++j) {
-      const long long e = base + j * TR_THREADS + threadIdx.x;
+    for (int j = 0; j < SWEEP_UNROLL; ++j) {
       if (t[j] < 0) {
         continue;
       }
-      int a = 0, b = count - 1;
-      while (a < b) {
-        const int mid = a + (b - a + 1) / 2;
-        if (rows[mid] <= e) {
-          a = mid;
-        } else {
-          b = mid - 1;
-        }
-      }
-      u[j] = first + a;
+      const long long e = base + j * SWEEP_THREADS + threadIdx.x;
       at[j] = atomicAdd(&next[t[j] >> bits], 1);
       recT[at[j]] = t[j];
-      recU[at[j]] = u[j];
+      recU[at[j]] = win.first + win.slot(e);
     }
     for (int k = 0; k < K; ++k) {
-      int w[TR_UNROLL];
-      for (int j = 0; j < TR_UNROLL; ++j) {
-        const long long e = base + j * TR_THREADS + threadIdx.x;
+      int w[SWEEP_UNROLL];
+      for (int j = 0; j < SWEEP_UNROLL; ++j) {
+        const long long e = base + j * SWEEP_THREADS + threadIdx.x;
         w[j] = t[j] >= 0 ? outW[k * stride + e] : 0;
       }
-      for (int j = 0; j < TR_UNROLL; ++j) {
+      for (int j = 0; j < SWEEP_UNROLL; ++j) {
         if (t[j] >= 0) {
           recW[k * recCap + at[j]] = w[j];
         }
       }
-    }
-    if (stop == end) {
-      want = first + count;  // rowPtr[first + count] == end == the next base
     }
     base = stop;
   }
@@ -355,7 +397,7 @@ __device__ __forceinline__ void tr_tile_targets(const int *span, int bits, int &
 }
 
 // (4) In-degrees (degree[] zeroed; m read from rowPtr[n]).
-__global__ void __launch_bounds__(TR_THREADS)
+__global__ void __launch_bounds__(SWEEP_THREADS)
 tr_degree_kernel(const int *rowPtr, int n, const int *recT, int bits, int *degree) {
   __shared__ int counts[TR_WINDOW];
   __shared__ int span[2];
@@ -366,7 +408,7 @@ tr_degree_kernel(const int *rowPtr, int n, const int *recT, int bits, int *degre
   }
   const long long i1 = std::min(i0 + TR_TILE, m);
   tr_tile_span(recT, i0, i1, bits, span);
-  for (int j = static_cast<int>(threadIdx.x); j < TR_WINDOW; j += TR_THREADS) {
+  for (int j = static_cast<int>(threadIdx.x); j < TR_WINDOW; j += SWEEP_THREADS) {
     counts[j] = 0;
   }
   __syncthreads();
@@ -374,13 +416,13 @@ tr_degree_kernel(const int *rowPtr, int n, const int *recT, int bits, int *degre
   long long width;
   tr_tile_targets(span, bits, lo, width);
   const bool local = width <= TR_WINDOW;
-  for (long long i = i0 + threadIdx.x; i < i1; i += TR_THREADS * TR_UNROLL) {
-    int t[TR_UNROLL];
-    for (int j = 0; j < TR_UNROLL; ++j) {
-      const long long r = i + j * TR_THREADS;
+  for (long long i = i0 + threadIdx.x; i < i1; i += SWEEP_THREADS * SWEEP_UNROLL) {
+    int t[SWEEP_UNROLL];
+    for (int j = 0; j < SWEEP_UNROLL; ++j) {
+      const long long r = i + j * SWEEP_THREADS;
       t[j] = r < i1 ? recT[r] : -1;
     }
-    for (int j = 0; j < TR_UNROLL; ++j) {
+    for (int j = 0; j < SWEEP_UNROLL; ++j) {
       if (t[j] < 0) {
         continue;
       }
@@ -393,7 +435,7 @@ tr_degree_kernel(const int *rowPtr, int n, const int *recT, int bits, int *degre
   }
   __syncthreads();
   if (local) {
-    for (int j = static_cast<int>(threadIdx.x); j < width; j += TR_THREADS) {
+    for (int j = static_cast<int>(threadIdx.x); j < width; j += SWEEP_THREADS) {
       if (counts[j] > 0) {
         atomicAdd(&degree[lo + j], counts[j]);
       }
@@ -404,7 +446,7 @@ tr_degree_kernel(const int *rowPtr, int n, const int *recT, int bits, int *degre
 // (5) In-row places: a tile reserves each target's run with one atomic on
 // the target's cursor (the in-row starts, advanced by reservations), then
 // hands out the run's places with LDS atomics.
-__global__ void __launch_bounds__(TR_THREADS)
+__global__ void __launch_bounds__(SWEEP_THREADS)
 tr_place_kernel(const int *rowPtr, int n, int K, const int *recT, const int *recU,
                 const int *recW, long long recCap, int bits, int *cursor, int *inCol, int *inW,
                 long long stride) {
@@ -417,7 +459,7 @@ tr_place_kernel(const int *rowPtr, int n, int K, const int *recT, const int *rec
   }
   const long long i1 = std::min(i0 + TR_TILE, m);
   tr_tile_span(recT, i0, i1, bits, span);
-  for (int j = static_cast<int>(threadIdx.x); j < TR_WINDOW; j += TR_THREADS) {
+  for (int j = static_cast<int>(threadIdx.x); j < TR_WINDOW; j += SWEEP_THREADS) {
     slots[j] = 0;
   }
   __syncthreads();
@@ -426,37 +468,37 @@ tr_place_kernel(const int *rowPtr, int n, int K, const int *recT, const int *rec
   tr_tile_targets(span, bits, lo, width);
   const bool local = width <= TR_WINDOW;
   if (local) {
-    for (long long i = i0 + threadIdx.x; i < i1; i += TR_THREADS) {
+    for (long long i = i0 + threadIdx.x; i < i1; i += SWEEP_THREADS) {
       atomicAdd(&slots[recT[i] - lo], 1);
     }
     __syncthreads();
-    for (int j = static_cast<int>(threadIdx.x); j < width; j += TR_THREADS) {
+    for (int j = static_cast<int>(threadIdx.x); j < width; j += SWEEP_THREADS) {
       if (slots[j] > 0) {
         slots[j] = atomicAdd(&cursor[lo + j], slots[j]);
       }
     }
     __syncthreads();
   }
-  for (long long i = i0 + threadIdx.x; i < i1; i += TR_THREADS * TR_UNROLL) {
-    int t[TR_UNROLL], u[TR_UNROLL], pos[TR_UNROLL];
-    for (int j = 0; j < TR_UNROLL; ++j) {
-      const long long r = i + j * TR_THREADS;
+  for (long long i = i0 + threadIdx.x; i < i1; i += SWEEP_THREADS * SWEEP_UNROLL) {
+    int t[SWEEP_UNROLL], u[SWEEP_UNROLL], pos[SWEEP_UNROLL];
+    for (int j = 0; j < SWEEP_UNROLL; ++j) {
+      const long long r = i + j * SWEEP_THREADS;
       t[j] = r < i1 ? recT[r] : -1;
       u[j] = r < i1 ? recU[r] : 0;
     }
-    for (int j = 0; j < TR_UNROLL; ++j) {
+    for (int j = 0; j < SWEEP_UNROLL; ++j) {
       if (t[j] >= 0) {
         pos[j] = local ? atomicAdd(&slots[t[j] - lo], 1) : atomicAdd(&cursor[t[j]], 1);
         inCol[pos[j]] = u[j];
       }
     }
     for (int k = 0; k < K; ++k) {
-      int w[TR_UNROLL];
-      for (int j = 0; j < TR_UNROLL; ++j) {
-        const long long r = i + j * TR_THREADS;
+      int w[SWEEP_UNROLL];
+      for (int j = 0; j < SWEEP_UNROLL; ++j) {
+        const long long r = i + j * SWEEP_THREADS;
         w[j] = t[j] >= 0 ? recW[k * recCap + r] : 0;
       }
-      for (int j = 0; j < TR_UNROLL; ++j) {
+      for (int j = 0; j < SWEEP_UNROLL; ++j) {
         if (t[j] >= 0) {
           inW[k * stride + pos[j]] = w[j];
         }
@@ -646,21 +688,54 @@ __global__ void apply_rows_kernel(int R, const int *rows, const int *rowPairs, c
   }
 }
 
-// Unchanged rows, one thread per original entry: column and K weights move to
-// the same offset in the row's new place.
-__global__ void __launch_bounds__(EDGE_BLOCK)
+// Unchanged rows: every original entry's column and K weights move to the
+// same offset in its row's new place (rows from a RowWindow).
+__global__ void __launch_bounds__(SWEEP_THREADS)
 apply_copy_kernel(int n, int K, const int *oldRow, const int *oldCol, const int *oldW,
                   long long oldStride, const int *rowTag, const int *newRow, int *newCol,
-                  int *newW, long long newStride) {
-  long long e;
-  int u;
-  if (!entry_row(oldRow, n, e, u) || rowTag[u] >= 0) {
+                  int *newW, long long newStride, long long chunk) {
+  __shared__ int rows[ROW_WINDOW];
+  const long long m = oldRow[n];
+  const long long c0 = static_cast<long long>(blockIdx.x) * chunk;
+  if (c0 >= m) {
     return;
   }
-  const long long pos = newRow[u] + (e - oldRow[u]);
-  newCol[pos] = oldCol[e];
-  for (int k = 0; k < K; ++k) {
-    newW[k * newStride + pos] = oldW[k * oldStride + e];
+  const long long c1 = std::min(c0 + chunk, m);
+  RowWindow win(oldRow, n, c0, rows);
+  for (long long base = c0; base < c1;) {
+    const long long stop = win.pass(base, SWEEP_PASS, c1);
+    long long pos[SWEEP_UNROLL];
+    int col[SWEEP_UNROLL];
+    for (int j = 0; j < SWEEP_UNROLL; ++j) {
+      const long long e = base + j * SWEEP_THREADS + threadIdx.x;
+      pos[j] = -1;
+      if (e < stop) {
+        const int a = win.slot(e);
+        const int u = win.first + a;
+        if (rowTag[u] < 0) {
+          pos[j] = newRow[u] + (e - rows[a]);
+          col[j] = oldCol[e];
+        }
+      }
+    }
+    for (int j = 0; j < SWEEP_UNROLL; ++j) {
+      if (pos[j] >= 0) {
+        newCol[pos[j]] = col[j];
+      }
+    }
+    for (int k = 0; k < K; ++k) {
+      int w[SWEEP_UNROLL];
+      for (int j = 0; j < SWEEP_UNROLL; ++j) {
+        const long long e = base + j * SWEEP_THREADS + threadIdx.x;
+        w[j] = pos[j] >= 0 ? oldW[k * oldStride + e] : 0;
+      }
+      for (int j = 0; j < SWEEP_UNROLL; ++j) {
+        if (pos[j] >= 0) {
+          newW[k * newStride + pos[j]] = w[j];
+        }
+      }
+    }
+    base = stop;
   }
 }
 
@@ -754,9 +829,6 @@ int grid_for(long long work) {
   return static_cast<int>((std::max(work, 1LL) + BLOCK - 1) / BLOCK);
 }
 
-int edge_grid(long long entries) {
-  return static_cast<int>((std::max(entries, 1LL) + EDGE_BLOCK - 1) / EDGE_BLOCK);
-}
 
 // Exclusive scan of count ints (count >= 1); `sums` holds scan_tiles(count).
 int scan_tiles(int count) { return (count + TILE - 1) / TILE; }
@@ -859,11 +931,8 @@ TransposePlan transpose_plan(int n, long long mBound) {
     ++p.bits;
   }
   p.buckets = static_cast<int>((static_cast<long long>(n) + (1LL << p.bits) - 1) >> p.bits);
-  // ~2048 chunks: enough workgroups to fill the GPU, few (bucket, chunk) counts.
-  long long chunk = std::max(4096LL, std::min(32768LL, (mBound + 2047) / 2048));
-  chunk = std::max(chunk, (mBound + (1LL << 20) - 1) >> 20);  // <= 2^20 chunks
-  p.chunk = (chunk + TR_THREADS - 1) / TR_THREADS * TR_THREADS;
-  p.chunks = static_cast<int>(std::max(1LL, (mBound + p.chunk - 1) / p.chunk));
+  p.chunk = sweep_chunk(mBound);
+  p.chunks = sweep_grid(mBound);
   p.recCap = std::max(mBound, 1LL);
   return p;
 }
@@ -897,8 +966,9 @@ int build_reverse_direct(mosp_device_graph *g, char *at, long long mBound, hipSt
   }
   hipLaunchKernelGGL(copy_ints_kernel, dim3(grid_for(n)), dim3(BLOCK), 0, stream, cursor,
                      g->inRow, n);
-  hipLaunchKernelGGL(fill_reverse_kernel, dim3(edge_grid(mBound)), dim3(EDGE_BLOCK), 0, stream,
-                     n, g->stride, K, g->outRow, g->outCol, g->outW, cursor, g->inCol, g->inW);
+  hipLaunchKernelGGL(fill_reverse_kernel, dim3(sweep_grid(mBound)), dim3(SWEEP_THREADS), 0, stream,
+                     n, g->stride, K, g->outRow, g->outCol, g->outW, sweep_chunk(mBound), cursor,
+                     g->inCol, g->inW);
   HIP_TRY(hipGetLastError());
   return 1;
 }
@@ -928,24 +998,24 @@ int build_reverse(mosp_device_graph *g, char *at, long long mBound, bool hubs,
   int *recW = c.take<int>(p.recCap * std::max(K, 1));
   const int tiles = static_cast<int>((p.recCap + TR_TILE - 1) / TR_TILE);
   const size_t countLds = sizeof(int) * p.buckets;
-  const size_t scatterLds = sizeof(int) * (p.buckets + TR_ROW_WINDOW);
-  hipLaunchKernelGGL(tr_count_kernel, dim3(p.chunks), dim3(TR_THREADS), countLds, stream,
+  const size_t scatterLds = sizeof(int) * (p.buckets + ROW_WINDOW);
+  hipLaunchKernelGGL(tr_count_kernel, dim3(p.chunks), dim3(SWEEP_THREADS), countLds, stream,
                      g->outRow, n, g->outCol, p.bits, p.buckets, p.chunk, p.chunks, counts);
   if (!exclusive_scan(counts, places, static_cast<int>(cells), cellSums, stream)) {
     return 0;
   }
-  hipLaunchKernelGGL(tr_scatter_kernel, dim3(p.chunks), dim3(TR_THREADS), scatterLds, stream,
+  hipLaunchKernelGGL(tr_scatter_kernel, dim3(p.chunks), dim3(SWEEP_THREADS), scatterLds, stream,
                      g->outRow, n, g->outCol, g->outW, g->stride, K, p.bits, p.buckets, p.chunk,
                      p.chunks, places, p.recCap, recT, recU, recW);
   hipLaunchKernelGGL(zero_kernel, dim3(grid_for(n + 1)), dim3(BLOCK), 0, stream, degree, n + 1);
-  hipLaunchKernelGGL(tr_degree_kernel, dim3(tiles), dim3(TR_THREADS), 0, stream, g->outRow, n,
+  hipLaunchKernelGGL(tr_degree_kernel, dim3(tiles), dim3(SWEEP_THREADS), 0, stream, g->outRow, n,
                      recT, p.bits, degree);
   if (!exclusive_scan(degree, g->inRow, n + 1, sums, stream)) {
     return 0;
   }
   hipLaunchKernelGGL(copy_ints_kernel, dim3(grid_for(n)), dim3(BLOCK), 0, stream, cursor,
                      g->inRow, n);
-  hipLaunchKernelGGL(tr_place_kernel, dim3(tiles), dim3(TR_THREADS), 0, stream, g->outRow, n, K,
+  hipLaunchKernelGGL(tr_place_kernel, dim3(tiles), dim3(SWEEP_THREADS), 0, stream, g->outRow, n, K,
                      recT, recU, recW, p.recCap, p.bits, cursor, g->inCol, g->inW, g->stride);
   HIP_TRY(hipGetLastError());
   return 1;
@@ -1245,9 +1315,9 @@ extern "C" int mosp_device_graph_apply(const mosp_device_graph *g, int32_t nIns,
   if (!exclusive_scan(newDeg, out->outRow, n + 1, sums, stream)) return 0;
   out->n = n;
   out->K = K;
-  hipLaunchKernelGGL(apply_copy_kernel, dim3(edge_grid(mOld)), dim3(EDGE_BLOCK), 0, stream, n, K,
-                     g->outRow, g->outCol, g->outW, g->stride, rowTag, out->outRow, out->outCol,
-                     out->outW, out->stride);
+  hipLaunchKernelGGL(apply_copy_kernel, dim3(sweep_grid(mOld)), dim3(SWEEP_THREADS), 0, stream,
+                     n, K, g->outRow, g->outCol, g->outW, g->stride, rowTag, out->outRow,
+                     out->outCol, out->outW, out->stride, sweep_chunk(mOld));
   if (R > 0) {
     hipLaunchKernelGGL(apply_compact_kernel, dim3(R), dim3(COMPACT_THREADS), 0, stream, R, K,
                        dRows, dRowPairs, g->outRow, g->outCol, g->outW, g->stride, slot, pV,
