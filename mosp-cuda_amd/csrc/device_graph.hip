@@ -623,23 +623,33 @@ __global__ void apply_pairs_kernel(int P, int K, const int *pairU, const int *pa
   const int u = pairU[p], v = pairV[p], D = pairDel[p], I = pairInsCnt[p], last = pairLast[p];
   const int need = D + (I > 0 ? 1 : 0);
   const int ob = oldRow[u], oe = oldRow[u + 1];
+  // PAIR_UNROLL groups of 64 entries a step, their loads issued together
+  // (entries read past the last needed occurrence are only read).
+  constexpr int PAIR_UNROLL = 8;
   int found = 0, first = -1;
-  for (int base = ob; base < oe && found < need; base += 64) {
-    const int x = base + lane;
-    const bool hit = x < oe && oldCol[x] == v;
-    const unsigned long long bal = __ballot(hit);
-    if (hit) {
-      const int j = found + __popcll(bal & below);
-      if (j < D) {
-        slot[x] = DEAD;
-      } else if (j == D && I > 0) {
-        slot[x] = last;
+  for (int base = ob; base < oe && found < need; base += 64 * PAIR_UNROLL) {
+    int col[PAIR_UNROLL];
+    for (int g = 0; g < PAIR_UNROLL; ++g) {
+      const int x = base + 64 * g + lane;
+      col[g] = x < oe ? oldCol[x] : -1;
+    }
+    for (int g = 0; g < PAIR_UNROLL; ++g) {
+      const int x = base + 64 * g + lane;
+      const bool hit = col[g] == v;
+      const unsigned long long bal = __ballot(hit);
+      if (hit) {
+        const int j = found + __popcll(bal & below);
+        if (j < D) {
+          slot[x] = DEAD;
+        } else if (j == D && I > 0) {
+          slot[x] = last;
+        }
       }
+      if (first < 0 && bal != 0) {
+        first = base + 64 * g + __ffsll(static_cast<long long>(bal)) - 1;
+      }
+      found += __popcll(bal);
     }
-    if (first < 0 && bal != 0) {
-      first = base + __ffsll(static_cast<long long>(bal)) - 1;
-    }
-    found += __popcll(bal);
   }
   if (lane == 0) {
     pairDead[p] = std::min(D, found);
