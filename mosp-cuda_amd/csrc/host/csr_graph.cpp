@@ -62,11 +62,13 @@ std::string at(const std::string &path, const char *begin, const char *pos) {
   return path + ":" + std::to_string(line_of(begin, pos));
 }
 
-// Cursor over one text buffer.
+// Cursor over one text buffer (or over [p, end) of it; `begin` is the
+// buffer's start, for line numbers).
 struct Cursor {
   const char *begin, *p, *end;
   explicit Cursor(const std::string &s)
       : begin(s.data()), p(s.data()), end(s.data() + s.size()) {}
+  Cursor(const char *b, const char *from, const char *to) : begin(b), p(from), end(to) {}
   void blanks() {  // spaces, tabs, carriage returns
     while (p < end && (*p == ' ' || *p == '\t' || *p == '\r')) ++p;
   }
@@ -84,29 +86,136 @@ struct Cursor {
 
 bool valid_weight(long long w) { return w >= 1 && w <= INT_MAX; }
 
+// Large files are parsed in parallel: cut into parts at line boundaries,
+// one thread per part, each into its own array, then concatenated.  Every
+// part stops at its first error, so the first failing part's error is the
+// file's first error, as in one sequential pass.
+constexpr size_t kMinPart = size_t{4} << 20;
+
+std::vector<const char *> line_cuts(const std::string &text) {
+  const unsigned hw = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+  const size_t parts = std::max<size_t>(1, std::min<size_t>(hw, text.size() / kMinPart));
+  const char *const b = text.data(), *const e = b + text.size();
+  std::vector<const char *> cuts{b};
+  for (size_t i = 1; i < parts; ++i) {
+    const char *c = std::max(cuts.back(), b + text.size() * i / parts);
+    c = std::find(c, e, '\n');
+    cuts.push_back(c < e ? c + 1 : e);
+  }
+  cuts.push_back(e);
+  return cuts;
+}
+
+struct Part {
+  std::vector<int> vals;
+  std::string err;
+  long long lines = 0;
+  bool ok = true;
+};
+
+// Runs parse(part, from, to) over every part; false with the first error.
+template <typename F>
+bool parse_parts(const std::string &text, std::vector<Part> &parts, std::string &err, F parse) {
+  const std::vector<const char *> cuts = line_cuts(text);
+  parts.assign(cuts.size() - 1, Part{});
+  std::vector<std::thread> threads;
+  for (size_t i = 0; i + 1 < parts.size(); ++i) {
+    threads.emplace_back([&, i] { parts[i].ok = parse(parts[i], cuts[i], cuts[i + 1]); });
+  }
+  parts.back().ok = parse(parts.back(), cuts[parts.size() - 1], cuts[parts.size()]);
+  for (auto &t : threads) t.join();
+  for (const Part &pt : parts) {
+    if (!pt.ok) {
+      err = pt.err;
+      return false;
+    }
+  }
+  return true;
+}
+
+// The parts' arrays in order into `out` (one copying thread per part).
+void concatenate(std::vector<Part> &parts, std::vector<int> &out) {
+  std::vector<size_t> at(parts.size() + 1, 0);
+  for (size_t i = 0; i < parts.size(); ++i) at[i + 1] = at[i] + parts[i].vals.size();
+  out.resize(at.back());
+  auto copy = [&](size_t i) {
+    std::copy(parts[i].vals.begin(), parts[i].vals.end(),
+              out.begin() + static_cast<std::ptrdiff_t>(at[i]));
+    std::vector<int>().swap(parts[i].vals);
+  };
+  std::vector<std::thread> threads;
+  for (size_t i = 1; i < parts.size(); ++i) threads.emplace_back(copy, i);
+  copy(0);
+  for (auto &t : threads) t.join();
+}
+
 bool read_ids(const std::string &path, std::vector<int> &out, std::string &err) {
   std::string text;
   if (!slurp(path, text)) {
     err = "could not read " + path;
     return false;
   }
-  Cursor c(text);
-  out.clear();
-  out.reserve(text.size() / 4);
-  c.space();
-  while (c.p < c.end) {
-    long long v = 0;
-    if (!c.number(v)) {
-      err = "invalid token at " + at(path, c.begin, c.p);
-      return false;
-    }
-    if (v < 0 || v > INT_MAX) {
-      err = "value " + std::to_string(v) + " out of range [0, " +
-            std::to_string(INT_MAX) + "] at " + at(path, c.begin, c.p);
-      return false;
-    }
-    out.push_back(static_cast<int>(v));
+  std::vector<Part> parts;
+  const char *const begin = text.data();
+  const bool ok = parse_parts(text, parts, err, [&](Part &pt, const char *from, const char *to) {
+    Cursor c(begin, from, to);
+    pt.vals.reserve(static_cast<size_t>(to - from) / 4);
     c.space();
+    while (c.p < c.end) {
+      long long v = 0;
+      if (!c.number(v)) {
+        pt.err = "invalid token at " + at(path, c.begin, c.p);
+        return false;
+      }
+      if (v < 0 || v > INT_MAX) {
+        pt.err = "value " + std::to_string(v) + " out of range [0, " +
+                 std::to_string(INT_MAX) + "] at " + at(path, c.begin, c.p);
+        return false;
+      }
+      pt.vals.push_back(static_cast<int>(v));
+      c.space();
+    }
+    return true;
+  });
+  if (!ok) {
+    return false;
+  }
+  concatenate(parts, out);
+  return true;
+}
+
+// Weights per line of [from, to): K of them (K == 0: the first non-empty
+// line sets it, as it does for the whole file).
+bool parse_values(const std::string &path, Cursor c, int &K, Part &pt) {
+  for (;;) {
+    c.blanks();
+    int onLine = 0;
+    while (!c.eol()) {
+      long long v = 0;
+      if (!c.number(v)) {
+        pt.err = "Invalid token in CSR values file at " + at(path, c.begin, c.p) + ".";
+        return false;
+      }
+      if (!valid_weight(v)) {
+        pt.err = "CSR weight " + std::to_string(v) + " out of range [1, " +
+                 std::to_string(INT_MAX) + "] at " + at(path, c.begin, c.p) + ".";
+        return false;
+      }
+      pt.vals.push_back(static_cast<int>(v));
+      ++onLine;
+      c.blanks();
+    }
+    if (onLine > 0) {
+      if (K == 0) {
+        K = onLine;
+      } else if (onLine != K) {
+        pt.err = "Inconsistent number of objectives at " + at(path, c.begin, c.p) + ".";
+        return false;
+      }
+      ++pt.lines;
+    }
+    if (c.p >= c.end) break;
+    ++c.p;  // newline
   }
   return true;
 }
@@ -118,41 +227,36 @@ bool read_values(const std::string &path, std::vector<int> &w, int &K,
     err = "Could not read CSR values file " + path + ".";
     return false;
   }
-  Cursor c(text);
-  w.clear();
-  w.reserve(text.size() / 3);
+  // K from the first non-empty line, read alone (its errors are the file's
+  // first ones); then every part checks its lines against it.
   K = 0;
-  lines = 0;
-  for (;;) {
-    c.blanks();
-    int onLine = 0;
-    while (!c.eol()) {
-      long long v = 0;
-      if (!c.number(v)) {
-        err = "Invalid token in CSR values file at " + at(path, c.begin, c.p) + ".";
+  {
+    Part first;
+    const char *const e = text.data() + text.size();
+    const char *p = text.data();
+    while (p < e && K == 0) {
+      const char *nl = std::find(p, e, '\n');
+      if (!parse_values(path, Cursor(text.data(), p, nl), K, first)) {
+        err = first.err;
         return false;
       }
-      if (!valid_weight(v)) {
-        err = "CSR weight " + std::to_string(v) + " out of range [1, " +
-              std::to_string(INT_MAX) + "] at " + at(path, c.begin, c.p) + ".";
-        return false;
-      }
-      w.push_back(static_cast<int>(v));
-      ++onLine;
-      c.blanks();
+      p = nl < e ? nl + 1 : e;
     }
-    if (onLine > 0) {
-      if (K == 0) {
-        K = onLine;
-      } else if (onLine != K) {
-        err = "Inconsistent number of objectives at " + at(path, c.begin, c.p) + ".";
-        return false;
-      }
-      ++lines;
-    }
-    if (c.p >= c.end) break;
-    ++c.p;  // newline
   }
+  std::vector<Part> parts;
+  const char *const begin = text.data();
+  const int k = K;
+  const bool ok = parse_parts(text, parts, err, [&](Part &pt, const char *from, const char *to) {
+    int partK = k;
+    pt.vals.reserve(static_cast<size_t>(to - from) / 3);
+    return parse_values(path, Cursor(begin, from, to), partK, pt);
+  });
+  if (!ok) {
+    return false;
+  }
+  lines = 0;
+  for (const Part &pt : parts) lines += pt.lines;
+  concatenate(parts, w);
   return true;
 }
 

@@ -4,6 +4,8 @@
 //   mosp_prep rmat  <prefix> scale edgeFactor K seed [wmin=1] [wmax=100]
 //   mosp_prep changes <prefix> <dir> changes insPct safe(0/1) seed [wmin] [wmax]
 //   mosp_prep init  <prefix> <dir> [source=0]
+//   mosp_prep check <prefix>   read the CSR text (host only); print n, m, K and
+//                              order-sensitive checksums of the arrays
 //   mosp_prep copy  <MiB>      one device-to-device copy of MiB mebibytes
 //                              (calibrates rocprofv3 FETCH_SIZE / WRITE_SIZE)
 // `init` writes <dir>/obj<k>/distancesOriginal.txt + SSSPTreeOriginal.txt:
@@ -92,6 +94,21 @@ int main(int argc, char **argv) {
   CsrGraph g;
   if (!readCsrGraph(prefix, g)) return 1;
   const int n = g.numberOfNodes, K = g.numberOfObjectives;
+  if (cmd == "check") {
+    // Position-weighted sums mod 2^64 (sum of (i + 1) * a[i]) of the row
+    // pointers, columns and edge-major weights: order-sensitive checksums.
+    auto sum = [](const std::vector<int> &a) {
+      unsigned long long s = 0;
+      for (size_t i = 0; i < a.size(); ++i) {
+        s += (i + 1) * static_cast<unsigned long long>(static_cast<unsigned>(a[i]));
+      }
+      return s;
+    };
+    std::cout << "n=" << n << " m=" << g.numberOfEdges() << " K=" << K
+              << " rows=" << sum(g.rowPtr) << " cols=" << sum(g.colInd)
+              << " weights=" << sum(g.weights) << "\n";
+    return 0;
+  }
   if (cmd == "changes" && argc >= 8) {
     const std::string dir = argv[3];
     const int changes = std::atoi(argv[4]);

@@ -18,6 +18,7 @@ from oracle_lib import Batch, i32
 
 ROOT = pathlib.Path(__file__).resolve().parent.parent
 MOSP_BIN = ROOT / "mosp-cuda_amd" / "bin" / "mosp"
+PREP_BIN = ROOT / "mosp-cuda_amd" / "bin" / "mosp_prep"
 
 
 def M():
@@ -196,3 +197,60 @@ def test_reference_readers_agree_on_the_same_literals(tmp_path):
     p = tmp_path / "d.txt"
     p.write_text("0 0\n1 4\n1 4\n2 6\n")
     assert O.ref().ref_read_distances_ok(str(p).encode(), 3) == 0
+
+
+def _weighted_sum(a) -> int:
+    a = np.asarray(a, dtype=np.int64).astype(np.uint64)
+    pos = np.arange(1, a.size + 1, dtype=np.uint64)
+    with np.errstate(over="ignore"):
+        return int((a * pos).sum(dtype=np.uint64))
+
+
+@pytest.mark.skipif(not PREP_BIN.exists(), reason="bin/mosp_prep not built")
+def test_large_csr_text_is_read_in_parallel_parts(tmp_path):
+    """Text files above 4 MiB are cut at line boundaries and parsed by one
+    thread per part: the arrays equal a sequential read (order-sensitive
+    checksums), and an error deep in a file reports its line, as one pass."""
+    rng = np.random.default_rng(3)
+    n, K = 200_000, 3
+    deg = rng.integers(0, 8, n)
+    row = np.concatenate([[0], np.cumsum(deg)])
+    m = int(row[-1])
+    col = rng.integers(0, n, m)
+    w = rng.integers(1, 2**31 - 1, (m, K))
+    pre = tmp_path / "g"
+    rows_txt = "\n".join(map(str, row)) + "\n"
+    cols_txt = "\n".join(map(str, col)) + "\n"
+    vals = [" ".join(map(str, r)) for r in w]
+    (tmp_path / "gRowPtr.txt").write_text(rows_txt)
+    (tmp_path / "gColInd.txt").write_text(cols_txt)
+    (tmp_path / "gValues.txt").write_text("\n".join(vals) + "\n")
+    assert (tmp_path / "gValues.txt").stat().st_size > 3 * (4 << 20)
+    r = subprocess.run([str(PREP_BIN), "check", str(pre)], capture_output=True, text=True,
+                       timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    want = (f"n={n} m={m} K={K} rows={_weighted_sum(row)} cols={_weighted_sum(col)} "
+            f"weights={_weighted_sum(w.reshape(-1))}")
+    assert r.stdout.strip() == want
+    # errors deep in the files: the first one, with its line number
+    bad = list(vals)
+    bad[m - 10] = "1 2"                      # too few objectives on line m - 9
+    bad[m - 5] = "1 x 3"                     # a later invalid token
+    (tmp_path / "gValues.txt").write_text("\n".join(bad) + "\n")
+    r = subprocess.run([str(PREP_BIN), "check", str(pre)], capture_output=True, text=True,
+                       timeout=120)
+    assert r.returncode != 0
+    assert f"Inconsistent number of objectives at {pre}Values.txt:{m - 9}" in r.stdout
+    bad = list(vals)
+    bad[m // 2] = "1 2 0"                    # weight 0, in a middle part
+    (tmp_path / "gValues.txt").write_text("\n".join(bad) + "\n")
+    r = subprocess.run([str(PREP_BIN), "check", str(pre)], capture_output=True, text=True,
+                       timeout=120)
+    assert r.returncode != 0 and f"{pre}Values.txt:{m // 2 + 1}" in r.stdout
+    (tmp_path / "gValues.txt").write_text("\n".join(vals) + "\n")
+    cols = cols_txt.split("\n")
+    cols[m - 3] = "-4"
+    (tmp_path / "gColInd.txt").write_text("\n".join(cols))
+    r = subprocess.run([str(PREP_BIN), "check", str(pre)], capture_output=True, text=True,
+                       timeout=120)
+    assert r.returncode != 0 and f"{pre}ColInd.txt:{m - 2}" in r.stdout
