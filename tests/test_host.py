@@ -19,6 +19,7 @@ from oracle_lib import Batch, i32
 ROOT = pathlib.Path(__file__).resolve().parent.parent
 MOSP_BIN = ROOT / "mosp-cuda_amd" / "bin" / "mosp"
 PREP_BIN = ROOT / "mosp-cuda_amd" / "bin" / "mosp_prep"
+LEGACY_BIN = ROOT / "mosp-cuda_amd" / "bin" / "mosp_legacy"
 
 
 def M():
@@ -254,3 +255,31 @@ def test_large_csr_text_is_read_in_parallel_parts(tmp_path):
     r = subprocess.run([str(PREP_BIN), "check", str(pre)], capture_output=True, text=True,
                        timeout=120)
     assert r.returncode != 0 and f"{pre}ColInd.txt:{m - 2}" in r.stdout
+
+
+@pytest.mark.skipif(not LEGACY_BIN.exists(), reason="bin/mosp_legacy not built")
+def test_file_wrappers_reject_bad_arguments_before_the_gpu(tmp_path):
+    """parallelSOSPUpdate / parallelCombinedGraph argument checks (the
+    reference's messages), all before any GPU work."""
+    g = tmp_path / "g"
+    g.mkdir()
+    (g / "gRowPtr.txt").write_text("0\n1\n2\n2\n")
+    (g / "gColInd.txt").write_text("1\n2\n")
+    (g / "gValues.txt").write_text("4 5\n3 7\n")
+    pre = str(g / "g")
+
+    def run(*args):
+        return subprocess.run([str(LEGACY_BIN), *map(str, args)], capture_output=True,
+                              text=True, timeout=60)
+    r = run("sosp", pre, "d", "t", "i", "x", 2, 0, tmp_path / "o1", tmp_path / "o2")
+    assert r.returncode == 1 and "objectiveIndex out of range" in r.stdout
+    r = run("sosp", pre, "d", "t", "i", "x", 0, 7, tmp_path / "o1", tmp_path / "o2")
+    assert r.returncode == 1 and "source vertex out of range" in r.stdout
+    r = run("sosp", str(tmp_path / "nope"), "d", "t", "i", "x", 0, 0, "a", "b")
+    assert r.returncode == 1 and "Could not read original CSR graph" in r.stdout
+    r = run("combined", pre, 2, 0, "a", "b", "1,0", "t0", "t1")
+    assert r.returncode == 1 and "invalid preference vector" in r.stdout
+    r = run("combined", pre, 2, 5, "a", "b", "-", "t0", "t1")
+    assert r.returncode == 1 and "out of range" in r.stdout
+    r = run("combined", pre, 2, 0, "a", "b", "-", "t0")
+    assert r.returncode == 2
