@@ -431,6 +431,26 @@ int mosp_update_host(int32_t n, int32_t kg, const int32_t *row_ptr,
                      mosp_sosp_stats *stats, mosp_combine_stats *cstats,
                      double *timings_ms);
 
+/* ------------------------------------------------- file-based wrappers --- */
+/* parallelSOSPUpdate (headers/parallelSOSPUpdate.cuh:32-39): reads the CSR
+ * files <csr_prefix>{RowPtr,ColInd,Values}.txt, objective `objective`'s old
+ * tree (distances + parent files), insert.txt / delete.txt; applies the
+ * batch, updates the tree on the GPU and writes it (tree file format).
+ * 1 on success; 0 after printing the reference's error message. */
+int mosp_parallel_sosp_update(const char *csr_prefix, const char *distances_in,
+                              const char *tree_in, const char *insert_path,
+                              const char *delete_path, int32_t objective, int32_t source,
+                              const char *distances_out, const char *tree_out);
+
+/* parallelCombinedGraph (headers/parallelCombinedGraph.cuh:46-53): the
+ * vertex count from <csr_prefix>RowPtr.txt, the K parent files tree_paths[k];
+ * the combined graph and its SOSP tree on the GPU, written in the tree file
+ * format (distances in units of 1/L).  pref: n_pref values (0: all ones).
+ * 1 on success; 0 after printing the reference's error message. */
+int mosp_parallel_combined_graph(const char *csr_prefix, const char *const *tree_paths,
+                                 int32_t k, int32_t source, const char *distances_out,
+                                 const char *tree_out, const int32_t *pref, int32_t n_pref);
+
 /* ----------------------------------------------------- synthetic inputs --- */
 /* Seeded generators (xoshiro256** + unbiased bounded integers): identical
  * instances on every host.  No reference counterpart (the reference's own

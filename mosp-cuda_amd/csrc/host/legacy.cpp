@@ -15,6 +15,7 @@
 #include "combinedGraphGpu.h"
 #include "csrGraph.h"
 #include "deviceArray.h"
+#include "mosp_hip.h"
 #include "deviceGraph.h"
 #include "parallelCombinedGraph.h"
 #include "parallelSOSPUpdate.h"
@@ -196,4 +197,46 @@ bool parallelCombinedGraph(const std::string &originalCsrPrefix,
             << "  Distances: " << distancesOutputPath << "\n"
             << "  SSSP Tree: " << treeOutputPath << "\n";
   return true;
+}
+
+extern "C" int mosp_parallel_sosp_update(const char *csr_prefix, const char *distances_in,
+                                         const char *tree_in, const char *insert_path,
+                                         const char *delete_path, int32_t objective,
+                                         int32_t source, const char *distances_out,
+                                         const char *tree_out) {
+  if (csr_prefix == nullptr || distances_in == nullptr || tree_in == nullptr ||
+      insert_path == nullptr || delete_path == nullptr || distances_out == nullptr ||
+      tree_out == nullptr) {
+    std::cout << "Error: a path is missing.\n";
+    return 0;
+  }
+  return parallelSOSPUpdate(csr_prefix, distances_in, tree_in, insert_path, delete_path,
+                            objective, source, distances_out, tree_out)
+             ? 1
+             : 0;
+}
+
+extern "C" int mosp_parallel_combined_graph(const char *csr_prefix,
+                                            const char *const *tree_paths, int32_t k,
+                                            int32_t source, const char *distances_out,
+                                            const char *tree_out, const int32_t *pref,
+                                            int32_t n_pref) {
+  if (csr_prefix == nullptr || distances_out == nullptr || tree_out == nullptr ||
+      (k > 0 && tree_paths == nullptr) || (n_pref > 0 && pref == nullptr)) {
+    std::cout << "Error: a path or the preference vector is missing.\n";
+    return 0;
+  }
+  std::vector<std::string> trees;
+  for (int i = 0; i < std::max(k, 0); ++i) {
+    if (tree_paths[i] == nullptr) {
+      std::cout << "Error: tree path " << i << " is missing.\n";
+      return 0;
+    }
+    trees.emplace_back(tree_paths[i]);
+  }
+  const std::vector<int> preferences(pref, pref + std::max(n_pref, 0));
+  return parallelCombinedGraph(csr_prefix, trees, k, source, "output/combinedGraph",
+                               distances_out, tree_out, preferences)
+             ? 1
+             : 0;
 }

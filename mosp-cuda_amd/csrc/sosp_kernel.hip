@@ -1494,6 +1494,24 @@ __device__ __forceinline__ void agg_add(int *keys, int *counts, int *global, int
   atomicAdd(&global[p], 1);
 }
 
+// Slot of key p in the table agg_add filled, or -1 (p took the global
+// path).  Insertion takes the first free slot of p's probe sequence, so an
+// empty slot before p means p is not in the table.
+__device__ __forceinline__ int agg_find(const int *keys, int p) {
+  unsigned h = (static_cast<unsigned>(p) * 2654435761u) >> (32 - AGG_BITS);
+  for (int probe = 0; probe < AGG_PROBES; ++probe) {
+    const int key = keys[h];
+    if (key == p) {
+      return static_cast<int>(h);
+    }
+    if (key == -1) {
+      return -1;
+    }
+    h = (h + 1) & (AGG_SLOTS - 1);
+  }
+  return -1;
+}
+
 // Parent k of v if it is the first occurrence of its value among the K
 // parents; its weight is base - sum of terms over the trees containing it.
 __device__ __forceinline__ bool combined_edge(Args &a, int v, int k,
@@ -1630,6 +1648,19 @@ __device__ void combined_build(Args &args, GridBarrier &grid,
   stamp(ctl, PHASE_COMB_SCAN, t.id);
 
   // Fill (row order is irrelevant: Step 3 breaks ties by the lowest id).
+  // Hub variant: the workgroup visits the same vertices as in the count, so
+  // its LDS table still holds its edge count per parent; one atomic per
+  // parent reserves the workgroup's run of the parent's row (hub parents
+  // have hundreds of thousands of children), and LDS atomics hand out the
+  // run's places.  Parents that took the global path count there.
+  if constexpr (HUBS) {
+    for (int i = threadIdx.x; i < AGG_SLOTS; i += BLOCK) {
+      if (sh.aggCount[i] > 0) {
+        sh.aggCount[i] = atomicAdd(&a.cDegree[sh.aggKey[i]], sh.aggCount[i]);
+      }
+    }
+    __syncthreads();
+  }
   for (int v = t.id; v < n; v += t.count) {
     if (v == a.source) {
       continue;
@@ -1637,7 +1668,13 @@ __device__ void combined_build(Args &args, GridBarrier &grid,
     for (int k = 0; k < a.cK; ++k) {
       int p, weight;
       if (combined_edge(a, v, k, p, weight)) {
-        const int pos = atomicAdd(&a.cDegree[p], 1);
+        int pos;
+        if constexpr (HUBS) {
+          const int slot = agg_find(sh.aggKey, p);
+          pos = slot >= 0 ? atomicAdd(&sh.aggCount[slot], 1) : atomicAdd(&a.cDegree[p], 1);
+        } else {
+          pos = atomicAdd(&a.cDegree[p], 1);
+        }
         a.cColInd[pos] = v;
         a.cWeights[pos] = weight;
       }

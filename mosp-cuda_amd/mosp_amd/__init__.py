@@ -86,7 +86,7 @@ EXPORTS = (
     "mosp_server_create", "mosp_server_add_graph", "mosp_server_start", "mosp_server_submit",
     "mosp_server_wait", "mosp_server_drain", "mosp_server_request", "mosp_server_request_trees",
     "mosp_server_release", "mosp_server_stats", "mosp_server_reset_stats", "mosp_server_clock",
-    "mosp_server_destroy",
+    "mosp_server_destroy", "mosp_parallel_sosp_update", "mosp_parallel_combined_graph",
 )
 
 _LIB: C.CDLL | None = None
@@ -160,6 +160,9 @@ def lib() -> C.CDLL:
     L.mosp_apply_batch.restype = _L
     L.mosp_apply_batch.argtypes = [_I, _I, _i32p, _i32p, _i32p, _I, _i32p, _i32p, _i32p, _I,
                                    _i32p, _i32p, _i32p, _i32p, _i32p, _u32p]
+    L.mosp_parallel_sosp_update.argtypes = [C.c_char_p] * 5 + [_I, _I, C.c_char_p, C.c_char_p]
+    L.mosp_parallel_combined_graph.argtypes = [C.c_char_p, C.POINTER(C.c_char_p), _I, _I,
+                                               C.c_char_p, C.c_char_p, _i32p, _I]
     L.mosp_update_host.argtypes = [_I, _I, _i32p, _i32p, _i32p, _I, _i32p, _i32p, _i32p, _I,
                                    _i32p, _i32p, _I, _I, _i32p, _I, _L, _I, _i64p, _i32p,
                                    C.POINTER(C.c_int64), _i32p, _i32p, _i32p, _i64p, _i32p,
@@ -574,6 +577,30 @@ class DeviceGraph:
             self.close()
         except Exception:
             pass
+
+
+def parallel_sosp_update(csr_prefix: str, distances_in: str, tree_in: str, insert_path: str,
+                         delete_path: str, objective: int, source: int, distances_out: str,
+                         tree_out: str) -> bool:
+    """parallelSOSPUpdate (the reference's file-based single-objective update)."""
+    _require_gpu()
+    enc = [str(x).encode() for x in (csr_prefix, distances_in, tree_in, insert_path,
+                                     delete_path)]
+    return bool(lib().mosp_parallel_sosp_update(*enc, int(objective), int(source),
+                                                 str(distances_out).encode(),
+                                                 str(tree_out).encode()))
+
+
+def parallel_combined_graph(csr_prefix: str, tree_paths, source: int, distances_out: str,
+                            tree_out: str, pref=()) -> bool:
+    """parallelCombinedGraph (the reference's file-based combined graph)."""
+    _require_gpu()
+    paths = (C.c_char_p * max(len(tree_paths), 1))(*[str(t).encode() for t in tree_paths])
+    pr = _i32(pref) if len(pref) else np.zeros(1, np.int32)
+    return bool(lib().mosp_parallel_combined_graph(str(csr_prefix).encode(), paths,
+                                                   len(tree_paths), int(source),
+                                                   str(distances_out).encode(),
+                                                   str(tree_out).encode(), pr, len(pref)))
 
 
 def _dev(a, dtype) -> DeviceBuffer:
