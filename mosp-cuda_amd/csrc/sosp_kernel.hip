@@ -1536,6 +1536,53 @@ __device__ __forceinline__ bool combined_edge(Args &a, int v, int k,
   return true;
 }
 
+constexpr int COMB_REG = 8;  // trees whose parents a lane holds in registers
+
+// f(p, weight) for every combined edge (p, v): v's parents are read once
+// (K <= COMB_REG: into registers, every test from them), in tree order.
+template <typename F>
+__device__ __forceinline__ void combined_edges(Args &a, int v, F &&f) {
+  const int K = a.cK;
+  if (K > COMB_REG) {
+    for (int k = 0; k < K; ++k) {
+      int p, weight;
+      if (combined_edge(a, v, k, p, weight)) {
+        f(p, weight);
+      }
+    }
+    return;
+  }
+  const int *par = a.cParents;
+  const size_t n = static_cast<size_t>(a.n);
+  int pv[COMB_REG];
+#pragma unroll
+  for (int k = 0; k < COMB_REG; ++k) {
+    pv[k] = k < K ? par[k * n + v] : -1;
+  }
+#pragma unroll
+  for (int k = 0; k < COMB_REG; ++k) {
+    if (k >= K || pv[k] < 0) {
+      continue;
+    }
+    bool first = true;
+#pragma unroll
+    for (int j = 0; j < k; ++j) {
+      first = first && pv[j] != pv[k];
+    }
+    if (!first) {
+      continue;
+    }
+    int weight = a.cBase - a.cTerms[k];
+#pragma unroll
+    for (int j = k + 1; j < COMB_REG; ++j) {
+      if (j < K && pv[j] == pv[k]) {
+        weight -= a.cTerms[j];
+      }
+    }
+    f(pv[k], weight);
+  }
+}
+
 template <bool HUBS>
 __device__ void combined_build(Args &args, GridBarrier &grid,
                                Threads t, Shared &sh) {
@@ -1561,18 +1608,15 @@ __device__ void combined_build(Args &args, GridBarrier &grid,
     if (v == a.source) {
       continue;
     }
-    for (int k = 0; k < a.cK; ++k) {
-      int p, weight;
-      if (combined_edge(a, v, k, p, weight)) {
-        if constexpr (HUBS) {
-          agg_add(sh.aggKey, sh.aggCount, a.cDegree, p);
-        } else {
-          atomicAdd(&a.cDegree[p], 1);
-        }
-        edges += 1;
-        wsum += static_cast<u64>(weight);
+    combined_edges(a, v, [&](int p, int weight) {
+      if constexpr (HUBS) {
+        agg_add(sh.aggKey, sh.aggCount, a.cDegree, p);
+      } else {
+        atomicAdd(&a.cDegree[p], 1);
       }
-    }
+      edges += 1;
+      wsum += static_cast<u64>(weight);
+    });
   }
   if constexpr (HUBS) {
     __syncthreads();
@@ -1665,20 +1709,17 @@ __device__ void combined_build(Args &args, GridBarrier &grid,
     if (v == a.source) {
       continue;
     }
-    for (int k = 0; k < a.cK; ++k) {
-      int p, weight;
-      if (combined_edge(a, v, k, p, weight)) {
-        int pos;
-        if constexpr (HUBS) {
-          const int slot = agg_find(sh.aggKey, p);
-          pos = slot >= 0 ? atomicAdd(&sh.aggCount[slot], 1) : atomicAdd(&a.cDegree[p], 1);
-        } else {
-          pos = atomicAdd(&a.cDegree[p], 1);
-        }
-        a.cColInd[pos] = v;
-        a.cWeights[pos] = weight;
+    combined_edges(a, v, [&](int p, int weight) {
+      int pos;
+      if constexpr (HUBS) {
+        const int slot = agg_find(sh.aggKey, p);
+        pos = slot >= 0 ? atomicAdd(&sh.aggCount[slot], 1) : atomicAdd(&a.cDegree[p], 1);
+      } else {
+        pos = atomicAdd(&a.cDegree[p], 1);
       }
-    }
+      a.cColInd[pos] = v;
+      a.cWeights[pos] = weight;
+    });
   }
   // Near-far width for Step 3: the caller's, or 32 * average weight /
   // average out-degree (src/sospUpdateGpu.cu:607-616), in double as there,
