@@ -452,6 +452,7 @@ int prepare(mosp_batch *b) {
     a.maxRounds = jump_rounds(n);
     a.anyFallback = anyFallback;
     a.laneBits = b->laneBits;
+    a.listCap = list_cap_override();
     a.genDev = b->dGen + i;
     a.ctl = b->dCtl + i;
     a.cParents = I.spec.d_parents;

@@ -128,8 +128,9 @@ struct mosp_ws {
   int laneBits = 2;         // 4 lanes per frontier vertex
   unsigned *degreeWord = nullptr;  // max-degree reduction of caller CSRs
   int combDeltaScale = 1;   // combined search: default width times this
-  int hopsUpdate = 1;       // local relaxation rounds per near-far iteration
-  int hopsSearch = 2;
+  int hopsUpdate = 2;       // local relaxation rounds per near-far iteration
+  int hopsSearch = 6;
+  int listCap = 0;          // near / far list entries: 0 = n
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
 };
 
@@ -193,6 +194,7 @@ int ws_init(mosp_ws *ws) {
     ws->laneBits = std::max(0, std::min(5, std::atoi(env)));
   }
   local_hops_override(ws->hopsUpdate, ws->hopsSearch);
+  ws->listCap = list_cap_override();
   HIP_TRY(hipMalloc(&ws->dDesc, sizeof(ObjDesc) * (MOSP_MAX_OBJ + 1)));
   HIP_TRY(hipHostMalloc(&ws->hDesc, sizeof(ObjDesc) * (MOSP_MAX_OBJ + 1),
                         hipHostMallocDefault));
@@ -354,6 +356,7 @@ int run_launch(mosp_ws *ws, LaunchArgs &args, int descCount,
   args.laneBits = ws->laneBits;
   args.hopsUpdate = ws->hopsUpdate;
   args.hopsSearch = ws->hopsSearch;
+  args.listCap = ws->listCap;
   args.cPartials = ws->cPartials;
   args.cDeltaScale = ws->combDeltaScale;
   if (descCount > 0) {
@@ -438,6 +441,13 @@ void local_hops_override(int &update, int &search) {
     update = std::max(0, std::min(8, u));
     search = std::max(0, std::min(8, c));
   }
+}
+
+// MOSP_LIST_CAP=c: near / far lists of c entries instead of n, so that
+// small graphs take the sweep that follows a full list (tests).
+int list_cap_override() {
+  const char *env = std::getenv("MOSP_LIST_CAP");
+  return env == nullptr ? 0 : std::max(0, std::atoi(env));
 }
 
 int hubs_override() {

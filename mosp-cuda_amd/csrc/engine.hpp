@@ -141,6 +141,8 @@ struct LaunchArgs {
   int hopsUpdate;          // local relaxation rounds per near-far iteration: K updates
   int hopsSearch;          // ... from-scratch and combined searches (single-update
                            // kernel without hub handling only)
+  int listCap;             // entries of each near / far list: 0 = n; a smaller
+                           // value (tests) forces the sweeps that follow a full list
   int hubs;                // host only: launch the HUB_DEGREE-aware variant
   int *genDev;             // device-resident stamp generation (persists)
   Ctl *ctl;

@@ -48,6 +48,9 @@ int hubs_override();
 // MOSP_LOCAL_HOPS=u,s overrides the local relaxation rounds per near-far
 // iteration of the K updates (u) and of the searches (s) (tuning).
 void local_hops_override(int &update, int &search);
+// MOSP_LIST_CAP=c caps the near / far lists at c entries instead of n, so
+// that small inputs take the sweep that follows a full list (tests); 0 unset.
+int list_cap_override();
 
 // Records the message for mosp_last_error() and returns 0.
 int fail(const std::string &message);

@@ -298,36 +298,48 @@ struct BlockQueue {
   int buf[QCAP];
   unsigned count[WAVES];  // entries reserved per slice
   unsigned base[WAVES];   // list offset of a slice being flushed
+  unsigned keep;          // entries of a whole-buffer flush that fit the list
 };
 
-// The two stores name their address spaces (LDS queue, global list):
-// otherwise the compiler merges them into one flat store through a pointer
-// that may be either.
+// Global lists hold listCap entries.  Every append still counts, so a list
+// that would run past its end has a count above listCap and loses the
+// entries beyond it: readers take such a count to mean "every vertex".  The
+// two stores name their address spaces (LDS queue, global list): otherwise
+// the compiler merges them into one flat store through a pointer that may
+// be either.
 __device__ __forceinline__ void slice_push(BlockQueue &q, unsigned slice, unsigned cap, int v,
-                                           int *list, unsigned *listCount) {
+                                           int *list, unsigned *listCount, unsigned listCap) {
   const unsigned slot = wave_reserve(&q.count[slice]);
   if (slot < cap) {
     *(__attribute__((address_space(3))) int *)(q.buf + slice * cap + slot) = v;
   } else {
-    *(__attribute__((address_space(1))) int *)(list + wave_reserve(listCount)) = v;
+    const unsigned at = wave_reserve(listCount);
+    if (at < listCap) {
+      *(__attribute__((address_space(1))) int *)(list + at) = v;
+    }
   }
 }
 
 __device__ __forceinline__ void queue_push(BlockQueue &q, int v, int *list,
-                                           unsigned *listCount) {
-  slice_push(q, 0, QCAP, v, list, listCount);
+                                           unsigned *listCount, unsigned listCap) {
+  slice_push(q, 0, QCAP, v, list, listCount, listCap);
 }
 
-// Workgroup-uniform: publish the whole-buffer queue and empty it.
-__device__ void queue_flush(BlockQueue &q, int *list, unsigned *listCount) {
+// Workgroup-uniform: publish the whole-buffer queue and empty it.  Thread 0
+// alone reads the count and empties the queue before the barrier; the
+// others read the published base and length after it.
+__device__ void queue_flush(BlockQueue &q, int *list, unsigned *listCount, unsigned listCap) {
   __syncthreads();
-  const unsigned count = q.count[0] < QCAP ? q.count[0] : QCAP;
   if (threadIdx.x == 0) {
-    q.base[0] = count > 0 ? atomicAdd(listCount, count) : 0;
+    const unsigned count = q.count[0] < QCAP ? q.count[0] : QCAP;
+    const unsigned base = count > 0 ? atomicAdd(listCount, count) : 0;
+    q.base[0] = base;
+    q.keep = base < listCap ? min(count, listCap - base) : 0;  // entries that fit
     q.count[0] = 0;
   }
   __syncthreads();
-  for (unsigned i = threadIdx.x; i < count; i += BLOCK) {
+  const unsigned keep = q.keep;
+  for (unsigned i = threadIdx.x; i < keep; i += BLOCK) {
     list[q.base[0] + i] = q.buf[i];
   }
   __syncthreads();
@@ -336,13 +348,14 @@ __device__ void queue_flush(BlockQueue &q, int *list, unsigned *listCount) {
 // Workgroup-uniform: publish slices [0, teams) of `cap` entries to their
 // lists (one global atomicAdd per slice) and empty them.
 __device__ void flush_slices(BlockQueue &q, unsigned teams, unsigned cap,
-                             int *const *lists, unsigned *const *counts) {
+                             int *const *lists, unsigned *const *counts, unsigned listCap) {
   __syncthreads();
   if (threadIdx.x < teams) {
     const unsigned s = threadIdx.x;
     const unsigned n = q.count[s] < cap ? q.count[s] : cap;
-    q.base[s] = n > 0 ? atomicAdd(counts[s], n) : 0;
-    q.count[s] = n;
+    const unsigned base = n > 0 ? atomicAdd(counts[s], n) : 0;
+    q.base[s] = base;
+    q.count[s] = base < listCap ? min(n, listCap - base) : 0;  // entries that fit
   }
   __syncthreads();
   for (unsigned s = 0; s < teams; ++s) {
@@ -527,8 +540,12 @@ struct Team {
 template <bool HUBS, bool LOCAL>
 __device__ void push_team(Desc &D, ObjCtl &c, Shared &sh, int k,
                           unsigned it, int source, int gen, unsigned blockRank,
-                          unsigned blocks, int laneBits, int localHops, const Team &tm) {
-  const unsigned cnt = sh.cnt[k];
+                          unsigned blocks, int laneBits, int localHops, unsigned n,
+                          unsigned listCap, const Team &tm) {
+  // A count above the lists' capacity means entries were lost: the frontier
+  // is then every vertex (relaxing a vertex again is always valid).
+  const bool all = sh.cnt[k] > listCap;
+  const unsigned cnt = all ? n : sh.cnt[k];
   const int e = sh.epoch[k];
   const int bits = D.parentBits;
   const u64 noParent = D.noParent;
@@ -565,11 +582,15 @@ __device__ void push_team(Desc &D, ObjCtl &c, Shared &sh, int k,
     // The claim word (stamp for the near list, far mark for the far pile) is
     // read before the atomic so both round trips overlap.  Claims need no
     // atomics: two lanes may both append w (a duplicate entry only repeats
-    // work), but an improvement is never missed: the stamp only ever holds
-    // this iteration's generation once w is appended in it (the entry reads
-    // w's word when processed, after this iteration's atomics), and inFar is
-    // only set during push iterations of this epoch.  Reading the claim word
-    // early can only see an older value, i.e. add a duplicate.
+    // work; racing lanes, and other XCDs, which see this XCD's stores only
+    // after the next barrier, can make many), but an improvement is never
+    // missed: the stamp only ever holds this iteration's generation once w is
+    // appended in it (the entry reads w's word when processed, after this
+    // iteration's atomics), and inFar is only set during push iterations of
+    // this epoch.  Reading the claim word early can only see an older value,
+    // i.e. add a duplicate.  Duplicates can run a list past its n entries (a
+    // small graph that local hops cover in one iteration does): the lost
+    // entries turn the next iteration into a sweep of every vertex.
     const bool near = nd < thr;
     const int claim = near ? stampArr[w] : inFar[w];
     const u64 prev = atomicMin(&packed[w], cand);
@@ -577,11 +598,11 @@ __device__ void push_team(Desc &D, ObjCtl &c, Shared &sh, int k,
       if (near) {
         if (claim != gen) {
           stampArr[w] = gen;
-          slice_push(sh.near, tm.slice, tm.cap, w, outNear, nextCnt);
+          slice_push(sh.near, tm.slice, tm.cap, w, outNear, nextCnt, listCap);
         }
       } else if (claim == 0) {
         inFar[w] = 1;
-        slice_push(sh.far, tm.slice, tm.cap, w, farBuf, farCnt);
+        slice_push(sh.far, tm.slice, tm.cap, w, farBuf, farCnt, listCap);
         atomicMin(&sh.sliceFarMin[tm.slice], nd);
       }
     }
@@ -594,7 +615,7 @@ __device__ void push_team(Desc &D, ObjCtl &c, Shared &sh, int k,
     for (unsigned base = lo; base < hi; base += perPass) {
       const unsigned i = base + (tm.tid >> laneBits);
       if (i < hi) {
-        const int u = in[i];
+        const int u = all ? static_cast<int>(i) : in[i];
         // Independent loads, issued together.
         const u64 word = packed[u];
         const int begin = outRow[u];
@@ -608,8 +629,8 @@ __device__ void push_team(Desc &D, ObjCtl &c, Shared &sh, int k,
       }
       if (tm.whole) {
         __syncthreads();
-        if (sh.near.count[0] > QCAP / 2) queue_flush(sh.near, outNear, nextCnt);
-        if (sh.far.count[0] > QCAP / 2) queue_flush(sh.far, farBuf, farCnt);
+        if (sh.near.count[0] > QCAP / 2) queue_flush(sh.near, outNear, nextCnt, listCap);
+        if (sh.far.count[0] > QCAP / 2) queue_flush(sh.far, farBuf, farCnt, listCap);
       }
     }
   } else {
@@ -629,7 +650,7 @@ __device__ void push_team(Desc &D, ObjCtl &c, Shared &sh, int k,
         const unsigned i = base + (tm.tid >> laneBits);
         bool heavy = false;
         if (i < hi) {
-          const int u = in[i];
+          const int u = all ? static_cast<int>(i) : in[i];
           // Independent loads, issued together.
           const u64 word = packed[u];
           const int begin = outRow[u];
@@ -676,8 +697,8 @@ __device__ void push_team(Desc &D, ObjCtl &c, Shared &sh, int k,
       }
       if (tm.whole) {
         __syncthreads();
-        if (sh.near.count[0] > QCAP / 2) queue_flush(sh.near, outNear, nextCnt);
-        if (sh.far.count[0] > QCAP / 2) queue_flush(sh.far, farBuf, farCnt);
+        if (sh.near.count[0] > QCAP / 2) queue_flush(sh.near, outNear, nextCnt, listCap);
+        if (sh.far.count[0] > QCAP / 2) queue_flush(sh.far, farBuf, farCnt, listCap);
       }
     }
   }
@@ -686,17 +707,20 @@ __device__ void push_team(Desc &D, ObjCtl &c, Shared &sh, int k,
   // list, localHops rounds.  They stay listed for the next iteration as well
   // (relaxing a vertex again is always valid), so this only brings
   // improvements forward: up to localHops + 1 hops advance per grid barrier,
-  // which halves the iterations.  No flush inside the rounds (the entries
-  // being read stay in place; a full queue spills to the global list).
-  // Heavy vertices are left to the next iteration's balanced handling.
-  // Compiled into the single-update kernel only: in the batched kernel the
-  // extra loop pushes register use past the budget and the spills cost more
-  // than the hops save.
+  // which halves the iterations.  A round takes at most one pass (perPass
+  // vertices): a workgroup that listed many must not hold the grid at the
+  // next barrier, and the capped rounds make more of them affordable.  No
+  // flush inside the rounds (the entries being read stay in place; a full
+  // queue spills to the global list).  Heavy vertices are left to the next
+  // iteration's balanced handling.  Compiled into the single-update kernel
+  // only: in the batched kernel the extra loop pushes register use past the
+  // budget and the spills cost more than the hops save.
   if (LOCAL && tm.whole) {
     unsigned from = 0;
     for (int hop = 0; hop < localHops; ++hop) {
       __syncthreads();
-      const unsigned to = sh.near.count[0] < QCAP ? sh.near.count[0] : QCAP;
+      unsigned to = sh.near.count[0] < QCAP ? sh.near.count[0] : QCAP;
+      to = to < from + perPass ? to : from + perPass;
       if (from >= to) {
         break;
       }
@@ -731,8 +755,11 @@ __device__ void push_team(Desc &D, ObjCtl &c, Shared &sh, int k,
 // next near frontier, keep the rest in the other far buffer.
 __device__ void split_team(Desc &D, ObjCtl &c, Shared &sh, int k,
                            unsigned it, int gen, unsigned blockRank,
-                           unsigned blocks, const Team &tm) {
-  const unsigned cnt = sh.cnt[k];
+                           unsigned blocks, unsigned n, unsigned listCap, const Team &tm) {
+  // A count above the lists' capacity means entries were lost: the pile is
+  // then read from the far marks of every vertex, which name the same set.
+  const bool all = sh.cnt[k] > listCap;
+  const unsigned cnt = all ? n : sh.cnt[k];
   const int e = sh.epoch[k];
   const int bits = D.parentBits;
   const u64 thr = sh.newThr[k];
@@ -748,25 +775,25 @@ __device__ void split_team(Desc &D, ObjCtl &c, Shared &sh, int k,
   const unsigned hi = static_cast<unsigned>(static_cast<u64>(cnt) * (blockRank + 1) / blocks);
   for (unsigned base = lo; base < hi; base += tm.size) {
     const unsigned i = base + tm.tid;
-    if (i < hi) {
-      const int v = in[i];
+    const int v = i < hi ? (all ? static_cast<int>(i) : in[i]) : -1;
+    if (v >= 0 && (!all || inFar[v] != 0)) {
       const u64 word = packed[v];
       const u64 d = word == PACKED_INF ? PACKED_INF : dist_of(word, bits);
       if (d < thr) {
         inFar[v] = 0;
         if (stampArr[v] != gen) {  // duplicates of v in the pile: append once
           stampArr[v] = gen;
-          slice_push(sh.near, tm.slice, tm.cap, v, outNear, nextCnt);
+          slice_push(sh.near, tm.slice, tm.cap, v, outNear, nextCnt, listCap);
         }
       } else {
-        slice_push(sh.far, tm.slice, tm.cap, v, far2, far2Cnt);
+        slice_push(sh.far, tm.slice, tm.cap, v, far2, far2Cnt, listCap);
         atomicMin(&sh.sliceFarMin[tm.slice], d);
       }
     }
     if (tm.whole) {
       __syncthreads();
-      if (sh.near.count[0] > QCAP - BLOCK) queue_flush(sh.near, outNear, nextCnt);
-      if (sh.far.count[0] > QCAP - BLOCK) queue_flush(sh.far, far2, far2Cnt);
+      if (sh.near.count[0] > QCAP - BLOCK) queue_flush(sh.near, outNear, nextCnt, listCap);
+      if (sh.far.count[0] > QCAP - BLOCK) queue_flush(sh.far, far2, far2Cnt, listCap);
     }
   }
   if (tm.tid == 0) {
@@ -782,8 +809,8 @@ template <bool HUBS, bool LOCAL>
 __device__ int near_far(Desc *objsArg, int nobj, ObjCtl *ctlsArg,
                         int source, int generation, u64 initThr,
                         const u64 *deltaOverride, int laneBits, int localHops,
-                        GridBarrier &grid, Threads t, Shared &sh,
-                        int profBase) {
+                        unsigned n, unsigned listCap, GridBarrier &grid, Threads t,
+                        Shared &sh, int profBase) {
   Desc *const objs = uniform(objsArg);
   ObjCtl *const ctls = uniform_global(ctlsArg);
   (void)profBase;
@@ -892,15 +919,15 @@ __device__ int near_far(Desc *objsArg, int nobj, ObjCtl *ctlsArg,
       const unsigned blocks = static_cast<unsigned>(sh.blockEnd[k] - sh.blockBegin[k]);
       if (sh.mode[k] == MODE_PUSH) {
         push_team<HUBS, LOCAL>(objs[k], ctls[k], sh, k, it, source, gen, rank, blocks,
-                               laneBits, localHops, tm);
+                               laneBits, localHops, n, listCap, tm);
       } else {
-        split_team(objs[k], ctls[k], sh, k, it, gen, rank, blocks, tm);
+        split_team(objs[k], ctls[k], sh, k, it, gen, rank, blocks, n, listCap, tm);
       }
       if (tm.tid == 0) {
         sh.sliceObj[team] = k;
       }
-      flush_slices(sh.near, teams, tm.cap, sh.sliceList[0], sh.sliceCount[0]);
-      flush_slices(sh.far, teams, tm.cap, sh.sliceList[1], sh.sliceCount[1]);
+      flush_slices(sh.near, teams, tm.cap, sh.sliceList[0], sh.sliceCount[0], listCap);
+      flush_slices(sh.far, teams, tm.cap, sh.sliceList[1], sh.sliceCount[1], listCap);
       if (threadIdx.x < teams && sh.sliceFarMin[threadIdx.x] != PACKED_INF) {
         atomicMin(&ctls[sh.sliceObj[threadIdx.x]].farMin[sh.sliceFarSlot[threadIdx.x]],
                   sh.sliceFarMin[threadIdx.x]);
@@ -1277,13 +1304,13 @@ __device__ int update_step1(Args &a, GridBarrier &grid, Threads t,
           if (i < cnt) {
             const int v = round == 0 ? static_cast<int>(i) : inList[i];
             if (round == 0 ? jump_first(anc, flag, v) : jump_next(anc, v)) {
-              queue_push(sh.near, v, outList, outCount);
+              queue_push(sh.near, v, outList, outCount, static_cast<unsigned>(n));
             }
           }
           __syncthreads();
-          if (sh.near.count[0] > QCAP - BLOCK) queue_flush(sh.near, outList, outCount);
+          if (sh.near.count[0] > QCAP - BLOCK) queue_flush(sh.near, outList, outCount, static_cast<unsigned>(n));
         }
-        queue_flush(sh.near, outList, outCount);
+        queue_flush(sh.near, outList, outCount, static_cast<unsigned>(n));
       }
       if (threadIdx.x < static_cast<unsigned>(a.nobj)) {
         sh.src[threadIdx.x] = &ctl->obj[threadIdx.x].jumpCount[(round + 1) % 3];
@@ -1341,7 +1368,7 @@ __device__ int update_step1(Args &a, GridBarrier &grid, Threads t,
       const u64 bd = dist_of(best, D.parentBits);
       if (bd < dist_of(prev, D.parentBits)) {
         D.s.inFar[v] = 1;
-        queue_push(sh.far, v, D.s.far[0], &c.farCount[0]);
+        queue_push(sh.far, v, D.s.far[0], &c.farCount[0], static_cast<unsigned>(n));
         atomicMin(&sh.farMin, bd);
       }
     }
@@ -1384,7 +1411,7 @@ __device__ int update_step1(Args &a, GridBarrier &grid, Threads t,
         }
       } else {
         if (want && end - begin > PULL_BLOCK) {
-          queue_push(sh.near, v, D.s.near[0], &c.pullHeavy);
+          queue_push(sh.near, v, D.s.near[0], &c.pullHeavy, static_cast<unsigned>(n));
           want = false;
           end = begin;
         }
@@ -1412,14 +1439,14 @@ __device__ int update_step1(Args &a, GridBarrier &grid, Threads t,
         pulled(D, c, v, best, current);
       }
       __syncthreads();
-      if (sh.far.count[0] > QCAP - BLOCK) queue_flush(sh.far, D.s.far[0], &c.farCount[0]);
+      if (sh.far.count[0] > QCAP - BLOCK) queue_flush(sh.far, D.s.far[0], &c.farCount[0], static_cast<unsigned>(n));
       if constexpr (HUBS) {
-        if (sh.near.count[0] > QCAP - BLOCK) queue_flush(sh.near, D.s.near[0], &c.pullHeavy);
+        if (sh.near.count[0] > QCAP - BLOCK) queue_flush(sh.near, D.s.near[0], &c.pullHeavy, static_cast<unsigned>(n));
       }
     }
-    queue_flush(sh.far, D.s.far[0], &c.farCount[0]);
+    queue_flush(sh.far, D.s.far[0], &c.farCount[0], static_cast<unsigned>(n));
     if constexpr (HUBS) {
-      queue_flush(sh.near, D.s.near[0], &c.pullHeavy);
+      queue_flush(sh.near, D.s.near[0], &c.pullHeavy, static_cast<unsigned>(n));
     }
     if (threadIdx.x == 0 && sh.farMin != PACKED_INF) {
       atomicMin(&c.farMin[0], sh.farMin);
@@ -1459,7 +1486,7 @@ __device__ int update_step1(Args &a, GridBarrier &grid, Threads t,
           pulled(D, c, v, best < current ? best : current, current);
         }
       }
-      queue_flush(sh.far, D.s.far[0], &c.farCount[0]);
+      queue_flush(sh.far, D.s.far[0], &c.farCount[0], static_cast<unsigned>(n));
       if (threadIdx.x == 0 && sh.farMin != PACKED_INF) {
         atomicMin(&c.farMin[0], sh.farMin);
       }
@@ -1781,10 +1808,14 @@ __device__ void run_instance(Args &args, const Threads t, Shared &sh) {
   }
   __syncthreads();
 
+  // Entries of each near / far list: n, or fewer when a caller asks for it
+  // (a smaller capacity exercises the sweep that follows a full list).
+  const unsigned listCap = static_cast<unsigned>(a.listCap > 0 && a.listCap < n ? a.listCap : n);
   if (a.phases & PH_UPDATE) {
     generation = update_step1<HUBS>(a, grid, t, generation, sh);
     generation = near_far<HUBS, LOCAL>(desc(a.objs), a.nobj, a.ctl->obj, a.source, generation, 0,
-                                nullptr, a.laneBits, a.hopsUpdate, grid, t, sh, 8);
+                                nullptr, a.laneBits, a.hopsUpdate, static_cast<unsigned>(n),
+                                listCap, grid, t, sh, 8);
     stamp(a.ctl, PHASE_NEARFAR, t.id);
     // One edge sweep for every distance-only objective when they are all
     // views of one out-CSR (the fused update's case), else one each.
@@ -1829,7 +1860,8 @@ __device__ void run_instance(Args &args, const Threads t, Shared &sh) {
     scratch_init(D, a.ctl->obj[0], n, a.source, t);
     grid.sync();
     generation = near_far<HUBS, LOCAL>(desc(a.objs), 1, a.ctl->obj, a.source, generation,
-                                D.delta, nullptr, a.laneBits, a.hopsSearch, grid, t, sh, 8);
+                                D.delta, nullptr, a.laneBits, a.hopsSearch,
+                                static_cast<unsigned>(n), listCap, grid, t, sh, 8);
     unpack(D, n, a.source, t);
     if (D.parentBits == 0) {
       grid.sync();
@@ -1849,7 +1881,8 @@ __device__ void run_instance(Args &args, const Threads t, Shared &sh) {
     stamp(a.ctl, PHASE_COMB_FILL, t.id);
     const u64 delta = static_cast<u64>(sh.snap[0]) | (static_cast<u64>(sh.snap[1]) << 32);
     generation = near_far<HUBS, LOCAL>(desc(a.cDesc), 1, &cc, a.source, generation, delta, &delta,
-                                a.laneBits, a.hopsSearch, grid, t, sh, 12);
+                                a.laneBits, a.hopsSearch, static_cast<unsigned>(n), listCap,
+                                grid, t, sh, 12);
     stamp(a.ctl, PHASE_COMB_SEARCH, t.id);
     unpack(C, n, a.source, t);
     if (C.parentBits == 0) {

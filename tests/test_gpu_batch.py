@@ -86,6 +86,22 @@ def test_batched_matches_oracle_and_fused(small):
     bu.close()
 
 
+def test_batched_full_lists_fall_back_to_sweeps(small, monkeypatch):
+    """Near / far lists capped far below n (MOSP_LIST_CAP) in the batched
+    kernel: the iterations after a full list sweep every vertex; results
+    equal the oracle's."""
+    specs, built = small
+    monkeypatch.setenv("MOSP_LIST_CAP", "8")
+    sessions = [s for _, s in built]
+    bu = B().BatchedUpdate(sessions)
+    say("batched with capped lists:", len(sessions), "instances")
+    bu.run(use_graph=False)
+    bu.check()
+    for i, (spec, (host, _)) in enumerate(zip(specs, built)):
+        assert_same(bu.results(i), oracle_of(spec, host), f"instance {i} (capped lists)")
+    bu.close()
+
+
 def test_replays_and_direct_launch_agree(small):
     specs, built = small
     sessions = [s for _, s in built]

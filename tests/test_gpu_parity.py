@@ -308,6 +308,35 @@ def test_kernel_variants_agree(monkeypatch):
             assert np.array_equal(runs[0][key], runs[1][key]), f"{tag}: {key}"
 
 
+def test_full_lists_fall_back_to_sweeps(monkeypatch):
+    """Near / far lists capped far below n (MOSP_LIST_CAP): appends past the
+    cap are lost, and the iteration after a full list relaxes every vertex
+    (near frontier) or splits every vertex with a far mark (far pile).
+    Results stay exact: from-scratch searches (whose local hops re-list most
+    of a small graph every iteration), and updates with the combined graph,
+    in both kernel variants."""
+    monkeypatch.setenv("MOSP_LIST_CAP", "16")
+    g = gen.random_graph(500, 1800, 2, 40, 2)
+    dg = M().DeviceGraph(mgraph(g))
+    ws = M().Workspace(g.n)
+    for k in range(g.K):
+        for src in (0, 7):
+            say(f"capped lists: from-scratch k={k} src={src}")
+            w = g.w.reshape(-1, g.K)[:, k]
+            d, p, st = M().from_scratch(dg, k, src, M().default_delta(g.m, g.n, int(w.sum())),
+                                        int(w.max()), ws)
+            rd, rp = O.dijkstra(g, k, src)
+            assert np.array_equal(d, rd) and np.array_equal(p, rp)
+    for forced in ("0", "1"):
+        monkeypatch.setenv("MOSP_HUBS", forced)
+        grid = gen.grid_graph(40, 30, 3, 90, 0.2, 81)
+        check_pipeline(grid, gen.uniform_batch(grid, 400, 50, 82), pref=[1, 2, 1],
+                       tag=f"capped-lists grid hubs={forced}")
+        hub = rmat(10, 16, 2, seed=83)
+        check_pipeline(hub, gen.make_batch("uniform-unsafe", hub, 300, 84), pref=[2, 1],
+                       tag=f"capped-lists rmat hubs={forced}")
+
+
 def test_rmat_large_weights_fallback():
     """R-MAT scale 17, weights up to 2^31 - 1: (n - 1) * maxW does not fit next
     to an 18-bit parent id, so the distance-only words and the lowest-id
