@@ -68,6 +68,8 @@ struct mosp_batch {
   int xcds = 1;
   int workgroups = 0;
   int laneBits = 2;
+  int hopsUpdate = kHopsUpdate;  // local relaxation rounds (MOSP_LOCAL_HOPS)
+  int hopsSearch = kHopsSearch;
   double deltaScale = 1.0;
   int xcdAware = 1;
   double latencyItems = 400;  // schedule cost model: a / b (see model_sizes)
@@ -198,6 +200,7 @@ int setup(mosp_batch *b, int maxInstances, int maxDescs, size_t slabBytes) {
   if (const char *env = std::getenv("MOSP_XCDS")) {
     b->xcds = std::max(1, std::atoi(env));
   }
+  local_hops_override(b->hopsUpdate, b->hopsSearch);
   b->workgroups = std::min(b->cus, b->maxBlocks);
   maxInstances = std::max(maxInstances, 1);
   maxDescs = std::max(maxDescs, 1);
@@ -453,6 +456,8 @@ int prepare(mosp_batch *b) {
     a.anyFallback = anyFallback;
     a.laneBits = b->laneBits;
     a.listCap = list_cap_override();
+    a.hopsUpdate = b->hopsUpdate;
+    a.hopsSearch = b->hopsSearch;
     a.genDev = b->dGen + i;
     a.ctl = b->dCtl + i;
     a.cParents = I.spec.d_parents;

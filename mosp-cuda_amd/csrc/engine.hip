@@ -128,8 +128,8 @@ struct mosp_ws {
   int laneBits = 2;         // 4 lanes per frontier vertex
   unsigned *degreeWord = nullptr;  // max-degree reduction of caller CSRs
   int combDeltaScale = 1;   // combined search: default width times this
-  int hopsUpdate = 2;       // local relaxation rounds per near-far iteration
-  int hopsSearch = 6;
+  int hopsUpdate = kHopsUpdate;  // local relaxation rounds per near-far iteration
+  int hopsSearch = kHopsSearch;
   int listCap = 0;          // near / far list entries: 0 = n
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
 };

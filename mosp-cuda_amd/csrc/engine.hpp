@@ -139,8 +139,8 @@ struct LaunchArgs {
   int anyFallback;         // some objective of objs uses distance-only words
   int laneBits;            // 2^laneBits lanes share a frontier vertex
   int hopsUpdate;          // local relaxation rounds per near-far iteration: K updates
-  int hopsSearch;          // ... from-scratch and combined searches (single-update
-                           // kernel without hub handling only)
+  int hopsSearch;          // ... from-scratch and combined searches (kernels
+                           // without hub handling only)
   int listCap;             // entries of each near / far list: 0 = n; a smaller
                            // value (tests) forces the sweeps that follow a full list
   int hubs;                // host only: launch the HUB_DEGREE-aware variant

@@ -45,6 +45,11 @@ int graph_max_degree(const int *outRow, const int *inRow, int n, unsigned *scrat
 
 // MOSP_HUBS=0|1 forces the kernel variant (tests, tuning); -1 when unset.
 int hubs_override();
+// Local relaxation rounds per near-far iteration: K updates, and the
+// from-scratch / combined searches (measured best on MI355X for road-like
+// graphs, single and batched launches).
+constexpr int kHopsUpdate = 2;
+constexpr int kHopsSearch = 6;
 // MOSP_LOCAL_HOPS=u,s overrides the local relaxation rounds per near-far
 // iteration of the K updates (u) and of the searches (s) (tuning).
 void local_hops_override(int &update, int &search);

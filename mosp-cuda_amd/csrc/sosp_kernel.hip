@@ -712,9 +712,10 @@ __device__ void push_team(Desc &D, ObjCtl &c, Shared &sh, int k,
   // next barrier, and the capped rounds make more of them affordable.  No
   // flush inside the rounds (the entries being read stay in place; a full
   // queue spills to the global list).  Heavy vertices are left to the next
-  // iteration's balanced handling.  Compiled into the single-update kernel
-  // only: in the batched kernel the extra loop pushes register use past the
-  // budget and the spills cost more than the hops save.
+  // iteration's balanced handling.  Compiled into the kernels without hub
+  // handling (single update and batched): in the hub variants the extra loop
+  // pushes register use past the budget and the spills cost more than the
+  // hops save.
   if (LOCAL && tm.whole) {
     unsigned from = 0;
     for (int hop = 0; hop < localHops; ++hop) {
@@ -1937,7 +1938,7 @@ __global__ void __launch_bounds__(BLOCK, WAVES_PER_SIMD)
     __syncthreads();
     // Read in place (uniform, read-only: scalar loads); an LDS copy would
     // put every field in vector registers and the kernel would spill.
-    run_instance<HUBS, false>(*(Args *)(insts + i), t, sh);
+    run_instance<HUBS, !HUBS>(*(Args *)(insts + i), t, sh);
   }
 }
 
