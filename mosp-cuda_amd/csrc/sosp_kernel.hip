@@ -537,7 +537,7 @@ struct Team {
 
 // Relax the out-edges of the frontier vertices of objective k: groups of
 // 2^laneBits lanes share a vertex and stride over its edges.
-template <bool HUBS, bool LOCAL>
+template <bool HUBS, int LOCAL>
 __device__ void push_team(Desc &D, ObjCtl &c, Shared &sh, int k,
                           unsigned it, int source, int gen, unsigned blockRank,
                           unsigned blocks, int laneBits, int localHops, unsigned n,
@@ -716,7 +716,7 @@ __device__ void push_team(Desc &D, ObjCtl &c, Shared &sh, int k,
   // handling (single update and batched): in the hub variants the extra loop
   // pushes register use past the budget and the spills cost more than the
   // hops save.
-  if (LOCAL && tm.whole) {
+  if (LOCAL != 0 && tm.whole) {
     unsigned from = 0;
     for (int hop = 0; hop < localHops; ++hop) {
       __syncthreads();
@@ -806,7 +806,7 @@ __device__ void split_team(Desc &D, ObjCtl &c, Shared &sh, int k,
   }
 }
 
-template <bool HUBS, bool LOCAL>
+template <bool HUBS, int LOCAL>
 __device__ int near_far(Desc *objsArg, int nobj, ObjCtl *ctlsArg,
                         int source, int generation, u64 initThr,
                         const u64 *deltaOverride, int laneBits, int localHops,
@@ -1775,7 +1775,11 @@ __device__ void combined_build(Args &args, GridBarrier &grid,
 
 // One update instance (graph, K objectives, combined step) on the group of
 // workgroups `t` describes; the group synchronizes through a.ctl.
-template <bool HUBS, bool LOCAL>
+// LOCAL: 0 no local hops (hub variants); 1 local hops, single-update
+// kernel; 2 local hops, batched kernel.  The two values with hops compile
+// the same code as separate instantiations down the call tree, so neither
+// kernel's register allocation depends on the other's.
+template <bool HUBS, int LOCAL>
 __device__ void run_instance(Args &args, const Threads t, Shared &sh) {
   Args &a = *uniform(&args);
   GridBarrier grid{a.ctl, 0, t.block, t.blocks};
@@ -1914,8 +1918,17 @@ __global__ void __launch_bounds__(BLOCK, WAVES_PER_SIMD) sosp_persistent(LaunchA
   __shared__ Shared sh;
   (void)a;  // read in place: the first kernel argument starts the kernarg segment
   Args &args = *(Args *)__builtin_amdgcn_kernarg_segment_ptr();
-  run_instance<HUBS, !HUBS>(args, Threads{blockIdx.x * BLOCK + threadIdx.x, gridDim.x * BLOCK,
+  run_instance<HUBS, HUBS ? 0 : 1>(args, Threads{blockIdx.x * BLOCK + threadIdx.x, gridDim.x * BLOCK,
                                           blockIdx.x, gridDim.x}, sh);
+}
+
+// One instance of a batched launch, as a real call: the single-update
+// kernel inlines its instance (120 VGPRs, no spills), but inlined into the
+// batched kernel's loop over instances the same code spills 78 VGPRs
+// (batched 64 road instances: 32.5 ms inlined, 31.4 ms called).
+template <bool HUBS>
+__device__ __noinline__ void run_batched_instance(Args &args, const Threads t, Shared &sh) {
+  run_instance<HUBS, HUBS ? 0 : 2>(args, t, sh);
 }
 
 // Many independent instances in one launch: workgroup b belongs to group
@@ -1938,7 +1951,7 @@ __global__ void __launch_bounds__(BLOCK, WAVES_PER_SIMD)
     __syncthreads();
     // Read in place (uniform, read-only: scalar loads); an LDS copy would
     // put every field in vector registers and the kernel would spill.
-    run_instance<HUBS, !HUBS>(*(Args *)(insts + i), t, sh);
+    run_batched_instance<HUBS>(*(Args *)(insts + i), t, sh);
   }
 }
 
