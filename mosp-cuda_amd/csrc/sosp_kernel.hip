@@ -1199,38 +1199,53 @@ __device__ void scratch_init(Desc &D, ObjCtl &c, int n, int source,
 // one 32-bit word carries both fields, so a read sees a consistent pair.
 constexpr int JUMP_FLAG = INT_MIN;
 
+// Jumps per round and vertex: a later word of an ancestor is as valid to
+// follow as an earlier one (every vertex between v and the ancestor its word
+// names is known not to be a root), so each round follows up to JUMPS words
+// instead of one and fewer rounds (grid barriers) are needed.  On a
+// 1.96 M-vertex road grid with 3 objectives and 50,000 changes: 0.468 /
+// 0.359 / 0.328 ms for the whole invalidation at 1 / 2 / 3 jumps.
+constexpr int JUMPS = 3;
+
 // Round 0, every vertex: roots[] marks the roots (heads of changed tree
 // edges), which get their flag here.  Returns true if v still jumps.
 __device__ __forceinline__ bool jump_first(int *word, const int *roots, int v) {
-  const int w = word[v];
+  const int w0 = word[v];
   if (roots[v]) {
-    word[v] = w | JUMP_FLAG;
+    word[v] = w0 | JUMP_FLAG;
     return false;
   }
-  if (w == 0) {
-    return false;
+  int w = w0;
+  for (int j = 0; j < JUMPS && w != 0; ++j) {
+    const int x = w - 1;
+    const int wx = word[x];
+    if (roots[x] || (wx & JUMP_FLAG)) {
+      word[v] = w0 | JUMP_FLAG;
+      return false;
+    }
+    w = wx;
   }
-  const int x = w - 1;
-  const int wx = word[x];
-  if (roots[x] || (wx & JUMP_FLAG)) {
-    word[v] = w | JUMP_FLAG;
-    return false;
+  if (w != w0) {
+    word[v] = w;
   }
-  word[v] = wx;
-  return wx != 0;
+  return w != 0;
 }
 
 // Later rounds, vertices still jumping (unflagged, with an ancestor): every
-// root carries its flag by now, so one read of the ancestor's word decides.
+// root carries its flag by now, so each ancestor's word alone decides.
 __device__ __forceinline__ bool jump_next(int *word, int v) {
-  const int w = word[v];
-  const int wx = word[w - 1];
-  if (wx & JUMP_FLAG) {
-    word[v] = w | JUMP_FLAG;
-    return false;
+  const int w0 = word[v];
+  int w = w0;
+  for (int j = 0; j < JUMPS && w != 0; ++j) {
+    const int wx = word[w - 1];
+    if (wx & JUMP_FLAG) {
+      word[v] = w0 | JUMP_FLAG;
+      return false;
+    }
+    w = wx;
   }
-  word[v] = wx;
-  return wx != 0;
+  word[v] = w;
+  return w != 0;
 }
 
 template <bool HUBS>
