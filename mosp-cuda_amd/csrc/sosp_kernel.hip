@@ -537,19 +537,26 @@ struct Team {
 
 // Relax the out-edges of the frontier vertices of objective k: groups of
 // 2^laneBits lanes share a vertex and stride over its edges.
-template <bool HUBS, int LOCAL>
+// SPLIT: the iteration raises the threshold past the far pile instead: its
+// entries below the new threshold move to the next near frontier, the rest
+// to the next epoch's pile.  In the kernels with local hops, the hop rounds
+// then relax the moved vertices at once, under the new threshold, instead of
+// leaving them all to the next iteration.
+template <bool HUBS, int LOCAL, bool SPLIT = false>
 __device__ void push_team(Desc &D, ObjCtl &c, Shared &sh, int k,
                           unsigned it, int source, int gen, unsigned blockRank,
                           unsigned blocks, int laneBits, int localHops, unsigned n,
                           unsigned listCap, const Team &tm) {
   // A count above the lists' capacity means entries were lost: the frontier
-  // is then every vertex (relaxing a vertex again is always valid).
+  // is then every vertex (relaxing a vertex again is always valid); a split's
+  // pile is then read from the far marks of every vertex, which name the
+  // same set.
   const bool all = sh.cnt[k] > listCap;
   const unsigned cnt = all ? n : sh.cnt[k];
-  const int e = sh.epoch[k];
+  const int e = sh.epoch[k] + (SPLIT ? 1 : 0);  // a split works for the next epoch
   const int bits = D.parentBits;
   const u64 noParent = D.noParent;
-  const u64 thr = sh.thr[k];
+  const u64 thr = SPLIT ? sh.newThr[k] : sh.thr[k];
   const u64 maxDist = D.maxDistance;
   // Locals, read once; a team is whole waves, so they are wave-uniform and
   // live in SGPRs (k, hence D, comes from LDS: the compiler cannot tell).
@@ -560,7 +567,7 @@ __device__ void push_team(Desc &D, ObjCtl &c, Shared &sh, int k,
   const int *const outRow = uniform_global(D.outRow);
   const int *const outCol = uniform_global(D.outCol);
   const int *const outW = uniform_global(D.outW);
-  const int *in = uniform_global(D.s.near[it & 1]);
+  const int *in = uniform_global(SPLIT ? D.s.far[(e + 1) & 1] : D.s.near[it & 1]);
   int *outNear = uniform_global(D.s.near[(it + 1) & 1]);
   int *farBuf = uniform_global(D.s.far[e & 1]);
   unsigned *nextCnt = &c.nearCount[(it + 1) % 3];
@@ -611,7 +618,32 @@ __device__ void push_team(Desc &D, ObjCtl &c, Shared &sh, int k,
   // workgroup of the objective (random loads cost per-CU address cycles).
   const unsigned lo = static_cast<unsigned>(static_cast<u64>(cnt) * blockRank / blocks);
   const unsigned hi = static_cast<unsigned>(static_cast<u64>(cnt) * (blockRank + 1) / blocks);
-  if constexpr (!HUBS) {
+  if constexpr (SPLIT) {
+    // One thread per pile entry; `in` is the old epoch's pile, farBuf the new one's.
+    for (unsigned base = lo; base < hi; base += tm.size) {
+      const unsigned i = base + tm.tid;
+      const int v = i < hi ? (all ? static_cast<int>(i) : in[i]) : -1;
+      if (v >= 0 && (!all || inFar[v] != 0)) {
+        const u64 word = packed[v];
+        const u64 d = word == PACKED_INF ? PACKED_INF : dist_of(word, bits);
+        if (d < thr) {
+          inFar[v] = 0;
+          if (stampArr[v] != gen) {  // duplicates of v in the pile: append once
+            stampArr[v] = gen;
+            slice_push(sh.near, tm.slice, tm.cap, v, outNear, nextCnt, listCap);
+          }
+        } else {
+          slice_push(sh.far, tm.slice, tm.cap, v, farBuf, farCnt, listCap);
+          atomicMin(&sh.sliceFarMin[tm.slice], d);
+        }
+      }
+      if (tm.whole) {
+        __syncthreads();
+        if (sh.near.count[0] > QCAP - BLOCK) queue_flush(sh.near, outNear, nextCnt, listCap);
+        if (sh.far.count[0] > QCAP - BLOCK) queue_flush(sh.far, farBuf, farCnt, listCap);
+      }
+    }
+  } else if constexpr (!HUBS) {
     for (unsigned base = lo; base < hi; base += perPass) {
       const unsigned i = base + (tm.tid >> laneBits);
       if (i < hi) {
@@ -752,60 +784,6 @@ __device__ void push_team(Desc &D, ObjCtl &c, Shared &sh, int k,
   }
 }
 
-// Raise the threshold past the far pile: move the entries below it to the
-// next near frontier, keep the rest in the other far buffer.
-__device__ void split_team(Desc &D, ObjCtl &c, Shared &sh, int k,
-                           unsigned it, int gen, unsigned blockRank,
-                           unsigned blocks, unsigned n, unsigned listCap, const Team &tm) {
-  // A count above the lists' capacity means entries were lost: the pile is
-  // then read from the far marks of every vertex, which name the same set.
-  const bool all = sh.cnt[k] > listCap;
-  const unsigned cnt = all ? n : sh.cnt[k];
-  const int e = sh.epoch[k];
-  const int bits = D.parentBits;
-  const u64 thr = sh.newThr[k];
-  const int *in = D.s.far[e & 1];
-  int *far2 = D.s.far[(e + 1) & 1];
-  int *outNear = D.s.near[(it + 1) & 1];
-  unsigned *nextCnt = &c.nearCount[(it + 1) % 3];
-  unsigned *far2Cnt = &c.farCount[(e + 1) % 3];
-  const u64 *const packed = D.s.packed;
-  int *const stampArr = D.s.stamp;
-  int *const inFar = D.s.inFar;
-  const unsigned lo = static_cast<unsigned>(static_cast<u64>(cnt) * blockRank / blocks);
-  const unsigned hi = static_cast<unsigned>(static_cast<u64>(cnt) * (blockRank + 1) / blocks);
-  for (unsigned base = lo; base < hi; base += tm.size) {
-    const unsigned i = base + tm.tid;
-    const int v = i < hi ? (all ? static_cast<int>(i) : in[i]) : -1;
-    if (v >= 0 && (!all || inFar[v] != 0)) {
-      const u64 word = packed[v];
-      const u64 d = word == PACKED_INF ? PACKED_INF : dist_of(word, bits);
-      if (d < thr) {
-        inFar[v] = 0;
-        if (stampArr[v] != gen) {  // duplicates of v in the pile: append once
-          stampArr[v] = gen;
-          slice_push(sh.near, tm.slice, tm.cap, v, outNear, nextCnt, listCap);
-        }
-      } else {
-        slice_push(sh.far, tm.slice, tm.cap, v, far2, far2Cnt, listCap);
-        atomicMin(&sh.sliceFarMin[tm.slice], d);
-      }
-    }
-    if (tm.whole) {
-      __syncthreads();
-      if (sh.near.count[0] > QCAP - BLOCK) queue_flush(sh.near, outNear, nextCnt, listCap);
-      if (sh.far.count[0] > QCAP - BLOCK) queue_flush(sh.far, far2, far2Cnt, listCap);
-    }
-  }
-  if (tm.tid == 0) {
-    sh.sliceList[0][tm.slice] = outNear;
-    sh.sliceCount[0][tm.slice] = nextCnt;
-    sh.sliceList[1][tm.slice] = far2;
-    sh.sliceCount[1][tm.slice] = far2Cnt;
-    sh.sliceFarSlot[tm.slice] = (e + 1) % 3;
-  }
-}
-
 template <bool HUBS, int LOCAL>
 __device__ int near_far(Desc *objsArg, int nobj, ObjCtl *ctlsArg,
                         int source, int generation, u64 initThr,
@@ -922,7 +900,8 @@ __device__ int near_far(Desc *objsArg, int nobj, ObjCtl *ctlsArg,
         push_team<HUBS, LOCAL>(objs[k], ctls[k], sh, k, it, source, gen, rank, blocks,
                                laneBits, localHops, n, listCap, tm);
       } else {
-        split_team(objs[k], ctls[k], sh, k, it, gen, rank, blocks, n, listCap, tm);
+        push_team<HUBS, LOCAL, true>(objs[k], ctls[k], sh, k, it, source, gen, rank, blocks,
+                                     laneBits, localHops, n, listCap, tm);
       }
       if (tm.tid == 0) {
         sh.sliceObj[team] = k;
