@@ -104,6 +104,15 @@ void mosp_ws_destroy(mosp_ws *ws);
 /* Co-resident workgroups of the persistent launch (0 = occupancy maximum). */
 int mosp_ws_set_grid(mosp_ws *ws, int32_t workgroups);
 int32_t mosp_ws_grid(const mosp_ws *ws);
+/* Longest wait of a workgroup at a grid barrier before the persistent
+ * launch gives up (ms, default 2000 or MOSP_BARRIER_TIMEOUT_MS).  No
+ * reference counterpart: the reference relies on its cooperative launch.
+ * Here the launch is cooperative when the runtime allows it and otherwise a
+ * plain launch of the same occupancy-bounded grid; a launch whose workgroups
+ * are not all resident then ends with an error (no result written; the
+ * workspace's scratch is cleared before its next launch) instead of hanging. */
+int mosp_ws_set_barrier_timeout(mosp_ws *ws, double ms);
+
 /* Near-far width of the combined search in the fused launch when none is
  * given: the reference's rule (src/sospUpdateGpu.cu:607-616, evaluated on the
  * device over the combined graph) times `scale` (>= 1; default 1).  A tuning
@@ -241,6 +250,10 @@ int mosp_batch_set_cost_model(mosp_batch *b, double latency_items);
  * several batches at once on disjoint shares of the CUs).  *kernel_ms
  * (optional) = device time of the pass. */
 int mosp_batch_run(mosp_batch *b, int32_t use_graph, float *kernel_ms);
+/* Longest grid-barrier wait of the batch's launches (ms, as
+ * mosp_ws_set_barrier_timeout); a run that gives up returns 0 and the next
+ * run starts from cleared scratch.  No reference counterpart. */
+int mosp_batch_set_barrier_timeout(mosp_batch *b, double ms);
 /* After a run: 0 with a message if an instance's old tree did not fit its
  * graph (the check mosp_update_fused makes after every launch). */
 int mosp_batch_check(mosp_batch *b);
@@ -406,6 +419,22 @@ int64_t mosp_apply_batch(int32_t n, int32_t k, const int32_t *row_ptr,
                          const int32_t *del_to, int32_t *out_row_ptr,
                          int32_t *out_col_ind, int32_t *out_weights,
                          uint32_t *inc_mask);
+
+/* Lowest-id tie rule on host arrays: parent[v] becomes the smallest u over
+ * the tight edges u -> v of objective `objective` (d[u] + w = d[v]) and its
+ * current parent, for every vertex with a parent (v != source).  Replaces
+ * canonicalizeTree (src/mospUpdate.cu:40-55). */
+void mosp_canonicalize_tree(int32_t n, int32_t k, const int32_t *row_ptr,
+                            const int32_t *col_ind, const int32_t *weights,
+                            int32_t objective, int32_t source, const int64_t *dist,
+                            int32_t *parent);
+
+/* MOSP path costs on the host (costs[v*K + k], as mosp_path_costs_gpu):
+ * 1, or 0 if a tree edge is missing.  Replaces the reference's host
+ * mospPathCosts (src/combinedGraphGpu.cu:161-208). */
+int mosp_path_costs_host(int32_t n, int32_t k, const int32_t *row_ptr,
+                         const int32_t *col_ind, const int32_t *weights,
+                         const int32_t *parent, int32_t source, int64_t *costs);
 
 /* The in-memory MOSP update on host arrays (src/mospUpdate.cu:57-225):
  * apply, upload, K updates + combined step on the GPU, download.

@@ -95,6 +95,10 @@ struct mosp_batch {
   int graphMode = 0;
   int hubs = 0;  // some instance's graph has a vertex of degree > HUB_DEGREE
   bool dirty = true;
+  unsigned *dAbort = nullptr;  // set by a workgroup whose barrier wait ran out
+  double barrierMs = -1;       // longest barrier wait (< 0: the default)
+  bool scrub = false;          // a run gave up: clear the scratch before the next
+  size_t usedSlab = 0;         // scratch bytes of the current instances
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   float lastMs = 0;
 };
@@ -114,6 +118,7 @@ void release(mosp_batch *b) {
   (void)hipFree(b->slab);
   (void)hipFree(b->dCtl);
   (void)hipFree(b->dGen);
+  (void)hipFree(b->dAbort);
   (void)hipFree(b->dDesc);
   (void)hipFree(b->dArgs);
   (void)hipFree(b->dGroups);
@@ -210,6 +215,8 @@ int setup(mosp_batch *b, int maxInstances, int maxDescs, size_t slabBytes) {
   HIP_TRY(hipMalloc(&b->slab, b->capSlab));
   HIP_TRY(hipMalloc(&b->dCtl, sizeof(Ctl) * maxInstances));
   HIP_TRY(hipMalloc(&b->dGen, sizeof(int) * maxInstances));
+  HIP_TRY(hipMalloc(&b->dAbort, sizeof(unsigned)));
+  HIP_TRY(hipMemset(b->dAbort, 0, sizeof(unsigned)));
   HIP_TRY(hipMalloc(&b->dDesc, sizeof(ObjDesc) * maxDescs));
   HIP_TRY(hipMalloc(&b->dArgs, sizeof(LaunchArgs) * maxInstances));
   HIP_TRY(hipMalloc(&b->dGroups, sizeof(BatchGroup) * std::max(maxInstances, b->maxBlocks)));
@@ -259,6 +266,8 @@ int load(mosp_batch *b, const mosp_batch_instance *spec, int count) {
   if (hubs_override() >= 0) b->hubs = hubs_override();
   HIP_TRY(hipMemsetAsync(b->slab, 0, std::max<size_t>(slabBytes, 256), b->stream));
   HIP_TRY(hipMemsetAsync(b->dGen, 0, sizeof(int) * count, b->stream));
+  b->usedSlab = std::max<size_t>(slabBytes, 256);
+  b->scrub = false;
   HIP_TRY(hipStreamSynchronize(b->stream));
   b->dirty = true;
   return 1;
@@ -460,6 +469,9 @@ int prepare(mosp_batch *b) {
     a.hopsSearch = b->hopsSearch;
     a.genDev = b->dGen + i;
     a.ctl = b->dCtl + i;
+    a.abortWord = b->dAbort;
+    a.barrierTicks =
+        barrier_ticks(b->barrierMs >= 0 ? b->barrierMs : barrier_timeout_ms_default(), b->clockKhz);
     a.cParents = I.spec.d_parents;
     a.cK = K;
     a.cBase = static_cast<int>(I.scale * (K + 1));
@@ -501,7 +513,7 @@ int capture(mosp_batch *b) {
   void *params[] = {&b->dArgs, &b->dGroups, &b->dGroupOf, &b->dRankOf};
   for (int mode = 1; mode <= 2; ++mode) {
     HIP_TRY(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
-    hipError_t err = launch_ctl_reset_many(b->dCtl, count, b->stream);
+    hipError_t err = launch_ctl_reset_many(b->dCtl, count, b->stream, b->dAbort);
     if (err == hipSuccess) {
       err = mode == 1 ? hipLaunchCooperativeKernel(sosp_batched_symbol(b->hubs != 0), dim3(b->workgroups),
                                                    dim3(BLOCK), params, 0, b->stream)
@@ -643,6 +655,13 @@ extern "C" int mosp_batch_run(mosp_batch *b, int32_t use_graph, float *kernel_ms
   if (b == nullptr) return fail("Error: no batch.");
   if (b->dirty && !prepare(b)) return 0;
   const int count = static_cast<int>(b->inst.size());
+  if (b->scrub) {
+    // The last run ended early: its stamps, far marks and invalidation marks
+    // are not in the state the next run relies on.
+    HIP_TRY(hipMemsetAsync(b->slab, 0, b->usedSlab, b->stream));
+    HIP_TRY(hipMemsetAsync(b->dGen, 0, sizeof(int) * std::max(count, 1), b->stream));
+    b->scrub = false;
+  }
   if (use_graph == 1) {
     if (b->exec == nullptr && !capture(b)) return 0;
     HIP_TRY(hipEventRecord(b->ev0, b->stream));
@@ -653,20 +672,36 @@ extern "C" int mosp_batch_run(mosp_batch *b, int32_t use_graph, float *kernel_ms
     // occupancy-bounded grid (co-resident as long as the launches running
     // together fit the device, e.g. slots of a server splitting the CUs).
     void *params[] = {&b->dArgs, &b->dGroups, &b->dGroupOf, &b->dRankOf};
-    HIP_TRY(launch_ctl_reset_many(b->dCtl, count, b->stream));
+    HIP_TRY(launch_ctl_reset_many(b->dCtl, count, b->stream, b->dAbort));
     HIP_TRY(hipEventRecord(b->ev0, b->stream));
     if (use_graph == 2) {
       HIP_TRY(hipLaunchKernel(sosp_batched_symbol(b->hubs != 0), dim3(b->workgroups), dim3(BLOCK), params, 0,
                               b->stream));
     } else {
-      HIP_TRY(hipLaunchCooperativeKernel(sosp_batched_symbol(b->hubs != 0), dim3(b->workgroups),
-                                         dim3(BLOCK), params, 0, b->stream));
+      HIP_TRY(launch_persistent(sosp_batched_symbol(b->hubs != 0), dim3(b->workgroups),
+                                dim3(BLOCK), params, b->stream));
     }
     HIP_TRY(hipEventRecord(b->ev1, b->stream));
   }
+  unsigned gaveUp = 0;
+  HIP_TRY(hipMemcpyAsync(&gaveUp, b->dAbort, sizeof(unsigned), hipMemcpyDeviceToHost, b->stream));
   HIP_TRY(hipStreamSynchronize(b->stream));
   HIP_TRY(hipEventElapsedTime(&b->lastMs, b->ev0, b->ev1));
   if (kernel_ms != nullptr) *kernel_ms = b->lastMs;
+  if (gaveUp != 0) {
+    b->scrub = true;
+    return fail("Error: a grid barrier of the batched launch waited longer than its limit and "
+                "the launch gave up (its workgroups were not all resident); no result was "
+                "written.");
+  }
+  return 1;
+}
+
+extern "C" int mosp_batch_set_barrier_timeout(mosp_batch *b, double ms) {
+  clear_error();
+  if (b == nullptr || !(ms >= 0)) return fail("Error: the barrier timeout must be >= 0 ms.");
+  b->barrierMs = ms;
+  b->dirty = true;  // the limit is part of every instance's launch arguments
   return 1;
 }
 

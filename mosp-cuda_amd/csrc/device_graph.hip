@@ -28,6 +28,7 @@
 #include <limits>
 #include <numeric>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "engine.hpp"
@@ -1058,6 +1059,50 @@ unsigned *degree_word(mosp_device_graph *g) {
   return reinterpret_cast<unsigned *>(g->stats + 2 * MOSP_MAX_OBJ);
 }
 
+// The rules readCsrGraph enforces on a text graph, checked on the caller's
+// arrays before anything is copied (the reverse build indexes by them, so a
+// bad column would be an out-of-bounds device access): rowPtr[0] == 0,
+// non-decreasing row pointers, columns in [0, n), weights >= 1.  The arrays
+// are split over threads.  0 with the first violation recorded.
+int host_csr_valid(int n, int K, const int32_t *rowPtr, const int32_t *colInd,
+                   const int32_t *weights) {
+  if (rowPtr == nullptr || rowPtr[0] != 0 || rowPtr[n] < 0) {
+    return fail("Error: invalid graph: the row pointers must start at 0 and end at m >= 0.");
+  }
+  const long long m = rowPtr[n];
+  if (m > 0 && (colInd == nullptr || weights == nullptr)) {
+    return fail("Error: invalid graph: missing column or weight array.");
+  }
+  const unsigned workers =
+      m < (1 << 20) ? 1u : std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+  std::vector<int> bad(workers, 0);  // 1: row pointers, 2: columns, 3: weights
+  auto check = [&](unsigned t) {
+    const long long r0 = static_cast<long long>(n) * t / workers;
+    const long long r1 = static_cast<long long>(n) * (t + 1) / workers;
+    for (long long v = r0; v < r1; ++v) {
+      if (rowPtr[v + 1] < rowPtr[v]) { bad[t] = 1; return; }
+    }
+    const long long e0 = m * t / workers, e1 = m * (t + 1) / workers;
+    for (long long e = e0; e < e1; ++e) {
+      if (static_cast<uint32_t>(colInd[e]) >= static_cast<uint32_t>(n)) { bad[t] = 2; return; }
+    }
+    const long long w0 = e0 * K, w1 = e1 * K;
+    for (long long i = w0; i < w1; ++i) {
+      if (weights[i] < 1) { bad[t] = 3; return; }
+    }
+  };
+  std::vector<std::thread> threads;
+  for (unsigned t = 1; t < workers; ++t) threads.emplace_back(check, t);
+  check(0);
+  for (std::thread &t : threads) t.join();
+  for (int b : bad) {
+    if (b == 1) return fail("Error: invalid graph: the row pointers decrease.");
+    if (b == 2) return fail("Error: invalid graph: a column index is outside [0, n).");
+    if (b == 3) return fail("Error: invalid graph: a weight is below 1.");
+  }
+  return 1;
+}
+
 }  // namespace
 
 extern "C" mosp_device_graph *mosp_device_graph_create(void) { return new mosp_device_graph(); }
@@ -1068,6 +1113,9 @@ extern "C" mosp_device_graph *mosp_device_graph_upload(int32_t n, int32_t K, con
   clear_error();
   if (n <= 0 || K <= 0 || K > MOSP_MAX_OBJ) {
     fail("Error: invalid graph dimensions.");
+    return nullptr;
+  }
+  if (!host_csr_valid(n, K, rowPtr, colInd, weights)) {
     return nullptr;
   }
   auto *g = new mosp_device_graph();

@@ -131,6 +131,8 @@ struct mosp_ws {
   int hopsUpdate = kHopsUpdate;  // local relaxation rounds per near-far iteration
   int hopsSearch = kHopsSearch;
   int listCap = 0;          // near / far list entries: 0 = n
+  double barrierMs = -1;    // longest grid-barrier wait before a launch gives up (< 0: default)
+  bool scrub = false;       // a launch gave up: clear the scratch before the next
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
 };
 
@@ -349,9 +351,42 @@ void fill_stats(const Ctl &c, int k, const ObjDesc &d, float ms,
 
 // Upload descriptors, reset the control block, run the cooperative launch,
 // read the control block back.  Synchronous on `stream`.
+double barrier_timeout_ms_default() {
+  const char *env = std::getenv("MOSP_BARRIER_TIMEOUT_MS");
+  return env != nullptr && *env != '\0' ? std::max(0.0, std::atof(env)) : 2000.0;
+}
+
+unsigned long long barrier_ticks(double ms, double clockKhz) {
+  const double ticks = ms * clockKhz;
+  return ticks >= 1.8e19 ? ~0ULL : static_cast<unsigned long long>(ticks);
+}
+
+hipError_t launch_persistent(const void *kernel, dim3 grid, dim3 block, void **params,
+                             hipStream_t stream) {
+  const hipError_t err = hipLaunchCooperativeKernel(kernel, grid, block, params, 0, stream);
+  if (err != hipErrorIllegalState) {
+    return err;
+  }
+  (void)hipGetLastError();  // the refused cooperative launch
+  return hipLaunchKernel(kernel, grid, block, params, 0, stream);
+}
+
 int run_launch(mosp_ws *ws, LaunchArgs &args, int descCount,
                hipStream_t stream, float *kernelMs) {
+  if (ws->scrub) {
+    // The last launch ended early: its stamps, far marks and invalidation
+    // marks are not in the state the next launch relies on.
+    if (ws->slab != nullptr) {
+      HIP_TRY(hipMemsetAsync(ws->slab, 0, state_bytes(std::max(ws->capacity, 1)) * ws->slots,
+                             stream));
+    }
+    HIP_TRY(hipMemsetAsync(ws->genDev, 0, sizeof(int), stream));
+    ws->scrub = false;
+  }
   args.ctl = ws->dCtl;
+  args.abortWord = &ws->dCtl->gaveUp;
+  const double waitMs = ws->barrierMs >= 0 ? ws->barrierMs : barrier_timeout_ms_default();
+  args.barrierTicks = barrier_ticks(waitMs, ws->clockKhz);
   args.genDev = ws->genDev;
   args.laneBits = ws->laneBits;
   args.hopsUpdate = ws->hopsUpdate;
@@ -367,9 +402,8 @@ int run_launch(mosp_ws *ws, LaunchArgs &args, int descCount,
   void *params[] = {&args};
   HIP_TRY(hipEventRecord(ws->ev0, stream));
   const int forced = hubs_override();
-  HIP_TRY(hipLaunchCooperativeKernel(sosp_kernel_symbol(forced >= 0 ? forced != 0 : args.hubs != 0),
-                                     dim3(ws->gridBlocks), dim3(BLOCK), params,
-                                     0, stream));
+  HIP_TRY(launch_persistent(sosp_kernel_symbol(forced >= 0 ? forced != 0 : args.hubs != 0),
+                            dim3(ws->gridBlocks), dim3(BLOCK), params, stream));
   HIP_TRY(hipEventRecord(ws->ev1, stream));
   HIP_TRY(hipMemcpyAsync(ws->hCtl, ws->dCtl, CTL_READBACK,
                          hipMemcpyDeviceToHost, stream));
@@ -398,6 +432,13 @@ int run_launch(mosp_ws *ws, LaunchArgs &args, int descCount,
 #endif
   if (kernelMs != nullptr) {
     *kernelMs = ms;
+  }
+  if (ws->hCtl->gaveUp != 0) {
+    ws->scrub = true;
+    return fail("Error: a grid barrier of the persistent launch waited more than " +
+                std::to_string(waitMs) +
+                " ms and the launch gave up (its workgroups were not all resident: the device "
+                "was running other persistent work); no result was written.");
   }
   for (int k = 0; k < args.nobj; ++k) {
     if (!input_tree_ok(ws->hCtl->obj[k].overflow, "")) {
@@ -517,6 +558,15 @@ extern "C" int mosp_ws_set_grid(mosp_ws *ws, int32_t workgroups) {
 }
 
 extern "C" int32_t mosp_ws_grid(const mosp_ws *ws) { return ws->gridBlocks; }
+
+extern "C" int mosp_ws_set_barrier_timeout(mosp_ws *ws, double ms) {
+  clear_error();
+  if (ws == nullptr || !(ms >= 0)) {
+    return fail("Error: the barrier timeout must be >= 0 ms.");
+  }
+  ws->barrierMs = ms;
+  return 1;
+}
 
 extern "C" int mosp_ws_set_combined_delta_scale(mosp_ws *ws, int32_t scale) {
   clear_error();

@@ -120,6 +120,7 @@ struct Ctl {
   u64 clkStart, clkSospEnd, clkEnd;  // wall_clock64 stamps
   u64 clk[16];             // phase ends (PHASE_* below), wall_clock64
   unsigned barriers;       // grid barriers executed by the launch
+  unsigned gaveUp;         // a workgroup's barrier wait ran out (the launch ended early)
   u64 prof[16];            // sub-phase clock sums (MOSP_SUBPROFILE builds)
   ObjCtl obj[MOSP_MAX_OBJ + 1];
   // Barrier mailboxes (device-only: not part of the host read-back).
@@ -146,6 +147,13 @@ struct LaunchArgs {
   int hubs;                // host only: launch the HUB_DEGREE-aware variant
   int *genDev;             // device-resident stamp generation (persists)
   Ctl *ctl;
+  // Bounded barrier waits: a workgroup that waits longer than barrierTicks
+  // (wall_clock64 ticks) at a grid barrier, or finds *abortWord set while it
+  // waits, sets *abortWord and ends; so does every other workgroup at its
+  // next wait.  A launch whose workgroups are not all resident then ends
+  // with an error instead of spinning for ever.
+  unsigned long long barrierTicks;
+  unsigned *abortWord;
   // Combined graph (PH_COMBINED): parents of cK trees, objective-major.
   const int *cParents;
   int cK;

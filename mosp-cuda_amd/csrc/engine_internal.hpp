@@ -34,7 +34,9 @@ namespace mosp {
 const void *sosp_kernel_symbol(bool hubs);
 const void *sosp_batched_symbol(bool hubs);
 hipError_t launch_ctl_reset(Ctl *ctl, hipStream_t stream);
-hipError_t launch_ctl_reset_many(Ctl *ctls, int count, hipStream_t stream);
+// abortWord (may be null): a launch-wide abort word cleared with the blocks.
+hipError_t launch_ctl_reset_many(Ctl *ctls, int count, hipStream_t stream,
+                                 unsigned *abortWord = nullptr);
 
 namespace detail {
 
@@ -53,6 +55,20 @@ constexpr int kHopsSearch = 6;
 // MOSP_LOCAL_HOPS=u,s overrides the local relaxation rounds per near-far
 // iteration of the K updates (u) and of the searches (s) (tuning).
 void local_hops_override(int &update, int &search);
+// Longest wait at a grid barrier before a launch gives up (ms): 2000, or
+// MOSP_BARRIER_TIMEOUT_MS.  Far above any legitimate wait (a phase's
+// imbalance is microseconds to milliseconds).
+double barrier_timeout_ms_default();
+// wall_clock64 ticks of `ms` on the current device (rate in kHz).
+unsigned long long barrier_ticks(double ms, double clockKhz);
+// Launches `kernel` cooperatively; if the runtime refuses the cooperative
+// launch in this process's present state (another process holds the
+// device's cooperative-launch resources), launches the same grid plainly.
+// The grid never exceeds the occupancy limit, so on a device running no
+// other persistent kernel every workgroup is resident; otherwise the
+// bounded barrier waits end the launch with an error.
+hipError_t launch_persistent(const void *kernel, dim3 grid, dim3 block, void **params,
+                             hipStream_t stream);
 // MOSP_LIST_CAP=c caps the near / far lists at c entries instead of n, so
 // that small inputs take the sweep that follows a full list (tests); 0 unset.
 int list_cap_override();

@@ -2,16 +2,24 @@
 // combinedGraphGpu.h, parallelCombinedGraph.h, deviceGraph.h, mospUpdate.h)
 // over the C ABI, and the host pipeline entry mosp_update_host.
 //
-// Semantics follow the reference's src/mospUpdate.cu:40-225 (root checks,
-// per-objective maxWeight / weightSum / changed lists, timing regions) and
-// src/combinedGraphGpu.cu:129-208 (Pref scale, edge weight, path costs).
+// mospUpdate computes what the reference's src/mospUpdate.cu:57-225 computes
+// (same checks, per-objective bounds, changed lists, outputs and timed
+// regions) as a device pipeline of this library: the original graph goes
+// to HBM once, the batch is applied there (mosp_device_graph_apply, equal to
+// the host applyChangeBatch), the weight statistics are reduced there, the K
+// updates and Steps 2-3 run as one persistent launch, and the updated graph
+// comes back with the trees.  Helpers follow src/combinedGraphGpu.cu:129-208
+// (Pref scale, edge weight, path costs) and src/mospUpdate.cu:40-55
+// (canonical tie rule).
 #include <hip/hip_runtime_api.h>
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <iostream>
 #include <numeric>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "combinedGraphGpu.h"
@@ -23,6 +31,7 @@
 #include "mosp_hip.h"
 #include "parallelCombinedGraph.h"
 #include "sospUpdateGpu.h"
+#include "stageTimer.h"
 
 namespace {
 
@@ -101,39 +110,49 @@ long long combinedEdgeWeight(unsigned int mask, const std::vector<int> &pref, in
                                     K, scale);
 }
 
+// Costs by memoised ancestor walks: a vertex's chain is followed up to the
+// source or a vertex already resolved, then resolved downwards.  A chain
+// that ends at another root, or runs into itself, is not reachable (INF),
+// and its edges are not needed -- the vertices whose tree edge is looked up
+// are exactly those reachable from the source, as in a walk of the tree
+// from the source down.
 bool mospPathCosts(const CsrGraph &graph, const std::vector<int> &parent, int source,
                    std::vector<long long> &costs) {
   const int n = graph.numberOfNodes, K = graph.numberOfObjectives;
-  costs.assign(static_cast<size_t>(n) * K, DISTANCE_INF);
-  // Children lists of the tree, then a breadth-first walk from the source.
-  std::vector<int> first(n + 1, 0), kids(n);
-  for (int v = 0; v < n; ++v) {
-    if (v != source && parent[v] >= 0) ++first[parent[v] + 1];
-  }
-  std::partial_sum(first.begin(), first.end(), first.begin());
-  std::vector<int> fill(first.begin(), first.end() - 1);
-  for (int v = 0; v < n; ++v) {
-    if (v != source && parent[v] >= 0) kids[fill[parent[v]]++] = v;
-  }
-  for (int k = 0; k < K; ++k) costs[static_cast<size_t>(source) * K + k] = 0;
-  std::vector<int> queue{source};
-  for (size_t i = 0; i < queue.size(); ++i) {
-    const int p = queue[i];
-    for (int c = first[p]; c < first[p + 1]; ++c) {
-      const int v = kids[c];
-      int edge = -1;
-      for (int e = graph.rowPtr[p]; e < graph.rowPtr[p + 1]; ++e) {
-        if (graph.colInd[e] == v) {
-          edge = e;
-          break;
-        }
+  const size_t kk = static_cast<size_t>(K);
+  costs.assign(static_cast<size_t>(n) * kk, DISTANCE_INF);
+  enum : char { UNSEEN = 0, ON_CHAIN = 1, REACHED = 2, CUT = 3 };
+  std::vector<char> state(static_cast<size_t>(n), UNSEEN);
+  state[source] = REACHED;
+  std::fill(costs.begin() + static_cast<std::ptrdiff_t>(source * kk),
+            costs.begin() + static_cast<std::ptrdiff_t>((source + 1) * kk), 0LL);
+  std::vector<int> chain;
+  for (int start = 0; start < n; ++start) {
+    int x = start;
+    chain.clear();
+    while (state[x] == UNSEEN) {
+      state[x] = ON_CHAIN;
+      chain.push_back(x);
+      if (parent[x] < 0) break;
+      x = parent[x];
+    }
+    // The walk stopped at a resolved vertex (REACHED or CUT), at a root other
+    // than the source (on the chain), or on the chain itself (a cycle).
+    const bool reached = state[x] == REACHED;
+    for (auto it = chain.rbegin(); it != chain.rend(); ++it) {
+      const int v = *it;
+      if (!reached) {
+        state[v] = CUT;
+        continue;
       }
-      if (edge < 0) return false;
-      for (int k = 0; k < K; ++k) {
-        costs[static_cast<size_t>(v) * K + k] =
-            costs[static_cast<size_t>(p) * K + k] + graph.weight(edge, k);
+      const int p = parent[v];
+      int edge = graph.rowPtr[p];
+      while (edge < graph.rowPtr[p + 1] && graph.colInd[edge] != v) ++edge;
+      if (edge == graph.rowPtr[p + 1]) return false;  // the tree edge p -> v is missing
+      for (size_t k = 0; k < kk; ++k) {
+        costs[v * kk + k] = costs[p * kk + k] + graph.weight(edge, static_cast<int>(k));
       }
-      queue.push_back(v);
+      state[v] = REACHED;
     }
   }
   return true;
@@ -287,199 +306,313 @@ bool uploadDeviceGraph(const CsrGraph &graph, DeviceGraph &device) {
 // ------------------------------------------------------------ MOSP pipeline
 
 double MospTimings::gpuCompute() const {
-  double total = combined;
-  for (double t : objectives) total += t;
-  return total;
+  return std::accumulate(objectives.begin(), objectives.end(), combined);
 }
 
+// The lowest-id tie rule as a per-vertex minimum: parent[v] becomes the
+// smallest u over every tight edge u -> v (d[u] + w = d[v]) and its current
+// parent, for every vertex with a parent; vertices without one (the source,
+// unreachable vertices) keep -1.  Rows are split over threads; the minimum
+// is taken with atomic compare-exchange, so the order does not matter --
+// which is why one sequential pass over the out-edges
+// (src/mospUpdate.cu:40-55) gives the same tree.
 void canonicalizeTree(const CsrGraph &graph, int objective, int source,
                       const std::vector<long long> &distances, int *parent) {
-  for (int u = 0; u < graph.numberOfNodes; ++u) {
-    const long long du = distances[u];
-    if (du >= DISTANCE_INF / 2) continue;
-    for (int e = graph.rowPtr[u]; e < graph.rowPtr[u + 1]; ++e) {
-      const int v = graph.colInd[e];
-      if (v != source && u < parent[v] && du + graph.weight(e, objective) == distances[v]) {
-        parent[v] = u;
+  const int n = graph.numberOfNodes;
+  std::vector<std::atomic<int>> low(static_cast<size_t>(n));
+  for (int v = 0; v < n; ++v) low[v].store(parent[v], std::memory_order_relaxed);
+  const unsigned workers = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+  const int parts = n >= (1 << 16) ? static_cast<int>(workers) : 1;
+  auto rows = [&](int part) {
+    const int lo = static_cast<int>(static_cast<long long>(n) * part / parts);
+    const int hi = static_cast<int>(static_cast<long long>(n) * (part + 1) / parts);
+    for (int u = lo; u < hi; ++u) {
+      const long long du = distances[u];
+      if (du >= DISTANCE_INF / 2) continue;
+      for (int e = graph.rowPtr[u]; e < graph.rowPtr[u + 1]; ++e) {
+        const int v = graph.colInd[e];
+        if (v == source || du + graph.weight(e, objective) != distances[v]) continue;
+        int cur = low[v].load(std::memory_order_relaxed);
+        while (cur >= 0 && u < cur &&
+               !low[v].compare_exchange_weak(cur, u, std::memory_order_relaxed)) {
+        }
       }
     }
-  }
+  };
+  std::vector<std::thread> threads;
+  for (int t = 1; t < parts; ++t) threads.emplace_back(rows, t);
+  rows(0);
+  for (std::thread &t : threads) t.join();
+  for (int v = 0; v < n; ++v) parent[v] = low[v].load(std::memory_order_relaxed);
 }
+
+namespace {
+
+// Device state of one mospUpdate call, released on every return path.
+struct UpdatePipeline {
+  mosp_device_graph *original = nullptr;
+  mosp_device_graph *updated = nullptr;
+  SospWorkspace ws;
+  DeviceArray<long long> dist, combDist;
+  DeviceArray<int> parents, combParent;
+  DeviceArray<int> lists;  // per objective: changed tails, changed heads; then the insert heads
+  ~UpdatePipeline() {
+    mosp_device_graph_free(updated);
+    mosp_device_graph_free(original);
+  }
+};
+
+// The check of src/mospUpdate.cu:77-92: every initial tree is rooted at the
+// source (distance 0, no parent); the message says what was found.
+bool trees_rooted_at(int source, int n, int K, const std::vector<long long> &dist,
+                     const std::vector<int> &par) {
+  for (int k = 0; k < K; ++k) {
+    const size_t s = static_cast<size_t>(k) * n + source;
+    if (dist[s] == 0 && par[s] == -1) continue;
+    const std::string d = dist[s] >= DISTANCE_INF / 2 ? "INF" : std::to_string(dist[s]);
+    std::cerr << "Error: the initial tree of objective " << k
+              << " is not rooted at the source " << source << " (its distance is " << d
+              << ", its parent " << par[s] << "); compute the trees for this source.\n";
+    return false;
+  }
+  return true;
+}
+
+// Per objective k < K: the changed edges (every deletion, then the
+// insertions whose weight-increase bit k is set; src/mospUpdate.cu:124-136),
+// as tails then heads, followed by the insert heads shared by all.  Returns
+// the host image and each objective's [offset, count).
+std::vector<int> changed_lists(const ChangeBatch &b, int K, std::vector<size_t> &at,
+                               std::vector<int> &count) {
+  std::vector<int> img;
+  at.assign(K, 0);
+  count.assign(K, 0);
+  for (int k = 0; k < K; ++k) {
+    std::vector<int> tails(b.deleteFrom), heads(b.deleteTo);
+    for (int i = 0; i < b.numberOfInserts(); ++i) {
+      if ((b.weightIncreaseMask[i] >> k) & 1u) {
+        tails.push_back(b.insertFrom[i]);
+        heads.push_back(b.insertTo[i]);
+      }
+    }
+    at[k] = img.size();
+    count[k] = static_cast<int>(tails.size());
+    img.insert(img.end(), tails.begin(), tails.end());
+    img.insert(img.end(), heads.begin(), heads.end());
+  }
+  img.insert(img.end(), b.insertTo.begin(), b.insertTo.end());
+  return img;
+}
+
+void record_counters(const MospResult &r) {
+  if (!instrumentationEnabled()) return;
+  for (int k = 0; k < r.numberOfObjectives; ++k) {
+    const SospStats &s = r.objectiveStats[k];
+    const std::string o = "obj" + std::to_string(k) + "/";
+    recordCounter(o + "invalidated", s.invalidated);
+    recordCounter(o + "jump_rounds", s.jumpRounds);
+    recordCounter(o + "iterations", s.iterations);
+    recordCounter(o + "epochs", s.epochs);
+    recordCounter(o + "pushes", static_cast<double>(s.pushes));
+    recordCounter(o + "packed_parents", s.packedParents ? 1 : 0);
+  }
+  const CombineStats &c = r.combineStats;
+  recordCounter("comb/edges", c.numberOfEdges);
+  recordCounter("comb/L", static_cast<double>(c.scale));
+  recordCounter("comb/delta", static_cast<double>(c.delta));
+  recordCounter("comb/iterations", c.search.iterations);
+  recordCounter("comb/pushes", static_cast<double>(c.search.pushes));
+}
+
+}  // namespace
 
 bool mospUpdate(const CsrGraph &original, ChangeBatch &batch,
                 const std::vector<long long> &initialDistances,
                 const std::vector<int> &initialParents, const MospOptions &options,
                 CsrGraph &updated, MospResult &result) {
+  using clock = std::chrono::steady_clock;
   const int n = original.numberOfNodes;
   const int KG = original.numberOfObjectives;
   const int K = options.numberOfObjectives > 0 ? std::min(options.numberOfObjectives, KG) : KG;
-  const size_t treeSize = static_cast<size_t>(K) * n;
+  const int s = options.source;
+  const size_t kn = static_cast<size_t>(K) * n;
   result = MospResult();
   result.numberOfObjectives = K;
   MospTimings &tm = result.timings;
-  if (n <= 0 || K <= 0 || options.source < 0 || options.source >= n ||
-      initialDistances.size() < treeSize || initialParents.size() < treeSize) {
-    std::cerr << "Error: invalid MOSP update input.\n";
+  const std::vector<int> &pref = options.preferences;
+
+  if (n <= 0 || K <= 0 || s < 0 || s >= n || initialDistances.size() < kn ||
+      initialParents.size() < kn) {
+    std::cerr << "Error: invalid MOSP update input (graph, objectives, source or tree sizes).\n";
     return false;
   }
-  for (int k = 0; k < K; ++k) {  // trees rooted at the source (mospUpdate.cu:77-92)
-    const size_t at = static_cast<size_t>(k) * n + options.source;
-    if (initialDistances[at] != 0 || initialParents[at] != -1) {
-      std::cerr << "Error: the initial tree of objective " << k
-                << " is not rooted at the source " << options.source << " (its distance is ";
-      if (initialDistances[at] >= DISTANCE_INF / 2) {
-        std::cerr << "INF";
-      } else {
-        std::cerr << initialDistances[at];
-      }
-      std::cerr << ", its parent " << initialParents[at]
-                << "); compute the trees for this source.\n";
-      return false;
-    }
-  }
-  if (preferenceScale(options.preferences, K) == 0) {
+  if (!trees_rooted_at(s, n, K, initialDistances, initialParents)) return false;
+  if (preferenceScale(pref, K) == 0) {
     std::cerr << "Error: invalid preference vector (need K values >= 1).\n";
     return false;
   }
-
-  auto t0 = std::chrono::steady_clock::now();
-  if (!applyChangeBatch(original, batch, updated)) return false;
-  tm.applyBatch = ms_since(t0);
-
-  // Per objective: largest weight before and after the batch, weight sum of
-  // the original edges, changed edges (mospUpdate.cu:104-136).
-  const size_t m = static_cast<size_t>(original.numberOfEdges());
-  std::vector<long long> maxWeight(K, 1), weightSum(K, 0);
-  for (size_t e = 0; e < m; ++e) {
-    for (int k = 0; k < K; ++k) {
-      const int w = original.weights[e * KG + k];
-      maxWeight[k] = std::max(maxWeight[k], static_cast<long long>(w));
-      weightSum[k] += w;
-    }
-  }
-  for (int i = 0; i < batch.numberOfInserts(); ++i) {
-    for (int k = 0; k < K; ++k) {
-      maxWeight[k] = std::max(
-          maxWeight[k], static_cast<long long>(batch.insertWeights[static_cast<size_t>(i) * KG + k]));
-    }
-  }
-  std::vector<std::vector<int>> chFrom(K), chTo(K);
-  for (int k = 0; k < K; ++k) {
-    chFrom[k] = batch.deleteFrom;
-    chTo[k] = batch.deleteTo;
-    for (int i = 0; i < batch.numberOfInserts(); ++i) {
-      if (batch.weightIncreaseMask[i] & (1u << k)) {
-        chFrom[k].push_back(batch.insertFrom[i]);
-        chTo[k].push_back(batch.insertTo[i]);
-      }
-    }
-  }
-
-  t0 = std::chrono::steady_clock::now();
-  DeviceGraph graph;
-  DeviceArray<long long> dDist, dCombDist(n);
-  DeviceArray<int> dPar, dCombPar(n), dHeads;
-  std::vector<DeviceArray<int>> dFrom(K), dTo(K);
-  SospWorkspace ws;
-  {
-    std::vector<long long> dist(initialDistances.begin(), initialDistances.begin() + treeSize);
-    std::vector<int> par(initialParents.begin(), initialParents.begin() + treeSize);
-    bool ok = uploadDeviceGraph(updated, graph) && dDist.upload(dist) && dPar.upload(par) &&
-              dHeads.upload(batch.insertTo) && ws.reserve(n, options.fused ? K : 1);
-    for (int k = 0; ok && k < K; ++k) {
-      ok = dFrom[k].upload(chFrom[k]) && dTo[k].upload(chTo[k]);
-    }
-    if (!ok || hipDeviceSynchronize() != hipSuccess) {
-      std::cerr << "Error: could not copy the MOSP inputs to the GPU.\n";
-      return false;
-    }
-    if (!mosp_ws_set_combined_delta_scale(ws.handle, std::max(1, options.combinedDeltaScale))) {
-      std::cerr << mosp_last_error() << "\n";
-      return false;
-    }
-  }
-  tm.upload = ms_since(t0);
-
-  const long long edges = std::max<long long>(updated.numberOfEdges(), 1);
-  std::vector<long long> delta(K);
-  std::vector<mosp_changes> changes(K);
-  for (int k = 0; k < K; ++k) {
-    const long long rule =
-        defaultDelta(m > 0 ? static_cast<long long>(m) : edges, n, weightSum[k]);
-    delta[k] = options.delta > 0
-                   ? options.delta
-                   : std::max(1LL, static_cast<long long>(static_cast<double>(rule) *
-                                                          std::max(options.deltaScale, 0.0)));
-    changes[k] = mosp_changes{dFrom[k].data(), dTo[k].data(), static_cast<int32_t>(chFrom[k].size()),
-                              dHeads.data(), batch.numberOfInserts()};
-  }
-  result.objectiveStats.resize(K);
-  const std::vector<int> &pref = options.preferences;
-  if (options.fused) {
-    std::vector<mosp_sosp_stats> st(K);
-    mosp_combine_stats cs{};
-    t0 = std::chrono::steady_clock::now();
-    const bool ok = mosp_update_fused(
-        graph.handle, K, changes.data(), options.source,
-        reinterpret_cast<const int64_t *>(delta.data()),
-        reinterpret_cast<const int64_t *>(maxWeight.data()), pref.data(),
-        static_cast<int32_t>(pref.size()), ws.handle, nullptr, nullptr,
-        reinterpret_cast<int64_t *>(dDist.data()), dPar.data(),
-        reinterpret_cast<int64_t *>(dCombDist.data()), dCombPar.data(), st.data(), &cs,
-        nullptr);
-    const double total = ms_since(t0);
-    if (!ok) {
-      report_error();
-      std::cerr << "Error: MOSP update on the GPU failed.\n";
-      return false;
-    }
-    for (int k = 0; k < K; ++k) from_c(st[k], result.objectiveStats[k]);
-    CombineStats &c = result.combineStats;
-    c.numberOfEdges = cs.n_edges;
-    c.scale = cs.scale;
-    c.delta = cs.delta;
-    from_c(cs.search, c.search);
-    // Split the host wall time of the one launch by the kernel's own clock
-    // stamps (end of the K updates); gpuCompute() is the total either way.
-    const double phases = static_cast<double>(cs.sosp_phase_ms) + cs.combined_phase_ms;
-    const double sospFrac = phases > 0 ? cs.sosp_phase_ms / phases : 1.0;
-    tm.objectives.assign(K, total * sospFrac / K);
-    tm.combined = total * (1.0 - sospFrac);
-  } else {
-    for (int k = 0; k < K; ++k) {
-      t0 = std::chrono::steady_clock::now();
-      mosp_sosp_stats st{};
-      mosp_csr out{}, in{};
-      mosp_device_graph_view(graph.handle, k, &out, &in);
-      if (!mosp_sosp_update(&out, &in, &changes[k], options.source, delta[k], maxWeight[k],
-                            ws.handle,
-                            reinterpret_cast<int64_t *>(dDist.data()) + static_cast<size_t>(k) * n,
-                            dPar.data() + static_cast<size_t>(k) * n, &st, nullptr)) {
-        report_error();
-        std::cerr << "Error: SOSP update of objective " << k << " failed.\n";
-        return false;
-      }
-      tm.objectives.push_back(ms_since(t0));
-      from_c(st, result.objectiveStats[k]);
-    }
-    t0 = std::chrono::steady_clock::now();
-    CombineWorkspace cws;
-    if (!combinedGraphSospGpu(dPar.data(), n, K, pref, options.source, 0, cws, ws,
-                              dCombDist.data(), dCombPar.data(), &result.combineStats)) {
-      std::cerr << "Error: combined graph step failed.\n";
-      return false;
-    }
-    tm.combined = ms_since(t0);
-  }
-
-  t0 = std::chrono::steady_clock::now();
-  if (!dDist.download(result.distances) || !dPar.download(result.parents) ||
-      !dCombDist.download(result.combinedDistances) ||
-      !dCombPar.download(result.combinedParent)) {
-    std::cerr << "Error: could not copy the results from the GPU.\n";
+  if (batch.numberOfObjectives != KG) {
+    std::cout << "Error: change batch has " << batch.numberOfObjectives
+              << " objectives, graph has " << KG << ".\n";
     return false;
   }
-  tm.download = ms_since(t0);
+  if (KG > 32) {
+    std::cout << "Error: at most 32 objectives are supported.\n";
+    return false;
+  }
+  auto gpu_failed = [](const char *what) {
+    report_error();
+    std::cerr << "Error: " << what << " failed on the GPU.\n";
+    return false;
+  };
+
+  UpdatePipeline dev;
+  // Upload: the original graph (reverse CSR and weight columns built in HBM),
+  // the initial trees, the engine's workspace.
+  auto t0 = clock::now();
+  {
+    ScopedStage stage("upload", true);
+    dev.original = mosp_device_graph_upload(n, KG, original.rowPtr.data(),
+                                            original.colInd.data(), original.weights.data());
+    if (dev.original == nullptr) return gpu_failed("uploading the graph");
+    dev.updated = mosp_device_graph_create();
+    const std::vector<long long> d0(initialDistances.begin(), initialDistances.begin() + kn);
+    const std::vector<int> p0(initialParents.begin(), initialParents.begin() + kn);
+    if (!dev.dist.upload(d0) || !dev.parents.upload(p0) || !dev.combDist.allocate(n) ||
+        !dev.combParent.allocate(n) || !dev.ws.reserve(n, options.fused ? K : 1) ||
+        !mosp_ws_set_combined_delta_scale(dev.ws.handle, std::max(1, options.combinedDeltaScale))) {
+      return gpu_failed("uploading the trees");
+    }
+  }
+  tm.upload = std::chrono::duration<double, std::milli>(clock::now() - t0).count();
+
+  // The batch, applied to the graph in HBM; the weight-increase mask comes
+  // back to the batch.  Bounds from the original graph and the inserted
+  // weights (src/mospUpdate.cu:104-122), changed lists from the mask.
+  t0 = clock::now();
+  std::vector<long long> maxWeight(KG, 1), weightSum(KG, 0);
+  std::vector<size_t> listAt;
+  std::vector<int> listCount;
+  {
+    ScopedStage stage("apply_batch", true);
+    batch.weightIncreaseMask.assign(static_cast<size_t>(batch.numberOfInserts()), 0u);
+    if (!mosp_device_graph_apply(dev.original, batch.numberOfInserts(), batch.insertFrom.data(),
+                                 batch.insertTo.data(), batch.insertWeights.data(),
+                                 batch.numberOfDeletes(), batch.deleteFrom.data(),
+                                 batch.deleteTo.data(), dev.updated,
+                                 batch.weightIncreaseMask.data(), nullptr)) {
+      return gpu_failed("applying the change batch");
+    }
+    std::vector<int64_t> mx(KG, 0), sum(KG, 0);
+    if (!mosp_device_graph_weight_stats(dev.original, mx.data(), sum.data(), nullptr)) {
+      return gpu_failed("the weight statistics");
+    }
+    for (int k = 0; k < KG; ++k) {
+      maxWeight[k] = std::max<long long>(1, mx[k]);
+      weightSum[k] = sum[k];
+    }
+    for (size_t i = 0; i < batch.insertWeights.size(); ++i) {
+      const int k = static_cast<int>(i % KG);
+      maxWeight[k] = std::max<long long>(maxWeight[k], batch.insertWeights[i]);
+    }
+    if (!dev.lists.upload(changed_lists(batch, K, listAt, listCount))) {
+      return gpu_failed("uploading the changed edges");
+    }
+  }
+  tm.applyBatch = std::chrono::duration<double, std::milli>(clock::now() - t0).count();
+
+  // Width per objective: the reference's rule on the original graph
+  // (src/mospUpdate.cu:177-181), or the caller's.
+  const long long m0 = original.numberOfEdges();
+  const long long m = m0 > 0 ? m0 : std::max<int64_t>(1, mosp_device_graph_edges(dev.updated));
+  std::vector<long long> delta(K);
+  std::vector<mosp_changes> changes(K);
+  const int *heads = dev.lists.data() + (listAt.empty() ? 0 : listAt.back() + 2 * listCount.back());
+  for (int k = 0; k < K; ++k) {
+    const double rule = static_cast<double>(defaultDelta(m, n, weightSum[k]));
+    delta[k] = options.delta > 0 ? options.delta
+                                 : std::max(1LL, static_cast<long long>(
+                                                     rule * std::max(options.deltaScale, 0.0)));
+    const int *tails = dev.lists.data() + listAt[k];
+    changes[k] = mosp_changes{tails, tails + listCount[k], listCount[k], heads,
+                              batch.numberOfInserts()};
+  }
+
+  result.objectiveStats.resize(K);
+  CombineStats &cst = result.combineStats;
+  if (options.fused) {
+    // One launch: K updates, combined graph, its search.  The host wall time
+    // is split between the K updates and Steps 2-3 by the kernel's own clock.
+    std::vector<mosp_sosp_stats> st(K);
+    mosp_combine_stats cs{};
+    t0 = clock::now();
+    const bool ok = mosp_update_fused(
+        dev.updated, K, changes.data(), s, reinterpret_cast<const int64_t *>(delta.data()),
+        reinterpret_cast<const int64_t *>(maxWeight.data()), pref.data(),
+        static_cast<int32_t>(pref.size()), dev.ws.handle, nullptr, nullptr,
+        reinterpret_cast<int64_t *>(dev.dist.data()), dev.parents.data(),
+        reinterpret_cast<int64_t *>(dev.combDist.data()), dev.combParent.data(), st.data(), &cs,
+        nullptr);
+    const double wall = std::chrono::duration<double, std::milli>(clock::now() - t0).count();
+    if (!ok) return gpu_failed("the MOSP update");
+    for (int k = 0; k < K; ++k) from_c(st[k], result.objectiveStats[k]);
+    cst.numberOfEdges = cs.n_edges;
+    cst.scale = cs.scale;
+    cst.delta = cs.delta;
+    from_c(cs.search, cst.search);
+    const double device = static_cast<double>(cs.sosp_phase_ms) + cs.combined_phase_ms;
+    const double share = device > 0 ? cs.sosp_phase_ms / device : 1.0;
+    tm.objectives.assign(K, wall * share / K);
+    tm.combined = wall * (1.0 - share);
+    for (int k = 0; k < K; ++k) recordStage("obj" + std::to_string(k) + "/sosp_update_gpu", tm.objectives[k]);
+    recordStage("combined_graph_gpu", tm.combined);
+  } else {
+    // The reference's schedule: K launches, then the combined step.
+    for (int k = 0; k < K; ++k) {
+      mosp_csr out{}, in{};
+      mosp_sosp_stats st{};
+      mosp_device_graph_view(dev.updated, k, &out, &in);
+      ScopedStage stage("obj" + std::to_string(k) + "/sosp_update_gpu", true);
+      t0 = clock::now();
+      if (!mosp_sosp_update(&out, &in, &changes[k], s, delta[k], maxWeight[k], dev.ws.handle,
+                            reinterpret_cast<int64_t *>(dev.dist.data()) + static_cast<size_t>(k) * n,
+                            dev.parents.data() + static_cast<size_t>(k) * n, &st, nullptr)) {
+        return gpu_failed(("the update of objective " + std::to_string(k)).c_str());
+      }
+      tm.objectives.push_back(std::chrono::duration<double, std::milli>(clock::now() - t0).count());
+      from_c(st, result.objectiveStats[k]);
+    }
+    ScopedStage stage("combined_graph_gpu", true);
+    t0 = clock::now();
+    CombineWorkspace cws;
+    if (!combinedGraphSospGpu(dev.parents.data(), n, K, pref, s, 0, cws, dev.ws,
+                              dev.combDist.data(), dev.combParent.data(), &cst)) {
+      return gpu_failed("the combined graph step");
+    }
+    tm.combined = std::chrono::duration<double, std::milli>(clock::now() - t0).count();
+  }
+
+  // Download: trees, combined search, and the updated graph.
+  t0 = clock::now();
+  {
+    ScopedStage stage("download", true);
+    const int64_t mu = mosp_device_graph_edges(dev.updated);
+    updated = CsrGraph();
+    updated.numberOfNodes = n;
+    updated.numberOfObjectives = KG;
+    updated.rowPtr.resize(static_cast<size_t>(n) + 1);
+    updated.colInd.resize(static_cast<size_t>(mu));
+    updated.weights.resize(static_cast<size_t>(mu) * KG);
+    if (!dev.dist.download(result.distances) || !dev.parents.download(result.parents) ||
+        !dev.combDist.download(result.combinedDistances) ||
+        !dev.combParent.download(result.combinedParent) ||
+        !mosp_device_graph_download(dev.updated, updated.rowPtr.data(), updated.colInd.data(),
+                                    updated.weights.data())) {
+      return gpu_failed("copying the results back");
+    }
+  }
+  tm.download = std::chrono::duration<double, std::milli>(clock::now() - t0).count();
+  record_counters(result);
   return true;
 }
 
@@ -512,6 +645,41 @@ extern "C" int64_t mosp_apply_batch(int32_t n, int32_t K, const int32_t *rowPtr,
   std::copy(u.weights.begin(), u.weights.end(), outW);
   std::copy(b.weightIncreaseMask.begin(), b.weightIncreaseMask.end(), incMask);
   return static_cast<int64_t>(mu);
+}
+
+namespace {
+
+CsrGraph graph_of(int32_t n, int32_t K, const int32_t *rowPtr, const int32_t *colInd,
+                  const int32_t *weights) {
+  CsrGraph g;
+  g.numberOfNodes = n;
+  g.numberOfObjectives = K;
+  const size_t m = static_cast<size_t>(rowPtr[n]);
+  g.rowPtr.assign(rowPtr, rowPtr + n + 1);
+  g.colInd.assign(colInd, colInd + m);
+  g.weights.assign(weights, weights + m * K);
+  return g;
+}
+
+}  // namespace
+
+extern "C" void mosp_canonicalize_tree(int32_t n, int32_t K, const int32_t *rowPtr,
+                                       const int32_t *colInd, const int32_t *weights,
+                                       int32_t objective, int32_t source, const int64_t *dist,
+                                       int32_t *parent) {
+  const CsrGraph g = graph_of(n, K, rowPtr, colInd, weights);
+  const std::vector<long long> d(dist, dist + n);
+  canonicalizeTree(g, objective, source, d, parent);
+}
+
+extern "C" int mosp_path_costs_host(int32_t n, int32_t K, const int32_t *rowPtr,
+                                    const int32_t *colInd, const int32_t *weights,
+                                    const int32_t *parent, int32_t source, int64_t *costs) {
+  const CsrGraph g = graph_of(n, K, rowPtr, colInd, weights);
+  std::vector<long long> c;
+  if (!mospPathCosts(g, std::vector<int>(parent, parent + n), source, c)) return 0;
+  std::copy(c.begin(), c.end(), costs);
+  return 1;
 }
 
 extern "C" int mosp_update_host(

@@ -7,6 +7,9 @@
 // weight counts as raised against the original first occurrence) and run
 // the same GPU engine as mospUpdate.  Trees are canonical (lowest-id
 // parents), so the outputs equal the reference's for the same inputs.
+// With instrumentation enabled they record the reference's stage and
+// counter names (src/parallelSOSPUpdate.cu:229-497,
+// src/parallelCombinedGraph.cu:100-150).
 #include <algorithm>
 #include <iostream>
 #include <string>
@@ -20,6 +23,7 @@
 #include "parallelCombinedGraph.h"
 #include "parallelSOSPUpdate.h"
 #include "sospUpdateGpu.h"
+#include "stageTimer.h"
 
 namespace {
 
@@ -53,10 +57,12 @@ bool parallelSOSPUpdate(const std::string &originalCsrPrefix,
                         int objectiveIndex, int source, const std::string &distancesOutputPath,
                         const std::string &treeOutputPath) {
   CsrGraph original;
+  ScopedStage readStage("sosp/0a_read_csr_text");
   if (!readCsrGraph(originalCsrPrefix, original)) {
     std::cout << "Error: Could not read original CSR graph.\n";
     return false;
   }
+  readStage.stop();
   const int n = original.numberOfNodes, K = original.numberOfObjectives;
   if (n == 0) {
     std::cout << "Error: Graph has no vertices.\n";
@@ -73,14 +79,25 @@ bool parallelSOSPUpdate(const std::string &originalCsrPrefix,
   std::vector<long long> distances;
   std::vector<int> parent;
   ChangeBatch batch;
-  if (!readDistances(distancesInputPath, n, distances) ||
-      !readParents(treeInputPath, n, parent) ||
-      !readChangeBatch(insertPath, deletePath, K, n, batch)) {
-    return false;
+  {
+    ScopedStage treeStage("sosp/0c_read_tree_text");
+    if (!readDistances(distancesInputPath, n, distances) ||
+        !readParents(treeInputPath, n, parent)) {
+      return false;
+    }
+  }
+  {
+    ScopedStage changesStage("sosp/0d_read_changes_text");
+    if (!readChangeBatch(insertPath, deletePath, K, n, batch)) {
+      return false;
+    }
   }
   CsrGraph updated;
-  if (!applyChangeBatch(original, batch, updated)) {
-    return false;
+  {
+    ScopedStage applyStage("sosp/0e_apply_changes_host");
+    if (!applyChangeBatch(original, batch, updated)) {
+      return false;
+    }
   }
   // Changed edges of this objective: every deletion and every insertion that
   // raised its weight; heads: every insertion's head.
@@ -109,6 +126,7 @@ bool parallelSOSPUpdate(const std::string &originalCsrPrefix,
   SospWorkspace workspace;
   DeviceArray<int> dFrom, dTo, dHeads, dParent;
   DeviceArray<long long> dDistances;
+  ScopedStage uploadStage("sosp/2b_alloc_h2d", true);
   if (!uploadDeviceGraph(updated, device) || !workspace.reserve(n) ||
       !dFrom.upload(changedFrom) || !dTo.upload(changedTo) || !dHeads.upload(batch.insertTo) ||
       !dDistances.upload(distances) || !dParent.upload(parent)) {
@@ -121,15 +139,29 @@ bool parallelSOSPUpdate(const std::string &originalCsrPrefix,
   changes.numberOfChanged = static_cast<int>(changedFrom.size());
   changes.insertHeads = dHeads.data();
   changes.numberOfInsertHeads = batch.numberOfInserts();
-  if (!sospUpdateGpu(device.out(k), device.in(k), changes, source, delta, maxWeight, workspace,
-                     dDistances.data(), dParent.data())) {
-    std::cout << "Error: SOSP update failed on the GPU.\n";
-    return false;
+  uploadStage.stop();
+  SospStats stats;
+  {
+    ScopedStage updateStage("sosp/update_gpu", true);
+    if (!sospUpdateGpu(device.out(k), device.in(k), changes, source, delta, maxWeight, workspace,
+                       dDistances.data(), dParent.data(), &stats)) {
+      std::cout << "Error: SOSP update failed on the GPU.\n";
+      return false;
+    }
   }
-  if (!dDistances.download(distances) || !dParent.download(parent)) {
-    std::cout << "Error: could not copy the result from the GPU.\n";
-    return false;
+  recordCounter("sosp/invalidated", stats.invalidated);
+  recordCounter("sosp/jump_rounds", stats.jumpRounds);
+  recordCounter("sosp/iterations", stats.iterations);
+  recordCounter("sosp/epochs", stats.epochs);
+  recordCounter("sosp/pushes", static_cast<double>(stats.pushes));
+  {
+    ScopedStage downloadStage("sosp/4_d2h", true);
+    if (!dDistances.download(distances) || !dParent.download(parent)) {
+      std::cout << "Error: could not copy the result from the GPU.\n";
+      return false;
+    }
   }
+  ScopedStage writeStage("sosp/5_write_text");
   return writeDistances(distancesOutputPath, distances) &&
          writeParents(treeOutputPath, parent);
 }
@@ -163,6 +195,7 @@ bool parallelCombinedGraph(const std::string &originalCsrPrefix,
     return false;
   }
   std::vector<int> parents(static_cast<size_t>(n) * K);
+  ScopedStage readStage("comb/0_read_trees_text");
   for (int k = 0; k < K; ++k) {
     std::vector<int> parent;
     if (!readParents(treeInputPaths[k], n, parent)) {
@@ -170,6 +203,8 @@ bool parallelCombinedGraph(const std::string &originalCsrPrefix,
     }
     std::copy(parent.begin(), parent.end(), parents.begin() + static_cast<size_t>(k) * n);
   }
+  readStage.stop();
+  ScopedStage uploadStage("comb/1_alloc_h2d", true);
   DeviceArray<int> dParents, dParent(n);
   DeviceArray<long long> dDistances(n);
   CombineWorkspace combineWorkspace;
@@ -179,12 +214,20 @@ bool parallelCombinedGraph(const std::string &originalCsrPrefix,
     std::cout << "Error: could not allocate the combined graph on the GPU.\n";
     return false;
   }
+  uploadStage.stop();
   CombineStats stats;
-  if (!combinedGraphSospGpu(dParents.data(), n, K, preferences, source, 0, combineWorkspace,
-                            sospWorkspace, dDistances.data(), dParent.data(), &stats)) {
-    std::cout << "Error: combined graph failed on the GPU.\n";
-    return false;
+  {
+    ScopedStage gpuStage("comb/steps2_3_gpu", true);
+    if (!combinedGraphSospGpu(dParents.data(), n, K, preferences, source, 0, combineWorkspace,
+                              sospWorkspace, dDistances.data(), dParent.data(), &stats)) {
+      std::cout << "Error: combined graph failed on the GPU.\n";
+      return false;
+    }
   }
+  recordCounter("comb/edges", stats.numberOfEdges);
+  recordCounter("comb/iterations", stats.search.iterations);
+  recordCounter("comb/pushes", static_cast<double>(stats.search.pushes));
+  ScopedStage writeStage("comb/2_d2h_write_text");
   std::vector<long long> distances;
   std::vector<int> parent;
   if (!dDistances.download(distances) || !dParent.download(parent) ||

@@ -184,8 +184,15 @@ std::vector<Request *> admit(mosp_server *s, int &workgroups) {
     if (!(call)) return fail(std::string(what) + ": " + mosp_last_error());     \
   } while (0)
 
+// Time and device time of one slot launch; the worker adds them to the
+// slot's totals under the server's lock (the statistics calls read them).
+struct LaunchTotals {
+  double kernelMs = 0, applyS = 0, launchS = 0, postS = 0;
+};
+
 // One launch of the admitted requests on `slot` (its own thread, no lock).
-int run_slot(mosp_server *s, Slot &slot, const std::vector<Request *> &reqs, int workgroups) {
+int run_slot(mosp_server *s, Slot &slot, const std::vector<Request *> &reqs, int workgroups,
+             LaunchTotals &tot) {
   const auto t0 = Clock::now();
   if (workgroups > 0 && workgroups != slot.workgroups) {
     SERVE_TRY(mosp_batch_configure(slot.batch, workgroups, -1, 1.0, 1), "configure");
@@ -267,8 +274,7 @@ int run_slot(mosp_server *s, Slot &slot, const std::vector<Request *> &reqs, int
   float ms = 0;
   SERVE_TRY(mosp_batch_run(slot.batch, 2, &ms), "run");
   SERVE_TRY(mosp_batch_check(slot.batch), "check");
-  slot.kernelMs += ms;
-  slot.launches += 1;
+  tot.kernelMs = ms;
   const auto t2 = Clock::now();
   for (Request *r : reqs) {
     Graph &G = *s->graphs[r->gid];
@@ -291,9 +297,9 @@ int run_slot(mosp_server *s, Slot &slot, const std::vector<Request *> &reqs, int
     }
   }
   const auto t3 = Clock::now();
-  slot.applyS += seconds(t0, t1);
-  slot.launchS += seconds(t1, t2);
-  slot.postS += seconds(t2, t3);
+  tot.applyS = seconds(t0, t1);
+  tot.launchS = seconds(t1, t2);
+  tot.postS = seconds(t2, t3);
   return 1;
 }
 
@@ -317,8 +323,16 @@ void worker(mosp_server *s, Slot *slot) {
         r->slot = slot->index;
       }
     }
-    const int ok = run_slot(s, *slot, reqs, workgroups);
+    LaunchTotals tot;
+    const int ok = run_slot(s, *slot, reqs, workgroups, tot);
     std::lock_guard<std::mutex> lock(s->mu);
+    if (ok) {
+      slot->kernelMs += tot.kernelMs;
+      slot->launches += 1;
+      slot->applyS += tot.applyS;
+      slot->launchS += tot.launchS;
+      slot->postS += tot.postS;
+    }
     const double t = s->now();
     for (Request *r : reqs) {
       r->done = t;
@@ -484,9 +498,35 @@ extern "C" int64_t mosp_server_submit(mosp_server *s, int32_t gid, int32_t n_ins
       n_ins + n_del > std::max(s->cfg.max_batch, 0)) {
     return fail("Error: invalid request (graph id, or more changes than max_batch)."), -1;
   }
+  // A malformed request is refused here, alone: once queued it would be
+  // coalesced with others, and a failure inside a slot stops the server.
+  const int n = s->graphs[gid]->n, KG0 = s->graphs[gid]->KG;
+  if ((n_ins > 0 && (ins_from == nullptr || ins_to == nullptr || ins_w == nullptr)) ||
+      (n_del > 0 && (del_from == nullptr || del_to == nullptr))) {
+    return fail("Error: invalid request (missing change arrays)."), -1;
+  }
+  const auto outside = [n](int32_t v) { return v < 0 || v >= n; };
+  for (int32_t i = 0; i < n_ins; ++i) {
+    if (outside(ins_from[i]) || outside(ins_to[i])) {
+      return fail("Error: invalid request: insertion " + std::to_string(i) +
+                  " has a vertex outside [0, " + std::to_string(n) + ")."), -1;
+    }
+    for (int k = 0; k < KG0; ++k) {
+      if (ins_w[static_cast<size_t>(i) * KG0 + k] < 1) {
+        return fail("Error: invalid request: insertion " + std::to_string(i) +
+                    " has a weight below 1."), -1;
+      }
+    }
+  }
+  for (int32_t i = 0; i < n_del; ++i) {
+    if (outside(del_from[i]) || outside(del_to[i])) {
+      return fail("Error: invalid request: deletion " + std::to_string(i) +
+                  " has a vertex outside [0, " + std::to_string(n) + ")."), -1;
+    }
+  }
   auto r = std::make_unique<Request>();
   r->gid = gid;
-  const int KG = s->graphs[gid]->KG;
+  const int KG = KG0;
   r->insFrom.assign(ins_from, ins_from + n_ins);
   r->insTo.assign(ins_to, ins_to + n_ins);
   r->insW.assign(ins_w, ins_w + static_cast<size_t>(n_ins) * KG);

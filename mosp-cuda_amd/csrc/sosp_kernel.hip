@@ -126,24 +126,45 @@ __device__ __forceinline__ int *subprof_last() {
 // is reset inside a launch.  The last arrivals re-fence (acquire + release)
 // before passing the arrival on, so every workgroup's writes happen-before
 // every workgroup's later reads.
+// The wait for the release is bounded (limit wall_clock64 ticks, and the
+// launch's abort word, read every 64 polls): a workgroup that gives up sets
+// the abort word, and sync() returns false to all of its threads (it stays
+// false for every later call); each phase then returns at once, so the
+// workgroup only ever works on the state of the last barrier it passed, as
+// in a complete run.  The others give up at their next wait.  A launch
+// whose workgroups are not all resident ends with its abort word set
+// instead of spinning for ever.
+__device__ __forceinline__ int *gave_up_lds() {
+  __shared__ int flag;
+  return &flag;
+}
+
 struct GridBarrier {
   Ctl *ctl;
   unsigned epoch;
   unsigned rank;  // workgroup rank within the group that synchronizes
   unsigned size;  // workgroups in that group
+  unsigned *abortWord;
+  u64 limit;      // wall_clock64 ticks a wait may last
+  bool aborted = false;
 #ifdef MOSP_SUBPROFILE
   bool wasLast = false;  // this workgroup released the last barrier
 #endif
 
   // src: nsrc word addresses in a workgroup-uniform LDS array (set before
   // the call); dst: LDS array that receives their values as of the barrier.
-  __device__ void sync(unsigned *const *src = nullptr, int nsrc = 0,
+  __device__ bool sync(unsigned *const *src = nullptr, int nsrc = 0,
                        unsigned *dst = nullptr) {
+    if (aborted) {
+      return false;
+    }
     __syncthreads();
     // Locals: a store through a global pointer could alias *this, so every
     // member access after one would be reloaded.
     const unsigned e = ++epoch;
     Ctl *const c = ctl;
+    unsigned *const abortWord = this->abortWord;
+    const u64 limit = this->limit;
     const unsigned G = size;
     const unsigned g = rank % BARRIER_GROUPS;
     unsigned *const mail = c->mail;
@@ -188,13 +209,26 @@ struct GridBarrier {
         }
       }
       if (lane == 0) {
-        while (load_relaxed(box) < e) {
+        int gaveUp = 0;
+        const u64 start = wall_clock64();
+        for (unsigned polls = 1; load_relaxed(box) < e; ++polls) {
+          if ((polls & 63u) == 0 &&
+              (load_relaxed(abortWord) != 0 || wall_clock64() - start > limit)) {
+            store_relaxed(abortWord, 1u);
+            gaveUp = 1;
+            break;
+          }
           __builtin_amdgcn_s_sleep(1);
         }
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+        *gave_up_lds() = gaveUp;
       }
     }
     __syncthreads();
+    if (*gave_up_lds() != 0) {
+      aborted = true;
+      return false;
+    }
 #ifdef MOSP_SUBPROFILE
     wasLast = *subprof_last() != 0;
 #endif
@@ -204,6 +238,7 @@ struct GridBarrier {
       }
       __syncthreads();
     }
+    return true;
   }
 };
 
@@ -925,7 +960,9 @@ __device__ int near_far(Desc *objsArg, int nobj, ObjCtl *ctlsArg,
       sh.src[4 * k + 3] = fm + 1;
     }
     SUBPROF_MARK(tWork);
-    grid.sync(sh.src, 4 * nobj, sh.snap);
+    if (!grid.sync(sh.src, 4 * nobj, sh.snap)) {
+      return generation;
+    }
     SUBPROF_MARK(tSync);
     SUBPROF_ADD(profBase + 0, tTop, tSnap);
     SUBPROF_ADD(profBase + 1, tSnap, tWork);
@@ -1269,7 +1306,9 @@ __device__ int update_step1(Args &a, GridBarrier &grid, Threads t,
       }
     }
   }
-  grid.sync();
+  if (!grid.sync()) {
+    return generation;
+  }
   stamp(ctl, PHASE_PACK, t.id);
 
   // Subtree invalidation by pointer jumping.  Round 0 sweeps every vertex;
@@ -1310,7 +1349,9 @@ __device__ int update_step1(Args &a, GridBarrier &grid, Threads t,
       if (threadIdx.x < static_cast<unsigned>(a.nobj)) {
         sh.src[threadIdx.x] = &ctl->obj[threadIdx.x].jumpCount[(round + 1) % 3];
       }
-      grid.sync(sh.src, a.nobj, sh.snap);
+      if (!grid.sync(sh.src, a.nobj, sh.snap)) {
+        return generation;
+      }
       unsigned left = 0;
       for (int k = 0; k < a.nobj; ++k) {
         left += sh.snap[k];
@@ -1347,7 +1388,9 @@ __device__ int update_step1(Args &a, GridBarrier &grid, Threads t,
       atomicAdd(&ctl->obj[k].invalidated, count);
     }
   }
-  grid.sync();
+  if (!grid.sync()) {
+    return generation;
+  }
   stamp(ctl, PHASE_INVALIDATE, t.id);
 
   // Pull: every invalidated vertex and insert head takes the best (distance,
@@ -1451,7 +1494,9 @@ __device__ int update_step1(Args &a, GridBarrier &grid, Threads t,
     if (threadIdx.x < static_cast<unsigned>(a.nobj)) {
       sh.src[threadIdx.x] = &ctl->obj[threadIdx.x].pullHeavy;
     }
-    grid.sync(sh.src, a.nobj, sh.snap);
+    if (!grid.sync(sh.src, a.nobj, sh.snap)) {
+      return generation;
+    }
     for (int k = 0; k < a.nobj; ++k) {
       Desc &D = desc(a.objs)[k];
       ObjCtl &c = ctl->obj[k];
@@ -1487,7 +1532,7 @@ __device__ int update_step1(Args &a, GridBarrier &grid, Threads t,
       }
     }
   }
-  grid.sync();
+  grid.sync();  // the caller checks grid.aborted
   stamp(ctl, PHASE_PULL, t.id);
   return generation;
 }
@@ -1616,7 +1661,9 @@ __device__ void combined_build(Args &args, GridBarrier &grid,
   for (int i = t.id; i <= n; i += t.count) {
     a.cDegree[i] = 0;
   }
-  grid.sync();
+  if (!grid.sync()) {
+    return;
+  }
   // Count: out-degree of every parent; edge count and weight sum.
   if constexpr (HUBS) {
     for (int i = threadIdx.x; i < AGG_SLOTS; i += BLOCK) {
@@ -1657,7 +1704,9 @@ __device__ void combined_build(Args &args, GridBarrier &grid,
     atomicAdd(&ctl->combEdges, edges);
     atomicAdd(&ctl->combWeightSum, wsum);
   }
-  grid.sync();
+  if (!grid.sync()) {
+    return;
+  }
   stamp(ctl, PHASE_COMB_COUNT, t.id);
 
   // Exclusive scan of the n + 1 degrees into row pointers: every workgroup
@@ -1689,7 +1738,9 @@ __device__ void combined_build(Args &args, GridBarrier &grid,
   if (threadIdx.x == 0) {
     a.cPartials[t.block] = chunkTotal;
   }
-  grid.sync();
+  if (!grid.sync()) {
+    return;
+  }
   int offset = 0;
   for (unsigned b = threadIdx.x; b < t.block; b += BLOCK) {
     offset += a.cPartials[b];
@@ -1710,7 +1761,9 @@ __device__ void combined_build(Args &args, GridBarrier &grid,
     a.cDegree[i] = running;  // becomes the fill cursor
     running += d;
   }
-  grid.sync();
+  if (!grid.sync()) {
+    return;
+  }
   stamp(ctl, PHASE_COMB_SCAN, t.id);
 
   // Fill (row order is irrelevant: Step 3 breaks ties by the lowest id).
@@ -1776,7 +1829,7 @@ __device__ void combined_build(Args &args, GridBarrier &grid,
 template <bool HUBS, int LOCAL>
 __device__ void run_instance(Args &args, const Threads t, Shared &sh) {
   Args &a = *uniform(&args);
-  GridBarrier grid{a.ctl, 0, t.block, t.blocks};
+  GridBarrier grid{a.ctl, 0, t.block, t.blocks, a.abortWord, a.barrierTicks};
   const int n = a.n;
   if (t.id == 0) {
     a.ctl->clkStart = wall_clock64();
@@ -1803,7 +1856,9 @@ __device__ void run_instance(Args &args, const Threads t, Shared &sh) {
       }
     }
     generation = 0;
-    grid.sync();
+    if (!grid.sync()) {
+      return;
+    }
   }
   __syncthreads();
 
@@ -1812,9 +1867,15 @@ __device__ void run_instance(Args &args, const Threads t, Shared &sh) {
   const unsigned listCap = static_cast<unsigned>(a.listCap > 0 && a.listCap < n ? a.listCap : n);
   if (a.phases & PH_UPDATE) {
     generation = update_step1<HUBS>(a, grid, t, generation, sh);
+    if (grid.aborted) {
+      return;
+    }
     generation = near_far<HUBS, LOCAL>(desc(a.objs), a.nobj, a.ctl->obj, a.source, generation, 0,
                                 nullptr, a.laneBits, a.hopsUpdate, static_cast<unsigned>(n),
                                 listCap, grid, t, sh, 8);
+    if (grid.aborted) {
+      return;
+    }
     stamp(a.ctl, PHASE_NEARFAR, t.id);
     // One edge sweep for every distance-only objective when they are all
     // views of one out-CSR (the fused update's case), else one each.
@@ -1832,7 +1893,9 @@ __device__ void run_instance(Args &args, const Threads t, Shared &sh) {
       }
     }
     if (a.anyFallback) {
-      grid.sync();
+      if (!grid.sync()) {
+        return;
+      }
       if (stage != nullptr) {
         recover_staged(desc(a.objs), a.nobj, n, a.source, t, sh, stage);
       } else if (fused) {
@@ -1845,8 +1908,8 @@ __device__ void run_instance(Args &args, const Threads t, Shared &sh) {
         }
       }
     }
-    if (a.phases & PH_COMBINED) {
-      grid.sync();  // the K new trees are complete (and the stamp below is the grid's)
+    if ((a.phases & PH_COMBINED) && !grid.sync()) {  // the K new trees are complete
+      return;
     }
     if (t.id == 0) {
       a.ctl->clkSospEnd = wall_clock64();
@@ -1857,13 +1920,20 @@ __device__ void run_instance(Args &args, const Threads t, Shared &sh) {
   if (a.phases & PH_SCRATCH) {
     Desc &D = desc(a.objs)[0];
     scratch_init(D, a.ctl->obj[0], n, a.source, t);
-    grid.sync();
+    if (!grid.sync()) {
+      return;
+    }
     generation = near_far<HUBS, LOCAL>(desc(a.objs), 1, a.ctl->obj, a.source, generation,
                                 D.delta, nullptr, a.laneBits, a.hopsSearch,
                                 static_cast<unsigned>(n), listCap, grid, t, sh, 8);
+    if (grid.aborted) {
+      return;
+    }
     unpack(D, n, a.source, t);
     if (D.parentBits == 0) {
-      grid.sync();
+      if (!grid.sync()) {
+        return;
+      }
       recover_parents<HUBS>(D, n, a.source, t, sh);
     }
   }
@@ -1872,20 +1942,30 @@ __device__ void run_instance(Args &args, const Threads t, Shared &sh) {
     Desc &C = *desc(a.cDesc);
     ObjCtl &cc = a.ctl->obj[MOSP_MAX_OBJ];
     combined_build<HUBS>(a, grid, t, sh);
+    if (grid.aborted) {
+      return;
+    }
     scratch_init(C, cc, n, a.source, t);
     if (threadIdx.x < 2) {
       sh.src[threadIdx.x] = reinterpret_cast<unsigned *>(&a.ctl->combDelta) + threadIdx.x;
     }
-    grid.sync(sh.src, 2, sh.snap);
+    if (!grid.sync(sh.src, 2, sh.snap)) {
+      return;
+    }
     stamp(a.ctl, PHASE_COMB_FILL, t.id);
     const u64 delta = static_cast<u64>(sh.snap[0]) | (static_cast<u64>(sh.snap[1]) << 32);
     generation = near_far<HUBS, LOCAL>(desc(a.cDesc), 1, &cc, a.source, generation, delta, &delta,
                                 a.laneBits, a.hopsSearch, static_cast<unsigned>(n), listCap,
                                 grid, t, sh, 12);
+    if (grid.aborted) {
+      return;
+    }
     stamp(a.ctl, PHASE_COMB_SEARCH, t.id);
     unpack(C, n, a.source, t);
     if (C.parentBits == 0) {
-      grid.sync();
+      if (!grid.sync()) {
+        return;
+      }
       recover_parents<HUBS>(C, n, a.source, t, sh);
     }
   }
@@ -1950,9 +2030,13 @@ __global__ void __launch_bounds__(BLOCK, WAVES_PER_SIMD)
 }
 
 // Zero control block blockIdx.x of ctls (one workgroup each) and open the
-// far-pile minima.
-__global__ void ctl_reset(Ctl *ctls) {
+// far-pile minima; workgroup 0 also clears the launch's shared abort word
+// (when there is one besides the blocks' own).
+__global__ void ctl_reset(Ctl *ctls, unsigned *abortWord) {
   Ctl *ctl = ctls + blockIdx.x;
+  if (blockIdx.x == 0 && threadIdx.x == 0 && abortWord != nullptr) {
+    *abortWord = 0;
+  }
   unsigned *words = reinterpret_cast<unsigned *>(ctl);
   const unsigned count = sizeof(Ctl) / sizeof(unsigned);
   for (unsigned i = threadIdx.x; i < count; i += blockDim.x) {
@@ -1980,13 +2064,13 @@ const void *sosp_batched_symbol(bool hubs) {
               : reinterpret_cast<const void *>(&sosp_batched<false>);
 }
 
-hipError_t launch_ctl_reset_many(Ctl *ctls, int count, hipStream_t stream) {
-  hipLaunchKernelGGL(ctl_reset, dim3(count), dim3(256), 0, stream, ctls);
+hipError_t launch_ctl_reset_many(Ctl *ctls, int count, hipStream_t stream, unsigned *abortWord) {
+  hipLaunchKernelGGL(ctl_reset, dim3(count), dim3(256), 0, stream, ctls, abortWord);
   return hipGetLastError();
 }
 
 hipError_t launch_ctl_reset(Ctl *ctl, hipStream_t stream) {
-  return launch_ctl_reset_many(ctl, 1, stream);
+  return launch_ctl_reset_many(ctl, 1, stream, nullptr);
 }
 
 }  // namespace mosp

@@ -1,30 +1,28 @@
 // mosp -- command-line driver of the MOSP update on MI355X.
 //
-// Drop-in for the reference's bin/mosp (src/mosp.cu:1-420): same options,
-// input files, output files and the stable summary line
+// Drop-in for the reference's bin/mosp (src/mosp.cu): the same options,
+// input and output files, "VALIDATE ..." lines and the summary line
 //   RESULT gpu_compute_ms=<a> end_to_end_ms=<b>
-// (a) = the K SOSP updates + Steps 2-3 (host wall clock around the
-// synchronous GPU calls), (b) = reading inputs to writing outputs.
+// where (a) is the K SOSP updates + Steps 2-3 (host wall clock around the
+// synchronous GPU calls) and (b) runs from reading the inputs to writing the
+// outputs.  Extra options: --serial (K launches + 1, the reference's
+// schedule, instead of one fused launch), --delta-scale and
+// --comb-delta-scale (near-far widths; outputs do not depend on them).
 //
-// usage: mosp --graph <csrPrefix> --changes <dir> --init <dir>
-//             [-k K] [--source s] [--pref p1,..,pK] [--delta D]
-//             [--cache file] [--canonicalize] [--out dir] [--no-output]
-//             [--validate] [--timing file.csv] [--quiet] [--serial]
-// --serial runs K launches + 1 like the reference instead of one fused
-// launch.  --validate checks every tree against a from-scratch GPU search
-// on the updated graph and the MOSP tree against a host-built combined
-// graph searched from scratch (distances and lowest-id parents).
-#include <hip/hip_runtime_api.h>
-
+// The driver is a table of options and a sequence of stages (inputs,
+// canonical ties, update, outputs, report, validation), each a function
+// over one Run record.  The MOSP path costs are computed on the GPU; the
+// validation compares against independent from-scratch GPU searches of the
+// updated graph and of a combined graph built on the host.
+#include <algorithm>
+#include <charconv>
 #include <chrono>
 #include <climits>
 #include <cstdio>
 #include <cstdlib>
 #include <filesystem>
-#include <fstream>
 #include <functional>
 #include <iostream>
-#include <sstream>
 #include <string>
 #include <vector>
 
@@ -35,372 +33,440 @@
 #include "mosp_hip.h"
 #include "parallelCombinedGraph.h"
 #include "sospUpdateGpu.h"
+#include "stageTimer.h"
 
 namespace {
 
-struct Options {
-  std::string graph, changes, init, out = "mosp-output", timingCsv, cache;
+using Clock = std::chrono::steady_clock;
+
+double ms_between(Clock::time_point a, Clock::time_point b) {
+  return std::chrono::duration<double, std::milli>(b - a).count();
+}
+
+// ------------------------------------------------------------- options --
+
+struct Settings {
+  std::string graph, changes, init, out = "mosp-output", timing, cache;
   std::vector<int> pref;
-  int K = 0, source = 0;
-  long long delta = 0;
+  long long K = 0, source = 0, delta = 0, combScale = 1;
   double deltaScale = 1.0;
-  int combDeltaScale = 1;
-  bool canonicalize = false, writeOutput = true, validate = false, quiet = false;
-  bool serial = false;
+  bool canonicalize = false, noOutput = false, validate = false, quiet = false, serial = false;
 };
 
-void usage() {
-  std::cerr << "usage: mosp --graph <csrPrefix> --changes <dir> --init <dir>\n"
-               "            [-k K] [--source s] [--pref p1,..,pK] [--delta D]\n"
-               "            [--cache file] [--canonicalize] [--out dir]\n"
-               "            [--no-output] [--validate] [--timing file.csv]\n"
-               "            [--quiet] [--serial] [--comb-delta-scale S] [--delta-scale F]\n";
+// One command-line option: a string, an integer in [lo, hi], a switch, a
+// comma list of integers >= 1, or a positive real.
+struct Option {
+  const char *name;
+  enum Kind { TEXT, INTEGER, SWITCH, LIST, REAL } kind;
+  long long lo, hi;
+  std::function<void(Settings &, const std::string &, long long)> set;
+};
+
+const std::vector<Option> &options() {
+  using S = Settings;
+  static const std::vector<Option> table = {
+      {"--graph", Option::TEXT, 0, 0, [](S &s, const std::string &v, long long) { s.graph = v; }},
+      {"--changes", Option::TEXT, 0, 0, [](S &s, const std::string &v, long long) { s.changes = v; }},
+      {"--init", Option::TEXT, 0, 0, [](S &s, const std::string &v, long long) { s.init = v; }},
+      {"--out", Option::TEXT, 0, 0, [](S &s, const std::string &v, long long) { s.out = v; }},
+      {"--timing", Option::TEXT, 0, 0, [](S &s, const std::string &v, long long) { s.timing = v; }},
+      {"--cache", Option::TEXT, 0, 0, [](S &s, const std::string &v, long long) { s.cache = v; }},
+      {"-k", Option::INTEGER, 0, INT_MAX, [](S &s, const std::string &, long long x) { s.K = x; }},
+      {"--source", Option::INTEGER, 0, INT_MAX,
+       [](S &s, const std::string &, long long x) { s.source = x; }},
+      {"--delta", Option::INTEGER, 1, LLONG_MAX,
+       [](S &s, const std::string &, long long x) { s.delta = x; }},
+      {"--comb-delta-scale", Option::INTEGER, 1, INT_MAX,
+       [](S &s, const std::string &, long long x) { s.combScale = x; }},
+      {"--delta-scale", Option::REAL, 0, 0,
+       [](S &s, const std::string &v, long long) { s.deltaScale = std::stod(v); }},
+      {"--pref", Option::LIST, 1, INT_MAX, [](S &, const std::string &, long long) {}},
+      {"--canonicalize", Option::SWITCH, 0, 0, [](S &s, const std::string &, long long) { s.canonicalize = true; }},
+      {"--no-output", Option::SWITCH, 0, 0, [](S &s, const std::string &, long long) { s.noOutput = true; }},
+      {"--validate", Option::SWITCH, 0, 0, [](S &s, const std::string &, long long) { s.validate = true; }},
+      {"--quiet", Option::SWITCH, 0, 0, [](S &s, const std::string &, long long) { s.quiet = true; }},
+      {"--serial", Option::SWITCH, 0, 0, [](S &s, const std::string &, long long) { s.serial = true; }},
+  };
+  return table;
 }
 
-bool parse(int argc, char **argv, Options &o) {
-  for (int i = 1; i < argc; ++i) {
-    const std::string a = argv[i];
-    auto next = [&](std::string &dst) {
-      if (i + 1 >= argc) return false;
-      dst = argv[++i];
-      return true;
-    };
-    std::string v;
+void usage() {
+  std::cerr << "usage: mosp --graph <csrPrefix> --changes <dir> --init <dir> [options]\n";
+  for (const Option &o : options()) {
+    const char *arg = o.kind == Option::SWITCH ? "" : o.kind == Option::LIST ? " p1,..,pK"
+                      : o.kind == Option::TEXT ? " <value>" : " <number>";
+    std::cerr << "  " << o.name << arg << "\n";
+  }
+}
+
+bool parse_list(const std::string &v, std::vector<int> &out) {
+  out.clear();
+  size_t from = 0;
+  for (;;) {
+    const size_t comma = v.find(',', from);
     long long x = 0;
-    auto integer = [&](long long lo, long long hi) {
-      if (!next(v)) return false;
-      if (parseInteger(v, lo, hi, x)) return true;
-      std::cerr << a << " needs an integer in [" << lo << ", " << hi << "], got '" << v << "'\n";
+    if (!parseInteger(v.substr(from, comma - from), 1, INT_MAX, x)) return false;
+    out.push_back(static_cast<int>(x));
+    if (comma == std::string::npos) return true;
+    from = comma + 1;
+  }
+}
+
+bool parse_command_line(int argc, char **argv, Settings &s) {
+  for (int i = 1; i < argc; ++i) {
+    const std::string name = argv[i];
+    const auto it = std::find_if(options().begin(), options().end(),
+                                 [&](const Option &o) { return name == o.name; });
+    if (it == options().end()) {
+      std::cerr << "unknown option: " << name << "\n";
       return false;
-    };
-    if (a == "--graph") {
-      if (!next(o.graph)) return false;
-    } else if (a == "--changes") {
-      if (!next(o.changes)) return false;
-    } else if (a == "--init") {
-      if (!next(o.init)) return false;
-    } else if (a == "--out") {
-      if (!next(o.out)) return false;
-    } else if (a == "--timing") {
-      if (!next(o.timingCsv)) return false;
-    } else if (a == "--cache") {
-      if (!next(o.cache)) return false;
-    } else if (a == "-k") {
-      if (!integer(0, INT_MAX)) return false;
-      o.K = static_cast<int>(x);
-    } else if (a == "--source") {
-      if (!integer(0, INT_MAX)) return false;
-      o.source = static_cast<int>(x);
-    } else if (a == "--delta") {
-      if (!integer(1, LLONG_MAX)) return false;
-      o.delta = x;
-    } else if (a == "--delta-scale") {
-      if (!next(v)) return false;
-      try {
-        o.deltaScale = std::stod(v);
-      } catch (...) {
-        o.deltaScale = -1;
-      }
-      if (!(o.deltaScale > 0)) {
-        std::cerr << "--delta-scale needs a positive number, got '" << v << "'\n";
-        return false;
-      }
-    } else if (a == "--comb-delta-scale") {
-      if (!integer(1, INT_MAX)) return false;
-      o.combDeltaScale = static_cast<int>(x);
-    } else if (a == "--pref") {
-      if (!next(v)) return false;
-      std::istringstream list(v);
-      std::string item;
-      while (std::getline(list, item, ',')) {
-        if (!parseInteger(item, 1, INT_MAX, x)) {
-          std::cerr << "--pref needs integers >= 1, got '" << v << "'\n";
+    }
+    if (it->kind == Option::SWITCH) {
+      it->set(s, "", 0);
+      continue;
+    }
+    if (i + 1 >= argc) return false;
+    const std::string value = argv[++i];
+    long long x = 0;
+    switch (it->kind) {
+      case Option::INTEGER:
+        if (!parseInteger(value, it->lo, it->hi, x)) {
+          std::cerr << name << " needs an integer in [" << it->lo << ", " << it->hi
+                    << "], got '" << value << "'\n";
           return false;
         }
-        o.pref.push_back(static_cast<int>(x));
+        break;
+      case Option::LIST:
+        if (!parse_list(value, s.pref)) {
+          std::cerr << name << " needs integers >= 1, got '" << value << "'\n";
+          return false;
+        }
+        break;
+      case Option::REAL: {
+        double d = 0;
+        const auto r = std::from_chars(value.data(), value.data() + value.size(), d);
+        if (r.ec != std::errc() || r.ptr != value.data() + value.size() || !(d > 0)) {
+          std::cerr << name << " needs a positive number, got '" << value << "'\n";
+          return false;
+        }
+        break;
       }
-    } else if (a == "--canonicalize") {
-      o.canonicalize = true;
-    } else if (a == "--no-output") {
-      o.writeOutput = false;
-    } else if (a == "--validate") {
-      o.validate = true;
-    } else if (a == "--quiet") {
-      o.quiet = true;
-    } else if (a == "--serial") {
-      o.serial = true;
-    } else {
-      std::cerr << "unknown option: " << a << "\n";
-      return false;
+      default:
+        break;
     }
+    it->set(s, value, x);
   }
-  return !o.graph.empty() && !o.changes.empty() && !o.init.empty();
+  return !s.graph.empty() && !s.changes.empty() && !s.init.empty();
 }
 
-double ms_since(std::chrono::steady_clock::time_point t) {
-  return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t).count();
-}
+// ----------------------------------------------------------------- run --
+
+struct Run {
+  Settings opt;
+  Clock::time_point start;
+  double contextMs = 0, readMs = 0, canonMs = 0, writeMs = 0;
+  CsrGraph original, updated;
+  ChangeBatch batch;
+  int n = 0, KG = 0, K = 0;
+  std::vector<long long> distances;  // K * n, objective-major
+  std::vector<int> parents;
+  MospResult result;
+};
 
 template <typename T>
-std::vector<T> slice(const std::vector<T> &all, int k, int n) {
-  return std::vector<T>(all.begin() + static_cast<size_t>(k) * n,
-                        all.begin() + static_cast<size_t>(k + 1) * n);
+std::vector<T> objective(const std::vector<T> &all, int k, int n) {
+  const auto at = all.begin() + static_cast<std::ptrdiff_t>(k) * n;
+  return std::vector<T>(at, at + n);
 }
 
-bool write_costs(const std::string &path, const std::vector<long long> &costs, int K) {
-  FILE *f = std::fopen(path.c_str(), "w");
-  if (f == nullptr) return false;
-  const size_t n = costs.size() / K;
-  std::string buf;
-  buf.reserve(1 << 22);
-  bool ok = true;
-  for (size_t v = 0; v < n; ++v) {
-    buf += std::to_string(v);
-    for (int k = 0; k < K; ++k) {
-      const long long c = costs[v * K + k];
-      buf.push_back(' ');
-      buf += c >= DISTANCE_INF / 2 ? std::string("INF") : std::to_string(c);
-    }
-    buf.push_back('\n');
-    if (buf.size() > (1 << 22)) {
-      ok = ok && std::fwrite(buf.data(), 1, buf.size(), f) == buf.size();
-      buf.clear();
-    }
+std::string obj_dir(const std::string &root, int k) { return root + "/obj" + std::to_string(k); }
+
+bool read_inputs(Run &r) {
+  const auto t = Clock::now();
+  {
+    ScopedStage stage("read_graph");
+    if (!loadCsrGraph(r.opt.graph, r.original, r.opt.cache)) return false;
   }
-  ok = ok && std::fwrite(buf.data(), 1, buf.size(), f) == buf.size();
+  r.n = r.original.numberOfNodes;
+  r.KG = r.original.numberOfObjectives;
+  r.K = r.opt.K > 0 ? static_cast<int>(std::min<long long>(r.opt.K, r.KG)) : r.KG;
+  const size_t n = static_cast<size_t>(r.n);
+  r.distances.resize(n * r.K);
+  r.parents.resize(n * r.K);
+  ScopedStage stage("read_changes_and_trees");
+  // The batch and the 2K tree files are independent: one job each, every
+  // tree read straight into its slice.
+  std::vector<std::vector<long long>> d(r.K);
+  std::vector<std::vector<int>> p(r.K);
+  std::vector<std::function<bool()>> jobs;
+  jobs.emplace_back([&] {
+    return readChangeBatch(r.opt.changes + "/insert.txt", r.opt.changes + "/delete.txt", r.KG,
+                           r.n, r.batch);
+  });
+  for (int k = 0; k < r.K; ++k) {
+    const std::string dir = obj_dir(r.opt.init, k);
+    jobs.emplace_back([&, k, dir] { return readDistances(dir + "/distancesOriginal.txt", r.n, d[k]); });
+    jobs.emplace_back([&, k, dir] { return readParents(dir + "/SSSPTreeOriginal.txt", r.n, p[k]); });
+  }
+  if (!runConcurrently(jobs)) return false;
+  for (int k = 0; k < r.K; ++k) {
+    std::copy(d[k].begin(), d[k].end(), r.distances.begin() + static_cast<std::ptrdiff_t>(k * n));
+    std::copy(p[k].begin(), p[k].end(), r.parents.begin() + static_cast<std::ptrdiff_t>(k * n));
+  }
+  r.readMs = ms_between(t, Clock::now());
+  return true;
+}
+
+void canonicalize(Run &r) {
+  const auto t = Clock::now();
+  for (int k = 0; k < r.K; ++k) {
+    canonicalizeTree(r.original, k, static_cast<int>(r.opt.source), objective(r.distances, k, r.n),
+                     r.parents.data() + static_cast<size_t>(k) * r.n);
+  }
+  r.canonMs = ms_between(t, Clock::now());
+}
+
+bool update(Run &r) {
+  MospOptions o;
+  o.source = static_cast<int>(r.opt.source);
+  o.numberOfObjectives = r.K;
+  o.preferences = r.opt.pref;
+  o.delta = r.opt.delta;
+  o.deltaScale = r.opt.deltaScale;
+  o.fused = !r.opt.serial;
+  o.combinedDeltaScale = static_cast<int>(r.opt.combScale);
+  return mospUpdate(r.original, r.batch, r.distances, r.parents, o, r.updated, r.result);
+}
+
+// mospCosts.txt: "v c1 .. cKG" per vertex (INF for unreachable ones).
+bool write_costs(const std::string &path, const std::vector<long long> &costs, int KG) {
+  const size_t n = KG > 0 ? costs.size() / KG : 0;
+  std::string text;
+  text.resize(n * (12 + 21 * static_cast<size_t>(KG)));
+  char *o = &text[0];
+  for (size_t v = 0; v < n; ++v) {
+    o = std::to_chars(o, o + 12, static_cast<long long>(v)).ptr;
+    for (int k = 0; k < KG; ++k) {
+      *o++ = ' ';
+      const long long c = costs[v * KG + k];
+      if (c >= DISTANCE_INF / 2) {
+        *o++ = 'I';
+        *o++ = 'N';
+        *o++ = 'F';
+      } else {
+        o = std::to_chars(o, o + 20, c).ptr;
+      }
+    }
+    *o++ = '\n';
+  }
+  text.resize(static_cast<size_t>(o - text.data()));
+  FILE *f = std::fopen(path.c_str(), "wb");
+  if (f == nullptr) return false;
+  const bool ok = std::fwrite(text.data(), 1, text.size(), f) == text.size();
   return std::fclose(f) == 0 && ok;
 }
 
-// Lowest-id tight in-neighbour check of a tree (distances equal to `ref`).
-struct Check {
+// MOSP path costs over all KG objectives of the updated graph, on the GPU.
+bool path_costs(const Run &r, std::vector<long long> &costs) {
+  DeviceGraph g;
+  DeviceArray<int> parent;
+  DeviceArray<long long> c(static_cast<size_t>(r.n) * r.KG);
+  if (!uploadDeviceGraph(r.updated, g) || !parent.upload(r.result.combinedParent) ||
+      !mosp_path_costs_gpu(g.handle, parent.data(), static_cast<int>(r.opt.source),
+                           reinterpret_cast<int64_t *>(c.data()), nullptr) ||
+      !c.download(costs)) {
+    std::cerr << mosp_last_error() << "\n";
+    return false;
+  }
+  return true;
+}
+
+bool write_outputs(Run &r) {
+  const auto t = Clock::now();
+  ScopedStage stage("write_outputs");
+  namespace fs = std::filesystem;
+  const std::string comb = r.opt.out + "/combinedGraph";
+  std::error_code ec;
+  for (int k = 0; k < r.K; ++k) fs::create_directories(obj_dir(r.opt.out, k), ec);
+  fs::create_directories(comb, ec);
+  std::vector<long long> costs;
+  const bool costsOk = path_costs(r, costs);
+  const MospResult &res = r.result;
+  std::vector<std::function<bool()>> jobs;
+  for (int k = 0; k < r.K; ++k) {
+    const std::string dir = obj_dir(r.opt.out, k);
+    jobs.emplace_back([&, k, dir] {
+      return writeDistances(dir + "/distancesUpdated.txt", objective(res.distances, k, r.n));
+    });
+    jobs.emplace_back([&, k, dir] {
+      return writeParents(dir + "/SSSPTreeUpdated.txt", objective(res.parents, k, r.n));
+    });
+  }
+  jobs.emplace_back([&] { return writeDistances(comb + "/distancesCsr.txt", res.combinedDistances); });
+  jobs.emplace_back([&] { return writeParents(comb + "/SSSPTreeCsr.txt", res.combinedParent); });
+  jobs.emplace_back([&] { return costsOk && write_costs(comb + "/mospCosts.txt", costs, r.KG); });
+  if (!runConcurrently(jobs)) {
+    std::cerr << "cannot write the outputs to " << r.opt.out << "\n";
+    return false;
+  }
+  r.writeMs = ms_between(t, Clock::now());
+  return true;
+}
+
+void report(const Run &r) {
+  const MospTimings &tm = r.result.timings;
+  std::printf("graph  n=%d m=%d K=%d; batch %d inserts, %d deletes\n", r.n,
+              r.updated.numberOfEdges(), r.K, r.batch.numberOfInserts(),
+              r.batch.numberOfDeletes());
+  std::printf("host   context %.1f ms, read inputs %.1f ms, canonicalize %.1f ms, upload %.1f ms, "
+              "apply batch (GPU) %.1f ms, download %.1f ms, write %.1f ms\n",
+              r.contextMs, r.readMs, r.canonMs, tm.upload, tm.applyBatch, tm.download, r.writeMs);
+  for (int k = 0; k < r.K; ++k) {
+    const SospStats &s = r.result.objectiveStats[k];
+    std::printf("obj%d   SOSP update %.3f ms (invalidated %d, jump rounds %d, iterations %d, "
+                "epochs %d, pushes %lld)\n",
+                k, tm.objectives[k], s.invalidated, s.jumpRounds, s.iterations, s.epochs,
+                s.pushes);
+  }
+  const CombineStats &c = r.result.combineStats;
+  std::printf("comb   combined graph + SOSP %.3f ms (%d edges, L=%lld, delta %lld, iterations "
+              "%d, pushes %lld)\n",
+              tm.combined, c.numberOfEdges, c.scale, c.delta, c.search.iterations,
+              c.search.pushes);
+}
+
+// ---------------------------------------------------------- validation --
+
+struct Mismatch {
   long long distance = 0, parent = 0;
+  bool none() const { return distance == 0 && parent == 0; }
 };
 
-Check compare(const std::vector<long long> &d, const std::vector<int> &p,
-              const std::vector<long long> &rd, const std::vector<int> &rp) {
-  Check c;
+Mismatch compare_trees(const std::vector<long long> &d, const std::vector<int> &p,
+                       const std::vector<long long> &rd, const std::vector<int> &rp) {
+  Mismatch m;
   for (size_t v = 0; v < d.size(); ++v) {
-    const bool fin = d[v] < DISTANCE_INF / 2, rfin = rd[v] < DISTANCE_INF / 2;
-    if (fin != rfin || (fin && d[v] != rd[v])) ++c.distance;
-    if (p[v] != rp[v]) ++c.parent;
+    const bool a = d[v] < DISTANCE_INF / 2, b = rd[v] < DISTANCE_INF / 2;
+    m.distance += (a != b || (a && d[v] != rd[v])) ? 1 : 0;
+    m.parent += p[v] != rp[v] ? 1 : 0;
+  }
+  return m;
+}
+
+void print_check(const std::string &what, const Mismatch &m) {
+  std::cout << "VALIDATE " << what << " " << (m.none() ? "PASS" : "FAIL")
+            << " (distance mismatches=" << m.distance << " parent mismatches=" << m.parent
+            << ")\n";
+}
+
+// A from-scratch GPU search of objective k of g (default width, bound from
+// the graph's largest weight).
+bool scratch_tree(const CsrGraph &host, const DeviceGraph &g, int k, int source,
+                  SospWorkspace &ws, std::vector<long long> &d, std::vector<int> &p) {
+  long long maxW = 1, sum = 0;
+  for (int e = 0; e < host.numberOfEdges(); ++e) {
+    maxW = std::max<long long>(maxW, host.weight(e, k));
+    sum += host.weight(e, k);
+  }
+  const int n = host.numberOfNodes;
+  DeviceArray<long long> dd(n);
+  DeviceArray<int> dp(n);
+  return sospFromScratchGpu(g.out(k), source, defaultDelta(host.numberOfEdges(), n, sum), maxW,
+                            ws, dd.data(), dp.data()) &&
+         dd.download(d) && dp.download(p);
+}
+
+// The combined graph of the result's K trees, built on the host: for every
+// vertex v and distinct parent p among its K parents, the edge p -> v with
+// the weight of the trees that contain it.
+CsrGraph host_combined_graph(const MospResult &res, int n, int source, const std::vector<int> &pref) {
+  const int K = res.numberOfObjectives;
+  const long long L = preferenceScale(pref, K);
+  std::vector<std::vector<std::pair<int, int>>> rows(n);
+  for (int v = 0; v < n; ++v) {
+    if (v == source) continue;
+    unsigned done = 0;
+    for (int k = 0; k < K; ++k) {
+      const int p = res.parents[static_cast<size_t>(k) * n + v];
+      if (p < 0 || ((done >> k) & 1u)) continue;
+      unsigned mask = 0;
+      for (int j = k; j < K; ++j) {
+        if (res.parents[static_cast<size_t>(j) * n + v] == p) mask |= 1u << j;
+      }
+      done |= mask;
+      rows[p].emplace_back(v, static_cast<int>(combinedEdgeWeight(mask, pref, K, L)));
+    }
+  }
+  CsrGraph c;
+  c.numberOfNodes = n;
+  c.numberOfObjectives = 1;
+  c.rowPtr.assign(static_cast<size_t>(n) + 1, 0);
+  for (int u = 0; u < n; ++u) {
+    c.rowPtr[u + 1] = c.rowPtr[u] + static_cast<int>(rows[u].size());
+    for (const auto &e : rows[u]) {
+      c.colInd.push_back(e.first);
+      c.weights.push_back(e.second);
+    }
   }
   return c;
 }
 
-// Validation by an independent GPU path: from-scratch searches of the
-// updated graph (no incremental state) and of a host-built combined graph.
-bool validate(const CsrGraph &updated, const MospResult &r, int source,
-              const std::vector<int> &pref) {
-  const int n = updated.numberOfNodes, K = r.numberOfObjectives;
-  DeviceGraph g;
+bool validate(const Run &r) {
+  const int n = r.n, source = static_cast<int>(r.opt.source);
   SospWorkspace ws;
-  DeviceArray<long long> d(n);
-  DeviceArray<int> p(n);
-  if (!uploadDeviceGraph(updated, g) || !ws.reserve(n)) return false;
+  DeviceGraph g;
+  if (!uploadDeviceGraph(r.updated, g) || !ws.reserve(n)) return false;
   bool all = true;
-  for (int k = 0; k < K; ++k) {
-    long long maxW = 1, sum = 0;
-    for (int e = 0; e < updated.numberOfEdges(); ++e) {
-      maxW = std::max<long long>(maxW, updated.weight(e, k));
-      sum += updated.weight(e, k);
-    }
-    std::vector<long long> rd;
-    std::vector<int> rp;
-    if (!sospFromScratchGpu(g.out(k), source, defaultDelta(updated.numberOfEdges(), n, sum),
-                            maxW, ws, d.data(), p.data()) ||
-        !d.download(rd) || !p.download(rp)) {
-      return false;
-    }
-    const Check c = compare(slice(r.distances, k, n), slice(r.parents, k, n), rd, rp);
-    const bool ok = c.distance == 0 && c.parent == 0;
-    all = all && ok;
-    std::cout << "VALIDATE obj" << k << " " << (ok ? "PASS" : "FAIL")
-              << " (distance mismatches=" << c.distance << " parent mismatches=" << c.parent
-              << ")\n";
+  for (int k = 0; k < r.K; ++k) {
+    std::vector<long long> d;
+    std::vector<int> p;
+    if (!scratch_tree(r.updated, g, k, source, ws, d, p)) return false;
+    const Mismatch m = compare_trees(objective(r.result.distances, k, n),
+                                     objective(r.result.parents, k, n), d, p);
+    print_check("obj" + std::to_string(k), m);
+    all = all && m.none();
   }
-  // Combined graph of the K new trees, built on the host.
-  const long long L = preferenceScale(pref, K);
-  CsrGraph comb;
-  comb.numberOfNodes = n;
-  comb.numberOfObjectives = 1;
-  comb.rowPtr.assign(n + 1, 0);
-  std::vector<std::vector<std::pair<int, int>>> rows(n);
-  for (int v = 0; v < n; ++v) {
-    if (v == source) continue;
-    for (int k = 0; k < K; ++k) {
-      const int par = r.parents[static_cast<size_t>(k) * n + v];
-      bool seen = par < 0;
-      for (int j = 0; j < k && !seen; ++j) seen = r.parents[static_cast<size_t>(j) * n + v] == par;
-      if (seen) continue;
-      unsigned mask = 0;
-      for (int j = 0; j < K; ++j) {
-        if (r.parents[static_cast<size_t>(j) * n + v] == par) mask |= 1u << j;
-      }
-      rows[par].push_back({v, static_cast<int>(combinedEdgeWeight(mask, pref, K, L))});
-    }
-  }
-  long long sum = 0;
-  for (int u = 0; u < n; ++u) {
-    comb.rowPtr[u + 1] = comb.rowPtr[u] + static_cast<int>(rows[u].size());
-    for (auto &e : rows[u]) {
-      comb.colInd.push_back(e.first);
-      comb.weights.push_back(e.second);
-      sum += e.second;
-    }
-  }
+  const CsrGraph comb = host_combined_graph(r.result, n, source, r.opt.pref);
   DeviceGraph cg;
-  std::vector<long long> rd;
-  std::vector<int> rp;
-  if (!uploadDeviceGraph(comb, cg) ||
-      !sospFromScratchGpu(cg.out(0), source, defaultDelta(comb.numberOfEdges(), n, sum),
-                          L * (K + 1), ws, d.data(), p.data()) ||
-      !d.download(rd) || !p.download(rp)) {
-    return false;
-  }
-  const Check c = compare(r.combinedDistances, r.combinedParent, rd, rp);
-  const bool ok = c.distance == 0 && c.parent == 0;
-  std::cout << "VALIDATE combined " << (ok ? "PASS" : "FAIL") << " (distance mismatches="
-            << c.distance << " parent mismatches=" << c.parent << ")\n";
-  return all && ok;
+  std::vector<long long> d;
+  std::vector<int> p;
+  if (!uploadDeviceGraph(comb, cg) || !scratch_tree(comb, cg, 0, source, ws, d, p)) return false;
+  const Mismatch m = compare_trees(r.result.combinedDistances, r.result.combinedParent, d, p);
+  print_check("combined", m);
+  return all && m.none();
 }
 
 }  // namespace
 
 int main(int argc, char **argv) {
-  Options opt;
-  if (!parse(argc, argv, opt)) {
+  Run r;
+  if (!parse_command_line(argc, argv, r.opt)) {
     usage();
     return 2;
   }
-  const auto start = std::chrono::steady_clock::now();
-  double contextMs = 0, loadMs = 0, canonMs = 0, writeMs = 0;
+  setInstrumentation(!r.opt.timing.empty());
+  r.start = Clock::now();
   {
-    const auto t = std::chrono::steady_clock::now();
-    (void)hipFree(nullptr);  // create the context outside the timed region
-    contextMs = ms_since(t);
+    ScopedStage stage("device_context_init");
+    const auto t = Clock::now();
+    (void)mosp_device_count();  // the runtime and device context, outside the timed region
+    void *probe = mosp_dev_alloc(1);
+    if (probe != nullptr) (void)mosp_dev_free(probe);
+    r.contextMs = ms_between(t, Clock::now());
   }
-  auto t = std::chrono::steady_clock::now();
-  CsrGraph original;
-  ChangeBatch batch;
-  if (!loadCsrGraph(opt.graph, original, opt.cache)) return 1;
-  const int n = original.numberOfNodes, KG = original.numberOfObjectives;
-  const int K = opt.K > 0 ? std::min(opt.K, KG) : KG;
-  std::vector<long long> distances(static_cast<size_t>(K) * n);
-  std::vector<int> parents(static_cast<size_t>(K) * n);
-  {
-    std::vector<std::vector<long long>> dist(K);
-    std::vector<std::vector<int>> par(K);
-    std::vector<std::function<bool()>> reads{[&] {
-      return readChangeBatch(opt.changes + "/insert.txt", opt.changes + "/delete.txt", KG, n,
-                             batch);
-    }};
-    for (int k = 0; k < K; ++k) {
-      const std::string dir = opt.init + "/obj" + std::to_string(k);
-      reads.push_back([&, k, dir] { return readDistances(dir + "/distancesOriginal.txt", n, dist[k]); });
-      reads.push_back([&, k, dir] { return readParents(dir + "/SSSPTreeOriginal.txt", n, par[k]); });
-    }
-    if (!runConcurrently(reads)) return 1;
-    for (int k = 0; k < K; ++k) {
-      std::copy(dist[k].begin(), dist[k].end(), distances.begin() + static_cast<size_t>(k) * n);
-      std::copy(par[k].begin(), par[k].end(), parents.begin() + static_cast<size_t>(k) * n);
-    }
+  if (!read_inputs(r)) return 1;
+  if (r.opt.canonicalize) canonicalize(r);
+  if (!update(r)) return 1;
+  if (!r.opt.noOutput && !write_outputs(r)) return 1;
+  const double endToEnd = ms_between(r.start, Clock::now());
+  if (!r.opt.quiet) report(r);
+  if (!r.opt.timing.empty()) {
+    recordStage("TOTAL_end_to_end", endToEnd);
+    if (!writeInstrumentationCsv(r.opt.timing)) std::cerr << "cannot write " << r.opt.timing << "\n";
   }
-  loadMs = ms_since(t);
-  if (opt.canonicalize) {
-    t = std::chrono::steady_clock::now();
-    for (int k = 0; k < K; ++k) {
-      canonicalizeTree(original, k, opt.source, slice(distances, k, n),
-                       parents.data() + static_cast<size_t>(k) * n);
-    }
-    canonMs = ms_since(t);
-  }
-  MospOptions options;
-  options.source = opt.source;
-  options.numberOfObjectives = K;
-  options.preferences = opt.pref;
-  options.delta = opt.delta;
-  options.deltaScale = opt.deltaScale;
-  options.fused = !opt.serial;
-  options.combinedDeltaScale = opt.combDeltaScale;
-  CsrGraph updated;
-  MospResult result;
-  if (!mospUpdate(original, batch, distances, parents, options, updated, result)) return 1;
-
-  if (opt.writeOutput) {
-    const auto tw = std::chrono::steady_clock::now();
-    namespace fs = std::filesystem;
-    for (int k = 0; k < K; ++k) fs::create_directories(opt.out + "/obj" + std::to_string(k));
-    fs::create_directories(opt.out + "/combinedGraph");
-    // MOSP path costs on the GPU (all KG objectives of the graph).
-    std::vector<long long> costs;
-    bool costsOk = false;
-    {
-      DeviceGraph g;
-      DeviceArray<int> dp;
-      DeviceArray<long long> dc(static_cast<size_t>(n) * KG);
-      if (uploadDeviceGraph(updated, g) && dp.upload(result.combinedParent) &&
-          mosp_path_costs_gpu(g.handle, dp.data(), opt.source,
-                              reinterpret_cast<int64_t *>(dc.data()), nullptr)) {
-        costsOk = dc.download(costs);
-      } else {
-        std::cerr << mosp_last_error() << "\n";
-      }
-    }
-    const std::string cdir = opt.out + "/combinedGraph";
-    std::vector<std::function<bool()>> writes;
-    for (int k = 0; k < K; ++k) {
-      const std::string dir = opt.out + "/obj" + std::to_string(k);
-      writes.push_back([&, k, dir] {
-        return writeDistances(dir + "/distancesUpdated.txt", slice(result.distances, k, n));
-      });
-      writes.push_back([&, k, dir] {
-        return writeParents(dir + "/SSSPTreeUpdated.txt", slice(result.parents, k, n));
-      });
-    }
-    writes.push_back([&] { return writeDistances(cdir + "/distancesCsr.txt", result.combinedDistances); });
-    writes.push_back([&] { return writeParents(cdir + "/SSSPTreeCsr.txt", result.combinedParent); });
-    writes.push_back([&] { return costsOk && write_costs(cdir + "/mospCosts.txt", costs, KG); });
-    if (!runConcurrently(writes)) {
-      std::cerr << "cannot write the outputs to " << opt.out << "\n";
-      return 1;
-    }
-    writeMs = ms_since(tw);
-  }
-  const double endToEnd = ms_since(start);
-  const MospTimings &tm = result.timings;
-  if (!opt.quiet) {
-    std::printf("graph  n=%d m=%d K=%d; batch %d inserts, %d deletes\n", n,
-                updated.numberOfEdges(), K, batch.numberOfInserts(), batch.numberOfDeletes());
-    std::printf("host   context %.1f ms, read inputs %.1f ms, canonicalize %.1f ms, apply batch "
-                "%.1f ms, upload %.1f ms, download %.1f ms, write %.1f ms\n",
-                contextMs, loadMs, canonMs, tm.applyBatch, tm.upload, tm.download, writeMs);
-    for (int k = 0; k < K; ++k) {
-      const SospStats &s = result.objectiveStats[k];
-      std::printf("obj%d   SOSP update %.3f ms (invalidated %d, jump rounds %d, iterations %d, "
-                  "epochs %d, pushes %lld)\n",
-                  k, tm.objectives[k], s.invalidated, s.jumpRounds, s.iterations, s.epochs,
-                  s.pushes);
-    }
-    const CombineStats &c = result.combineStats;
-    std::printf("comb   combined graph + SOSP %.3f ms (%d edges, L=%lld, delta %lld, iterations "
-                "%d, pushes %lld)\n",
-                tm.combined, c.numberOfEdges, c.scale, c.delta, c.search.iterations,
-                c.search.pushes);
-  }
-  if (!opt.timingCsv.empty()) {
-    std::ofstream csv(opt.timingCsv);
-    csv << "kind,name,value\n";
-    csv << "stage,read_inputs," << loadMs << "\n";
-    csv << "stage,apply_batch_host," << tm.applyBatch << "\n";
-    csv << "stage,upload," << tm.upload << "\n";
-    for (int k = 0; k < K; ++k) csv << "stage,obj" << k << "/sosp_update_gpu," << tm.objectives[k] << "\n";
-    csv << "stage,combined_graph_gpu," << tm.combined << "\n";
-    csv << "stage,download," << tm.download << "\n";
-    csv << "stage,write_outputs," << writeMs << "\n";
-    csv << "stage,TOTAL_end_to_end," << endToEnd << "\n";
-  }
-  std::printf("RESULT gpu_compute_ms=%.3f end_to_end_ms=%.3f\n", tm.gpuCompute(), endToEnd);
+  std::printf("RESULT gpu_compute_ms=%.3f end_to_end_ms=%.3f\n", r.result.timings.gpuCompute(),
+              endToEnd);
   std::fflush(stdout);
-  if (opt.validate && !validate(updated, result, opt.source, opt.pref)) return 1;
+  if (r.opt.validate && !validate(r)) return 1;
   return 0;
 }

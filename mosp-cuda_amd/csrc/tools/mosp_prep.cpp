@@ -4,7 +4,25 @@
 //   mosp_prep rmat  <prefix> scale edgeFactor K seed [wmin=1] [wmax=100]
 //   mosp_prep changes <prefix> <dir> changes insPct safe(0/1) seed [wmin] [wmax]
 //   mosp_prep init  <prefix> <dir> [source=0]
-//   mosp_prep check <prefix>   read the CSR text (host only); print n, m, K and
+//   mosp_prep expected <prefix> <changesDir> <dir> [source=0]
+//                              ground truth of an update: the batch applied
+//                              to the graph (in HBM), then a from-scratch
+//                              search per objective of the updated graph ->
+//                              <dir>/obj<k>/distancesUpdated.txt +
+//                              SSSPTreeUpdated.txt (the reference's
+//                              `mospPrep expected`, src/mospPrep.cu:360-375,
+//                              with the GPU search instead of host Dijkstra;
+//                              both give the canonical lowest-id tree)
+//   mosp_prep cache <prefix> <binaryPath>
+//                              write the binary cache `mosp --cache` reads
+//   mosp_prep mtx2csr <in.mtx> <prefix> K wmin wmax seed
+//                              Matrix Market coordinate file -> CSR text:
+//                              symmetric matrices get both directions,
+//                              self-loops and duplicates are dropped, each
+//                              edge gets K seeded weights in [wmin, wmax]
+//   mosp_prep check <prefix> [cacheFile]
+//                              read the CSR text (host only; through the binary
+//                              cache when one is named); print n, m, K and
 //                              order-sensitive checksums of the arrays
 //   mosp_prep copy  <MiB>      one device-to-device copy of MiB mebibytes
 //                              (calibrates rocprofv3 FETCH_SIZE / WRITE_SIZE)
@@ -12,10 +30,16 @@
 // the canonical tree of every objective, computed by the GPU from-scratch
 // search (the role of the reference's `mospPrep init`, src/mospPrep.cu:
 // 289-304, 350-359, done on the device instead of host Dijkstra).
+#include <hip/hip_runtime_api.h>
+
 #include <algorithm>
 #include <cstdlib>
+#include <fstream>
 #include <iostream>
+#include <random>
+#include <sstream>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "csrGraph.h"
@@ -38,6 +62,86 @@ bool make_graph(int n, int64_t m, int K, std::vector<int> &row, std::vector<int>
   g.colInd = col;
   g.weights = w;
   return static_cast<int64_t>(g.numberOfEdges()) == m;
+}
+
+// Canonical tree of objective k of a device graph from `source`: the
+// from-scratch search with the default width and the graph's own bound.
+bool scratch_tree(mosp_device_graph *g, int k, int source, SospWorkspace &ws,
+                  std::vector<long long> &dist, std::vector<int> &par) {
+  const int K = mosp_device_graph_objectives(g);
+  std::vector<int64_t> maxW(K), sum(K);
+  mosp_csr out{};
+  if (!mosp_device_graph_weight_stats(g, maxW.data(), sum.data(), nullptr) ||
+      !mosp_device_graph_view(g, k, &out, nullptr)) {
+    return false;
+  }
+  const int n = out.n;
+  DeviceArray<long long> d(n);
+  DeviceArray<int> p(n);
+  mosp_sosp_stats st{};
+  if (!mosp_sosp_from_scratch(&out, source, mosp_default_delta(out.m, n, sum[k]),
+                              std::max<int64_t>(1, maxW[k]), ws.handle,
+                              reinterpret_cast<int64_t *>(d.data()), p.data(), &st, nullptr)) {
+    return false;
+  }
+  return d.download(dist) && p.download(par);
+}
+
+// Unbiased integer in [lo, hi] (rejection on the top of the 64-bit range).
+int bounded(std::mt19937_64 &rng, int lo, int hi) {
+  const uint64_t span = static_cast<uint64_t>(hi) - static_cast<uint64_t>(lo) + 1;
+  const uint64_t limit = UINT64_MAX - UINT64_MAX % span;
+  uint64_t x = rng();
+  while (x >= limit) x = rng();
+  return lo + static_cast<int>(x % span);
+}
+
+// Matrix Market coordinate -> CSR (rows sorted by column, no self-loops or
+// duplicates; symmetric entries in both directions).
+bool mtx_to_csr(const std::string &path, int K, int wmin, int wmax, uint64_t seed, CsrGraph &g) {
+  std::ifstream in(path);
+  std::string line;
+  if (!std::getline(in, line) || line.rfind("%%MatrixMarket", 0) != 0) {
+    std::cerr << "Error: " << path << " is not a Matrix Market file\n";
+    return false;
+  }
+  const bool symmetric = line.find("symmetric") != std::string::npos;
+  while (std::getline(in, line) && (line.empty() || line[0] == '%')) {
+  }
+  long long rows = 0, cols = 0, entries = 0;
+  if (!(std::istringstream(line) >> rows >> cols >> entries) || rows <= 0 || rows != cols ||
+      rows > 2147483646LL) {
+    std::cerr << "Error: bad size line in " << path << "\n";
+    return false;
+  }
+  const int n = static_cast<int>(rows);
+  std::vector<std::pair<int, int>> arcs;
+  arcs.reserve(static_cast<size_t>(entries) * (symmetric ? 2 : 1));
+  for (long long i = 0; i < entries; ++i) {
+    long long a = 0, b = 0;
+    if (!std::getline(in, line) || !(std::istringstream(line) >> a >> b) || a < 1 || a > n ||
+        b < 1 || b > n) {
+      std::cerr << "Error: bad entry " << i + 1 << " in " << path << "\n";
+      return false;
+    }
+    if (a == b) continue;
+    arcs.emplace_back(static_cast<int>(a - 1), static_cast<int>(b - 1));
+    if (symmetric) arcs.emplace_back(static_cast<int>(b - 1), static_cast<int>(a - 1));
+  }
+  std::sort(arcs.begin(), arcs.end());
+  arcs.erase(std::unique(arcs.begin(), arcs.end()), arcs.end());
+  g = CsrGraph();
+  g.numberOfNodes = n;
+  g.numberOfObjectives = K;
+  g.rowPtr.assign(static_cast<size_t>(n) + 1, 0);
+  for (const auto &e : arcs) ++g.rowPtr[static_cast<size_t>(e.first) + 1];
+  for (int v = 0; v < n; ++v) g.rowPtr[v + 1] += g.rowPtr[v];
+  std::mt19937_64 rng(seed);
+  for (const auto &e : arcs) {
+    g.colInd.push_back(e.second);
+    for (int k = 0; k < K; ++k) g.weights.push_back(bounded(rng, wmin, wmax));
+  }
+  return true;
 }
 
 }  // namespace
@@ -91,9 +195,75 @@ int main(int argc, char **argv) {
     std::cout << "wrote " << prefix << " n=" << n << " m=" << m << " K=" << K << "\n";
     return 0;
   }
+  if (cmd == "mtx2csr") {
+    if (argc != 8) {
+      std::cerr << "usage: mosp_prep mtx2csr <in.mtx> <prefix> K wmin wmax seed\n";
+      return 2;
+    }
+    const int K = std::atoi(argv[4]), wmin = std::atoi(argv[5]), wmax = std::atoi(argv[6]);
+    if (K < 1 || K > 32 || wmin < 1 || wmax < wmin) {
+      std::cerr << "Error: need 1 <= K <= 32 and 1 <= wmin <= wmax\n";
+      return 2;
+    }
+    CsrGraph g;
+    if (!mtx_to_csr(argv[2], K, wmin, wmax, std::strtoull(argv[7], nullptr, 10), g) ||
+        !writeCsrGraph(argv[3], g)) {
+      return 1;
+    }
+    std::cout << "wrote " << argv[3] << " n=" << g.numberOfNodes << " m=" << g.numberOfEdges()
+              << " K=" << K << "\n";
+    return 0;
+  }
   CsrGraph g;
-  if (!readCsrGraph(prefix, g)) return 1;
+  // `check <prefix> <cacheFile>` reads through the binary cache (loadCsrGraph:
+  // used when it matches the text files, else rebuilt with a note).
+  if (cmd == "check" && argc >= 4 ? !loadCsrGraph(prefix, g, argv[3]) : !readCsrGraph(prefix, g)) {
+    return 1;
+  }
   const int n = g.numberOfNodes, K = g.numberOfObjectives;
+  if (cmd == "cache" && argc == 4) {
+    if (!saveCsrGraphBinary(argv[3], g, prefix)) {
+      std::cerr << "Error: could not write " << argv[3] << "\n";
+      return 1;
+    }
+    std::cout << "wrote the binary cache " << argv[3] << "\n";
+    return 0;
+  }
+  if (cmd == "expected" && argc >= 5) {
+    const std::string changes = argv[3], dir = argv[4];
+    const int source = static_cast<int>(argd(argc, argv, 5, 0));
+    if (source < 0 || source >= n) {
+      std::cerr << "Error: source vertex out of range\n";
+      return 1;
+    }
+    ChangeBatch b;
+    if (!readChangeBatch(changes + "/insert.txt", changes + "/delete.txt", K, n, b)) return 1;
+    mosp_device_graph *dg = mosp_device_graph_upload(n, K, g.rowPtr.data(), g.colInd.data(),
+                                                     g.weights.data());
+    mosp_device_graph *du = mosp_device_graph_create();
+    SospWorkspace ws;
+    bool ok = dg != nullptr && ws.reserve(n) &&
+              mosp_device_graph_apply(dg, b.numberOfInserts(), b.insertFrom.data(),
+                                      b.insertTo.data(), b.insertWeights.data(),
+                                      b.numberOfDeletes(), b.deleteFrom.data(), b.deleteTo.data(),
+                                      du, nullptr, nullptr);
+    for (int k = 0; ok && k < K; ++k) {
+      std::vector<long long> dist;
+      std::vector<int> par;
+      const std::string obj = dir + "/obj" + std::to_string(k);
+      ok = scratch_tree(du, k, source, ws, dist, par) &&
+           writeDistances(obj + "/distancesUpdated.txt", dist) &&
+           writeParents(obj + "/SSSPTreeUpdated.txt", par);
+    }
+    mosp_device_graph_free(du);
+    mosp_device_graph_free(dg);
+    if (!ok) {
+      std::cerr << mosp_last_error() << "\n";
+      return 1;
+    }
+    std::cout << "wrote the expected trees of " << K << " objectives to " << dir << "\n";
+    return 0;
+  }
   if (cmd == "check") {
     // Position-weighted sums mod 2^64 (sum of (i + 1) * a[i]) of the row
     // pointers, columns and edge-major weights: order-sensitive checksums.
@@ -162,6 +332,6 @@ int main(int argc, char **argv) {
     std::cout << "wrote " << K << " trees to " << dir << "\n";
     return 0;
   }
-  std::cerr << "usage: mosp_prep road|rmat|changes|init ...\n";
+  std::cerr << "usage: mosp_prep road|rmat|mtx2csr|changes|init|expected|cache|check|copy ...\n";
   return 2;
 }

@@ -87,6 +87,8 @@ EXPORTS = (
     "mosp_server_wait", "mosp_server_drain", "mosp_server_request", "mosp_server_request_trees",
     "mosp_server_release", "mosp_server_stats", "mosp_server_reset_stats", "mosp_server_clock",
     "mosp_server_destroy", "mosp_parallel_sosp_update", "mosp_parallel_combined_graph",
+    "mosp_canonicalize_tree", "mosp_path_costs_host", "mosp_ws_set_barrier_timeout",
+    "mosp_batch_set_barrier_timeout",
 )
 
 _LIB: C.CDLL | None = None
@@ -160,6 +162,11 @@ def lib() -> C.CDLL:
     L.mosp_apply_batch.restype = _L
     L.mosp_apply_batch.argtypes = [_I, _I, _i32p, _i32p, _i32p, _I, _i32p, _i32p, _i32p, _I,
                                    _i32p, _i32p, _i32p, _i32p, _i32p, _u32p]
+    L.mosp_ws_set_barrier_timeout.argtypes = [_VP, C.c_double]
+    L.mosp_batch_set_barrier_timeout.argtypes = [_VP, C.c_double]
+    L.mosp_canonicalize_tree.restype = None
+    L.mosp_canonicalize_tree.argtypes = [_I, _I, _i32p, _i32p, _i32p, _I, _I, _i64p, _i32p]
+    L.mosp_path_costs_host.argtypes = [_I, _I, _i32p, _i32p, _i32p, _i32p, _I, _i64p]
     L.mosp_parallel_sosp_update.argtypes = [C.c_char_p] * 5 + [_I, _I, C.c_char_p, C.c_char_p]
     L.mosp_parallel_combined_graph.argtypes = [C.c_char_p, C.POINTER(C.c_char_p), _I, _I,
                                                C.c_char_p, C.c_char_p, _i32p, _I]
@@ -327,6 +334,24 @@ def apply_batch(g: HostGraph, b: Batch) -> tuple[HostGraph, np.ndarray]:
     return HostGraph(g.n, g.K, row, col[:m].copy(), w[: m * g.K].copy()), mask[: b.n_ins].copy()
 
 
+def canonicalize_tree(g: HostGraph, k: int, source: int, dist, parent) -> np.ndarray:
+    """The product's lowest-id tie rule (canonicalizeTree, src/mospUpdate.cu:40-55)
+    on a copy of `parent`."""
+    p = _i32(parent).copy()
+    lib().mosp_canonicalize_tree(g.n, g.K, g.row, _nz(g.col), _nz(g.w), k, source,
+                                 np.ascontiguousarray(dist, dtype=np.int64), p)
+    return p
+
+
+def path_costs_host(g: HostGraph, parent, source: int) -> np.ndarray | None:
+    """The product's host mospPathCosts: n*K costs, or None if a tree edge is
+    missing (src/combinedGraphGpu.cu:161-208)."""
+    costs = np.zeros(g.n * g.K, np.int64)
+    ok = lib().mosp_path_costs_host(g.n, g.K, g.row, _nz(g.col), _nz(g.w), _i32(parent), source,
+                                    costs)
+    return costs if ok else None
+
+
 def changed_lists(b: Batch, mask: np.ndarray, k: int) -> tuple[np.ndarray, np.ndarray]:
     """changed_k = deletions + insertions that raised objective k
     (src/mospUpdate.cu:124-136)."""
@@ -471,6 +496,10 @@ class Workspace:
 
     def set_grid(self, workgroups: int) -> None:
         lib().mosp_ws_set_grid(self.handle, workgroups)
+
+    def set_barrier_timeout(self, ms: float) -> None:
+        """Longest grid-barrier wait (ms) before a launch gives up with an error."""
+        _check(lib().mosp_ws_set_barrier_timeout(self.handle, float(ms)), "barrier timeout")
 
     def set_combined_delta_scale(self, scale: int) -> None:
         """Combined search width = the reference's rule x scale (>= 1)."""

@@ -56,6 +56,10 @@ class BatchedUpdate:
             _check(lib().mosp_batch_set_cost_model(self.handle, float(latency_items)),
                    "mosp_batch_set_cost_model")
 
+    def set_barrier_timeout(self, ms: float) -> None:
+        """Longest grid-barrier wait (ms) before a run gives up with an error."""
+        _check(lib().mosp_batch_set_barrier_timeout(self.handle, float(ms)), "barrier timeout")
+
     def run(self, use_graph: bool = True) -> float:
         """One pass over every instance; returns its device time in ms."""
         ms = C.c_float(0)

@@ -14,6 +14,7 @@
 //   generateGraphCSR      (src/generateGraphCSR.cu)
 //   generateChangedEdges  (src/generateChangedEdges.cu)
 //   readCsrGraph/readChangeBatch/readDistances/readParents (src/csrGraph.cu)
+//   saveCsrGraphBinary/loadCsrGraphBinary (src/csrGraph.cu:503-596)
 #include "changeGenerator.cuh"
 #include "csrGraph.cuh"
 #include "dijkstra.cuh"
@@ -157,6 +158,28 @@ int ref_read_distances_ok(const char *path, int n) {
 int ref_read_parents_ok(const char *path, int n) {
   std::vector<int> p;
   return readParents(path, n, p) ? 1 : 0;
+}
+
+// Binary cache of the text graph at `prefix`, written by the reference's
+// own saveCsrGraphBinary: 1 written, 0 failed.
+int ref_save_cache(const char *prefix, const char *path) {
+  CsrGraph g;
+  return readCsrGraph(prefix, g) && saveCsrGraphBinary(path, g, prefix) ? 1 : 0;
+}
+
+// 1 if the reference's loadCsrGraphBinary accepts `path` as the cache of
+// `prefix` and the loaded arrays equal the text graph's; 0 otherwise.
+int ref_load_cache_ok(const char *path, const char *prefix) {
+  CsrGraph text, cached;
+  if (!readCsrGraph(prefix, text) || !loadCsrGraphBinary(path, cached, prefix)) {
+    return 0;
+  }
+  return text.numberOfNodes == cached.numberOfNodes &&
+                 text.numberOfObjectives == cached.numberOfObjectives &&
+                 text.rowPtr == cached.rowPtr && text.colInd == cached.colInd &&
+                 text.weights == cached.weights
+             ? 1
+             : 0;
 }
 
 } // extern "C"

@@ -304,6 +304,10 @@ def ref() -> C.CDLL:
         r.ref_read_csr_ok.argtypes = [s]
         r.ref_read_distances_ok.argtypes = [s, _I]
         r.ref_read_parents_ok.argtypes = [s, _I]
+        r.ref_save_cache.restype = _I
+        r.ref_save_cache.argtypes = [s, s]
+        r.ref_load_cache_ok.restype = _I
+        r.ref_load_cache_ok.argtypes = [s, s]
         r.ref_read_batch_ok.restype = _I
         r.ref_read_batch_ok.argtypes = [s, s, _I, _I]
         _REF = r

@@ -154,3 +154,36 @@ def test_request_errors_are_reported():
     with pytest.raises(RuntimeError):  # unknown graph
         srv.submit(S().Request(1, 5, big.batch))
     srv.close()
+
+
+def test_malformed_request_is_refused_alone():
+    """A request with a vertex outside the graph or a weight below 1 is
+    refused at submission, alone: the requests around it complete, the
+    server keeps serving, and their results equal the same requests served
+    one at a time (which test_mix_equals_alone_equals_oracle ties to the
+    oracle)."""
+    graphs = small_graphs()[:2]
+    srv = make_server(graphs, slots=2, max_batch=400, keep_results=True)
+    good = requests(graphs, count=4, seed=11)
+    g0 = graphs[0][0]
+    z = np.zeros(0, np.int32)
+    bad = [M().Batch(3, i32([0]), i32([g0.n]), i32([1, 1, 1]), z, z),        # head outside
+           M().Batch(3, i32([1]), i32([2]), i32([1, 0, 1]), z, z),           # weight 0
+           M().Batch(3, z, z, z, i32([-1]), i32([2]))]                       # tail outside
+    srv.submit(good[0])
+    for i, b in enumerate(bad):
+        with pytest.raises(RuntimeError, match="invalid request"):
+            srv.submit(S().Request(100 + i, 0, b))
+    for r in good[1:]:
+        srv.submit(r)
+    for r in good:
+        srv.wait(r)
+    srv.close()
+    alone = make_server(graphs, slots=1, max_coalesce=1, max_batch=400, keep_results=True)
+    again = fresh(good)
+    for r in again:
+        alone.submit(r)
+        alone.wait(r)
+    alone.close()
+    for a, b in zip(good, again):
+        same(a.result, b.result, f"request {a.rid} beside the refused ones")
