@@ -141,6 +141,21 @@ int mosp_sosp_update(const mosp_csr *out, const mosp_csr *in,
                      int64_t max_weight, mosp_ws *ws, int64_t *d_dist,
                      int32_t *d_parent, mosp_sosp_stats *stats, void *stream);
 
+/* mosp_sosp_update that allocates nothing.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (serving
+ * loops, harnesses that forbid hipMalloc).  Same arguments, semantics and
+ * results as mosp_sosp_update, without the workspace: the scratch is one
+ * static device array (40 bytes per vertex, for at most
+ * MOSP_NOALLOC_MAX_VERTICES = 2,097,152 vertices; about 84 MB that each GPU
+ * using the library holds until the process ends).  Larger graphs are
+ * refused with an error.  Every call on a GPU uses that GPU's array, so
+ * calls that overlap on one GPU race: run them one at a time per device.
+ * Synchronous on `stream`; stats->kernel_ms is taken from the device clock. */
+int mosp_sosp_update_noalloc(const mosp_csr *out, const mosp_csr *in,
+                             const mosp_changes *changes, int32_t source,
+                             int64_t delta, int64_t max_weight, int64_t *d_dist,
+                             int32_t *d_parent, mosp_sosp_stats *stats, void *stream);
+
 /* K objectives updated in ONE persistent launch (no reference counterpart;
  * it replaces the K serial sospUpdateGpu calls of mospUpdate.cu:170-194).
  * Arrays of length k; d_dist_in may equal d_dist_out (in place). */

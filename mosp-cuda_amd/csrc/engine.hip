@@ -13,6 +13,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -1067,5 +1068,197 @@ extern "C" int mosp_update_fused(const mosp_device_graph *g, int32_t K,
     fill_stats(*ws->hCtl, k, ws->hDesc[k], ms, stats ? &stats[k] : nullptr);
   }
   fill_combined_stats(ws, ws->hDesc[K], ms, cstats);
+  return 1;
+}
+
+// ---------------------------------------------------------------------------
+// mosp_sosp_update without allocation: the same launch as mosp_sosp_update,
+// its scratch in one file-scope device array instead of a workspace.
+// ---------------------------------------------------------------------------
+
+namespace {
+
+constexpr int NOALLOC_MAX_VERTICES = MOSP_NOALLOC_MAX_VERTICES;
+
+// Byte layout of the array: control block, one descriptor, the generation
+// word, the degree word, then one objective's state for up to
+// NOALLOC_MAX_VERTICES vertices.
+constexpr size_t NA_CTL = 0;
+constexpr size_t NA_DESC = (sizeof(Ctl) + 255) / 256 * 256;
+constexpr size_t NA_WORDS = NA_DESC + (sizeof(ObjDesc) + 255) / 256 * 256;
+constexpr size_t NA_STATE = NA_WORDS + 256;
+constexpr size_t NA_STATE_BYTES =
+    (static_cast<size_t>(NOALLOC_MAX_VERTICES) * sizeof(u64) + 255) / 256 * 256 +
+    8 * ((static_cast<size_t>(NOALLOC_MAX_VERTICES) * sizeof(int) + 255) / 256 * 256);
+constexpr size_t NA_BYTES = NA_STATE + NA_STATE_BYTES;
+
+// The scratch of every mosp_sosp_update_noalloc call on a device (each
+// device holds its own copy until the process ends).  Only its address as
+// returned by hipGetSymbolAddress is used on the host.
+static __device__ __align__(256) unsigned char scratch[NA_BYTES];
+
+// Per device: the array's device address and the launch shape, taken once
+// (the entry point's first call on that device).
+struct NoAllocDevice {
+  char *ptr = nullptr;
+  int gridBlocks = 0;
+  double clockKhz = 100000;
+};
+constexpr int NA_DEVICES = 64;
+NoAllocDevice g_noalloc[NA_DEVICES];
+std::mutex g_noalloc_mu;
+
+int noalloc_device(NoAllocDevice **out) {
+  int device = 0;
+  HIP_TRY(hipGetDevice(&device));
+  if (device < 0 || device >= NA_DEVICES) {
+    return fail("Error: device " + std::to_string(device) + " is beyond the no-allocation table.");
+  }
+  std::lock_guard<std::mutex> lock(g_noalloc_mu);
+  NoAllocDevice &d = g_noalloc[device];
+  if (d.ptr == nullptr) {
+    void *ptr = nullptr;
+    HIP_TRY(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+    int cus = 0, perCu = 0, clockKhz = 0;
+    HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device));
+    HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCu, sosp_kernel_symbol(true), BLOCK, 0));
+    if (perCu < 1) {
+      return fail("Error: the persistent kernel does not fit on a CU.");
+    }
+    if (hipDeviceGetAttribute(&clockKhz, hipDeviceAttributeWallClockRate, device) == hipSuccess &&
+        clockKhz > 0) {
+      d.clockKhz = clockKhz;
+    }
+    d.gridBlocks = std::max(1, std::min(cus, perCu * cus));
+    d.ptr = static_cast<char *>(ptr);
+  }
+  *out = &d;
+  return 1;
+}
+
+// Stamps, far marks and invalidation marks of the first n vertices, and the
+// generation word, back to zero: the last call on this device left them
+// as it ended (or in any state, had it given up).
+__global__ void noalloc_clear(int *stamp, int *inFar, int *flag, int n, int *gen) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) {
+    stamp[i] = 0;
+    inFar[i] = 0;
+    flag[i] = 0;
+  }
+  if (i == 0) {
+    *gen = 0;
+  }
+}
+
+}  // namespace
+
+extern "C" int mosp_sosp_update_noalloc(const mosp_csr *out, const mosp_csr *in,
+                                        const mosp_changes *changes, int32_t source,
+                                        int64_t delta, int64_t maxWeight, int64_t *d_dist,
+                                        int32_t *d_parent, mosp_sosp_stats *stats,
+                                        void *streamArg) {
+  g_error.clear();
+  if (stats != nullptr) {
+    *stats = mosp_sosp_stats{};
+    stats->packed_parents = 1;
+  }
+  if (out == nullptr || in == nullptr || changes == nullptr) {
+    return fail("Error: missing graph or change arguments.");
+  }
+  const int n = out->n;
+  if (n == 0) {
+    return 1;
+  }
+  if (n < 0 || n > NOALLOC_MAX_VERTICES) {
+    return fail("Error: the no-allocation update holds at most " +
+                std::to_string(NOALLOC_MAX_VERTICES) + " vertices; this graph has " +
+                std::to_string(n) + ".");
+  }
+  if (source < 0 || source >= n) {
+    return fail("Error: source vertex out of range.");
+  }
+  if (delta <= 0) {
+    return fail("Error: the near-far bucket width must be positive.");
+  }
+  PackChoice pc{};
+  if (!choose_packing(n, maxWeight, pc)) {
+    return 0;
+  }
+  NoAllocDevice *dev = nullptr;
+  if (!noalloc_device(&dev)) {
+    return 0;
+  }
+  hipStream_t stream = as_stream(streamArg);
+  char *base = dev->ptr;
+  Ctl *dCtl = reinterpret_cast<Ctl *>(base + NA_CTL);
+  ObjDesc *dDesc = reinterpret_cast<ObjDesc *>(base + NA_DESC);
+  int *genDev = reinterpret_cast<int *>(base + NA_WORDS);
+  unsigned *degreeWord = reinterpret_cast<unsigned *>(base + NA_WORDS + 128);
+  char *p = base + NA_STATE;
+  const ObjState st = carve_state(p, NOALLOC_MAX_VERTICES);
+
+  // Variant choice as in mosp_sosp_update: any vertex of more than
+  // HUB_DEGREE out- or in-edges.
+  int hubs = 0;
+  for (const int *row : {out->row_ptr, in->row_ptr}) {
+    int degree = 0;
+    if (!graph_max_degree(row, nullptr, n, degreeWord, stream, &degree)) {
+      return 0;
+    }
+    hubs |= degree > HUB_DEGREE ? 1 : 0;
+  }
+  const int forced = hubs_override();
+  if (forced >= 0) {
+    hubs = forced;
+  }
+
+  ObjDesc d = make_desc(*out, in, changes, pc, static_cast<u64>(delta), st);
+  d.distIn = d_dist;
+  d.parentIn = d_parent;
+  d.distOut = d_dist;
+  d.parentOut = d_parent;
+  LaunchArgs args{};
+  args.objs = dDesc;
+  args.nobj = 1;
+  args.n = n;
+  args.source = source;
+  args.phases = PH_UPDATE;
+  args.maxRounds = jump_rounds(n);
+  args.anyFallback = pc.bits == 0 ? 1 : 0;
+  args.laneBits = 2;
+  args.hopsUpdate = kHopsUpdate;
+  args.hopsSearch = kHopsSearch;
+  local_hops_override(args.hopsUpdate, args.hopsSearch);
+  args.listCap = list_cap_override();
+  args.hubs = hubs;
+  args.genDev = genDev;
+  args.ctl = dCtl;
+  args.abortWord = &dCtl->gaveUp;
+  args.barrierTicks = barrier_ticks(barrier_timeout_ms_default(), dev->clockKhz);
+  args.cDeltaScale = 1;
+
+  HIP_TRY(hipMemcpyAsync(dDesc, &d, sizeof(ObjDesc), hipMemcpyHostToDevice, stream));
+  hipLaunchKernelGGL(noalloc_clear, dim3((n + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, stream,
+                     st.stamp, st.inFar, st.flag, n, genDev);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(launch_ctl_reset(dCtl, stream));
+  void *params[] = {&args};
+  HIP_TRY(launch_persistent(sosp_kernel_symbol(hubs != 0), dim3(dev->gridBlocks), dim3(BLOCK),
+                            params, stream));
+  static thread_local Ctl hCtl;  // host copy of the control block (no allocation per call)
+  HIP_TRY(hipMemcpyAsync(&hCtl, dCtl, CTL_READBACK, hipMemcpyDeviceToHost, stream));
+  HIP_TRY(hipStreamSynchronize(stream));
+  if (hCtl.gaveUp != 0) {
+    return fail("Error: a grid barrier of the persistent launch gave up waiting (its workgroups "
+                "were not all resident); the tree was not updated.");
+  }
+  if (!input_tree_ok(hCtl.obj[0].overflow, "")) {
+    return 0;
+  }
+  const float ms = hCtl.clkEnd > hCtl.clkStart
+                       ? static_cast<float>((hCtl.clkEnd - hCtl.clkStart) / dev->clockKhz)
+                       : 0.0f;
+  fill_stats(hCtl, 0, d, ms, stats);
   return 1;
 }

@@ -30,6 +30,11 @@ constexpr int BLOCK = MOSP_BLOCK;    // threads per workgroup (wave64 multiple)
 #ifndef MOSP_MIN_BLOCKS
 #define MOSP_MIN_BLOCKS 1            // co-resident workgroups per CU the code is built for
 #endif
+// Vertices of the largest graph mosp_sosp_update_noalloc takes: its scratch
+// (40 bytes per vertex) is a static device array of that size.
+#ifndef MOSP_NOALLOC_MAX_VERTICES
+#define MOSP_NOALLOC_MAX_VERTICES (1 << 21)
+#endif
 // Graphs with a vertex of more than HUB_DEGREE out- or in-edges run the
 // kernel variant that hands such vertices to whole waves.
 constexpr int HUB_DEGREE = 32;

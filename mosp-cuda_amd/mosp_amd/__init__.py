@@ -88,7 +88,7 @@ EXPORTS = (
     "mosp_server_release", "mosp_server_stats", "mosp_server_reset_stats", "mosp_server_clock",
     "mosp_server_destroy", "mosp_parallel_sosp_update", "mosp_parallel_combined_graph",
     "mosp_canonicalize_tree", "mosp_path_costs_host", "mosp_ws_set_barrier_timeout",
-    "mosp_batch_set_barrier_timeout",
+    "mosp_batch_set_barrier_timeout", "mosp_sosp_update_noalloc",
 )
 
 _LIB: C.CDLL | None = None
@@ -162,6 +162,8 @@ def lib() -> C.CDLL:
     L.mosp_apply_batch.restype = _L
     L.mosp_apply_batch.argtypes = [_I, _I, _i32p, _i32p, _i32p, _I, _i32p, _i32p, _i32p, _I,
                                    _i32p, _i32p, _i32p, _i32p, _i32p, _u32p]
+    L.mosp_sosp_update_noalloc.argtypes = [C.POINTER(Csr), C.POINTER(Csr), C.POINTER(Changes),
+                                           _I, _L, _L, _VP, _VP, C.POINTER(SospStats), _VP]
     L.mosp_ws_set_barrier_timeout.argtypes = [_VP, C.c_double]
     L.mosp_batch_set_barrier_timeout.argtypes = [_VP, C.c_double]
     L.mosp_canonicalize_tree.restype = None
@@ -675,6 +677,22 @@ def sosp_update(dg: DeviceGraph, k: int, changes_from, changes_to, heads, source
     st = SospStats()
     _check(lib().mosp_sosp_update(C.byref(out), C.byref(inn), C.byref(ch), source, delta, max_w,
                                   ws.handle, d.ptr, p.ptr, C.byref(st), None), "sospUpdateGpu")
+    return d.to_numpy(np.int64, dg.n), p.to_numpy(np.int32, dg.n), st.as_dict()
+
+
+def sosp_update_noalloc(dg: DeviceGraph, k: int, changes_from, changes_to, heads, source: int,
+                        delta: int, max_w: int, dist, parent):
+    """mosp_sosp_update_noalloc on objective k (no workspace: the library's
+    static scratch) with host arrays in/out."""
+    out, inn = dg.view(k)
+    cf, ct, hd = (_dev(_nz(_i32(a)), np.int32) for a in (changes_from, changes_to, heads))
+    ch = Changes(cf.ptr, ct.ptr, len(changes_from), hd.ptr, len(heads))
+    d = _dev(dist, np.int64)
+    p = _dev(parent, np.int32)
+    st = SospStats()
+    _check(lib().mosp_sosp_update_noalloc(C.byref(out), C.byref(inn), C.byref(ch), source, delta,
+                                          max_w, d.ptr, p.ptr, C.byref(st), None),
+           "mosp_sosp_update_noalloc")
     return d.to_numpy(np.int64, dg.n), p.to_numpy(np.int32, dg.n), st.as_dict()
 
 
