@@ -51,12 +51,51 @@ def cpu_model() -> str:
     return platform.processor() or "unknown"
 
 
+def host_cpus() -> dict:
+    """The host's CPU as the process sees it before any pinning."""
+    allowed = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else None
+    sockets, cores = set(), set()
+    try:
+        phys = core = None
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("physical id"):
+                phys = line.split(":", 1)[1].strip()
+            elif line.startswith("core id"):
+                core = line.split(":", 1)[1].strip()
+                sockets.add(phys)
+                cores.add((phys, core))
+    except OSError:
+        pass
+    return {"cpu": cpu_model(), "logical_cpus": os.cpu_count(), "cpus_allowed": allowed,
+            "physical_cores": len(cores) or None, "sockets": len(sockets) or None}
+
+
+def copy_bandwidth(M, gib: int = 1, reps: int = 10) -> float:
+    """Device-to-device copy of `gib` GiB (hipMemcpy), GB/s of HBM traffic
+    (bytes read + bytes written over the wall time of `reps` copies)."""
+    nbytes = gib << 30
+    a, b = M.DeviceBuffer(nbytes), M.DeviceBuffer(nbytes)
+    L = M.lib()
+    L.mosp_memcpy(b.ptr, a.ptr, nbytes, 3)
+    M.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        L.mosp_memcpy(b.ptr, a.ptr, nbytes, 3)
+    M.synchronize()
+    dt = time.perf_counter() - t0
+    a.free()
+    b.free()
+    return 2 * nbytes * reps / dt / 1e9
+
+
 def cpu_baseline(g, b, d0, p0, gpu_result) -> dict:
     """The oracle (CPU restatement of the reference's sequential path) on one
     pinned core, over the same workload; also checks the GPU outputs."""
     sys.path.insert(0, str(ROOT / "tests"))
     import oracle_lib as O
 
+    host = host_cpus()
+    saved = os.sched_getaffinity(0) if hasattr(os, "sched_getaffinity") else None
     try:
         os.sched_setaffinity(0, {sorted(os.sched_getaffinity(0))[0]})
     except (AttributeError, OSError):
@@ -69,15 +108,16 @@ def cpu_baseline(g, b, d0, p0, gpu_result) -> dict:
     compute_ms = float(r["times_ms"][1] + r["times_ms"][2])  # updates + combined
     same = all(np.array_equal(gpu_result[k], r[k]) for k in
                ("dist", "parents", "comb_dist", "comb_parent"))
+    if saved is not None:
+        os.sched_setaffinity(0, saved)
     return {"value": (b.n_ins + b.n_del) / (compute_ms / 1e3),
             "unit": "changes/s", "cores": 1, "kind": "port",
             "sample": ("full workload once: K=3 sequential SOSP updates (Phases 1-2) + host "
                        "combined graph + Dijkstra, apply excluded "
                        f"(apply {r['times_ms'][0]:.1f} ms, updates {r['times_ms'][1]:.1f} ms, "
                        f"combined {r['times_ms'][2]:.1f} ms, wall {wall:.0f} ms)"),
-            "compute_ms": compute_ms, "cpu": cpu_model(),
-            "host_threads_visible": len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity")
-            else os.cpu_count(),
+            "compute_ms": compute_ms, **host,
+            "pinned_to": "one core (sched_setaffinity) for the single-threaded reference path",
             "outputs_equal_gpu": bool(same)}
 
 
@@ -119,18 +159,61 @@ def main() -> int:
     ap.add_argument("--tune-cache", default=str(ROOT / "profiles" / "autotune_batched.json"),
                     help="autotuned batched launch choices (read; --retune re-times)")
     ap.add_argument("--retune", action="store_true")
+    ap.add_argument("--headline-only", action="store_true",
+                    help="plain mode: the 1.96M-vertex update alone, without the nested "
+                         "batched / served / rmat summaries")
+    ap.add_argument("--equal-requests", type=int, default=16,
+                    help="served: requests replayed in the mix and alone to compare outputs")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: this build runs the update on a single MI355X "
               "(multi-GPU sharding is described, not built); use --gpus 1", file=sys.stderr)
         return 2
-    if args.mode == "batched":
-        return bench_batched(args)
-    if args.mode == "rmat":
-        return bench_rmat(args)
-    if args.mode == "served":
-        return bench_served(args)
+    if args.mode != "plain":
+        fn = {"batched": batched_line, "rmat": rmat_line, "served": served_line}[args.mode]
+        print(json.dumps(fn(args)), flush=True)
+        return 0
+    line, host = plain_line(args)
+    if not args.headline_only:
+        # The rest of the metric, each on its own workload: the batched and
+        # served forms (speed-up over serial launches, p50 / p99 latency) and
+        # the power-law stress update.  Compact passes, same code as the
+        # dedicated modes.
+        sub = argparse.Namespace(**vars(args))
+        sub.steps, sub.warmup = 3, 1
+        b = batched_line(sub)
+        line["batched"] = {
+            "workload": b["config"]["workload"], "instances": b["config"]["instances"],
+            "changes": b["config"]["changes"], "ms_per_pass": b["ms_per_step"],
+            "changes_per_s": b["value"], "serial_ms_per_pass": b["serial"]["ms_per_step"],
+            "speedup_vs_serial": b["speedup_vs_serial"],
+            "outputs_equal_serial": b["outputs_equal_serial"],
+            "graph_mode": b["config"]["graph_mode"]}
+        sub.requests = 96
+        v = served_line(sub)
+        line["served"] = {
+            "workload": v["config"]["workload"], "p50_ms": v["p50_ms"], "p99_ms": v["p99_ms"],
+            "changes_per_s": v["value"],
+            "poisson_changes_per_s": v["config"]["poisson_changes_per_s"],
+            "rate_per_s": v["config"]["rate_per_s"], "requests": v["config"]["requests"],
+            "mean_coalesced": v["config"]["mean_coalesced"],
+            "outputs_equal_alone": v["outputs_equal_alone"],
+            "equal_requests": v["equal_requests"]}
+        r = rmat_line(sub)
+        line["rmat"] = {
+            "workload": r["config"]["workload"], "ms_per_step": r["ms_per_step"],
+            "changes_per_s": r["value"], "kernel_ms": r["kernel_ms"],
+            "roofline_frac": r["roofline"]["frac"], "alg_bytes": r["roofline"]["alg_bytes"],
+            "packed_parents": r["config"]["packed_parents"]}
+    if host is not None:
+        log("cpu baseline: oracle CPU path on one core (about 20 s) ...")
+        line["cpu_baseline"] = cpu_baseline(*host)
+    print(json.dumps(line), flush=True)
+    return 0
 
+
+def plain_line(args):
+    """The headline: one MOSP update of the 1.96M-vertex road grid, resident."""
     import mosp_amd as M
     from mosp_amd.roofline import alg_bytes
 
@@ -201,17 +284,18 @@ def main() -> int:
                    "combined_delta": int(sess.cstats.delta)},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS,
                      "unit": "GB/s", "frac": achieved / PEAK_HBM_GBS, "traffic": traffic,
+                     "traffic_source": f"rocprofv3 FETCH_SIZE + WRITE_SIZE, {tp.name}",
                      "alg_bytes": ab, "kernel": "sosp_persistent (one launch per update)"},
     }
-    if not args.no_cpu_baseline:
-        log("cpu baseline: oracle CPU path on one core (about 20 s) ...")
-        line["cpu_baseline"] = cpu_baseline(g, b, d0, p0, res)
     sess.close()
-    print(json.dumps(line), flush=True)
-    return 0
+    line["roofline"]["copy_gbs"] = copy_bandwidth(M)
+    line["roofline"]["copy_frac"] = line["roofline"]["copy_gbs"] / PEAK_HBM_GBS
+    log(f"device copy: {line['roofline']['copy_gbs']:.0f} GB/s")
+    host = None if args.no_cpu_baseline else (g, b, d0, p0, res)
+    return line, host
 
 
-def bench_rmat(args) -> int:
+def rmat_line(args) -> dict:
     """Power-law stress: R-MAT scale 22 (a=0.57, b=c=0.19, edge factor 16,
     self-loops and duplicates removed), K=5 weights U[1, 2^24] -- (n - 1) *
     maxW does not fit next to a 23-bit parent id, so every objective runs the
@@ -283,11 +367,10 @@ def bench_rmat(args) -> int:
                      "kernel": "sosp_persistent (hub variant, one launch per update)"},
     }
     sess.close()
-    print(json.dumps(line), flush=True)
-    return 0
+    return line
 
 
-def bench_batched(args) -> int:
+def batched_line(args) -> dict:
     """64 independent update instances of uneven size (road-like grids with
     n log-uniform in [10k, 500k], K uniform in {2..8}, batches log-uniform in
     [1k, 50k] with 40% insertions, Pref random in [1, 4]) in ONE persistent
@@ -396,11 +479,10 @@ def bench_batched(args) -> int:
     bu.close()
     for s in sessions:
         s.close()
-    print(json.dumps(line), flush=True)
-    return 0
+    return line
 
 
-def bench_served(args) -> int:
+def served_line(args) -> dict:
     """8 resident graphs (4 road-like 1400x1400 grids K=3, 4 R-MAT scale 20
     K=4, weights U[1,100]); requests with batch sizes log-uniform in [100,
     100k] (50% insertions, 10% disconnecting deletions), served by the
@@ -415,9 +497,7 @@ def bench_served(args) -> int:
     M.set_device(0)
     t = time.perf_counter()
     budget = int(args.hbm_budget_gb * 2**30) if args.hbm_budget_gb > 0 else None
-    srv = S.Server(slots=args.slots, max_coalesce=args.coalesce, max_batch=100_000,
-                   cu_policy=args.cu_policy, delta_scale=args.delta_scale, hbm_budget=budget)
-    originals = []
+    originals, resident = [], []
     for gid in range(8):
         if gid < 4:
             g = M.gen_road_grid(1400, 1400, 3, seed=100 + gid)
@@ -427,9 +507,18 @@ def bench_served(args) -> int:
             K, pref = 4, (1, 2, 3, 4)
         d, p = M.initial_trees(g, K)
         originals.append(g)
-        srv.add_graph(g, d, p, K=K, pref=pref)
-        log(f"graph {gid}: n={g.n} m={g.m} K={K} resident ({time.perf_counter() - t:.0f} s)")
-    srv.start()
+        resident.append((g, d, p, K, pref))
+        log(f"graph {gid}: n={g.n} m={g.m} K={K} ({time.perf_counter() - t:.0f} s)")
+
+    def server(**kw):
+        srv = S.Server(max_batch=100_000, delta_scale=args.delta_scale, **kw)
+        for g, d, p, K, pref in resident:
+            srv.add_graph(g, d, p, K=K, pref=pref)
+        srv.start()
+        return srv
+
+    srv = server(slots=args.slots, max_coalesce=args.coalesce, cu_policy=args.cu_policy,
+                 hbm_budget=budget)
     st, _ = srv.counters()
     log(f"server: {args.slots} slots, {st.cus} CUs, policy {args.cu_policy}, coalesce <= "
         f"{args.coalesce}, HBM budget {st.hbm_budget / 2**30:.2f} GiB (resident "
@@ -461,6 +550,27 @@ def bench_served(args) -> int:
     reqs = S.poisson_requests(originals, args.requests, rate, seed=12)
     stats = srv.play(reqs, timeout=900)
     report(f"poisson {rate:.1f}/s", stats)
+    srv.close()
+    # Outputs: a sample of requests served in the mix (all queued at once, so
+    # launches overlap and coalesce) and one at a time must give identical
+    # trees, combined distances and MOSP trees.
+    sample = S.poisson_requests(originals, args.equal_requests, 1.0, seed=13)
+    runs = []
+    for kw in ({"slots": args.slots, "max_coalesce": args.coalesce, "cu_policy": args.cu_policy},
+               {"slots": 1, "max_coalesce": 1}):
+        eq = server(keep_results=True, **kw)
+        rs = [S.Request(r.rid, r.gid, r.batch) for r in sample]
+        if kw["slots"] > 1:
+            eq.play(rs, timeout=900)
+        else:
+            for r in rs:
+                eq.submit(r)
+                eq.wait(r)
+        eq.close()
+        runs.append(rs)
+    equal = all(np.array_equal(a.result[k], b.result[k]) for a, b in zip(*runs)
+                for k in ("dist", "parents", "comb_dist", "comb_parent"))
+    log(f"outputs of {len(sample)} requests in the mix and alone identical: {equal}")
     line = {
         "metric": "changes/s and p50/p99 latency, served: 8 resident graphs, Poisson requests",
         "value": sat.changes_per_s, "unit": "changes/s",
@@ -481,10 +591,9 @@ def bench_served(args) -> int:
                    "resident_bytes": st.resident_bytes,
                    "mean_coalesced": stats.mean_coalesced,
                    "poisson_changes_per_s": stats.changes_per_s},
+        "outputs_equal_alone": bool(equal), "equal_requests": len(sample),
     }
-    srv.close()
-    print(json.dumps(line), flush=True)
-    return 0
+    return line
 
 
 if __name__ == "__main__":

@@ -364,6 +364,13 @@ unsigned long long barrier_ticks(double ms, double clockKhz) {
 
 hipError_t launch_persistent(const void *kernel, dim3 grid, dim3 block, void **params,
                              hipStream_t stream) {
+  static const bool plain = [] {
+    const char *env = std::getenv("MOSP_LAUNCH");
+    return env != nullptr && std::strcmp(env, "plain") == 0;
+  }();
+  if (plain) {
+    return hipLaunchKernel(kernel, grid, block, params, 0, stream);
+  }
   const hipError_t err = hipLaunchCooperativeKernel(kernel, grid, block, params, 0, stream);
   if (err != hipErrorIllegalState) {
     return err;

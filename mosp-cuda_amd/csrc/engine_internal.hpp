@@ -66,7 +66,8 @@ unsigned long long barrier_ticks(double ms, double clockKhz);
 // device's cooperative-launch resources), launches the same grid plainly.
 // The grid never exceeds the occupancy limit, so on a device running no
 // other persistent kernel every workgroup is resident; otherwise the
-// bounded barrier waits end the launch with an error.
+// bounded barrier waits end the launch with an error.  MOSP_LAUNCH=plain
+// makes every such launch plain (the same grid, without the cooperative API).
 hipError_t launch_persistent(const void *kernel, dim3 grid, dim3 block, void **params,
                              hipStream_t stream);
 // MOSP_LIST_CAP=c caps the near / far lists at c entries instead of n, so

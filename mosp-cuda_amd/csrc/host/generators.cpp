@@ -8,7 +8,8 @@
 //     tree kept (Kruskal on a random edge order), a fraction of the other
 //     grid edges dropped, random diagonal shortcuts added; both directions
 //     stored, K iid weights per arc;
-//   * R-MAT (a, b, c, d) with self-loops and duplicate arcs removed;
+//   * R-MAT (a, b, c, d) with self-loops and duplicate arcs removed, drawn
+//     in parallel over fixed chunks with their own streams;
 //   * change batches: insertions join random pairs u != v, deletions pick
 //     distinct existing arcs; optionally connectivity-safe (deletions that
 //     cut a reachable vertex off the source are dropped and replaced, the
@@ -17,6 +18,7 @@
 #include <cstdint>
 #include <cstring>
 #include <numeric>
+#include <thread>
 #include <unordered_set>
 #include <vector>
 
@@ -182,34 +184,110 @@ void road_grid(int width, int height, int K, int wmin, int wmax, double drop,
   to_csr(n, K, us, vs, ws, g);
 }
 
+// R-MAT draws in fixed chunks, each with its own stream (seeded from the
+// seed and the chunk index), so the instance does not depend on how many
+// threads draw it; arcs are then bucketed by the top bits of their source,
+// each bucket sorted and de-duplicated on its own (duplicates share a
+// bucket), and each bucket's weights drawn from its own stream.
 void rmat(int scale, int edgeFactor, double a, double b, double c, int K, int wmin, int wmax,
           uint64_t seed, CsrGraph &g) {
-  Rng rng(seed);
   const int n = 1 << scale;
   const uint64_t m = static_cast<uint64_t>(edgeFactor) << scale;
-  std::vector<uint64_t> arcs;
-  arcs.reserve(m);
-  for (uint64_t i = 0; i < m; ++i) {
-    uint64_t u = 0, v = 0;
-    for (int bit = 0; bit < scale; ++bit) {
-      const double r = rng.unit();
-      const uint64_t du = r >= a + b ? 1 : 0;
-      const uint64_t dv = (r >= a && r < a + b) || r >= a + b + c ? 1 : 0;
-      u = (u << 1) | du;
-      v = (v << 1) | dv;
+  constexpr uint64_t NONE = ~0ULL;
+  constexpr int CHUNKS = 256;
+  const unsigned threads = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+  auto parallel = [&](int count, auto &&f) {
+    std::vector<std::thread> pool;
+    for (unsigned t = 1; t < threads; ++t) {
+      pool.emplace_back([&, t] {
+        for (int i = static_cast<int>(t); i < count; i += static_cast<int>(threads)) f(i);
+      });
     }
-    if (u != v) arcs.push_back((u << 32) | v);
+    for (int i = 0; i < count; i += static_cast<int>(threads)) f(i);
+    for (std::thread &th : pool) th.join();
+  };
+  std::vector<uint64_t> draws(m);
+  parallel(CHUNKS, [&](int ch) {
+    Rng rng(seed ^ (0x5851f42d4c957f2dULL * static_cast<uint64_t>(ch + 1)));
+    const uint64_t lo = m * ch / CHUNKS, hi = m * (ch + 1) / CHUNKS;
+    for (uint64_t i = lo; i < hi; ++i) {
+      uint64_t u = 0, v = 0;
+      for (int bit = 0; bit < scale; ++bit) {
+        const double r = rng.unit();
+        const uint64_t du = r >= a + b ? 1 : 0;
+        const uint64_t dv = (r >= a && r < a + b) || r >= a + b + c ? 1 : 0;
+        u = (u << 1) | du;
+        v = (v << 1) | dv;
+      }
+      draws[i] = u != v ? (u << 32) | v : NONE;
+    }
+  });
+  // Buckets by the source's top bits: count per (chunk, bucket), scan, scatter.
+  const int bucketBits = std::min(scale, 8);
+  const int buckets = 1 << bucketBits;
+  const int shift = 32 + scale - bucketBits;
+  std::vector<uint64_t> counts(static_cast<size_t>(CHUNKS) * buckets, 0);
+  parallel(CHUNKS, [&](int ch) {
+    const uint64_t lo = m * ch / CHUNKS, hi = m * (ch + 1) / CHUNKS;
+    uint64_t *cnt = &counts[static_cast<size_t>(ch) * buckets];
+    for (uint64_t i = lo; i < hi; ++i) {
+      if (draws[i] != NONE) ++cnt[draws[i] >> shift];
+    }
+  });
+  std::vector<uint64_t> bucketStart(buckets + 1, 0), at(counts.size());
+  {
+    uint64_t total = 0;
+    for (int bk = 0; bk < buckets; ++bk) {
+      bucketStart[bk] = total;
+      for (int ch = 0; ch < CHUNKS; ++ch) {
+        at[static_cast<size_t>(ch) * buckets + bk] = total;
+        total += counts[static_cast<size_t>(ch) * buckets + bk];
+      }
+    }
+    bucketStart[buckets] = total;
   }
-  std::sort(arcs.begin(), arcs.end());
-  arcs.erase(std::unique(arcs.begin(), arcs.end()), arcs.end());
-  std::vector<int> us(arcs.size()), vs(arcs.size()), ws;
-  ws.reserve(arcs.size() * K);
-  for (size_t i = 0; i < arcs.size(); ++i) {
-    us[i] = static_cast<int>(arcs[i] >> 32);
-    vs[i] = static_cast<int>(arcs[i] & 0xffffffffu);
-    for (int k = 0; k < K; ++k) ws.push_back(static_cast<int>(rng.between(wmin, wmax)));
+  std::vector<uint64_t> arcs(bucketStart[buckets]);
+  parallel(CHUNKS, [&](int ch) {
+    const uint64_t lo = m * ch / CHUNKS, hi = m * (ch + 1) / CHUNKS;
+    uint64_t *pos = &at[static_cast<size_t>(ch) * buckets];
+    for (uint64_t i = lo; i < hi; ++i) {
+      if (draws[i] != NONE) arcs[pos[draws[i] >> shift]++] = draws[i];
+    }
+  });
+  std::vector<uint64_t>().swap(draws);
+  std::vector<uint64_t> kept(buckets, 0);
+  parallel(buckets, [&](int bk) {
+    auto first = arcs.begin() + static_cast<std::ptrdiff_t>(bucketStart[bk]);
+    auto last = arcs.begin() + static_cast<std::ptrdiff_t>(bucketStart[bk + 1]);
+    std::sort(first, last);
+    kept[bk] = static_cast<uint64_t>(std::unique(first, last) - first);
+  });
+  std::vector<uint64_t> out(buckets + 1, 0);
+  for (int bk = 0; bk < buckets; ++bk) out[bk + 1] = out[bk] + kept[bk];
+  const size_t arcsKept = out[buckets];
+  g = CsrGraph();
+  g.numberOfNodes = n;
+  g.numberOfObjectives = K;
+  g.rowPtr.assign(static_cast<size_t>(n) + 1, 0);
+  g.colInd.resize(arcsKept);
+  g.weights.resize(arcsKept * K);
+  parallel(buckets, [&](int bk) {
+    Rng rng(seed ^ (0x9e3779b97f4a7c15ULL * static_cast<uint64_t>(bk + 1)) ^ 0xd1b54a32d192ed03ULL);
+    const uint64_t *src = &arcs[bucketStart[bk]];
+    for (uint64_t j = 0; j < kept[bk]; ++j) {
+      const size_t e = out[bk] + j;
+      g.colInd[e] = static_cast<int>(src[j] & 0xffffffffu);
+      for (int k = 0; k < K; ++k) {
+        g.weights[e * K + k] = static_cast<int>(rng.between(wmin, wmax));
+      }
+    }
+  });
+  // Row pointers: arcs are sorted by source across buckets.
+  for (int bk = 0; bk < buckets; ++bk) {
+    const uint64_t *src = &arcs[bucketStart[bk]];
+    for (uint64_t j = 0; j < kept[bk]; ++j) ++g.rowPtr[static_cast<size_t>(src[j] >> 32) + 1];
   }
-  to_csr(n, K, us, vs, ws, g);
+  for (int v = 0; v < n; ++v) g.rowPtr[v + 1] += g.rowPtr[v];
 }
 
 std::vector<char> reachable(const CsrGraph &g, int source) {

@@ -9,9 +9,11 @@ the library is missing -- there is no CPU fallback.
 """
 from __future__ import annotations
 
+import atexit
 import ctypes as C
 import os
 import pathlib
+import weakref
 from dataclasses import dataclass, field
 
 import numpy as np
@@ -92,6 +94,34 @@ EXPORTS = (
 )
 
 _LIB: C.CDLL | None = None
+
+# Every live object that owns library or device resources, released at
+# interpreter exit in dependency order (servers and their worker threads
+# first, plain device buffers last) while the HIP runtime is still intact,
+# instead of by finalizers that may run after it has been torn down.
+_LIVE: "weakref.WeakSet" = weakref.WeakSet()
+_RELEASE_ORDER = ("Server", "BatchedUpdate", "Session", "Workspace", "DeviceGraph",
+                  "DeviceBuffer")
+
+
+def _track(obj) -> None:
+    _LIVE.add(obj)
+
+
+@atexit.register
+def _release_all() -> None:
+    if _LIB is None:
+        return
+    rank = {name: i for i, name in enumerate(_RELEASE_ORDER)}
+    for obj in sorted(list(_LIVE), key=lambda o: rank.get(type(o).__name__, len(rank))):
+        try:
+            (getattr(obj, "close", None) or getattr(obj, "free"))()
+        except Exception:
+            pass
+    try:
+        _LIB.mosp_device_synchronize()
+    except Exception:
+        pass
 
 
 def lib() -> C.CDLL:
@@ -279,31 +309,37 @@ class Batch:
 
 def gen_road_grid(width: int, height: int, K: int, seed: int, drop: float = 0.3,
                   diag: float = 0.02, wmin: int = 1, wmax: int = 100) -> HostGraph:
+    """One call of the generator into buffers of an upper bound of its arcs
+    (both directions of every grid edge and diagonal)."""
     L = lib()
     n = C.c_int32(0)
     m = C.c_int64(0)
-    args = (width, height, K, wmin, wmax, drop, diag, seed)
-    _check(L.mosp_gen_road_grid(*args, C.byref(n), C.byref(m), None, None, None, 0), "road grid")
-    row = np.zeros(n.value + 1, np.int32)
-    col = np.zeros(max(m.value, 1), np.int32)
-    w = np.zeros(max(m.value, 1) * K, np.int32)
-    _check(L.mosp_gen_road_grid(*args, C.byref(n), C.byref(m), row.ctypes.data,
-                                col.ctypes.data, w.ctypes.data, m.value), "road grid")
-    return HostGraph(n.value, K, row, col[: m.value].copy(), w[: m.value * K].copy())
+    cells = width * height
+    cap = 2 * (width * (height - 1) + height * (width - 1)) + 2 * int(diag * cells + 2) + 16
+    row = np.zeros(cells + 1, np.int32)
+    col = np.zeros(max(cap, 1), np.int32)
+    w = np.zeros(max(cap, 1) * K, np.int32)
+    _check(L.mosp_gen_road_grid(width, height, K, wmin, wmax, drop, diag, seed, C.byref(n),
+                                C.byref(m), row.ctypes.data, col.ctypes.data, w.ctypes.data,
+                                cap), "road grid")
+    return HostGraph(n.value, K, row[: n.value + 1], col[: m.value].copy(),
+                     w[: m.value * K].copy())
 
 
 def gen_rmat(scale: int, edge_factor: int, K: int, seed: int, wmin: int = 1, wmax: int = 100,
              a: float = 0.57, b: float = 0.19, c: float = 0.19) -> HostGraph:
+    """One call of the generator into buffers for every draw (an upper
+    bound: self-loops and duplicates are removed)."""
     L = lib()
     n = C.c_int32(0)
     m = C.c_int64(0)
-    args = (scale, edge_factor, a, b, c, K, wmin, wmax, seed)
-    _check(L.mosp_gen_rmat(*args, C.byref(n), C.byref(m), None, None, None, 0), "rmat")
-    row = np.zeros(n.value + 1, np.int32)
-    col = np.zeros(max(m.value, 1), np.int32)
-    w = np.zeros(max(m.value, 1) * K, np.int32)
-    _check(L.mosp_gen_rmat(*args, C.byref(n), C.byref(m), row.ctypes.data, col.ctypes.data,
-                           w.ctypes.data, m.value), "rmat")
+    cap = edge_factor << scale
+    row = np.zeros((1 << scale) + 1, np.int32)
+    col = np.zeros(max(cap, 1), np.int32)
+    w = np.zeros(max(cap, 1) * K, np.int32)
+    _check(L.mosp_gen_rmat(scale, edge_factor, a, b, c, K, wmin, wmax, seed, C.byref(n),
+                           C.byref(m), row.ctypes.data, col.ctypes.data, w.ctypes.data, cap),
+           "rmat")
     return HostGraph(n.value, K, row, col[: m.value].copy(), w[: m.value * K].copy())
 
 
@@ -457,6 +493,7 @@ class DeviceBuffer:
         self.ptr = lib().mosp_dev_alloc(max(self.nbytes, 1))
         if not self.ptr:
             raise RuntimeError(last_error())
+        _track(self)
 
     @classmethod
     def from_numpy(cls, a: np.ndarray) -> "DeviceBuffer":
@@ -488,6 +525,7 @@ class Workspace:
     def __init__(self, capacity: int = 0, slots: int = 1, grid: int = 0):
         _require_gpu()
         self.handle = lib().mosp_ws_create()
+        _track(self)
         if grid:
             lib().mosp_ws_set_grid(self.handle, grid)
         if capacity:
@@ -541,6 +579,7 @@ class DeviceGraph:
 
     def __init__(self, g: HostGraph | None = None):
         _require_gpu()
+        _track(self)
         if g is None:  # empty: the target of apply()
             self.n = self.K = self.m = 0
             self.handle = lib().mosp_device_graph_create()

@@ -19,8 +19,8 @@ from dataclasses import dataclass
 
 import numpy as np
 
-from . import (BatchInstance, Changes, CombineStats, Session, SospStats, _check, gen_batch,
-               gen_road_grid, initial_trees, last_error, lib)
+from . import (BatchInstance, Changes, CombineStats, Session, SospStats, _check, _track,
+               gen_batch, gen_road_grid, initial_trees, last_error, lib)
 
 
 class BatchedUpdate:
@@ -43,6 +43,7 @@ class BatchedUpdate:
         self.handle = lib().mosp_batch_create(arr, len(self.sessions))
         if not self.handle:
             raise RuntimeError(f"mosp_batch_create failed: {last_error()}")
+        _track(self)
         self.configure(workgroups, lane_bits, delta_scale, xcd_aware)
 
     def configure(self, workgroups: int = 0, lane_bits: int = -1, delta_scale: float = 1.0,

@@ -33,7 +33,7 @@ from dataclasses import dataclass, field
 
 import numpy as np
 
-from . import Batch, HostGraph, _check, _i32, _nz, lib, last_error
+from . import Batch, HostGraph, _check, _i32, _nz, _track, lib, last_error
 
 _POLICIES = {"static": 0, "greedy": 1}
 
@@ -118,6 +118,7 @@ class Server:
         self.handle = lib().mosp_server_create(C.byref(cfg))
         if not self.handle:
             raise RuntimeError(f"mosp_server_create: {last_error()}")
+        _track(self)
         self.slots = slots
         self.keep_results = keep_results
         self.graphs: list[tuple[int, int]] = []   # (n, K) per graph id
