@@ -441,6 +441,7 @@ int prepare(mosp_batch *b) {
       const double scaled = std::llround(static_cast<double>(I.delta[k]) * b->deltaScale);
       const u64 delta = static_cast<u64>(std::max(1.0, scaled));
       ObjDesc d = make_desc(I.out[k], &I.in[k], &I.changes[k], I.pack[k], delta, states[k]);
+      graph_edges(I.spec.graph, k, &d.outEdge, &d.inEdge);
       const size_t off = static_cast<size_t>(k) * nn;
       d.distIn = I.spec.d_dist_in + off;
       d.parentIn = I.spec.d_parents_in + off;
@@ -449,8 +450,9 @@ int prepare(mosp_batch *b) {
       b->hDesc[I.descOffset + k] = d;
       anyFallback |= I.pack[k].bits == 0 ? 1 : 0;
     }
-    mosp_csr view{n, 0, cRowPtr, cColInd, cWeights};
+    mosp_csr view{n, 0, cRowPtr, nullptr, nullptr};
     ObjDesc c = make_desc(view, nullptr, nullptr, I.combPack, 0, states[0]);
+    c.outEdge = reinterpret_cast<int2 *>(cColInd);  // K * n records over cColInd + cWeights
     c.distOut = I.spec.d_comb_dist;
     c.parentOut = I.spec.d_comb_parent;
     b->hDesc[I.descOffset + K] = c;
@@ -483,6 +485,7 @@ int prepare(mosp_batch *b) {
     a.cRowPtr = cRowPtr;
     a.cColInd = cColInd;
     a.cWeights = cWeights;
+    a.cEdge = reinterpret_cast<int2 *>(cColInd);
     a.cPartials = cPartials;
     a.stage = reinterpret_cast<u64 *>(cColInd);  // cColInd, cWeights: 8 * K * n bytes in a row
     a.cDesc = b->dDesc + I.descOffset + K;

@@ -56,6 +56,20 @@ __global__ void split_weights_kernel(const int *edgeMajor, long long total, long
 }
 
 
+// {column, weight} records of every objective: edge[k * stride + e] =
+// {col[e], w[k * stride + e]} for e < m = rowPtr[n] (read on the device, so
+// the host need not know m yet).
+__global__ void pair_edges_kernel(const int *rowPtr, int n, const int *col, const int *w,
+                                  long long stride, int K, int2 *edge) {
+  const long long m = rowPtr[n];
+  const long long total = m * K;
+  for (long long i = blockIdx.x * static_cast<long long>(blockDim.x) + threadIdx.x; i < total;
+       i += static_cast<long long>(gridDim.x) * blockDim.x) {
+    const long long k = i / m, e = i - k * m;
+    edge[k * stride + e] = make_int2(col[e], w[k * stride + e]);
+  }
+}
+
 __global__ void copy_ints_kernel(int *dst, const int *src, int count) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < count) {
@@ -896,7 +910,10 @@ void release_arrays(mosp_device_graph *g) {
   (void)hipFree(g->inRow);
   (void)hipFree(g->inCol);
   (void)hipFree(g->inW);
+  (void)hipFree(g->outEdge);
+  (void)hipFree(g->inEdge);
   g->outRow = g->outCol = g->outW = g->inRow = g->inCol = g->inW = nullptr;
+  g->outEdge = g->inEdge = nullptr;
   g->stride = 0;
   g->rowCap = 0;
 }
@@ -919,6 +936,8 @@ int reserve(mosp_device_graph *g, int n, long long m, int K) {
   HIP_TRY(hipMalloc(&g->inRow, rows));
   HIP_TRY(hipMalloc(&g->inCol, cols));
   HIP_TRY(hipMalloc(&g->inW, cols * K));
+  HIP_TRY(hipMalloc(&g->outEdge, sizeof(int2) * static_cast<size_t>(stride) * K));
+  HIP_TRY(hipMalloc(&g->inEdge, sizeof(int2) * static_cast<size_t>(stride) * K));
   g->stride = stride;
   g->rowCap = n + 1;
   g->K = K;
@@ -1047,6 +1066,19 @@ int queue_max_degree(const int *outRow, const int *inRow, int n, unsigned *scrat
   return 1;
 }
 
+// The out- and in-edge records of g from its arrays (after an upload or an
+// apply, once the reverse CSR is built); mBound >= m sizes the grid.
+int pair_edges(mosp_device_graph *g, long long mBound, hipStream_t stream) {
+  const long long total = std::max(mBound, 1LL) * g->K;
+  const int blocks = static_cast<int>(std::min<long long>((total + BLOCK - 1) / BLOCK, 8192));
+  hipLaunchKernelGGL(pair_edges_kernel, dim3(blocks), dim3(BLOCK), 0, stream, g->outRow, g->n,
+                     g->outCol, g->outW, g->stride, g->K, g->outEdge);
+  hipLaunchKernelGGL(pair_edges_kernel, dim3(blocks), dim3(BLOCK), 0, stream, g->inRow, g->n,
+                     g->inCol, g->inW, g->stride, g->K, g->inEdge);
+  HIP_TRY(hipGetLastError());
+  return 1;
+}
+
 // Statistics words of a graph: 2 * MOSP_MAX_OBJ weight words, then the degree.
 int ensure_stats(mosp_device_graph *g) {
   if (g->stats == nullptr) {
@@ -1142,7 +1174,7 @@ extern "C" mosp_device_graph *mosp_device_graph_upload(int32_t n, int32_t K, con
     const long long total = static_cast<long long>(m) * K;
     hipLaunchKernelGGL(split_weights_kernel, dim3(grid_for(total)), dim3(BLOCK), 0, 0, edgeMajor,
                        total, g->stride, K, g->outW);
-    if (!build_reverse(g, g->scratch + wBytes, m, hubs, 0)) return 0;
+    if (!build_reverse(g, g->scratch + wBytes, m, hubs, 0) || !pair_edges(g, m, 0)) return 0;
     if (!ensure_stats(g) ||
         !graph_max_degree(g->outRow, g->inRow, n, degree_word(g), 0, &g->maxDegree)) {
       return 0;
@@ -1383,7 +1415,10 @@ extern "C" int mosp_device_graph_apply(const mosp_device_graph *g, int32_t nIns,
                        out->stride);
   }
   HIP_TRY(hipGetLastError());
-  if (!build_reverse(out, reverseAt, mOld + nIns, hubs, stream)) return 0;
+  if (!build_reverse(out, reverseAt, mOld + nIns, hubs, stream) ||
+      !pair_edges(out, mOld + nIns, stream)) {
+    return 0;
+  }
   if (!ensure_stats(out) ||
       !queue_max_degree(out->outRow, out->inRow, n, degree_word(out), stream)) {
     return 0;
@@ -1407,6 +1442,12 @@ extern "C" int mosp_device_graph_apply(const mosp_device_graph *g, int32_t nIns,
 
 namespace mosp {
 namespace detail {
+
+void graph_edges(const mosp_device_graph *g, int k, const int2 **out, const int2 **in) {
+  const size_t off = static_cast<size_t>(k) * g->stride;
+  *out = g->outEdge + off;
+  *in = g->inEdge + off;
+}
 
 int graph_max_degree(const int *outRow, const int *inRow, int n, unsigned *scratch,
                      hipStream_t stream, int *result) {

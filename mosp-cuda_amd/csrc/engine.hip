@@ -800,11 +800,13 @@ int setup_combined(mosp_ws *ws, LaunchArgs &args, const int32_t *d_parents,
   args.cRowPtr = ws->cRowPtr;
   args.cColInd = ws->cColInd;
   args.cWeights = ws->cWeights;
+  args.cEdge = reinterpret_cast<int2 *>(ws->cColInd);  // K * n records over both arrays
   if (!choose_packing(n, base, pc)) {
     return 0;
   }
-  mosp_csr view{n, 0, ws->cRowPtr, ws->cColInd, ws->cWeights};
+  mosp_csr view{n, 0, ws->cRowPtr, nullptr, nullptr};
   ObjDesc d = make_desc(view, nullptr, nullptr, pc, 0, ws->state[0]);
+  d.outEdge = args.cEdge;
   d.distOut = d_dist;
   d.parentOut = d_parent;
   ws->hDesc[descIndex] = d;
@@ -1043,6 +1045,7 @@ extern "C" int mosp_update_fused(const mosp_device_graph *g, int32_t K,
     mosp_device_graph_view(g, k, &out, &in);
     ObjDesc d = make_desc(out, &in, &changes[k], pc, static_cast<u64>(delta[k]),
                           ws->state[k]);
+    graph_edges(g, k, &d.outEdge, &d.inEdge);
     const size_t off = static_cast<size_t>(k) * n;
     d.distIn = distIn + off;
     d.parentIn = parIn + off;

@@ -13,6 +13,7 @@
 // reduction pass.
 #pragma once
 
+#include <hip/hip_runtime.h>
 #include <stddef.h>
 #include <stdint.h>
 
@@ -68,6 +69,10 @@ struct ObjState {
 struct ObjDesc {
   const int *outRow, *outCol, *outW;  // out-CSR + this objective's weights
   const int *inRow, *inCol, *inW;     // in-CSR + weights (update only)
+  // The same edges as {column, weight} records (one 8-byte load per edge),
+  // when the graph holds them (device graphs, the combined graph); null:
+  // the separate arrays above are read.
+  const int2 *outEdge, *inEdge;
   const int *chFrom, *chTo;           // deleted / weight-increased edges
   int nChanged;
   const int *heads;                   // heads of inserted edges
@@ -170,6 +175,7 @@ struct LaunchArgs {
   int *cRowPtr;            // n + 1 (written here, read through cDesc->outRow)
   int *cColInd;            // K * n
   int *cWeights;           // K * n
+  int2 *cEdge;             // K * n {column, weight} records (the memory of cColInd + cWeights)
   int *cPartials;          // one per workgroup (scan)
   u64 *stage;              // nobj * n words (may alias cColInd..cWeights, which are
                            // written only after it is read): the fused parent

@@ -21,6 +21,9 @@ struct mosp_device_graph {
   int rowCap = 0;          // vertex capacity of the row arrays
   int *outRow = nullptr, *outCol = nullptr, *outW = nullptr;
   int *inRow = nullptr, *inCol = nullptr, *inW = nullptr;
+  // Objective k's edges as {column, weight} records at k * stride, out and
+  // in (built after every upload and apply).
+  int2 *outEdge = nullptr, *inEdge = nullptr;
   char *scratch = nullptr;  // build / apply temporaries (grow-only)
   size_t scratchBytes = 0;
   unsigned long long *stats = nullptr;  // weight statistics (2 * MOSP_MAX_OBJ words) + degree
@@ -39,6 +42,9 @@ hipError_t launch_ctl_reset_many(Ctl *ctls, int count, hipStream_t stream,
                                  unsigned *abortWord = nullptr);
 
 namespace detail {
+
+// Objective k's {column, weight} record arrays of a device graph (out, in).
+void graph_edges(const mosp_device_graph *g, int k, const int2 **out, const int2 **in);
 
 // Largest out- or in-degree (inRow may be null) of a CSR with n rows, via
 // one device word `scratch`; synchronous on `stream` (device_graph.hip).

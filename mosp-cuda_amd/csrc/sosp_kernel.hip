@@ -418,6 +418,25 @@ __device__ __forceinline__ u64 dist_of(u64 word, int bits) {
   return word >> bits;
 }
 
+// Edge x of an objective as {column, weight}: one 8-byte record when the
+// graph holds records (device graphs, the combined graph), else the
+// separate column and weight arrays.  The choice is wave-uniform.
+__device__ __forceinline__ int2 out_edge(Desc &D, long long x) {
+  const int2 *rec = D.outEdge;
+  if (rec != nullptr) {
+    return rec[x];
+  }
+  return make_int2(D.outCol[x], D.outW[x]);
+}
+
+__device__ __forceinline__ int2 in_edge(Desc &D, long long x) {
+  const int2 *rec = D.inEdge;
+  if (rec != nullptr) {
+    return rec[x];
+  }
+  return make_int2(D.inCol[x], D.inW[x]);
+}
+
 // Stamp the end of a phase (called right after a grid barrier).
 __device__ __forceinline__ void stamp(Ctl *ctl, int phase, unsigned tid) {
   if (tid == 0) {
@@ -602,6 +621,7 @@ __device__ void push_team(Desc &D, ObjCtl &c, Shared &sh, int k,
   const int *const outRow = uniform_global(D.outRow);
   const int *const outCol = uniform_global(D.outCol);
   const int *const outW = uniform_global(D.outW);
+  const int2 *const outRec = uniform_global(D.outEdge);
   const int *in = uniform_global(SPLIT ? D.s.far[(e + 1) & 1] : D.s.near[it & 1]);
   int *outNear = uniform_global(D.s.near[(it + 1) & 1]);
   int *farBuf = uniform_global(D.s.far[e & 1]);
@@ -612,11 +632,20 @@ __device__ void push_team(Desc &D, ObjCtl &c, Shared &sh, int k,
   const unsigned perPass = tm.size >> laneBits;
   const int lane = static_cast<int>(__lane_id());
   auto relax = [&](int u, u64 du, int x) {
-    const int w = outCol[x];
+    // One record load when the graph holds {column, weight} records.
+    int w, weight;
+    if (outRec != nullptr) {
+      const int2 r = outRec[x];
+      w = r.x;
+      weight = r.y;
+    } else {
+      w = outCol[x];
+      weight = outW[x];
+    }
     if (w == source) {
       return;  // the source is never relaxed
     }
-    const u64 nd = du + static_cast<u64>(outW[x]);
+    const u64 nd = du + static_cast<u64>(weight);
     if (nd > maxDist) {
       return;  // never shortest; would not fit the packed word
     }
@@ -1081,16 +1110,15 @@ __device__ void sweep_edges(const int *rowPtr, int n, Threads t, Shared &sh, F &
 // column read).
 __device__ void recover_edges(Desc *objs, int count, int n, int source, Threads t,
                               Shared &sh) {
-  const int *const outCol = objs[0].outCol;
   sweep_edges(objs[0].outRow, n, t, sh, [&](int u, long long e) {
-    const int w = outCol[e];
+    const int w = out_edge(objs[0], e).x;
     if (w == source) {
       return;
     }
     for (int k = 0; k < count; ++k) {
       Desc &D = objs[k];
       const u64 du = D.s.packed[u];
-      if (du != PACKED_INF && du + static_cast<u64>(D.outW[e]) == D.s.packed[w]) {
+      if (du != PACKED_INF && du + static_cast<u64>(out_edge(D, e).y) == D.s.packed[w]) {
         atomicMin(&D.parentOut[w], u);
       }
     }
@@ -1169,9 +1197,9 @@ template <bool HUBS>
 __device__ void recover_parents(Desc &D, int n, int source,
                                 Threads t, Shared &sh) {
   auto scan = [&](int u, u64 du, int x) {
-    const int w = D.outCol[x];
-    if (w != source && du + static_cast<u64>(D.outW[x]) == D.s.packed[w]) {
-      atomicMin(&D.parentOut[w], u);
+    const int2 ew = out_edge(D, x);
+    if (ew.x != source && du + static_cast<u64>(ew.y) == D.s.packed[ew.x]) {
+      atomicMin(&D.parentOut[ew.x], u);
     }
   };
   if constexpr (!HUBS) {
@@ -1420,12 +1448,13 @@ __device__ int update_step1(Args &a, GridBarrier &grid, Threads t,
     }
     __syncthreads();
     auto candidate = [&](int x) -> u64 {
-      const int u = D.inCol[x];
+      const int2 ew = in_edge(D, x);
+      const int u = ew.x;
       const u64 word = D.s.packed[u];
       if (word == PACKED_INF) {
         return PACKED_INF;
       }
-      const u64 nd = dist_of(word, bits) + static_cast<u64>(D.inW[x]);
+      const u64 nd = dist_of(word, bits) + static_cast<u64>(ew.y);
       // Larger than maxDistance: never shortest, may not fit the word.
       return nd <= D.maxDistance ? pack(nd, u, bits, D.noParent) : PACKED_INF;
     };
@@ -1511,10 +1540,11 @@ __device__ int update_step1(Args &a, GridBarrier &grid, Threads t,
         const u64 current = D.s.packed[v];
         u64 part = PACKED_INF;
         for (int x = D.inRow[v] + static_cast<int>(threadIdx.x); x < D.inRow[v + 1]; x += BLOCK) {
-          const int u = D.inCol[x];
+          const int2 ew = in_edge(D, x);
+          const int u = ew.x;
           const u64 word = D.s.packed[u];
           if (word != PACKED_INF) {
-            const u64 nd = dist_of(word, bits) + static_cast<u64>(D.inW[x]);
+            const u64 nd = dist_of(word, bits) + static_cast<u64>(ew.y);
             if (nd <= D.maxDistance) {
               const u64 cand = pack(nd, u, bits, D.noParent);
               part = cand < part ? cand : part;
@@ -1792,8 +1822,7 @@ __device__ void combined_build(Args &args, GridBarrier &grid,
       } else {
         pos = atomicAdd(&a.cDegree[p], 1);
       }
-      a.cColInd[pos] = v;
-      a.cWeights[pos] = weight;
+      a.cEdge[pos] = make_int2(v, weight);
     });
   }
   // Near-far width for Step 3: the caller's, or 32 * average weight /
