@@ -467,6 +467,7 @@ int prepare(mosp_batch *b) {
     a.anyFallback = anyFallback;
     a.laneBits = b->laneBits;
     a.listCap = list_cap_override();
+    a.hubEdges = hub_edges_override();
     a.hopsUpdate = b->hopsUpdate;
     a.hopsSearch = b->hopsSearch;
     a.genDev = b->dGen + i;

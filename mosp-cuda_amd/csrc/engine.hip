@@ -239,7 +239,8 @@ int ws_warm_up(mosp_ws *ws) {
 
 size_t state_bytes(int n) {
   const size_t count = static_cast<size_t>(std::max(n, 1));
-  return align256(count * sizeof(u64)) + 8 * align256(count * sizeof(int));
+  return align256(count * sizeof(u64)) + 8 * align256(count * sizeof(int)) +
+         align256(2 * static_cast<size_t>(HUB_LIST) * sizeof(int));
 }
 
 ObjState carve_state(char *&p, int n) {
@@ -255,6 +256,8 @@ ObjState carve_state(char *&p, int n) {
     *ints[i] = reinterpret_cast<int *>(p);
     p += intBytes;
   }
+  st.hubs = reinterpret_cast<int *>(p);
+  p += align256(2 * static_cast<size_t>(HUB_LIST) * sizeof(int));
   return st;
 }
 
@@ -400,6 +403,7 @@ int run_launch(mosp_ws *ws, LaunchArgs &args, int descCount,
   args.hopsUpdate = ws->hopsUpdate;
   args.hopsSearch = ws->hopsSearch;
   args.listCap = ws->listCap;
+  args.hubEdges = hub_edges_override();
   args.cPartials = ws->cPartials;
   args.cDeltaScale = ws->combDeltaScale;
   if (descCount > 0) {
@@ -497,6 +501,11 @@ void local_hops_override(int &update, int &search) {
 int list_cap_override() {
   const char *env = std::getenv("MOSP_LIST_CAP");
   return env == nullptr ? 0 : std::max(0, std::atoi(env));
+}
+
+int hub_edges_override() {
+  const char *env = std::getenv("MOSP_HUB_EDGES");
+  return env == nullptr || *env == '\0' ? HUB_GRID : std::max(64, std::atoi(env));
 }
 
 int hubs_override() {
@@ -1099,7 +1108,8 @@ constexpr size_t NA_WORDS = NA_DESC + (sizeof(ObjDesc) + 255) / 256 * 256;
 constexpr size_t NA_STATE = NA_WORDS + 256;
 constexpr size_t NA_STATE_BYTES =
     (static_cast<size_t>(NOALLOC_MAX_VERTICES) * sizeof(u64) + 255) / 256 * 256 +
-    8 * ((static_cast<size_t>(NOALLOC_MAX_VERTICES) * sizeof(int) + 255) / 256 * 256);
+    8 * ((static_cast<size_t>(NOALLOC_MAX_VERTICES) * sizeof(int) + 255) / 256 * 256) +
+    (2 * static_cast<size_t>(HUB_LIST) * sizeof(int) + 255) / 256 * 256;
 constexpr size_t NA_BYTES = NA_STATE + NA_STATE_BYTES;
 
 // The scratch of every mosp_sosp_update_noalloc call on a device (each
@@ -1241,6 +1251,7 @@ extern "C" int mosp_sosp_update_noalloc(const mosp_csr *out, const mosp_csr *in,
   args.hopsSearch = kHopsSearch;
   local_hops_override(args.hopsUpdate, args.hopsSearch);
   args.listCap = list_cap_override();
+  args.hubEdges = hub_edges_override();
   args.hubs = hubs;
   args.genDev = genDev;
   args.ctl = dCtl;

@@ -39,6 +39,12 @@ constexpr int BLOCK = MOSP_BLOCK;    // threads per workgroup (wave64 multiple)
 // Graphs with a vertex of more than HUB_DEGREE out- or in-edges run the
 // kernel variant that hands such vertices to whole waves.
 constexpr int HUB_DEGREE = 32;
+// Hub variant: frontier vertices of more than HUB_GRID out-edges are listed
+// for the next near-far iteration, in which every workgroup of their
+// objective relaxes an even share of their edges (at most HUB_LIST of them
+// per iteration and objective; the others are relaxed by one workgroup).
+constexpr int HUB_GRID = 4096;
+constexpr int HUB_LIST = 512;
 
 // Phases of one launch.
 enum : int {
@@ -63,6 +69,7 @@ struct ObjState {
   int *ancestor;   // pointer-jumping ancestors, n
   int *near[2];    // near frontiers (ping-pong), n each
   int *far[2];     // far piles (ping-pong), n each
+  int *hubs;       // grid-relaxed hubs of the next iteration (ping-pong), 2 * HUB_LIST
 };
 
 // One objective of a launch (device-resident descriptor).
@@ -101,14 +108,16 @@ struct ObjCtl {
   unsigned epochs;
   unsigned jumpCount[3];   // pointer jumping: still-jumping list, slot = round % 3
   unsigned pullHeavy;      // pull candidates left for whole workgroups (hub variant)
+  unsigned hubCount[3];    // grid-relaxed hubs listed for an iteration, slot = iteration % 3
   u64 pushes;
 };
 
 constexpr int BARRIER_GROUPS = 16;  // first-level arrival counters
 constexpr int LINE_WORDS = 32;      // one 128-byte line per counter
-// Words a barrier can publish with its epoch: four per objective (near count,
-// far count, far minimum as two words) for MOSP_MAX_OBJ + 1 searches.
-constexpr int MAIL_WORDS = 4 * (MOSP_MAX_OBJ + 1);
+// Words a barrier can publish with its epoch: five per objective (near count,
+// far count, far minimum as two words, listed hubs) for MOSP_MAX_OBJ + 1
+// searches.
+constexpr int MAIL_WORDS = 5 * (MOSP_MAX_OBJ + 1);
 // One mailbox per barrier group: the epoch line, then the published words.
 constexpr int MAIL_STRIDE =
     LINE_WORDS * ((2 * LINE_WORDS + MAIL_WORDS - 1) / LINE_WORDS);
@@ -155,6 +164,8 @@ struct LaunchArgs {
   int listCap;             // entries of each near / far list: 0 = n; a smaller
                            // value (tests) forces the sweeps that follow a full list
   int hubs;                // host only: launch the HUB_DEGREE-aware variant
+  int hubEdges;            // hub variant: push vertices of more out-edges go to
+                           // the objective's hub list (HUB_GRID; tests lower it)
   int *genDev;             // device-resident stamp generation (persists)
   Ctl *ctl;
   // Bounded barrier waits: a workgroup that waits longer than barrierTicks

@@ -79,6 +79,9 @@ hipError_t launch_persistent(const void *kernel, dim3 grid, dim3 block, void **p
 // MOSP_LIST_CAP=c caps the near / far lists at c entries instead of n, so
 // that small inputs take the sweep that follows a full list (tests); 0 unset.
 int list_cap_override();
+// MOSP_HUB_EDGES=e: the hub list takes push vertices of more than e edges
+// (at least 64; default HUB_GRID).
+int hub_edges_override();
 
 // Records the message for mosp_last_error() and returns 0.
 int fail(const std::string &message);

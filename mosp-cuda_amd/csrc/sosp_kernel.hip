@@ -263,6 +263,13 @@ __device__ __forceinline__ unsigned wave_reserve(unsigned *counter) {
 // barrier.
 constexpr int HEAVY_GROUP = 64;   // lane groups (push): more than one wave pass
 constexpr unsigned HEAVY_LIST = 128;  // per-wave LDS list of heavy push vertices
+// Push vertices of more edges than this go to the workgroup's list (drained
+// by all its threads) rather than the wave's, when the team is the whole
+// workgroup; those of more than the launch's hubEdges (HUB_GRID unless a
+// test lowers it) to the objective's hub list.
+constexpr int WAVE_EDGES = 512;
+constexpr unsigned WG_LIST = 256;     // workgroup LDS list of heavy push vertices
+static_assert(HUB_LIST <= BLOCK, "one thread per listed hub");
 constexpr int HEAVY_THREAD = HUB_DEGREE;  // one lane per vertex (parent recovery)
 // Combined-graph count in the hub variant: increments of a parent are first
 // summed in a per-workgroup LDS table (open addressing, AGG_PROBES probes;
@@ -291,6 +298,18 @@ __device__ __forceinline__ u64 wave_min_u64(u64 x) {
     x = y < x ? y : x;
   }
   return x;
+}
+
+// Sum of x over the lanes below this one.
+__device__ __forceinline__ u64 wave_exclusive_sum_u64(u64 x) {
+  const int lane = static_cast<int>(__lane_id());
+  u64 run = x;
+#pragma unroll
+  for (int offset = 1; offset < 64; offset <<= 1) {
+    const u64 y = shfl_u64(run, lane >= offset ? lane - offset : lane);
+    run += lane >= offset ? y : 0;
+  }
+  return run - x;
 }
 
 // Calls f(src) for every lane src of the wave with `mine` set, in lane order;
@@ -516,7 +535,18 @@ struct Shared {
   u64 statPush[MOSP_MAX_OBJ];
   unsigned red[BLOCK / 64];
   int scan[BLOCK];
-  int heavy[WAVES][HEAVY_LIST];  // push: per-wave list of heavy vertices
+  union {
+    int heavy[WAVES][HEAVY_LIST];  // push: per-wave list of heavy vertices
+    struct {                       // push: this iteration's hubs (before the lists fill)
+      u64 end[HUB_LIST];           // end of each hub's edges in their concatenation
+      int u[HUB_LIST];
+      int begin[HUB_LIST];         // its first edge
+    } hub;
+  };
+  unsigned hubs[MOSP_MAX_OBJ];   // hubs listed for this iteration (hub variant)
+  int hubEdges;                  // push: more out-edges -> the hub list (hub variant)
+  unsigned wgCount;              // push: entries of wgList (hub variant)
+  int wgList[WG_LIST];
   int span[2];                   // edge-parallel passes: rows of the first/last edge
   int aggKey[AGG_SLOTS];         // combined count (hub variant): parent ids ...
   int aggCount[AGG_SLOTS];       // ... and their increments in this workgroup
@@ -529,16 +559,20 @@ struct Shared {
 // proportion to their item counts, leftovers to the largest.  Integer
 // arithmetic only, so every workgroup derives the same partition.  With more
 // active objectives than workgroups, objective k goes to workgroup k % G.
+// A listed hub (hub variant) weighs as much as HUB_WEIGHT items.
+constexpr u64 HUB_WEIGHT = HUB_GRID / 16;
+
 __device__ void partition_blocks(Shared &sh, int nobj, unsigned G) {
   u64 total = 0;
   unsigned active = 0;
   int largest = -1;
+  auto weight = [&](int k) { return sh.cnt[k] + HUB_WEIGHT * sh.hubs[k]; };
   for (int k = 0; k < nobj; ++k) {
     sh.blockBegin[k] = sh.blockEnd[k] = 0;
     if (sh.mode[k] != MODE_DONE) {
-      total += sh.cnt[k];
+      total += weight(k);
       ++active;
-      if (largest < 0 || sh.cnt[k] > sh.cnt[largest]) largest = k;
+      if (largest < 0 || weight(k) > weight(largest)) largest = k;
     }
   }
   if (active == 0) return;
@@ -555,7 +589,7 @@ __device__ void partition_blocks(Shared &sh, int nobj, unsigned G) {
   unsigned given = 0;
   for (int k = 0; k < nobj; ++k) {
     if (sh.mode[k] != MODE_DONE) {
-      sh.blockEnd[k] = 1 + static_cast<int>(spare * sh.cnt[k] / (total > 0 ? total : 1));
+      sh.blockEnd[k] = 1 + static_cast<int>(spare * weight(k) / (total > 0 ? total : 1));
       given += sh.blockEnd[k];
     }
   }
@@ -730,11 +764,118 @@ __device__ void push_team(Desc &D, ObjCtl &c, Shared &sh, int k,
       }
     }
   } else {
-    // Heavy vertices wait in this wave's LDS list; the whole wave relaxes
-    // them one by one when the list could overflow and after the last pass
-    // (their distance is re-read then: a newer one is as valid a source).  One
-    // relax site serves both: each lane walks (vertex, first edge, stride, end)
-    // -- its group's light vertex, then the list's vertices in turn.
+    // Out-degree tiers.  Up to HEAVY_GROUP edges: the vertex's lane group, as
+    // above.  Up to WAVE_EDGES (up to hubEdges when the team is not the whole
+    // workgroup): this wave's LDS list, whose vertices the whole wave relaxes
+    // one by one when the list could overflow and after the last pass.  Up
+    // to hubEdges: the workgroup's LDS list, relaxed by all its threads, one
+    // vertex after the other, when it is half full and after the last pass.
+    // More: the objective's hub list, relaxed in the next iteration by every
+    // workgroup of the objective, each over an even share of the listed
+    // hubs' edges (the next iteration is then a push iteration, under the
+    // same threshold).  A full list hands a vertex down a tier.  Listed
+    // vertices' distances are re-read when they are relaxed: a newer one is
+    // as valid a source.
+    // First the hubs listed for this iteration.
+    const unsigned hc = sh.hubs[k] < static_cast<unsigned>(HUB_LIST) ? sh.hubs[k] : HUB_LIST;
+    if (hc > 0) {
+      const int *const hubIn = uniform_global(D.s.hubs + (it & 1) * HUB_LIST);
+      if (tm.whole) {
+        // The listed hubs' edges as one range: each hub's end in it, in LDS.
+        if (threadIdx.x < hc) {
+          const int u = hubIn[threadIdx.x];
+          const u64 word = packed[u];
+          const int b = outRow[u];
+          const int e2 = outRow[u + 1];
+          sh.hub.u[threadIdx.x] = u;
+          sh.hub.begin[threadIdx.x] = b;
+          sh.hub.end[threadIdx.x] = word != PACKED_INF ? static_cast<u64>(e2 - b) : 0;
+        }
+        __syncthreads();
+        if (threadIdx.x < 64) {
+          constexpr unsigned per = HUB_LIST / 64;  // entries per lane of wave 0
+          u64 own = 0;
+          for (unsigned j = 0; j < per; ++j) {
+            const unsigned at = static_cast<unsigned>(lane) * per + j;
+            own += at < hc ? sh.hub.end[at] : 0;
+          }
+          u64 run = wave_exclusive_sum_u64(own);
+          for (unsigned j = 0; j < per; ++j) {
+            const unsigned at = static_cast<unsigned>(lane) * per + j;
+            if (at < hc) {
+              run += sh.hub.end[at];
+              sh.hub.end[at] = run;
+            }
+          }
+        }
+        __syncthreads();
+        const u64 total = sh.hub.end[hc - 1];
+        const u64 from = total * blockRank / blocks;
+        const u64 to = total * (blockRank + 1) / blocks;
+        for (u64 base = from; base < to; base += BLOCK) {
+          const u64 g = base + threadIdx.x;
+          if (g < to) {
+            // The first hub whose end is past g.
+            unsigned j = 0;
+#pragma unroll
+            for (unsigned step = HUB_LIST / 2; step > 0; step >>= 1) {
+              if (j + step < hc && sh.hub.end[j + step - 1] <= g) {
+                j += step;
+              }
+            }
+            const u64 start = j > 0 ? sh.hub.end[j - 1] : 0;
+            const int u = sh.hub.u[j];
+            const u64 word = packed[u];
+            relax(u, dist_of(word, bits), sh.hub.begin[j] + static_cast<int>(g - start));
+          }
+          __syncthreads();
+          if (sh.near.count[0] > QCAP / 2) queue_flush(sh.near, outNear, nextCnt, listCap);
+          if (sh.far.count[0] > QCAP / 2) queue_flush(sh.far, farBuf, farCnt, listCap);
+        }
+        __syncthreads();  // the wave lists overlay the hub table
+      } else {
+        for (unsigned j = 0; j < hc; ++j) {
+          const int u = hubIn[j];
+          const u64 word = packed[u];
+          const int b = outRow[u];
+          const int len = word != PACKED_INF ? outRow[u + 1] - b : 0;
+          const int x0 = b + static_cast<int>(static_cast<u64>(len) * blockRank / blocks);
+          const int x1 = b + static_cast<int>(static_cast<u64>(len) * (blockRank + 1) / blocks);
+          const u64 du = dist_of(word, bits);
+          for (int x = x0 + static_cast<int>(tm.tid); x < x1; x += static_cast<int>(tm.size)) {
+            relax(u, du, x);
+          }
+        }
+      }
+    }
+    // Workgroup-uniform, after a __syncthreads: relax the workgroup list.
+    auto drainGroup = [&]() {
+      const unsigned m = sh.wgCount < WG_LIST ? sh.wgCount : WG_LIST;
+      for (unsigned j = 0; j < m; ++j) {
+        const int u = sh.wgList[j];
+        const u64 word = packed[u];
+        const int b = outRow[u];
+        const int e2 = word != PACKED_INF ? outRow[u + 1] : b;
+        const u64 du = dist_of(word, bits);
+        for (int x = b + static_cast<int>(threadIdx.x); x < e2; x += BLOCK) {
+          relax(u, du, x);
+        }
+        __syncthreads();
+        if (sh.near.count[0] > QCAP / 2) queue_flush(sh.near, outNear, nextCnt, listCap);
+        if (sh.far.count[0] > QCAP / 2) queue_flush(sh.far, farBuf, farCnt, listCap);
+      }
+      if (threadIdx.x == 0) {
+        sh.wgCount = 0;
+      }
+      __syncthreads();
+    };
+    const int hubEdges = __builtin_amdgcn_readfirstlane(sh.hubEdges);
+    const int groupEdges = hubEdges < WAVE_EDGES ? hubEdges : WAVE_EDGES;
+    int *const hubOut = uniform_global(D.s.hubs + ((it + 1) & 1) * HUB_LIST);
+    unsigned *const hubCnt = &c.hubCount[(it + 1) % 3];
+    // One relax site serves the lane groups and the wave list: each lane
+    // walks (vertex, first edge, stride, end) -- its group's light vertex,
+    // then the list's vertices in turn.
     int *const heavyList = sh.heavy[threadIdx.x / 64];
     unsigned heavyCount = 0;  // wave-uniform
     const unsigned long long below = (1ull << lane) - 1ull;
@@ -752,12 +893,28 @@ __device__ void push_team(Desc &D, ObjCtl &c, Shared &sh, int k,
           const int begin = outRow[u];
           const int end = outRow[u + 1];
           if (word != PACKED_INF) {
-            heavy = end - begin > HEAVY_GROUP;
+            const int deg = end - begin;
+            heavy = deg > HEAVY_GROUP;
             cu = u;
             if (!heavy) {
               cdu = dist_of(word, bits);
               cx = begin + static_cast<int>(sub);
               cend = end;
+            } else if (sub == 0) {
+              if (deg > hubEdges) {
+                const unsigned slot = atomicAdd(hubCnt, 1u);
+                if (slot < static_cast<unsigned>(HUB_LIST)) {
+                  hubOut[slot] = u;
+                  heavy = false;
+                }
+              }
+              if (heavy && tm.whole && deg > groupEdges) {
+                const unsigned slot = atomicAdd(&sh.wgCount, 1u);
+                if (slot < WG_LIST) {
+                  sh.wgList[slot] = u;
+                  heavy = false;
+                }
+              }
             }
           }
         }
@@ -793,9 +950,14 @@ __device__ void push_team(Desc &D, ObjCtl &c, Shared &sh, int k,
       }
       if (tm.whole) {
         __syncthreads();
+        if (sh.wgCount > WG_LIST / 2) drainGroup();
         if (sh.near.count[0] > QCAP / 2) queue_flush(sh.near, outNear, nextCnt, listCap);
         if (sh.far.count[0] > QCAP / 2) queue_flush(sh.far, farBuf, farCnt, listCap);
       }
+    }
+    if (tm.whole) {
+      __syncthreads();
+      if (sh.wgCount > 0) drainGroup();
     }
   }
   // Local hops: relax, within this iteration, the near vertices this
@@ -857,6 +1019,9 @@ __device__ int near_far(Desc *objsArg, int nobj, ObjCtl *ctlsArg,
   Desc *const objs = uniform(objsArg);
   ObjCtl *const ctls = uniform_global(ctlsArg);
   (void)profBase;
+  // Words the barrier publishes per objective: near count, far count, far
+  // minimum (two words) and, in the hub variant, the listed hubs.
+  constexpr int MW = HUBS ? 5 : 4;
   if (threadIdx.x < static_cast<unsigned>(nobj)) {
     const int k = threadIdx.x;
     sh.thr[k] = initThr;
@@ -880,19 +1045,22 @@ __device__ int near_far(Desc *objsArg, int nobj, ObjCtl *ctlsArg,
       const int k = threadIdx.x;
       ObjCtl &c = ctls[k];
       const int e = sh.epoch[k];
-      unsigned nc, fc;
+      unsigned nc, fc, hc = 0;
       u64 fm;
       if (it == 0) {
         nc = load_relaxed(&c.nearCount[0]);
         fc = load_relaxed(&c.farCount[e % 3]);
         fm = load_relaxed(&c.farMin[e % 3]);
+        if (HUBS) hc = load_relaxed(&c.hubCount[0]);
       } else {
-        nc = sh.snap[4 * k];
-        fc = sh.snap[4 * k + 1];
-        fm = static_cast<u64>(sh.snap[4 * k + 2]) |
-             (static_cast<u64>(sh.snap[4 * k + 3]) << 32);
+        nc = sh.snap[MW * k];
+        fc = sh.snap[MW * k + 1];
+        fm = static_cast<u64>(sh.snap[MW * k + 2]) |
+             (static_cast<u64>(sh.snap[MW * k + 3]) << 32);
+        if (HUBS) hc = sh.snap[MW * k + 4];
       }
-      if (nc > 0) {
+      sh.hubs[k] = hc;
+      if (nc > 0 || hc > 0) {
         sh.mode[k] = MODE_PUSH;
         sh.cnt[k] = nc;
         sh.any = 1;
@@ -923,6 +1091,7 @@ __device__ int near_far(Desc *objsArg, int nobj, ObjCtl *ctlsArg,
       const int k = threadIdx.x;
       ObjCtl &c = ctls[k];
       store_relaxed(&c.nearCount[old], 0u);
+      if (HUBS) store_relaxed(&c.hubCount[old], 0u);
       if (sh.mode[k] == MODE_SPLIT) {
         const int e2 = (sh.epoch[k] + 2) % 3;
         store_relaxed(&c.farCount[e2], 0u);
@@ -983,13 +1152,14 @@ __device__ int near_far(Desc *objsArg, int nobj, ObjCtl *ctlsArg,
       ObjCtl &c = ctls[k];
       const int eNext = sh.epoch[k] + (sh.mode[k] == MODE_SPLIT ? 1 : 0);
       unsigned *fm = reinterpret_cast<unsigned *>(&c.farMin[eNext % 3]);
-      sh.src[4 * k] = &c.nearCount[(it + 1) % 3];
-      sh.src[4 * k + 1] = &c.farCount[eNext % 3];
-      sh.src[4 * k + 2] = fm;
-      sh.src[4 * k + 3] = fm + 1;
+      sh.src[MW * k] = &c.nearCount[(it + 1) % 3];
+      sh.src[MW * k + 1] = &c.farCount[eNext % 3];
+      sh.src[MW * k + 2] = fm;
+      sh.src[MW * k + 3] = fm + 1;
+      if (HUBS) sh.src[MW * k + 4] = &c.hubCount[(it + 1) % 3];
     }
     SUBPROF_MARK(tWork);
-    if (!grid.sync(sh.src, 4 * nobj, sh.snap)) {
+    if (!grid.sync(sh.src, MW * nobj, sh.snap)) {
       return generation;
     }
     SUBPROF_MARK(tSync);
@@ -1866,6 +2036,10 @@ __device__ void run_instance(Args &args, const Threads t, Shared &sh) {
   if (threadIdx.x < WAVES) {
     sh.near.count[threadIdx.x] = 0;
     sh.far.count[threadIdx.x] = 0;
+  }
+  if (threadIdx.x == 0) {
+    sh.wgCount = 0;
+    sh.hubEdges = a.hubEdges;
   }
 #ifdef MOSP_SUBPROFILE
   if (threadIdx.x < 16) subprof_lds()[threadIdx.x] = 0;

@@ -308,6 +308,29 @@ def test_kernel_variants_agree(monkeypatch):
             assert np.array_equal(runs[0][key], runs[1][key]), f"{tag}: {key}"
 
 
+def test_hub_tiers(monkeypatch):
+    """Hub variant, push tiers: vertices of more out-edges than the hub
+    threshold (MOSP_HUB_EDGES, lowered here so that a small graph has many)
+    are relaxed in the next iteration by every workgroup of their objective
+    over one range of all listed hubs' edges; a full hub list (512 entries)
+    hands them to the workgroup's list, a full one of those to the wave's.
+    With 2 workgroups for 3 objectives, teams of waves share each hub's
+    edges instead.  Every threshold and grid gives the oracle's results."""
+    g = rmat(12, 16, 3, seed=45)
+    assert int(np.diff(g.row).max()) > 512
+    monkeypatch.setenv("MOSP_HUBS", "1")
+    for kind in ("uniform-unsafe", "disconnecting"):
+        b = gen.make_batch(kind, g, 700, 46)
+        for edges, grid in (("64", None), ("300", None), ("1000", None), ("64", "2"),
+                            ("200", "3"), ("1000000", None)):
+            monkeypatch.setenv("MOSP_HUB_EDGES", edges)
+            if grid is None:
+                monkeypatch.delenv("MOSP_GRID", raising=False)
+            else:
+                monkeypatch.setenv("MOSP_GRID", grid)
+            check_pipeline(g, b, pref=[1, 2, 3], tag=f"hub tiers {kind} edges={edges} grid={grid}")
+
+
 def test_full_lists_fall_back_to_sweeps(monkeypatch):
     """Near / far lists capped far below n (MOSP_LIST_CAP): appends past the
     cap are lost, and the iteration after a full list relaxes every vertex
