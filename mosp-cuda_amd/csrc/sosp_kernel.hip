@@ -650,6 +650,7 @@ __device__ void push_team(Desc &D, ObjCtl &c, Shared &sh, int k,
   // live in SGPRs (k, hence D, comes from LDS: the compiler cannot tell).
   // All are device memory: global, not flat, accesses.
   u64 *const packed = uniform_global(D.s.packed);
+  u64 *const relaxedArr = uniform_global(D.s.relaxed);
   int *const stampArr = uniform_global(D.s.stamp);
   int *const inFar = uniform_global(D.s.inFar);
   const int *const outRow = uniform_global(D.outRow);
@@ -665,7 +666,13 @@ __device__ void push_team(Desc &D, ObjCtl &c, Shared &sh, int k,
   const unsigned sub = tm.tid & (lanes - 1);
   const unsigned perPass = tm.size >> laneBits;
   const int lane = static_cast<int>(__lane_id());
-  auto relax = [&](int u, u64 du, int x) {
+  // Relax edge x of u, whose word is `word`.  The edge back to u's parent
+  // (the vertex whose relaxation gave u this word) is skipped: it can never
+  // improve the parent (weights are at least 1).
+  const u64 parentMask = bits != 0 ? (1ull << bits) - 1ull : 0ull;
+  auto relax = [&](int u, u64 word, int x) {
+    const u64 du = dist_of(word, bits);
+    const int pu = bits != 0 ? static_cast<int>(word & parentMask) : -1;
     // One record load when the graph holds {column, weight} records.
     int w, weight;
     if (outRec != nullptr) {
@@ -676,7 +683,7 @@ __device__ void push_team(Desc &D, ObjCtl &c, Shared &sh, int k,
       w = outCol[x];
       weight = outW[x];
     }
-    if (w == source) {
+    if (w == source || w == pu) {
       return;  // the source is never relaxed
     }
     const u64 nd = du + static_cast<u64>(weight);
@@ -750,10 +757,12 @@ __device__ void push_team(Desc &D, ObjCtl &c, Shared &sh, int k,
         const u64 word = packed[u];
         const int begin = outRow[u];
         const int end = outRow[u + 1];
-        if (word != PACKED_INF) {
-          const u64 du = dist_of(word, bits);
+        // A vertex that a local-hop round relaxed with its present word
+        // needs nothing more.
+        const u64 done = LOCAL != 0 ? relaxedArr[u] : PACKED_INF;
+        if (word != PACKED_INF && word != done) {
           for (int x = begin + static_cast<int>(sub); x < end; x += static_cast<int>(lanes)) {
-            relax(u, du, x);
+            relax(u, word, x);
           }
         }
       }
@@ -825,8 +834,7 @@ __device__ void push_team(Desc &D, ObjCtl &c, Shared &sh, int k,
             }
             const u64 start = j > 0 ? sh.hub.end[j - 1] : 0;
             const int u = sh.hub.u[j];
-            const u64 word = packed[u];
-            relax(u, dist_of(word, bits), sh.hub.begin[j] + static_cast<int>(g - start));
+            relax(u, packed[u], sh.hub.begin[j] + static_cast<int>(g - start));
           }
           __syncthreads();
           if (sh.near.count[0] > QCAP / 2) queue_flush(sh.near, outNear, nextCnt, listCap);
@@ -841,9 +849,8 @@ __device__ void push_team(Desc &D, ObjCtl &c, Shared &sh, int k,
           const int len = word != PACKED_INF ? outRow[u + 1] - b : 0;
           const int x0 = b + static_cast<int>(static_cast<u64>(len) * blockRank / blocks);
           const int x1 = b + static_cast<int>(static_cast<u64>(len) * (blockRank + 1) / blocks);
-          const u64 du = dist_of(word, bits);
           for (int x = x0 + static_cast<int>(tm.tid); x < x1; x += static_cast<int>(tm.size)) {
-            relax(u, du, x);
+            relax(u, word, x);
           }
         }
       }
@@ -856,9 +863,8 @@ __device__ void push_team(Desc &D, ObjCtl &c, Shared &sh, int k,
         const u64 word = packed[u];
         const int b = outRow[u];
         const int e2 = word != PACKED_INF ? outRow[u + 1] : b;
-        const u64 du = dist_of(word, bits);
         for (int x = b + static_cast<int>(threadIdx.x); x < e2; x += BLOCK) {
-          relax(u, du, x);
+          relax(u, word, x);
         }
         __syncthreads();
         if (sh.near.count[0] > QCAP / 2) queue_flush(sh.near, outNear, nextCnt, listCap);
@@ -882,7 +888,7 @@ __device__ void push_team(Desc &D, ObjCtl &c, Shared &sh, int k,
     for (unsigned base = lo;; base += perPass) {
       const bool more = base < hi;  // team-uniform
       int cu = -1, cx = 0, cend = 0, cstep = static_cast<int>(lanes);
-      u64 cdu = 0;
+      u64 cword = 0;
       if (more) {
         const unsigned i = base + (tm.tid >> laneBits);
         bool heavy = false;
@@ -897,7 +903,7 @@ __device__ void push_team(Desc &D, ObjCtl &c, Shared &sh, int k,
             heavy = deg > HEAVY_GROUP;
             cu = u;
             if (!heavy) {
-              cdu = dist_of(word, bits);
+              cword = word;
               cx = begin + static_cast<int>(sub);
               cend = end;
             } else if (sub == 0) {
@@ -928,7 +934,7 @@ __device__ void push_team(Desc &D, ObjCtl &c, Shared &sh, int k,
       const bool drainNow = !more || heavyCount > HEAVY_LIST - 64;
       for (unsigned j = 0;;) {
         for (; cx < cend; cx += cstep) {
-          relax(cu, cdu, cx);
+          relax(cu, cword, cx);
         }
         if (!drainNow || j >= heavyCount) {
           break;
@@ -938,7 +944,7 @@ __device__ void push_team(Desc &D, ObjCtl &c, Shared &sh, int k,
         const u64 word = packed[cu];
         cx = outRow[cu] + lane;
         cend = word != PACKED_INF ? outRow[cu + 1] : 0;
-        cdu = dist_of(word, bits);
+        cword = word;
         cstep = 64;
       }
       if (drainNow) {
@@ -991,9 +997,11 @@ __device__ void push_team(Desc &D, ObjCtl &c, Shared &sh, int k,
           const int begin = outRow[u];
           const int end = outRow[u + 1];
           if (word != PACKED_INF && (!HUBS || end - begin <= HEAVY_GROUP)) {
-            const u64 du = dist_of(word, bits);
+            if (sub == 0) {
+              relaxedArr[u] = word;
+            }
             for (int x = begin + static_cast<int>(sub); x < end; x += static_cast<int>(lanes)) {
-              relax(u, du, x);
+              relax(u, word, x);
             }
           }
         }
@@ -1393,6 +1401,7 @@ __device__ void scratch_init(Desc &D, ObjCtl &c, int n, int source,
   for (int v = t.id; v < n; v += t.count) {
     D.s.packed[v] = v == source ? pack(0, -1, D.parentBits, D.noParent)
                                 : PACKED_INF;
+    D.s.relaxed[v] = PACKED_INF;
   }
   if (t.id == 0) {
     D.s.near[0][0] = source;
@@ -1490,6 +1499,9 @@ __device__ int update_step1(Args &a, GridBarrier &grid, Threads t,
         word = pack(static_cast<u64>(d), p, D.parentBits, D.noParent);
       }
       D.s.packed[v] = word;
+      if constexpr (!HUBS) {
+        D.s.relaxed[v] = PACKED_INF;  // no vertex relaxed by a local-hop round yet
+      }
       if (D.nChanged > 0) {
         D.s.ancestor[v] = p + 1;  // jump word: ancestor + 1, no flag
       }
