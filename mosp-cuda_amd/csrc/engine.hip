@@ -308,7 +308,7 @@ int ws_reserve_combined(mosp_ws *ws, int n, int K) {
 }
 
 
-int choose_packing(int n, long long maxWeight, PackChoice &pc) {
+int choose_packing(int n, long long maxWeight, PackChoice &pc, bool optimistic) {
   const u64 outputMax = static_cast<u64>(DISTANCE_INF / 2 - 1);
   const u64 weight = static_cast<u64>(std::max(maxWeight, 1LL));
   const u64 hops = static_cast<u64>(std::max(n - 1, 1));
@@ -321,10 +321,24 @@ int choose_packing(int n, long long maxWeight, PackChoice &pc) {
   while ((1ULL << bits) - 1 < static_cast<u64>(n)) {
     ++bits;
   }
-  const u64 maxDistance = (PACKED_INF >> bits) - 1;
-  if (pc.bound <= maxDistance) {
+  u64 maxDistance = (PACKED_INF >> bits) - 1;
+  // MOSP_PACK_LIMIT=d: packed words hold distances up to d only, so that
+  // small graphs take the optimistic packing and its redo (tests).
+  if (const char *env = std::getenv("MOSP_PACK_LIMIT")) {
+    const long long limit = std::atoll(env);
+    if (limit > 0 && static_cast<u64>(limit) < maxDistance) {
+      maxDistance = static_cast<u64>(limit);
+    }
+  }
+  pc.optimistic = false;
+  pc.maxDistance = pc.bound;
+  if (pc.bound <= maxDistance || optimistic) {
     pc.bits = bits;
     pc.noParent = (1ULL << bits) - 1;
+    if (pc.bound > maxDistance) {
+      pc.optimistic = true;
+      pc.maxDistance = maxDistance;
+    }
   } else {
     pc.bits = 0;
     pc.noParent = 0;
@@ -353,6 +367,17 @@ void fill_stats(const Ctl &c, int k, const ObjDesc &d, float ms,
   stats->pushes = static_cast<int64_t>(o.pushes);
   stats->packed_parents = d.parentBits > 0 ? 1 : 0;
   stats->kernel_ms = ms;
+}
+
+// Whether an optimistic launch met a distance past its packing's limit for
+// one of objectives [0, count) (it then wrote no result).
+bool launch_dropped(const Ctl &c, int count) {
+  for (int k = 0; k < count; ++k) {
+    if (c.obj[k].dropped != 0) {
+      return true;
+    }
+  }
+  return false;
 }
 
 // Upload descriptors, reset the control block, run the cooperative launch,
@@ -483,7 +508,8 @@ ObjDesc make_desc(const mosp_csr &out, const mosp_csr *in,
   }
   d.parentBits = pc.bits;
   d.noParent = pc.noParent;
-  d.maxDistance = pc.bound;
+  d.maxDistance = pc.maxDistance;
+  d.optimistic = pc.optimistic ? 1 : 0;
   d.delta = delta;
   d.s = st;
   return d;
@@ -668,38 +694,50 @@ extern "C" int mosp_sosp_update_multi(
   if (!ws_reserve(ws, n, K)) {
     return 0;
   }
-  int anyFallback = 0;
-  for (int k = 0; k < K; ++k) {
-    if (delta[k] <= 0) {
-      return fail("Error: the near-far bucket width must be positive.");
-    }
-    PackChoice pc{};
-    if (!choose_packing(n, maxWeight[k], pc)) {
-      return 0;
-    }
-    ObjDesc d = make_desc(out[k], &in[k], &changes[k], pc,
-                          static_cast<u64>(delta[k]), ws->state[k]);
-    d.distIn = distIn[k];
-    d.parentIn = parentIn[k];
-    d.distOut = distOut[k];
-    d.parentOut = parentOut[k];
-    ws->hDesc[k] = d;
-    anyFallback |= pc.bits == 0 ? 1 : 0;
-  }
-  LaunchArgs args{};
-  args.objs = ws->dDesc;
-  args.nobj = K;
-  args.n = n;
-  args.source = source;
-  args.phases = PH_UPDATE;
-  args.maxRounds = jump_rounds(n);
-  args.anyFallback = anyFallback;
-  if (!caller_hubs(ws, out, in, K, as_stream(stream), args.hubs)) {
+  int hubs = 0;
+  if (!caller_hubs(ws, out, in, K, as_stream(stream), hubs)) {
     return 0;
   }
   float ms = 0;
-  if (!run_launch(ws, args, K, as_stream(stream), &ms)) {
-    return 0;
+  // Packed words first, also where only their own limit bounds the
+  // distances; a launch that meets a larger one wrote nothing and is redone
+  // with the conservative packing.
+  for (int attempt = 0; attempt < 2; ++attempt) {
+    int anyFallback = 0, anyOptimistic = 0;
+    for (int k = 0; k < K; ++k) {
+      if (delta[k] <= 0) {
+        return fail("Error: the near-far bucket width must be positive.");
+      }
+      PackChoice pc{};
+      if (!choose_packing(n, maxWeight[k], pc, attempt == 0)) {
+        return 0;
+      }
+      ObjDesc d = make_desc(out[k], &in[k], &changes[k], pc,
+                            static_cast<u64>(delta[k]), ws->state[k]);
+      d.distIn = distIn[k];
+      d.parentIn = parentIn[k];
+      d.distOut = distOut[k];
+      d.parentOut = parentOut[k];
+      ws->hDesc[k] = d;
+      anyFallback |= pc.bits == 0 ? 1 : 0;
+      anyOptimistic |= pc.optimistic ? 1 : 0;
+    }
+    LaunchArgs args{};
+    args.objs = ws->dDesc;
+    args.nobj = K;
+    args.n = n;
+    args.source = source;
+    args.phases = PH_UPDATE;
+    args.maxRounds = jump_rounds(n);
+    args.anyFallback = anyFallback;
+    args.anyOptimistic = anyOptimistic;
+    args.hubs = hubs;
+    if (!run_launch(ws, args, K, as_stream(stream), &ms)) {
+      return 0;
+    }
+    if (!anyOptimistic || !launch_dropped(*ws->hCtl, K)) {
+      break;
+    }
   }
   for (int k = 0; k < K; ++k) {
     fill_stats(*ws->hCtl, k, ws->hDesc[k], ms, stats ? &stats[k] : nullptr);
@@ -740,28 +778,37 @@ extern "C" int mosp_sosp_from_scratch(const mosp_csr *out, int32_t source,
   if (!ws_reserve(ws, n, 1)) {
     return 0;
   }
-  PackChoice pc{};
-  if (!choose_packing(n, maxWeight, pc)) {
-    return 0;
-  }
-  ObjDesc d = make_desc(*out, nullptr, nullptr, pc, static_cast<u64>(delta),
-                        ws->state[0]);
-  d.distOut = d_dist;
-  d.parentOut = d_parent;
-  ws->hDesc[0] = d;
-  LaunchArgs args{};
-  args.objs = ws->dDesc;
-  args.nobj = 1;
-  args.n = n;
-  args.source = source;
-  args.phases = PH_SCRATCH;
-  args.maxRounds = jump_rounds(n);
-  if (!caller_hubs(ws, out, nullptr, 1, as_stream(stream), args.hubs)) {
+  int hubs = 0;
+  if (!caller_hubs(ws, out, nullptr, 1, as_stream(stream), hubs)) {
     return 0;
   }
   float ms = 0;
-  if (!run_launch(ws, args, 1, as_stream(stream), &ms)) {
-    return 0;
+  ObjDesc d{};
+  // Optimistic packed words first, as in mosp_sosp_update_multi.
+  for (int attempt = 0; attempt < 2; ++attempt) {
+    PackChoice pc{};
+    if (!choose_packing(n, maxWeight, pc, attempt == 0)) {
+      return 0;
+    }
+    d = make_desc(*out, nullptr, nullptr, pc, static_cast<u64>(delta), ws->state[0]);
+    d.distOut = d_dist;
+    d.parentOut = d_parent;
+    ws->hDesc[0] = d;
+    LaunchArgs args{};
+    args.objs = ws->dDesc;
+    args.nobj = 1;
+    args.n = n;
+    args.source = source;
+    args.phases = PH_SCRATCH;
+    args.maxRounds = jump_rounds(n);
+    args.anyOptimistic = pc.optimistic ? 1 : 0;
+    args.hubs = hubs;
+    if (!run_launch(ws, args, 1, as_stream(stream), &ms)) {
+      return 0;
+    }
+    if (!pc.optimistic || !launch_dropped(*ws->hCtl, 1)) {
+      break;
+    }
   }
   fill_stats(*ws->hCtl, 0, d, ms, stats);
   return 1;
@@ -1043,27 +1090,39 @@ extern "C" int mosp_update_fused(const mosp_device_graph *g, int32_t K,
   }
   const int64_t *distIn = d_dist_in != nullptr ? d_dist_in : d_dist;
   const int32_t *parIn = d_parents_in != nullptr ? d_parents_in : d_parents;
-  int anyFallback = 0;
   for (int k = 0; k < K; ++k) {
     if (delta[k] <= 0) {
       return fail("Error: the near-far bucket width must be positive.");
     }
-    PackChoice pc{};
-    if (!choose_packing(n, maxWeight[k], pc)) {
-      return 0;
+  }
+  // The K objectives' descriptors; `optimistic`: packed words also where
+  // only their own limit bounds the distances.
+  auto describe = [&](bool optimistic, int &anyFallback, int &anyOptimistic) {
+    anyFallback = anyOptimistic = 0;
+    for (int k = 0; k < K; ++k) {
+      PackChoice pc{};
+      if (!choose_packing(n, maxWeight[k], pc, optimistic)) {
+        return 0;
+      }
+      mosp_csr out{}, in{};
+      mosp_device_graph_view(g, k, &out, &in);
+      ObjDesc d = make_desc(out, &in, &changes[k], pc, static_cast<u64>(delta[k]),
+                            ws->state[k]);
+      graph_edges(g, k, &d.outEdge, &d.inEdge);
+      const size_t off = static_cast<size_t>(k) * n;
+      d.distIn = distIn + off;
+      d.parentIn = parIn + off;
+      d.distOut = d_dist + off;
+      d.parentOut = d_parents + off;
+      ws->hDesc[k] = d;
+      anyFallback |= pc.bits == 0 ? 1 : 0;
+      anyOptimistic |= pc.optimistic ? 1 : 0;
     }
-    mosp_csr out{}, in{};
-    mosp_device_graph_view(g, k, &out, &in);
-    ObjDesc d = make_desc(out, &in, &changes[k], pc, static_cast<u64>(delta[k]),
-                          ws->state[k]);
-    graph_edges(g, k, &d.outEdge, &d.inEdge);
-    const size_t off = static_cast<size_t>(k) * n;
-    d.distIn = distIn + off;
-    d.parentIn = parIn + off;
-    d.distOut = d_dist + off;
-    d.parentOut = d_parents + off;
-    ws->hDesc[k] = d;
-    anyFallback |= pc.bits == 0 ? 1 : 0;
+    return 1;
+  };
+  int anyFallback = 0, anyOptimistic = 0;
+  if (!describe(true, anyFallback, anyOptimistic)) {
+    return 0;
   }
   LaunchArgs args{};
   PackChoice cpc{};
@@ -1079,11 +1138,24 @@ extern "C" int mosp_update_fused(const mosp_device_graph *g, int32_t K,
   args.phases = PH_UPDATE | PH_COMBINED;
   args.maxRounds = jump_rounds(n);
   args.anyFallback = anyFallback;
+  args.anyOptimistic = anyOptimistic;
   // A combined-graph vertex has at most its out-degree in the graph.
   args.hubs = g->maxDegree > HUB_DEGREE ? 1 : 0;
   float ms = 0;
   if (!run_launch(ws, args, K + 1, as_stream(stream), &ms)) {
     return 0;
+  }
+  if (anyOptimistic && launch_dropped(*ws->hCtl, K)) {
+    // A distance past the optimistic packing's limit: nothing was written;
+    // again with the conservative packing.
+    if (!describe(false, anyFallback, anyOptimistic)) {
+      return 0;
+    }
+    args.anyFallback = anyFallback;
+    args.anyOptimistic = 0;
+    if (!run_launch(ws, args, K + 1, as_stream(stream), &ms)) {
+      return 0;
+    }
   }
   for (int k = 0; k < K; ++k) {
     fill_stats(*ws->hCtl, k, ws->hDesc[k], ms, stats ? &stats[k] : nullptr);
@@ -1204,7 +1276,7 @@ extern "C" int mosp_sosp_update_noalloc(const mosp_csr *out, const mosp_csr *in,
     return fail("Error: the near-far bucket width must be positive.");
   }
   PackChoice pc{};
-  if (!choose_packing(n, maxWeight, pc)) {
+  if (!choose_packing(n, maxWeight, pc, true)) {
     return 0;
   }
   NoAllocDevice *dev = nullptr;
@@ -1235,49 +1307,61 @@ extern "C" int mosp_sosp_update_noalloc(const mosp_csr *out, const mosp_csr *in,
     hubs = forced;
   }
 
-  ObjDesc d = make_desc(*out, in, changes, pc, static_cast<u64>(delta), st);
-  d.distIn = d_dist;
-  d.parentIn = d_parent;
-  d.distOut = d_dist;
-  d.parentOut = d_parent;
-  LaunchArgs args{};
-  args.objs = dDesc;
-  args.nobj = 1;
-  args.n = n;
-  args.source = source;
-  args.phases = PH_UPDATE;
-  args.maxRounds = jump_rounds(n);
-  args.anyFallback = pc.bits == 0 ? 1 : 0;
-  args.laneBits = 2;
-  args.hopsUpdate = kHopsUpdate;
-  args.hopsSearch = kHopsSearch;
-  local_hops_override(args.hopsUpdate, args.hopsSearch);
-  args.listCap = list_cap_override();
-  args.hubEdges = hub_edges_override();
-  args.hubs = hubs;
-  args.genDev = genDev;
-  args.ctl = dCtl;
-  args.abortWord = &dCtl->gaveUp;
-  args.barrierTicks = barrier_ticks(barrier_timeout_ms_default(), dev->clockKhz);
-  args.cDeltaScale = 1;
-
-  HIP_TRY(hipMemcpyAsync(dDesc, &d, sizeof(ObjDesc), hipMemcpyHostToDevice, stream));
-  hipLaunchKernelGGL(noalloc_clear, dim3((n + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, stream,
-                     st.stamp, st.inFar, st.flag, n, genDev);
-  HIP_TRY(hipGetLastError());
-  HIP_TRY(launch_ctl_reset(dCtl, stream));
-  void *params[] = {&args};
-  HIP_TRY(launch_persistent(sosp_kernel_symbol(hubs != 0), dim3(dev->gridBlocks), dim3(BLOCK),
-                            params, stream));
   static thread_local Ctl hCtl;  // host copy of the control block (no allocation per call)
-  HIP_TRY(hipMemcpyAsync(&hCtl, dCtl, CTL_READBACK, hipMemcpyDeviceToHost, stream));
-  HIP_TRY(hipStreamSynchronize(stream));
-  if (hCtl.gaveUp != 0) {
-    return fail("Error: a grid barrier of the persistent launch gave up waiting (its workgroups "
-                "were not all resident); the tree was not updated.");
-  }
-  if (!input_tree_ok(hCtl.obj[0].overflow, "")) {
-    return 0;
+  ObjDesc d{};
+  // Optimistic packed words first, as in mosp_sosp_update_multi; again with
+  // the conservative packing when the launch met a larger distance.
+  for (int attempt = 0; attempt < 2; ++attempt) {
+    if (attempt > 0 && !choose_packing(n, maxWeight, pc, false)) {
+      return 0;
+    }
+    d = make_desc(*out, in, changes, pc, static_cast<u64>(delta), st);
+    d.distIn = d_dist;
+    d.parentIn = d_parent;
+    d.distOut = d_dist;
+    d.parentOut = d_parent;
+    LaunchArgs args{};
+    args.objs = dDesc;
+    args.nobj = 1;
+    args.n = n;
+    args.source = source;
+    args.phases = PH_UPDATE;
+    args.maxRounds = jump_rounds(n);
+    args.anyFallback = pc.bits == 0 ? 1 : 0;
+    args.anyOptimistic = pc.optimistic ? 1 : 0;
+    args.laneBits = 2;
+    args.hopsUpdate = kHopsUpdate;
+    args.hopsSearch = kHopsSearch;
+    local_hops_override(args.hopsUpdate, args.hopsSearch);
+    args.listCap = list_cap_override();
+    args.hubEdges = hub_edges_override();
+    args.hubs = hubs;
+    args.genDev = genDev;
+    args.ctl = dCtl;
+    args.abortWord = &dCtl->gaveUp;
+    args.barrierTicks = barrier_ticks(barrier_timeout_ms_default(), dev->clockKhz);
+    args.cDeltaScale = 1;
+
+    HIP_TRY(hipMemcpyAsync(dDesc, &d, sizeof(ObjDesc), hipMemcpyHostToDevice, stream));
+    hipLaunchKernelGGL(noalloc_clear, dim3((n + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, stream,
+                       st.stamp, st.inFar, st.flag, n, genDev);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(launch_ctl_reset(dCtl, stream));
+    void *params[] = {&args};
+    HIP_TRY(launch_persistent(sosp_kernel_symbol(hubs != 0), dim3(dev->gridBlocks), dim3(BLOCK),
+                              params, stream));
+    HIP_TRY(hipMemcpyAsync(&hCtl, dCtl, CTL_READBACK, hipMemcpyDeviceToHost, stream));
+    HIP_TRY(hipStreamSynchronize(stream));
+    if (hCtl.gaveUp != 0) {
+      return fail("Error: a grid barrier of the persistent launch gave up waiting (its workgroups "
+                  "were not all resident); the tree was not updated.");
+    }
+    if (!input_tree_ok(hCtl.obj[0].overflow, "")) {
+      return 0;
+    }
+    if (!pc.optimistic || hCtl.obj[0].dropped == 0) {
+      break;
+    }
   }
   const float ms = hCtl.clkEnd > hCtl.clkStart
                        ? static_cast<float>((hCtl.clkEnd - hCtl.clkStart) / dev->clockKhz)

@@ -87,9 +87,11 @@ struct ObjDesc {
   const int *heads;                   // heads of inserted edges
   int nHeads;
   int parentBits;                     // 0: distance-only words
-  int pad0;
+  int optimistic;                     // maxDistance is below (n-1)*maxWeight: a dropped
+                                      // candidate sets ObjCtl::dropped
   u64 noParent;                       // all-ones parent field
-  u64 maxDistance;                    // (n-1)*maxWeight: larger candidates dropped
+  u64 maxDistance;                    // larger candidates are dropped: (n-1)*maxWeight,
+                                      // or the packed word's limit when optimistic
   u64 delta;                          // near-far bucket width
   const int64_t *distIn;              // old tree (update) -- may alias distOut
   const int *parentIn;
@@ -111,6 +113,8 @@ struct ObjCtl {
   unsigned jumpCount[3];   // pointer jumping: still-jumping list, slot = round % 3
   unsigned pullHeavy;      // pull candidates left for whole workgroups (hub variant)
   unsigned hubCount[3];    // grid-relaxed hubs listed for an iteration, slot = iteration % 3
+  unsigned dropped;        // optimistic packing: a candidate or input distance was past
+                           // maxDistance (the launch stops before writing results)
   u64 pushes;
 };
 
@@ -159,6 +163,7 @@ struct LaunchArgs {
   int phases;
   int maxRounds;           // pointer-jumping rounds: ceil(log2 n) + 1
   int anyFallback;         // some objective of objs uses distance-only words
+  int anyOptimistic;       // some objective of objs uses optimistic packing
   int laneBits;            // 2^laneBits lanes share a frontier vertex
   int hopsUpdate;          // local relaxation rounds per near-far iteration: K updates
   int hopsSearch;          // ... from-scratch and combined searches (kernels

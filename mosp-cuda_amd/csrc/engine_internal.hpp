@@ -107,10 +107,16 @@ size_t align256(size_t bytes);
 struct PackChoice {
   int bits;
   u64 noParent;
-  u64 bound;
+  u64 bound;        // (n-1) * maxWeight: no shortest distance is larger
+  u64 maxDistance;  // candidates beyond it are dropped
+  bool optimistic;  // maxDistance < bound: a launch that drops one is redone
 };
 
-int choose_packing(int n, long long maxWeight, PackChoice &pc);
+// Packed (distance << bits | parent) words when every distance up to the
+// bound fits beside the parent; otherwise, with `optimistic`, packed words
+// that hold distances up to their own limit (a launch that meets a larger
+// one stops before writing and is redone without), else distance-only words.
+int choose_packing(int n, long long maxWeight, PackChoice &pc, bool optimistic = false);
 // Pointer-jumping rounds for n vertices: ceil(log2 n) + 1.
 int jump_rounds(int n);
 void fill_stats(const Ctl &c, int k, const ObjDesc &d, float ms,

@@ -646,6 +646,7 @@ __device__ void push_team(Desc &D, ObjCtl &c, Shared &sh, int k,
   const u64 noParent = D.noParent;
   const u64 thr = SPLIT ? sh.newThr[k] : sh.thr[k];
   const u64 maxDist = D.maxDistance;
+  const bool optimistic = D.optimistic != 0;
   // Locals, read once; a team is whole waves, so they are wave-uniform and
   // live in SGPRs (k, hence D, comes from LDS: the compiler cannot tell).
   // All are device memory: global, not flat, accesses.
@@ -688,7 +689,12 @@ __device__ void push_team(Desc &D, ObjCtl &c, Shared &sh, int k,
     }
     const u64 nd = du + static_cast<u64>(weight);
     if (nd > maxDist) {
-      return;  // never shortest; would not fit the packed word
+      // Never shortest, unless the packing is optimistic (then the launch
+      // is redone with a packing that holds it); would not fit the word.
+      if (optimistic) {
+        store_relaxed(&c.dropped, 1u);
+      }
+      return;
     }
     const u64 cand = pack(nd, u, bits, noParent);
     // The claim word (stamp for the near list, far mark for the far pile) is
@@ -1492,6 +1498,9 @@ __device__ int update_step1(Args &a, GridBarrier &grid, Threads t,
       u64 word;
       if (d >= DISTANCE_INF / 2) {
         word = PACKED_INF;
+      } else if (d >= 0 && static_cast<u64>(d) > D.maxDistance && D.optimistic) {
+        store_relaxed(&ctl->obj[k].dropped, 1u);  // redone with a wider packing
+        word = PACKED_INF;
       } else if (d < 0 || static_cast<u64>(d) > D.maxDistance) {
         overflow |= 1;
         word = PACKED_INF;
@@ -1637,8 +1646,15 @@ __device__ int update_step1(Args &a, GridBarrier &grid, Threads t,
         return PACKED_INF;
       }
       const u64 nd = dist_of(word, bits) + static_cast<u64>(ew.y);
-      // Larger than maxDistance: never shortest, may not fit the word.
-      return nd <= D.maxDistance ? pack(nd, u, bits, D.noParent) : PACKED_INF;
+      // Larger than maxDistance: never shortest (unless the packing is
+      // optimistic: the launch is redone), may not fit the word.
+      if (nd > D.maxDistance) {
+        if (D.optimistic) {
+          store_relaxed(&c.dropped, 1u);
+        }
+        return PACKED_INF;
+      }
+      return pack(nd, u, bits, D.noParent);
     };
     for (unsigned base = t.block * BLOCK; base < static_cast<unsigned>(n);
          base += t.count) {
@@ -1730,6 +1746,8 @@ __device__ int update_step1(Args &a, GridBarrier &grid, Threads t,
             if (nd <= D.maxDistance) {
               const u64 cand = pack(nd, u, bits, D.noParent);
               part = cand < part ? cand : part;
+            } else if (D.optimistic) {
+              store_relaxed(&c.dropped, 1u);
             }
           }
         }
@@ -2037,10 +2055,33 @@ __device__ void combined_build(Args &args, GridBarrier &grid,
 // kernel; 2 local hops, batched kernel.  The two values with hops compile
 // the same code as separate instantiations down the call tree, so neither
 // kernel's register allocation depends on the other's.
+// Optimistic packing: whether some objective of the first `count` met a
+// distance past its packed words' limit.  Read after the last barrier of the
+// search (every drop happened before it), so every workgroup agrees; then
+// the launch ends without writing results, clearing the root marks that
+// Step 1 set (the host redoes it with a wider packing).
+__device__ bool dropped_any(Args &a, int count, Threads t) {
+  if (!a.anyOptimistic) {
+    return false;
+  }
+  bool dropped = false;
+  for (int k = 0; k < count; ++k) {
+    dropped = dropped || load_relaxed(&a.ctl->obj[k].dropped) != 0;
+  }
+  if (dropped) {
+    for (int k = 0; k < count; ++k) {
+      clear_roots(desc(a.objs)[k], t);
+    }
+  }
+  return dropped;
+}
+
+// Every phase of one launch (instance); `generation` is the stamp
+// generation, advanced as the phases use it.  Returns early when a barrier
+// gives up or an optimistic packing meets a larger distance.
 template <bool HUBS, int LOCAL>
-__device__ void run_instance(Args &args, const Threads t, Shared &sh) {
-  Args &a = *uniform(&args);
-  GridBarrier grid{a.ctl, 0, t.block, t.blocks, a.abortWord, a.barrierTicks};
+__device__ int run_phases(Args &a, GridBarrier &grid, const Threads t, Shared &sh,
+                          int generation) {
   const int n = a.n;
   if (t.id == 0) {
     a.ctl->clkStart = wall_clock64();
@@ -2058,7 +2099,6 @@ __device__ void run_instance(Args &args, const Threads t, Shared &sh) {
 #endif
   // Stamp generations persist in device memory across launches; restart
   // them (and clear the stamps) well before the counter could wrap.
-  int generation = *a.genDev;
   if (generation > (1 << 30)) {
     for (int k = 0; k < a.nobj; ++k) {
       for (int v = t.id; v < n; v += t.count) {
@@ -2072,7 +2112,7 @@ __device__ void run_instance(Args &args, const Threads t, Shared &sh) {
     }
     generation = 0;
     if (!grid.sync()) {
-      return;
+      return generation;
     }
   }
   __syncthreads();
@@ -2083,13 +2123,13 @@ __device__ void run_instance(Args &args, const Threads t, Shared &sh) {
   if (a.phases & PH_UPDATE) {
     generation = update_step1<HUBS>(a, grid, t, generation, sh);
     if (grid.aborted) {
-      return;
+      return generation;
     }
     generation = near_far<HUBS, LOCAL>(desc(a.objs), a.nobj, a.ctl->obj, a.source, generation, 0,
                                 nullptr, a.laneBits, a.hopsUpdate, static_cast<unsigned>(n),
                                 listCap, grid, t, sh, 8);
-    if (grid.aborted) {
-      return;
+    if (grid.aborted || dropped_any(a, a.nobj, t)) {
+      return generation;
     }
     stamp(a.ctl, PHASE_NEARFAR, t.id);
     // One edge sweep for every distance-only objective when they are all
@@ -2109,7 +2149,7 @@ __device__ void run_instance(Args &args, const Threads t, Shared &sh) {
     }
     if (a.anyFallback) {
       if (!grid.sync()) {
-        return;
+        return generation;
       }
       if (stage != nullptr) {
         recover_staged(desc(a.objs), a.nobj, n, a.source, t, sh, stage);
@@ -2124,7 +2164,7 @@ __device__ void run_instance(Args &args, const Threads t, Shared &sh) {
       }
     }
     if ((a.phases & PH_COMBINED) && !grid.sync()) {  // the K new trees are complete
-      return;
+      return generation;
     }
     if (t.id == 0) {
       a.ctl->clkSospEnd = wall_clock64();
@@ -2136,18 +2176,18 @@ __device__ void run_instance(Args &args, const Threads t, Shared &sh) {
     Desc &D = desc(a.objs)[0];
     scratch_init(D, a.ctl->obj[0], n, a.source, t);
     if (!grid.sync()) {
-      return;
+      return generation;
     }
     generation = near_far<HUBS, LOCAL>(desc(a.objs), 1, a.ctl->obj, a.source, generation,
                                 D.delta, nullptr, a.laneBits, a.hopsSearch,
                                 static_cast<unsigned>(n), listCap, grid, t, sh, 8);
-    if (grid.aborted) {
-      return;
+    if (grid.aborted || dropped_any(a, 1, t)) {
+      return generation;
     }
     unpack(D, n, a.source, t);
     if (D.parentBits == 0) {
       if (!grid.sync()) {
-        return;
+        return generation;
       }
       recover_parents<HUBS>(D, n, a.source, t, sh);
     }
@@ -2158,14 +2198,14 @@ __device__ void run_instance(Args &args, const Threads t, Shared &sh) {
     ObjCtl &cc = a.ctl->obj[MOSP_MAX_OBJ];
     combined_build<HUBS>(a, grid, t, sh);
     if (grid.aborted) {
-      return;
+      return generation;
     }
     scratch_init(C, cc, n, a.source, t);
     if (threadIdx.x < 2) {
       sh.src[threadIdx.x] = reinterpret_cast<unsigned *>(&a.ctl->combDelta) + threadIdx.x;
     }
     if (!grid.sync(sh.src, 2, sh.snap)) {
-      return;
+      return generation;
     }
     stamp(a.ctl, PHASE_COMB_FILL, t.id);
     const u64 delta = static_cast<u64>(sh.snap[0]) | (static_cast<u64>(sh.snap[1]) << 32);
@@ -2173,25 +2213,38 @@ __device__ void run_instance(Args &args, const Threads t, Shared &sh) {
                                 a.laneBits, a.hopsSearch, static_cast<unsigned>(n), listCap,
                                 grid, t, sh, 12);
     if (grid.aborted) {
-      return;
+      return generation;
     }
     stamp(a.ctl, PHASE_COMB_SEARCH, t.id);
     unpack(C, n, a.source, t);
     if (C.parentBits == 0) {
       if (!grid.sync()) {
-        return;
+        return generation;
       }
       recover_parents<HUBS>(C, n, a.source, t, sh);
     }
   }
 
+  return generation;
+}
+
+template <bool HUBS, int LOCAL>
+__device__ void run_instance(Args &args, const Threads t, Shared &sh) {
+  Args &a = *uniform(&args);
+  GridBarrier grid{a.ctl, 0, t.block, t.blocks, a.abortWord, a.barrierTicks};
+  const int generation = run_phases<HUBS, LOCAL>(a, grid, t, sh, *a.genDev);
 #ifdef MOSP_SUBPROFILE
   __syncthreads();
   if (threadIdx.x < 16) atomicAdd(&a.ctl->prof[threadIdx.x], subprof_lds()[threadIdx.x]);
 #endif
   if (t.id == 0) {
-    a.ctl->generationOut = static_cast<unsigned>(generation);
-    *a.genDev = generation;
+    // Every exit records the generations it used, so that the next launch
+    // never meets one of its stamps: after a barrier gave up, other
+    // workgroups may have passed one more barrier than this one and used a
+    // few more generations.
+    const int used = grid.aborted ? generation + 16 : generation;
+    a.ctl->generationOut = static_cast<unsigned>(used);
+    *a.genDev = used;
     a.ctl->clkEnd = wall_clock64();
     a.ctl->clk[PHASE_COMB_UNPACK] = a.ctl->clkEnd;
     a.ctl->barriers = grid.epoch;

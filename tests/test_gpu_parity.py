@@ -227,15 +227,64 @@ def test_packing_boundary():
         assert np.array_equal(d, rd) and np.array_equal(p, rp)
 
 
-def test_large_weights_fallback_with_ties():
-    """(n-1) * maxW does not fit next to the parent ids: distance-only words
-    and lowest-id parent recovery (src/mospTest.cu:597-700)."""
+def test_large_weights_fallback_with_ties(monkeypatch):
+    """(n-1) * maxW does not fit next to the parent ids
+    (src/mospTest.cu:597-700).  The launch first packs words that hold
+    distances up to their own limit (here far above every distance: packed
+    parents); with MOSP_PACK_LIMIT=1 it meets larger ones, writes nothing
+    and is redone with distance-only words and the lowest-id parent
+    recovery.  Both equal the oracle, ties included."""
     for W, tag in ((2**31 - 1, "int-max"), (2_000_000_000, "ties")):
         wmin = W if tag == "ties" else 1
         g = gen.grid_graph(320, 320, 2, W, 0.1, 13, wmin=wmin)
         b = gen.uniform_batch(g, 2000, 50, 15, wmax=W, wmin=wmin)
-        got, _ = check_pipeline(g, b, tag=f"large-weights/{tag}")
-        assert all(s["packed_parents"] == 0 for s in got["stats"])
+        for limit, packed in ((None, 1), ("1", 0)):
+            if limit is None:
+                monkeypatch.delenv("MOSP_PACK_LIMIT", raising=False)
+            else:
+                monkeypatch.setenv("MOSP_PACK_LIMIT", limit)
+            for fused in (True, False):
+                got, _ = check_pipeline(g, b, fused=fused,
+                                        tag=f"large-weights/{tag} limit={limit} fused={fused}")
+                assert all(s["packed_parents"] == packed for s in got["stats"])
+
+
+def test_optimistic_packing_redo_mid_search(monkeypatch):
+    """The optimistic packing meets a larger distance only during the search
+    (every input distance fits): the launch stops before writing and is
+    redone with distance-only words.  From-scratch searches (limit: half the
+    longest distance) and updates (limit: the old trees' longest distance,
+    below the updated graph's), both kernel variants; results equal the
+    oracle's."""
+    g = gen.grid_graph(60, 50, 2, 1000, 0.1, 21)
+    b = gen.make_batch("deletions-only", g, 300, 22)  # distances only grow
+    upd, _ = O.apply_batch(g, b)
+    d0, _ = gen.initial_trees(g, 2)
+    old_longest = int(d0[d0 < INF // 2].max())
+    final = [O.dijkstra(upd, k, 0) for k in range(2)]
+    new_longest = max(int(d[d < INF // 2].max()) for d, _ in final)
+    assert new_longest > old_longest
+    dg = M().DeviceGraph(mgraph(upd))
+    ws = M().Workspace(upd.n)
+    for hubs in ("0", "1"):
+        monkeypatch.setenv("MOSP_HUBS", hubs)
+        for k in range(2):
+            rd, rp = final[k]
+            w = upd.w.reshape(-1, 2)[:, k]
+            delta = M().default_delta(upd.m, upd.n, int(w.sum()))
+            monkeypatch.setenv("MOSP_PACK_LIMIT", str(int(rd[rd < INF // 2].max()) // 2))
+            d, p, st = M().from_scratch(dg, k, 0, delta, 1000, ws)
+            say(f"redo from-scratch hubs={hubs} k={k} packed={st['packed_parents']}")
+            assert st["packed_parents"] == 0
+            assert np.array_equal(d, rd) and np.array_equal(p, rp)
+        monkeypatch.setenv("MOSP_PACK_LIMIT", str(old_longest))
+        for fused in (True, False):
+            got, _ = check_pipeline(g, b, pref=[1, 1], fused=fused,
+                                    tag=f"redo update hubs={hubs} fused={fused}")
+            assert all(s["packed_parents"] == 0 for s in got["stats"])
+    monkeypatch.delenv("MOSP_PACK_LIMIT")
+    ws.close()
+    dg.close()
 
 
 def test_edge_cases():
@@ -362,14 +411,15 @@ def test_full_lists_fall_back_to_sweeps(monkeypatch):
 
 def test_rmat_large_weights_fallback():
     """R-MAT scale 17, weights up to 2^31 - 1: (n - 1) * maxW does not fit next
-    to an 18-bit parent id, so the distance-only words and the lowest-id
-    parent recovery run -- over hub vertices (the power-law stress workload
-    in miniature)."""
+    to an 18-bit parent id; the words packed up to their own limit (46 bits)
+    hold every distance, over hub vertices (the power-law stress workload in
+    miniature).  The distance-only words and parent recovery on hub graphs
+    are covered by test_full_lists_fall_back_to_sweeps and the redo tests."""
     g = rmat(17, 16, 2, seed=91, wmax=2**31 - 1)
     assert max_degree(g) > 32
     b = gen.uniform_batch(g, 4000, 50, 92, wmax=2**31 - 1)
     got, _ = check_pipeline(g, b, pref=[1, 2], tag="rmat-large-weights")
-    assert all(s["packed_parents"] == 0 for s in got["stats"])
+    assert all(s["packed_parents"] == 1 for s in got["stats"])  # optimistic packing held
 
 
 # ------------------------------------------------------ benchmark workloads --
@@ -428,7 +478,7 @@ def test_rmat_scale22_k5_full_size():
     rd, rp = O.dijkstra(g, 0, 0)
     assert np.array_equal(d0[0], rd) and np.array_equal(p0[0], rp), "initial tree 0"
     r = M().update_host(mg, mb, d0, p0, pref=pref, fused=True)
-    assert all(s["packed_parents"] == 0 for s in r["stats"])
+    assert all(s["packed_parents"] == 1 for s in r["stats"])  # optimistic packing held
     del g, mg
     upd = O.Graph(r["updated"].n, K, r["updated"].row, r["updated"].col, r["updated"].w)
     unreachable = 0

@@ -2,9 +2,11 @@
 scratch is one static device array) against mosp_sosp_update, the entry
 point it varies, through the C ABI: two calls in a row, the second on a
 smaller graph, over both kernel variants (a grid and an R-MAT graph with
-hubs) and the distance-only words of large weights.  Results must be
-bit-identical, and equal canonical Dijkstra on the updated graph.  Graphs
-above the stated maximum are refused before any device work."""
+hubs) and large weights: packed words bounded by their own limit (the
+optimistic packing), and distance-only words (MOSP_PACK_LIMIT=1 makes the
+optimistic launch meet a larger distance and be redone with them).  Results
+must be bit-identical, and equal canonical Dijkstra on the updated graph.
+Graphs above the stated maximum are refused before any device work."""
 from __future__ import annotations
 
 import ctypes as C
@@ -40,11 +42,11 @@ def case(kind: str, seed: int):
         g = Mm.gen_road_grid(40, 40, 2, seed=seed)
     elif kind == "rmat":
         g = Mm.gen_rmat(14, 8, 2, seed=seed)
-    else:  # distance-only words: (n - 1) * max weight does not fit beside the parent bits
+    else:  # large weights: (n - 1) * max weight does not fit beside the parent bits
         g = Mm.gen_road_grid(320, 320, 2, seed=seed, wmin=1 << 29, wmax=(1 << 31) - 1)
+    large = kind in ("fallback", "optimistic")
     b = Mm.gen_batch(g, max(50, g.n // 20), 50.0, seed=seed + 1, safe=False,
-                     wmin=1 if kind != "fallback" else 1 << 29,
-                     wmax=100 if kind != "fallback" else (1 << 31) - 1)
+                     wmin=1 << 29 if large else 1, wmax=(1 << 31) - 1 if large else 100)
     return g, b
 
 
@@ -70,9 +72,13 @@ def run_both(g, b, k: int):
 
 
 @pytest.mark.parametrize("kinds", [("grid-large", "grid-small"), ("rmat", "grid-small"),
-                                   ("fallback", "grid-small")])
-def test_noalloc_equals_workspace_entry_point(kinds):
+                                   ("fallback", "grid-small"), ("optimistic", "grid-small")])
+def test_noalloc_equals_workspace_entry_point(kinds, monkeypatch):
     for i, kind in enumerate(kinds):
+        if kind == "fallback":
+            monkeypatch.setenv("MOSP_PACK_LIMIT", "1")
+        else:
+            monkeypatch.delenv("MOSP_PACK_LIMIT", raising=False)
         g, b = case(kind, 11 + 7 * i)
         for k in range(2):
             (gd, gp, st), (wd, wp, sw), (rd, rp) = run_both(g, b, k)
@@ -82,8 +88,8 @@ def test_noalloc_equals_workspace_entry_point(kinds):
             assert np.array_equal(gd, rd) and np.array_equal(gp, rp), f"{kind} obj{k} vs Dijkstra"
             for key in ("invalidated", "jump_rounds", "packed_parents"):
                 assert st[key] == sw[key], (kind, k, key)
-            if kind == "fallback":
-                assert st["packed_parents"] == 0
+            if kind in ("fallback", "optimistic"):
+                assert st["packed_parents"] == (0 if kind == "fallback" else 1)
 
 
 def test_noalloc_refuses_graphs_above_its_maximum():
