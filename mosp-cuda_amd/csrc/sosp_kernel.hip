@@ -1477,8 +1477,10 @@ __device__ __forceinline__ bool jump_next(int *word, int v) {
   return w != 0;
 }
 
+// Inlined into its instance: as a call (its own register allocation and
+// frame) the update was ~10 % slower.
 template <bool HUBS>
-__device__ int update_step1(Args &a, GridBarrier &grid, Threads t,
+__device__ __forceinline__ int update_step1(Args &a, GridBarrier &grid, Threads t,
                             int generation, Shared &sh) {
   const int n = a.n;
   Ctl *ctl = a.ctl;
@@ -1880,8 +1882,10 @@ __device__ __forceinline__ void combined_edges(Args &a, int v, F &&f) {
   }
 }
 
+// A call, not inlined: it then does not raise the register use of the
+// near-far loops around it.
 template <bool HUBS>
-__device__ void combined_build(Args &args, GridBarrier &grid,
+__device__ __attribute__((noinline)) void combined_build(Args &args, GridBarrier &grid,
                                Threads t, Shared &sh) {
   Args &a = *uniform(&args);
   int *sScan = sh.scan;
@@ -2080,7 +2084,7 @@ __device__ bool dropped_any(Args &a, int count, Threads t) {
 // generation, advanced as the phases use it.  Returns early when a barrier
 // gives up or an optimistic packing meets a larger distance.
 template <bool HUBS, int LOCAL>
-__device__ int run_phases(Args &a, GridBarrier &grid, const Threads t, Shared &sh,
+__device__ __forceinline__ int run_phases(Args &a, GridBarrier &grid, const Threads t, Shared &sh,
                           int generation) {
   const int n = a.n;
   if (t.id == 0) {
