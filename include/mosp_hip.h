@@ -112,6 +112,14 @@ int32_t mosp_ws_grid(const mosp_ws *ws);
  * are not all resident then ends with an error (no result written; the
  * workspace's scratch is cleared before its next launch) instead of hanging. */
 int mosp_ws_set_barrier_timeout(mosp_ws *ws, double ms);
+/* Workgroup shape of the persistent launches: `threads` 1024 (one
+ * workgroup per CU, 16 waves) or 512 (two per CU), 0 = automatic (512 for
+ * graphs with a vertex of more than 32 edges, else 1024); `queue_cap` =
+ * LDS queue entries per list the push loop uses, 256..4096 (0 = all).
+ * Also MOSP_WG_THREADS / MOSP_QUEUE_CAP at workspace creation.  No
+ * reference counterpart: tuning knobs of this implementation; results do
+ * not depend on them. */
+int mosp_ws_set_shape(mosp_ws *ws, int32_t threads, int32_t queue_cap);
 
 /* Near-far width of the combined search in the fused launch when none is
  * given: the reference's rule (src/sospUpdateGpu.cu:607-616, evaluated on the
@@ -247,7 +255,8 @@ int mosp_batch_reset(mosp_batch *b, const mosp_batch_instance *instances,
 int64_t mosp_batch_instance_bytes(int32_t n, int32_t k);
 /* The batch's own HIP stream (non-blocking; its launches run there). */
 void *mosp_batch_stream(const mosp_batch *b);
-/* Launch shape: workgroups (0 = one per CU), lanes per frontier vertex as
+/* Launch shape: workgroups (0 = every co-resident workgroup of the
+ * batch's workgroup shape: one per CU at 1024 threads), lanes per frontier vertex as
  * 2^lane_bits (< 0 = keep), each instance's widths scaled by delta_scale
  * (> 0; widths stay >= 1), xcd_aware != 0 places small groups on one XCD. */
 int mosp_batch_configure(mosp_batch *b, int32_t workgroups, int32_t lane_bits,
@@ -258,6 +267,11 @@ int mosp_batch_configure(mosp_batch *b, int32_t workgroups, int32_t lane_bits,
  * instance finishes at the same time.  latency_items (default 400) is the
  * number of vertices a workgroup relaxes in one iteration's latency. */
 int mosp_batch_set_cost_model(mosp_batch *b, double latency_items);
+/* Workgroup shape of the batched launch: 1024 threads (one workgroup per
+ * CU; the default) or 512 (two per CU), keeping the batch's share of the
+ * device; `queue_cap` as in mosp_ws_set_shape.  No reference counterpart;
+ * results do not depend on it. */
+int mosp_batch_set_shape(mosp_batch *b, int32_t threads, int32_t queue_cap);
 /* One pass over every instance, synchronous.  use_graph 1 replays a HIP
  * graph of the launch sequence (captured on first use and after
  * mosp_batch_configure / mosp_batch_reset); 0 launches cooperatively; 2

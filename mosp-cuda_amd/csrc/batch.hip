@@ -64,7 +64,10 @@ struct mosp_batch {
   int capDescs = 0;
   size_t capSlab = 0;
   int cus = 0;
-  int maxBlocks = 0;  // co-resident workgroups of sosp_batched
+  int maxBlocks = 0;  // co-resident workgroups of sosp_batched (of the shape)
+  int shapeThreads = 1024;  // workgroup shape: 1024 or 512 threads
+  int partialsCap = 0;      // combined-scan partials per instance: either shape's largest grid
+  int queueCap = 0;         // LDS queue entries of the push loop (0: all)
   int xcds = 1;
   int workgroups = 0;
   int laneBits = 2;
@@ -191,11 +194,19 @@ int setup(mosp_batch *b, int maxInstances, int maxDescs, size_t slabBytes) {
     return fail("Error: the GPU does not support cooperative launches.");
   }
   HIP_TRY(hipDeviceGetAttribute(&b->cus, hipDeviceAttributeMultiprocessorCount, device));
-  HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCu, sosp_batched_symbol(true), BLOCK, 0));
-  if (perCu < 1) {
+  if (const char *env = std::getenv("MOSP_WG_THREADS")) {
+    b->shapeThreads = std::atoi(env) == 512 ? 512 : 1024;
+  }
+  if (const char *env = std::getenv("MOSP_QUEUE_CAP")) {
+    b->queueCap = std::max(0, std::min(MOSP_QUEUE_CAP_MAX, std::atoi(env)));
+  }
+  b->maxBlocks = shape_max_blocks(kernel_shape(b->shapeThreads), true);
+  b->partialsCap = std::max(shape_max_blocks(kernel_shape(1024), true),
+                            shape_max_blocks(kernel_shape(512), true));
+  (void)perCu;
+  if (b->maxBlocks < 1) {
     return fail("Error: the batched kernel does not fit on a CU.");
   }
-  b->maxBlocks = perCu * b->cus;
   if (hipDeviceGetAttribute(&clockKhz, hipDeviceAttributeWallClockRate, device) == hipSuccess &&
       clockKhz > 0) {
     b->clockKhz = clockKhz;
@@ -206,7 +217,7 @@ int setup(mosp_batch *b, int maxInstances, int maxDescs, size_t slabBytes) {
     b->xcds = std::max(1, std::atoi(env));
   }
   local_hops_override(b->hopsUpdate, b->hopsSearch);
-  b->workgroups = std::min(b->cus, b->maxBlocks);
+  b->workgroups = b->maxBlocks;  // every co-resident workgroup of the shape
   maxInstances = std::max(maxInstances, 1);
   maxDescs = std::max(maxDescs, 1);
   b->capInstances = maxInstances;
@@ -219,9 +230,9 @@ int setup(mosp_batch *b, int maxInstances, int maxDescs, size_t slabBytes) {
   HIP_TRY(hipMemset(b->dAbort, 0, sizeof(unsigned)));
   HIP_TRY(hipMalloc(&b->dDesc, sizeof(ObjDesc) * maxDescs));
   HIP_TRY(hipMalloc(&b->dArgs, sizeof(LaunchArgs) * maxInstances));
-  HIP_TRY(hipMalloc(&b->dGroups, sizeof(BatchGroup) * std::max(maxInstances, b->maxBlocks)));
-  HIP_TRY(hipMalloc(&b->dGroupOf, sizeof(int) * b->maxBlocks));
-  HIP_TRY(hipMalloc(&b->dRankOf, sizeof(int) * b->maxBlocks));
+  HIP_TRY(hipMalloc(&b->dGroups, sizeof(BatchGroup) * std::max(maxInstances, b->partialsCap)));
+  HIP_TRY(hipMalloc(&b->dGroupOf, sizeof(int) * b->partialsCap));
+  HIP_TRY(hipMalloc(&b->dRankOf, sizeof(int) * b->partialsCap));
   HIP_TRY(hipHostMalloc(&b->hCtl, sizeof(Ctl), hipHostMallocDefault));
   HIP_TRY(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
   HIP_TRY(hipEventCreate(&b->ev0));
@@ -247,7 +258,7 @@ int load(mosp_batch *b, const mosp_batch_instance *spec, int count) {
       return 0;
     }
     I.slabOffset = slabBytes;
-    slabBytes += instance_bytes(I.n, I.K, b->maxBlocks);
+    slabBytes += instance_bytes(I.n, I.K, b->partialsCap);
     I.descOffset = descs;
     descs += I.K + 1;
   }
@@ -467,6 +478,7 @@ int prepare(mosp_batch *b) {
     a.anyFallback = anyFallback;
     a.laneBits = b->laneBits;
     a.listCap = list_cap_override();
+    a.queueCap = b->queueCap;
     a.hubEdges = hub_edges_override();
     a.hopsUpdate = b->hopsUpdate;
     a.hopsSearch = b->hopsSearch;
@@ -519,10 +531,12 @@ int capture(mosp_batch *b) {
     HIP_TRY(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
     hipError_t err = launch_ctl_reset_many(b->dCtl, count, b->stream, b->dAbort);
     if (err == hipSuccess) {
-      err = mode == 1 ? hipLaunchCooperativeKernel(sosp_batched_symbol(b->hubs != 0), dim3(b->workgroups),
-                                                   dim3(BLOCK), params, 0, b->stream)
-                      : hipLaunchKernel(sosp_batched_symbol(b->hubs != 0), dim3(b->workgroups), dim3(BLOCK),
-                                        params, 0, b->stream);
+      const KernelShape &shape = kernel_shape(b->shapeThreads);
+      const void *kernel = shape.batched(b->hubs != 0);
+      err = mode == 1 ? hipLaunchCooperativeKernel(kernel, dim3(b->workgroups),
+                                                   dim3(shape.threads), params, 0, b->stream)
+                      : hipLaunchKernel(kernel, dim3(b->workgroups), dim3(shape.threads), params,
+                                        0, b->stream);
     }
     hipGraph_t graph = nullptr;
     const hipError_t endErr = hipStreamEndCapture(b->stream, &graph);
@@ -564,7 +578,7 @@ extern "C" mosp_batch *mosp_batch_create(const mosp_batch_instance *instances, i
       release(b.get());
       return nullptr;
     }
-    slab += instance_bytes(probe.n, probe.K, b->maxBlocks);
+    slab += instance_bytes(probe.n, probe.K, b->partialsCap);
   }
   (void)hipFree(b->slab);
   b->slab = nullptr;
@@ -608,14 +622,13 @@ extern "C" void *mosp_batch_stream(const mosp_batch *b) {
 }
 
 extern "C" int64_t mosp_batch_instance_bytes(int32_t n, int32_t k) {
-  int device = 0, cus = 0, perCu = 0;
-  if (n <= 0 || k <= 0 || hipGetDevice(&device) != hipSuccess ||
-      hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess ||
-      hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCu, sosp_batched_symbol(true), BLOCK, 0) !=
-          hipSuccess) {
+  // Room for the combined scan's partials of either shape's largest grid.
+  const int most = std::max(shape_max_blocks(kernel_shape(1024), true),
+                            shape_max_blocks(kernel_shape(512), true));
+  if (n <= 0 || k <= 0 || most < 1) {
     return -1;
   }
-  return static_cast<int64_t>(instance_bytes(n, k, std::max(perCu, 1) * cus));
+  return static_cast<int64_t>(instance_bytes(n, k, most));
 }
 
 extern "C" void mosp_batch_destroy(mosp_batch *b) {
@@ -631,7 +644,7 @@ extern "C" int mosp_batch_configure(mosp_batch *b, int32_t workgroups, int32_t l
   if (b == nullptr || !(delta_scale > 0)) {
     return fail("Error: invalid batch configuration.");
   }
-  const int wg = workgroups > 0 ? workgroups : b->cus;
+  const int wg = workgroups > 0 ? workgroups : b->maxBlocks;
   if (wg > b->maxBlocks) {
     return fail("Error: " + std::to_string(wg) + " workgroups exceed the " +
                 std::to_string(b->maxBlocks) + " that can be co-resident.");
@@ -640,6 +653,30 @@ extern "C" int mosp_batch_configure(mosp_batch *b, int32_t workgroups, int32_t l
   if (lane_bits >= 0) b->laneBits = std::min(5, static_cast<int>(lane_bits));
   b->deltaScale = delta_scale;
   b->xcdAware = xcd_aware;
+  b->dirty = true;
+  return 1;
+}
+
+extern "C" int mosp_batch_set_shape(mosp_batch *b, int32_t threads, int32_t queue_cap) {
+  clear_error();
+  if (b == nullptr || (threads != 512 && threads != 1024) || queue_cap < 0 ||
+      (queue_cap > 0 && (queue_cap < 256 || queue_cap > MOSP_QUEUE_CAP_MAX))) {
+    return fail("Error: workgroups of 512 or 1024 threads and an LDS queue of 0 (all) or "
+                "256.." + std::to_string(MOSP_QUEUE_CAP_MAX) + " entries.");
+  }
+  if (threads != b->shapeThreads) {
+    const int most = shape_max_blocks(kernel_shape(threads), true);
+    if (most < 1) {
+      return fail("Error: the batched kernel does not fit on a CU.");
+    }
+    // The same share of the device in the new shape.
+    const double share = static_cast<double>(b->workgroups) / std::max(b->maxBlocks, 1);
+    b->shapeThreads = threads;
+    b->maxBlocks = most;
+    b->workgroups = std::max(1, std::min(most, static_cast<int>(share * most + 0.5)));
+    drop_graph(b);
+  }
+  b->queueCap = queue_cap;
   b->dirty = true;
   return 1;
 }
@@ -678,12 +715,14 @@ extern "C" int mosp_batch_run(mosp_batch *b, int32_t use_graph, float *kernel_ms
     void *params[] = {&b->dArgs, &b->dGroups, &b->dGroupOf, &b->dRankOf};
     HIP_TRY(launch_ctl_reset_many(b->dCtl, count, b->stream, b->dAbort));
     HIP_TRY(hipEventRecord(b->ev0, b->stream));
+    const KernelShape &shape = kernel_shape(b->shapeThreads);
+    const void *kernel = shape.batched(b->hubs != 0);
     if (use_graph == 2) {
-      HIP_TRY(hipLaunchKernel(sosp_batched_symbol(b->hubs != 0), dim3(b->workgroups), dim3(BLOCK), params, 0,
+      HIP_TRY(hipLaunchKernel(kernel, dim3(b->workgroups), dim3(shape.threads), params, 0,
                               b->stream));
     } else {
-      HIP_TRY(launch_persistent(sosp_batched_symbol(b->hubs != 0), dim3(b->workgroups),
-                                dim3(BLOCK), params, b->stream));
+      HIP_TRY(launch_persistent(kernel, dim3(b->workgroups), dim3(shape.threads), params,
+                                b->stream));
     }
     HIP_TRY(hipEventRecord(b->ev1, b->stream));
   }

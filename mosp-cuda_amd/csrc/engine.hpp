@@ -28,6 +28,8 @@ constexpr int MOSP_MAX_OBJ = 32;     // K <= 32 (csrGraph.cu:736-739)
 #define MOSP_BLOCK 1024
 #endif
 constexpr int BLOCK = MOSP_BLOCK;    // threads per workgroup (wave64 multiple)
+// LDS queue entries per list and workgroup (the most a launch can use).
+#define MOSP_QUEUE_CAP_MAX 4096
 #ifndef MOSP_MIN_BLOCKS
 #define MOSP_MIN_BLOCKS 1            // co-resident workgroups per CU the code is built for
 #endif
@@ -171,6 +173,8 @@ struct LaunchArgs {
   int listCap;             // entries of each near / far list: 0 = n; a smaller
                            // value (tests) forces the sweeps that follow a full list
   int hubs;                // host only: launch the HUB_DEGREE-aware variant
+  int queueCap;            // LDS queue entries the push loop uses: 256..MOSP_QUEUE_CAP_MAX
+                           // (0: all of them)
   int hubEdges;            // hub variant: push vertices of more out-edges go to
                            // the objective's hub list (HUB_GRID; tests lower it)
   int *genDev;             // device-resident stamp generation (persists)

@@ -34,14 +34,32 @@ namespace mosp {
 
 // Kernels and their launchers (sosp_kernel.hip).
 // hubs: the variant for graphs with vertices of more than HUB_DEGREE edges.
+// The persistent kernels are built for two workgroup shapes: 1024 threads,
+// one workgroup per CU (16 waves per CU), and 512 threads, up to two per CU.
+namespace shape1024 {
 const void *sosp_kernel_symbol(bool hubs);
 const void *sosp_batched_symbol(bool hubs);
+}  // namespace shape1024
+namespace shape512 {
+const void *sosp_kernel_symbol(bool hubs);
+const void *sosp_batched_symbol(bool hubs);
+}  // namespace shape512
+
 hipError_t launch_ctl_reset(Ctl *ctl, hipStream_t stream);
 // abortWord (may be null): a launch-wide abort word cleared with the blocks.
 hipError_t launch_ctl_reset_many(Ctl *ctls, int count, hipStream_t stream,
                                  unsigned *abortWord = nullptr);
 
 namespace detail {
+struct KernelShape {
+  int threads;  // per workgroup
+  const void *(*persistent)(bool hubs);
+  const void *(*batched)(bool hubs);
+};
+// The shape of `threads` threads per workgroup (512; anything else: 1024).
+const KernelShape &kernel_shape(int threads);
+// Workgroups of a shape that fit on the device at once (0 on error).
+int shape_max_blocks(const KernelShape &shape, bool batched);
 
 // Objective k's {column, weight} record arrays of a device graph (out, in).
 void graph_edges(const mosp_device_graph *g, int k, const int2 **out, const int2 **in);

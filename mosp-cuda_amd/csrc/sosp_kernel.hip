@@ -72,7 +72,7 @@ __device__ __forceinline__ T *uniform_global(T *p) {
   return (T *)reinterpret_cast<G *>((static_cast<unsigned long long>(hi) << 32) | lo);
 }
 
-constexpr unsigned QCAP = 4096;  // LDS queue entries per list per workgroup
+constexpr unsigned QCAP = MOSP_QUEUE_CAP_MAX;  // LDS queue entries per list per workgroup
 
 // ---------------------------------------------------------------------------
 // Device helpers
@@ -344,6 +344,10 @@ __device__ __forceinline__ int row_of(const int *rowPtr, int lo, int hi, long lo
 // (one LDS atomic per wave) and a slice's entries are published with ONE
 // global atomicAdd.  A full slice sends entries straight to the global list.
 constexpr unsigned WAVES = BLOCK / 64;
+// Every per-wave LDS array below is dimensioned by WAVES and indexed by
+// threadIdx.x / 64 (or a team rank below WAVES): a workgroup of BLOCK
+// threads has exactly WAVES full waves.
+static_assert(BLOCK % 64 == 0 && WAVES * 64 == BLOCK && WAVES <= 16, "whole wave64s, at most 16");
 // HIP's second launch bound is the minimum number of waves per SIMD: 4 SIMDs
 // per CU share the workgroups' waves.
 constexpr unsigned WAVES_PER_SIMD = MOSP_MIN_BLOCKS * WAVES / 4;
@@ -379,13 +383,15 @@ __device__ __forceinline__ void queue_push(BlockQueue &q, int v, int *list,
   slice_push(q, 0, QCAP, v, list, listCount, listCap);
 }
 
-// Workgroup-uniform: publish the whole-buffer queue and empty it.  Thread 0
-// alone reads the count and empties the queue before the barrier; the
-// others read the published base and length after it.
-__device__ void queue_flush(BlockQueue &q, int *list, unsigned *listCount, unsigned listCap) {
+// Workgroup-uniform: publish the whole-buffer queue (its first `cap`
+// entries hold appends; later ones went straight to the list) and empty it.
+// Thread 0 alone reads the count and empties the queue before the barrier;
+// the others read the published base and length after it.
+__device__ void queue_flush(BlockQueue &q, int *list, unsigned *listCount, unsigned listCap,
+                            unsigned cap = QCAP) {
   __syncthreads();
   if (threadIdx.x == 0) {
-    const unsigned count = q.count[0] < QCAP ? q.count[0] : QCAP;
+    const unsigned count = q.count[0] < cap ? q.count[0] : cap;
     const unsigned base = count > 0 ? atomicAdd(listCount, count) : 0;
     q.base[0] = base;
     q.keep = base < listCap ? min(count, listCap - base) : 0;  // entries that fit
@@ -545,6 +551,7 @@ struct Shared {
   };
   unsigned hubs[MOSP_MAX_OBJ];   // hubs listed for this iteration (hub variant)
   int hubEdges;                  // push: more out-edges -> the hub list (hub variant)
+  unsigned queueCap;             // push: entries of the LDS queues used (<= QCAP)
   unsigned wgCount;              // push: entries of wgList (hub variant)
   int wgList[WG_LIST];
   int span[2];                   // edge-parallel passes: rows of the first/last edge
@@ -750,8 +757,8 @@ __device__ void push_team(Desc &D, ObjCtl &c, Shared &sh, int k,
       }
       if (tm.whole) {
         __syncthreads();
-        if (sh.near.count[0] > QCAP - BLOCK) queue_flush(sh.near, outNear, nextCnt, listCap);
-        if (sh.far.count[0] > QCAP - BLOCK) queue_flush(sh.far, farBuf, farCnt, listCap);
+        if (sh.near.count[0] + BLOCK > tm.cap) queue_flush(sh.near, outNear, nextCnt, listCap, tm.cap);
+        if (sh.far.count[0] + BLOCK > tm.cap) queue_flush(sh.far, farBuf, farCnt, listCap, tm.cap);
       }
     }
   } else if constexpr (!HUBS) {
@@ -774,8 +781,8 @@ __device__ void push_team(Desc &D, ObjCtl &c, Shared &sh, int k,
       }
       if (tm.whole) {
         __syncthreads();
-        if (sh.near.count[0] > QCAP / 2) queue_flush(sh.near, outNear, nextCnt, listCap);
-        if (sh.far.count[0] > QCAP / 2) queue_flush(sh.far, farBuf, farCnt, listCap);
+        if (sh.near.count[0] > tm.cap / 2) queue_flush(sh.near, outNear, nextCnt, listCap, tm.cap);
+        if (sh.far.count[0] > tm.cap / 2) queue_flush(sh.far, farBuf, farCnt, listCap, tm.cap);
       }
     }
   } else {
@@ -843,8 +850,8 @@ __device__ void push_team(Desc &D, ObjCtl &c, Shared &sh, int k,
             relax(u, packed[u], sh.hub.begin[j] + static_cast<int>(g - start));
           }
           __syncthreads();
-          if (sh.near.count[0] > QCAP / 2) queue_flush(sh.near, outNear, nextCnt, listCap);
-          if (sh.far.count[0] > QCAP / 2) queue_flush(sh.far, farBuf, farCnt, listCap);
+          if (sh.near.count[0] > tm.cap / 2) queue_flush(sh.near, outNear, nextCnt, listCap, tm.cap);
+          if (sh.far.count[0] > tm.cap / 2) queue_flush(sh.far, farBuf, farCnt, listCap, tm.cap);
         }
         __syncthreads();  // the wave lists overlay the hub table
       } else {
@@ -873,8 +880,8 @@ __device__ void push_team(Desc &D, ObjCtl &c, Shared &sh, int k,
           relax(u, word, x);
         }
         __syncthreads();
-        if (sh.near.count[0] > QCAP / 2) queue_flush(sh.near, outNear, nextCnt, listCap);
-        if (sh.far.count[0] > QCAP / 2) queue_flush(sh.far, farBuf, farCnt, listCap);
+        if (sh.near.count[0] > tm.cap / 2) queue_flush(sh.near, outNear, nextCnt, listCap, tm.cap);
+        if (sh.far.count[0] > tm.cap / 2) queue_flush(sh.far, farBuf, farCnt, listCap, tm.cap);
       }
       if (threadIdx.x == 0) {
         sh.wgCount = 0;
@@ -963,8 +970,8 @@ __device__ void push_team(Desc &D, ObjCtl &c, Shared &sh, int k,
       if (tm.whole) {
         __syncthreads();
         if (sh.wgCount > WG_LIST / 2) drainGroup();
-        if (sh.near.count[0] > QCAP / 2) queue_flush(sh.near, outNear, nextCnt, listCap);
-        if (sh.far.count[0] > QCAP / 2) queue_flush(sh.far, farBuf, farCnt, listCap);
+        if (sh.near.count[0] > tm.cap / 2) queue_flush(sh.near, outNear, nextCnt, listCap, tm.cap);
+        if (sh.far.count[0] > tm.cap / 2) queue_flush(sh.far, farBuf, farCnt, listCap, tm.cap);
       }
     }
     if (tm.whole) {
@@ -990,7 +997,7 @@ __device__ void push_team(Desc &D, ObjCtl &c, Shared &sh, int k,
     unsigned from = 0;
     for (int hop = 0; hop < localHops; ++hop) {
       __syncthreads();
-      unsigned to = sh.near.count[0] < QCAP ? sh.near.count[0] : QCAP;
+      unsigned to = sh.near.count[0] < tm.cap ? sh.near.count[0] : tm.cap;
       to = to < from + perPass ? to : from + perPass;
       if (from >= to) {
         break;
@@ -1135,7 +1142,7 @@ __device__ int near_far(Desc *objsArg, int nobj, ObjCtl *ctlsArg,
       const unsigned team = (threadIdx.x / 64) * teams / WAVES;
       const unsigned w0 = (team * WAVES + teams - 1) / teams;
       const unsigned w1 = ((team + 1) * WAVES + teams - 1) / teams;
-      const Team tm{team, threadIdx.x - 64 * w0, 64 * (w1 - w0), QCAP / teams, teams == 1};
+      const Team tm{team, threadIdx.x - 64 * w0, 64 * (w1 - w0), sh.queueCap / teams, teams == 1};
       const int k = sh.mine[first + static_cast<int>(team)];
       if (threadIdx.x < teams) {
         sh.sliceFarMin[threadIdx.x] = PACKED_INF;
@@ -2097,6 +2104,7 @@ __device__ __forceinline__ int run_phases(Args &a, GridBarrier &grid, const Thre
   if (threadIdx.x == 0) {
     sh.wgCount = 0;
     sh.hubEdges = a.hubEdges;
+    sh.queueCap = a.queueCap >= 256 && a.queueCap <= QCAP ? a.queueCap : QCAP;
   }
 #ifdef MOSP_SUBPROFILE
   if (threadIdx.x < 16) subprof_lds()[threadIdx.x] = 0;
@@ -2325,7 +2333,15 @@ __global__ void ctl_reset(Ctl *ctls, unsigned *abortWord) {
 
 }  // namespace
 
-// Host-visible handles of the kernels (used by engine.hip).
+// Host-visible handles of the kernels (used by engine.hip and batch.hip).
+// The library builds this file once per workgroup shape (MOSP_BLOCK threads,
+// MOSP_MIN_BLOCKS per CU); each build's handles live in the namespace
+// MOSP_SHAPE_NS names.
+#ifndef MOSP_SHAPE_NS
+#define MOSP_SHAPE_NS shape1024
+#endif
+namespace MOSP_SHAPE_NS {
+
 const void *sosp_kernel_symbol(bool hubs) {
   return hubs ? reinterpret_cast<const void *>(&sosp_persistent<true>)
               : reinterpret_cast<const void *>(&sosp_persistent<false>);
@@ -2336,6 +2352,11 @@ const void *sosp_batched_symbol(bool hubs) {
               : reinterpret_cast<const void *>(&sosp_batched<false>);
 }
 
+}  // namespace MOSP_SHAPE_NS
+
+// The control-block reset does not depend on the shape: the first build
+// alone defines it.
+#ifndef MOSP_EXTRA_SHAPE
 hipError_t launch_ctl_reset_many(Ctl *ctls, int count, hipStream_t stream, unsigned *abortWord) {
   hipLaunchKernelGGL(ctl_reset, dim3(count), dim3(256), 0, stream, ctls, abortWord);
   return hipGetLastError();
@@ -2344,5 +2365,6 @@ hipError_t launch_ctl_reset_many(Ctl *ctls, int count, hipStream_t stream, unsig
 hipError_t launch_ctl_reset(Ctl *ctl, hipStream_t stream) {
   return launch_ctl_reset_many(ctl, 1, stream, nullptr);
 }
+#endif
 
 }  // namespace mosp

@@ -90,7 +90,8 @@ EXPORTS = (
     "mosp_server_release", "mosp_server_stats", "mosp_server_reset_stats", "mosp_server_clock",
     "mosp_server_destroy", "mosp_parallel_sosp_update", "mosp_parallel_combined_graph",
     "mosp_canonicalize_tree", "mosp_path_costs_host", "mosp_ws_set_barrier_timeout",
-    "mosp_batch_set_barrier_timeout", "mosp_sosp_update_noalloc",
+    "mosp_batch_set_barrier_timeout", "mosp_sosp_update_noalloc", "mosp_ws_set_shape",
+    "mosp_batch_set_shape",
 )
 
 _LIB: C.CDLL | None = None
@@ -195,6 +196,8 @@ def lib() -> C.CDLL:
     L.mosp_sosp_update_noalloc.argtypes = [C.POINTER(Csr), C.POINTER(Csr), C.POINTER(Changes),
                                            _I, _L, _L, _VP, _VP, C.POINTER(SospStats), _VP]
     L.mosp_ws_set_barrier_timeout.argtypes = [_VP, C.c_double]
+    L.mosp_ws_set_shape.argtypes = [_VP, _I, _I]
+    L.mosp_batch_set_shape.argtypes = [_VP, _I, _I]
     L.mosp_batch_set_barrier_timeout.argtypes = [_VP, C.c_double]
     L.mosp_canonicalize_tree.restype = None
     L.mosp_canonicalize_tree.argtypes = [_I, _I, _i32p, _i32p, _i32p, _I, _I, _i64p, _i32p]
@@ -540,6 +543,11 @@ class Workspace:
     def set_barrier_timeout(self, ms: float) -> None:
         """Longest grid-barrier wait (ms) before a launch gives up with an error."""
         _check(lib().mosp_ws_set_barrier_timeout(self.handle, float(ms)), "barrier timeout")
+
+    def set_shape(self, threads: int = 0, queue_cap: int = 0) -> None:
+        """Workgroups of 1024 or 512 threads (0: automatic) and the push loop's
+        LDS queue entries (256..4096, 0: all); results do not depend on them."""
+        _check(lib().mosp_ws_set_shape(self.handle, int(threads), int(queue_cap)), "shape")
 
     def set_combined_delta_scale(self, scale: int) -> None:
         """Combined search width = the reference's rule x scale (>= 1)."""

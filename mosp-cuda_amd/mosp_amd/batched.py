@@ -47,9 +47,17 @@ class BatchedUpdate:
         self.configure(workgroups, lane_bits, delta_scale, xcd_aware)
 
     def configure(self, workgroups: int = 0, lane_bits: int = -1, delta_scale: float = 1.0,
-                  xcd_aware: bool = True, latency_items: float | None = None) -> None:
+                  xcd_aware: bool = True, latency_items: float | None = None,
+                  threads: int = 1024, queue_cap: int = 0) -> None:
+        """Launch shape: workgroups (0: every co-resident one of the shape),
+        lanes per frontier vertex (2^lane_bits), width scale, XCD placement,
+        group-sizing model, threads per workgroup (1024: one per CU; 512: two)
+        and the push loop's LDS queue entries (256..4096, 0: all)."""
         self.config = {"workgroups": workgroups, "lane_bits": lane_bits,
-                       "delta_scale": delta_scale, "xcd_aware": bool(xcd_aware)}
+                       "delta_scale": delta_scale, "xcd_aware": bool(xcd_aware),
+                       "threads": threads, "queue_cap": queue_cap}
+        _check(lib().mosp_batch_set_shape(self.handle, int(threads), int(queue_cap)),
+               "mosp_batch_set_shape")
         _check(lib().mosp_batch_configure(self.handle, workgroups, lane_bits, delta_scale,
                                           int(bool(xcd_aware))), "mosp_batch_configure")
         if latency_items is not None:

@@ -142,6 +142,28 @@ def test_schedule_does_not_change_results(small, workgroups, lane_bits, delta_sc
     bu.close()
 
 
+@pytest.mark.parametrize("threads,queue_cap,workgroups", [
+    (512, 0, 0),       # two 512-thread workgroups per CU
+    (512, 256, 7),     # a queue of 256 entries: most appends go straight to the lists
+    (1024, 1024, 0),
+    (512, 2048, 300),
+])
+def test_workgroup_shapes_do_not_change_results(small, threads, queue_cap, workgroups):
+    """The batched kernel's two workgroup shapes and smaller LDS push queues
+    (the autotuner's second stage) give the oracle's results."""
+    specs, built = small
+    sessions = [s for _, s in built]
+    bu = B().BatchedUpdate(sessions)
+    bu.configure(workgroups=workgroups, threads=threads, queue_cap=queue_cap)
+    sizes, mode = bu.schedule()
+    say(f"shape {threads} threads, queue {queue_cap}, wg {workgroups}: groups {sizes}")
+    bu.run(use_graph=True)
+    bu.run(use_graph=True)
+    for i, (spec, (host, s)) in enumerate(zip(specs, built)):
+        assert_same(bu.results(i), oracle_of(spec, host), f"instance {i}")
+    bu.close()
+
+
 def test_uneven_mix_subset_against_fused():
     """Eight instances drawn from the batched mix (10k-120k vertices)."""
     specs = B().road_specs(count=8, n_max=120_000, ch_max=20_000, seed=9)

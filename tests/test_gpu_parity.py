@@ -380,6 +380,22 @@ def test_hub_tiers(monkeypatch):
             check_pipeline(g, b, pref=[1, 2, 3], tag=f"hub tiers {kind} edges={edges} grid={grid}")
 
 
+def test_workgroup_shapes_and_queue_caps(monkeypatch):
+    """The persistent kernels' two workgroup shapes (MOSP_WG_THREADS: 1024
+    threads, one workgroup per CU; 512, two per CU; 0, by kernel variant)
+    and smaller LDS push queues (MOSP_QUEUE_CAP) give the oracle's results:
+    fused and serial updates on a grid and an R-MAT graph."""
+    cases = (("grid", gen.grid_graph(60, 50, 3, 100, 0.2, 61)), ("rmat", rmat(12, 16, 3, seed=62)))
+    for tag, g in cases:
+        b = gen.uniform_batch(g, 500, 50, 63)
+        for threads, cap in ((512, 0), (512, 256), (1024, 1024), (0, 2048)):
+            monkeypatch.setenv("MOSP_WG_THREADS", str(threads))
+            monkeypatch.setenv("MOSP_QUEUE_CAP", str(cap))
+            for fused in (True, False):
+                check_pipeline(g, b, pref=[1, 2, 3], fused=fused,
+                               tag=f"shape {tag} threads={threads} queue={cap} fused={fused}")
+
+
 def test_full_lists_fall_back_to_sweeps(monkeypatch):
     """Near / far lists capped far below n (MOSP_LIST_CAP): appends past the
     cap are lost, and the iteration after a full list relaxes every vertex
