@@ -284,7 +284,8 @@ def plain_line(args):
                    "combined_delta": int(sess.cstats.delta)},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS,
                      "unit": "GB/s", "frac": achieved / PEAK_HBM_GBS, "traffic": traffic,
-                     "traffic_source": f"rocprofv3 FETCH_SIZE + WRITE_SIZE, {tp.name}",
+                     "traffic_source": f"rocprofv3 TCC_EA0_RDREQ_64B/128B + WRITE_SIZE per "
+                                       f"update dispatch, {tp.name}",
                      "alg_bytes": ab, "kernel": "sosp_persistent (one launch per update)"},
     }
     sess.close()
@@ -339,17 +340,18 @@ def rmat_line(args) -> dict:
     np.add.at(in_row, sess.updated.col.astype(np.int64) + 1, 1)
     in_row = np.cumsum(in_row)
     ab = alg_bytes(g.n, sess.updated.row, in_row, 0, p0, d0, res["dist"], changed, b.ins_to,
-                   res["combined_edges"])
+                   res["combined_edges"],
+                   distance_only=[s["packed_parents"] == 0 for s in res["stats"]])
     k_ms = float(np.mean(kernel_ms))
     achieved = ab / (k_ms * 1e-3) / 1e9
-    # Measured HBM bytes per launch (rocprofv3 FETCH_SIZE / WRITE_SIZE, scale 22).
+    # Measured HBM bytes per launch (rocprofv3 read requests by size + WRITE_SIZE, scale 22).
     traffic = None
     tp = ROOT / "profiles" / "traffic_rmat22_k5.json"
     if args.scale == 22 and tp.exists():
         traffic = json.loads(tp.read_text()).get("bytes_per_launch")
     line = {
-        "metric": "changes/s, power-law stress update (R-MAT scale 22, K=5, distance-only "
-                  "fallback)",
+        "metric": "changes/s, power-law stress update (R-MAT scale 22, K=5, weights up to "
+                  "2^24)",
         "value": changes / (ms_per_step / 1e3), "unit": "changes/s",
         "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": ms_per_step, "gpu_compute_ms": ms_per_step, "kernel_ms": k_ms,

@@ -127,6 +127,10 @@ def test_roofline_bytes_match_oracle(seed):
     want = O.alg_bytes(upd, rev, 2, 0, p0, d0, d1, changed, b.ins_to, 123)
     got = alg_bytes(g.n, upd.row, rev.row, 0, p0, d0, d1, changed, b.ins_to, 123)
     assert got == want
+    # The parent-recovery term of distance-only objectives: 12 n + 16 m each.
+    fb = alg_bytes(g.n, upd.row, rev.row, 0, p0, d0, d1, changed, b.ins_to, 123,
+                   distance_only=[True, False])
+    assert fb - got == 12 * upd.n + 16 * int(upd.row[-1])
 
 
 @pytest.mark.skipif(not MOSP_BIN.exists(), reason="bin/mosp not built")
