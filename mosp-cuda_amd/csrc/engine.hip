@@ -252,7 +252,7 @@ int ws_warm_up(mosp_ws *ws) {
 
 size_t state_bytes(int n) {
   const size_t count = static_cast<size_t>(std::max(n, 1));
-  return 2 * align256(count * sizeof(u64)) + 8 * align256(count * sizeof(int)) +
+  return align256(count * sizeof(u64)) + 8 * align256(count * sizeof(int)) +
          align256(2 * static_cast<size_t>(HUB_LIST) * sizeof(int));
 }
 
@@ -262,8 +262,6 @@ ObjState carve_state(char *&p, int n) {
   const size_t intBytes = align256(count * sizeof(int));
   ObjState st{};
   st.packed = reinterpret_cast<u64 *>(p);
-  p += wordBytes;
-  st.relaxed = reinterpret_cast<u64 *>(p);
   p += wordBytes;
   int **ints[8] = {&st.stamp, &st.inFar, &st.flag, &st.ancestor, &st.near[0],
                    &st.near[1], &st.far[0], &st.far[1]};
@@ -1244,7 +1242,7 @@ constexpr size_t NA_DESC = (sizeof(Ctl) + 255) / 256 * 256;
 constexpr size_t NA_WORDS = NA_DESC + (sizeof(ObjDesc) + 255) / 256 * 256;
 constexpr size_t NA_STATE = NA_WORDS + 256;
 constexpr size_t NA_STATE_BYTES =
-    2 * ((static_cast<size_t>(NOALLOC_MAX_VERTICES) * sizeof(u64) + 255) / 256 * 256) +
+    (static_cast<size_t>(NOALLOC_MAX_VERTICES) * sizeof(u64) + 255) / 256 * 256 +
     8 * ((static_cast<size_t>(NOALLOC_MAX_VERTICES) * sizeof(int) + 255) / 256 * 256) +
     (2 * static_cast<size_t>(HUB_LIST) * sizeof(int) + 255) / 256 * 256;
 constexpr size_t NA_BYTES = NA_STATE + NA_STATE_BYTES;

@@ -65,8 +65,6 @@ enum : int {
 // Scratch of one objective (a slice of the workspace).
 struct ObjState {
   u64 *packed;     // (distance << b | parent) words, n
-  u64 *relaxed;    // word a local-hop round last relaxed the vertex with, n
-                   // (PACKED_INF: none in this search)
   int *stamp;      // generation a vertex was last listed in, n
   int *inFar;      // vertex is in the far pile, n (left all-zero)
   int *flag;       // invalidation marks, n (left all-zero)

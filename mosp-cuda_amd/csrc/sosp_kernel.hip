@@ -658,7 +658,6 @@ __device__ void push_team(Desc &D, ObjCtl &c, Shared &sh, int k,
   // live in SGPRs (k, hence D, comes from LDS: the compiler cannot tell).
   // All are device memory: global, not flat, accesses.
   u64 *const packed = uniform_global(D.s.packed);
-  u64 *const relaxedArr = uniform_global(D.s.relaxed);
   int *const stampArr = uniform_global(D.s.stamp);
   int *const inFar = uniform_global(D.s.inFar);
   const int *const outRow = uniform_global(D.outRow);
@@ -770,10 +769,7 @@ __device__ void push_team(Desc &D, ObjCtl &c, Shared &sh, int k,
         const u64 word = packed[u];
         const int begin = outRow[u];
         const int end = outRow[u + 1];
-        // A vertex that a local-hop round relaxed with its present word
-        // needs nothing more.
-        const u64 done = LOCAL != 0 ? relaxedArr[u] : PACKED_INF;
-        if (word != PACKED_INF && word != done) {
+        if (word != PACKED_INF) {
           for (int x = begin + static_cast<int>(sub); x < end; x += static_cast<int>(lanes)) {
             relax(u, word, x);
           }
@@ -1010,9 +1006,6 @@ __device__ void push_team(Desc &D, ObjCtl &c, Shared &sh, int k,
           const int begin = outRow[u];
           const int end = outRow[u + 1];
           if (word != PACKED_INF && (!HUBS || end - begin <= HEAVY_GROUP)) {
-            if (sub == 0) {
-              relaxedArr[u] = word;
-            }
             for (int x = begin + static_cast<int>(sub); x < end; x += static_cast<int>(lanes)) {
               relax(u, word, x);
             }
@@ -1414,7 +1407,6 @@ __device__ void scratch_init(Desc &D, ObjCtl &c, int n, int source,
   for (int v = t.id; v < n; v += t.count) {
     D.s.packed[v] = v == source ? pack(0, -1, D.parentBits, D.noParent)
                                 : PACKED_INF;
-    D.s.relaxed[v] = PACKED_INF;
   }
   if (t.id == 0) {
     D.s.near[0][0] = source;
@@ -1517,9 +1509,6 @@ __device__ __forceinline__ int update_step1(Args &a, GridBarrier &grid, Threads 
         word = pack(static_cast<u64>(d), p, D.parentBits, D.noParent);
       }
       D.s.packed[v] = word;
-      if constexpr (!HUBS) {
-        D.s.relaxed[v] = PACKED_INF;  // no vertex relaxed by a local-hop round yet
-      }
       if (D.nChanged > 0) {
         D.s.ancestor[v] = p + 1;  // jump word: ancestor + 1, no flag
       }
