@@ -703,25 +703,35 @@ __device__ void push_team(Desc &D, ObjCtl &c, Shared &sh, int k,
       return;
     }
     const u64 cand = pack(nd, u, bits, noParent);
-    // The claim word (stamp for the near list, far mark for the far pile) is
-    // read before the atomic so both round trips overlap.  Claims need no
-    // atomics: two lanes may both append w (a duplicate entry only repeats
-    // work; racing lanes, and other XCDs, which see this XCD's stores only
-    // after the next barrier, can make many), but an improvement is never
-    // missed: the stamp only ever holds this iteration's generation once w is
-    // appended in it (the entry reads w's word when processed, after this
-    // iteration's atomics), and inFar is only set during push iterations of
-    // this epoch.  Reading the claim word early can only see an older value,
-    // i.e. add a duplicate.  Duplicates can run a list past its n entries (a
-    // small graph that local hops cover in one iteration does): the lost
-    // entries turn the next iteration into a sweep of every vertex.
+    // Every improvement of w below the threshold lists w for the next
+    // iteration.  In the light variant (at most HUB_DEGREE edges a vertex)
+    // that is all: a vertex improved twice in an iteration is listed twice,
+    // and its second entry relaxes a few edges again (about 3 % more entries
+    // on a road grid), which costs less than reading a claim word -- one
+    // more random line -- on every relaxation (4.17 -> 3.75 ms per update).
+    // The hub variant claims near appends with w's stamp, and every variant
+    // claims far appends with w's far mark (the pile lives for an epoch).
+    // The claim word is read before the atomic so both round trips overlap.
+    // Claims need no atomics: two lanes may both append w (a duplicate entry
+    // only repeats work; racing lanes, and other XCDs, which see this XCD's
+    // stores only after the next barrier, can make many), but an improvement
+    // is never missed: the stamp only ever holds this iteration's generation
+    // once w is appended in it (the entry reads w's word when processed,
+    // after this iteration's atomics), and inFar is only set during push
+    // iterations of this epoch.  Reading the claim word early can only see
+    // an older value, i.e. add a duplicate.  Duplicates can run a list past
+    // its n entries (a small graph that local hops cover in one iteration
+    // does): the lost entries turn the next iteration into a sweep of every
+    // vertex.
     const bool near = nd < thr;
-    const int claim = near ? stampArr[w] : inFar[w];
+    const int claim = near ? (HUBS ? stampArr[w] : 0) : inFar[w];
     const u64 prev = atomicMin(&packed[w], cand);
     if (nd < dist_of(prev, bits)) {
       if (near) {
-        if (claim != gen) {
-          stampArr[w] = gen;
+        if (!HUBS || claim != gen) {
+          if (HUBS) {
+            stampArr[w] = gen;
+          }
           slice_push(sh.near, tm.slice, tm.cap, w, outNear, nextCnt, listCap);
         }
       } else if (claim == 0) {
