@@ -130,8 +130,9 @@ def main() -> int:
     ap.add_argument("--height", type=int, default=1400)
     ap.add_argument("--changes", type=int, default=50_000)
     ap.add_argument("--grid", type=int, default=0, help="workgroups of the persistent launch")
-    ap.add_argument("--delta-scale", type=float, default=3.0,
-                    help="multiply the default near-far width (tuning)")
+    ap.add_argument("--delta-scale", type=float, default=1.0,
+                    help="multiply the default near-far width (tuning; the asynchronous "
+                         "loop measured best at 1, the iteration-synchronous one at 3)")
     ap.add_argument("--comb-delta-scale", type=int, default=10,
                     help="multiply the combined search's default near-far width (tuning)")
     ap.add_argument("--no-cpu-baseline", action="store_true")

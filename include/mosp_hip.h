@@ -136,6 +136,16 @@ int mosp_ws_set_combined_delta_scale(mosp_ws *ws, int32_t scale);
 int mosp_ws_last_profile(const mosp_ws *ws, double *ms, int32_t n_phases,
                          int32_t *barriers);
 
+/* Asynchronous near-far loop of the last launch on this workspace (no
+ * reference counterpart: the loop is this engine's own schedule).  Graphs
+ * without hub vertices run their near-far phases without per-iteration grid
+ * barriers (MOSP_ASYNC=0 selects the iteration-synchronous loop instead);
+ * events[0] = batches that dropped near entries for lack of local queue
+ * room, events[1] = spills that found less pool room than asked, events[2] =
+ * phases that listed every vertex under the threshold again (after a drop),
+ * events[3] = 1 if the launch ran the asynchronous loop.  Returns 1. */
+int mosp_ws_async_events(const mosp_ws *ws, int32_t events[4]);
+
 /* Replaces defaultDelta (src/sospUpdateGpu.cu:607-616). */
 int64_t mosp_default_delta(int64_t m, int32_t n, int64_t weight_sum);
 

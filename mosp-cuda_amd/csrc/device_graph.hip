@@ -70,6 +70,21 @@ __global__ void pair_edges_kernel(const int *rowPtr, int n, const int *col, cons
   }
 }
 
+// {column, weight, rowPtr[column], rowPtr[column + 1]} records of every
+// objective, laid out as pair_edges_kernel's.
+__global__ void records4_kernel(const int *rowPtr, int n, const int *col, const int *w,
+                                long long stride, int K, int4 *edge) {
+  const long long m = rowPtr[n];
+  for (long long e = blockIdx.x * static_cast<long long>(blockDim.x) + threadIdx.x; e < m;
+       e += static_cast<long long>(gridDim.x) * blockDim.x) {
+    const int c = col[e];
+    const int b = rowPtr[c], end = rowPtr[c + 1];
+    for (int k = 0; k < K; ++k) {
+      edge[k * stride + e] = make_int4(c, w[k * stride + e], b, end);
+    }
+  }
+}
+
 __global__ void copy_ints_kernel(int *dst, const int *src, int count) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < count) {
@@ -912,8 +927,11 @@ void release_arrays(mosp_device_graph *g) {
   (void)hipFree(g->inW);
   (void)hipFree(g->outEdge);
   (void)hipFree(g->inEdge);
+  (void)hipFree(g->outEdge4);
   g->outRow = g->outCol = g->outW = g->inRow = g->inCol = g->inW = nullptr;
   g->outEdge = g->inEdge = nullptr;
+  g->outEdge4 = nullptr;
+  g->rec4Valid = false;
   g->stride = 0;
   g->rowCap = 0;
 }
@@ -1076,6 +1094,7 @@ int pair_edges(mosp_device_graph *g, long long mBound, hipStream_t stream) {
   hipLaunchKernelGGL(pair_edges_kernel, dim3(blocks), dim3(BLOCK), 0, stream, g->inRow, g->n,
                      g->inCol, g->inW, g->stride, g->K, g->inEdge);
   HIP_TRY(hipGetLastError());
+  g->rec4Valid = false;
   return 1;
 }
 
@@ -1442,6 +1461,22 @@ extern "C" int mosp_device_graph_apply(const mosp_device_graph *g, int32_t nIns,
 
 namespace mosp {
 namespace detail {
+
+int graph_records4(const mosp_device_graph *g, hipStream_t stream) {
+  if (g->rec4Valid) {
+    return 1;
+  }
+  if (g->outEdge4 == nullptr) {
+    HIP_TRY(hipMalloc(&g->outEdge4, sizeof(int4) * static_cast<size_t>(g->stride) * g->K));
+  }
+  const long long m = std::max(g->m, 1);
+  const int blocks = static_cast<int>(std::min<long long>((m + BLOCK - 1) / BLOCK, 8192));
+  hipLaunchKernelGGL(records4_kernel, dim3(blocks), dim3(BLOCK), 0, stream, g->outRow, g->n,
+                     g->outCol, g->outW, g->stride, g->K, g->outEdge4);
+  HIP_TRY(hipGetLastError());
+  g->rec4Valid = true;
+  return 1;
+}
 
 void graph_edges(const mosp_device_graph *g, int k, const int2 **out, const int2 **in) {
   const size_t off = static_cast<size_t>(k) * g->stride;

@@ -122,6 +122,7 @@ struct mosp_ws {
   int cK = 0;
   int *cDegree = nullptr, *cRowPtr = nullptr, *cColInd = nullptr,
       *cWeights = nullptr, *cPartials = nullptr;
+  int4 *cEdge4 = nullptr;   // combined records with row ranges (asynchronous loop)
   double clockKhz = 100000;  // wall_clock64 rate
   int maxBlocks = 0;        // cooperative residency limit (1024-thread shape)
   int maxBlocks512 = 0;     // ... of the 512-thread shape
@@ -139,6 +140,18 @@ struct mosp_ws {
   double barrierMs = -1;    // longest grid-barrier wait before a launch gives up (< 0: default)
   bool scrub = false;       // a launch gave up: clear the scratch before the next
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  // Asynchronous near-far (light single-update kernel): its counters, the
+  // entries of each pool segment of the slab's states, the spill thresholds.
+  AsyncCtl *dAsync = nullptr;
+  bool async = true;
+  unsigned segCap = 0;
+  unsigned spillHi = 0, spillLo = 0;  // 0: from the queue capacity
+  // Defaults measured on a 1.96M-vertex road grid (K = 3): spills to XCDs
+  // with idle workgroups on, listed words re-read (carrying them raised the
+  // work by 12 % for no net gain).
+  unsigned asyncFlags = ASYNC_HUNGRY;
+  bool lastAsync = false;  // the last launch ran the asynchronous loop
+  unsigned keepHungry = 0;            // 0: one batch (BLOCK >> laneBits)
 };
 
 namespace mosp {
@@ -150,12 +163,15 @@ void ws_release_slab(mosp_ws *ws) {
   ws->state.clear();
   ws->capacity = 0;
   ws->slots = 0;
+  ws->segCap = 0;
 }
 
 void ws_release_combined(mosp_ws *ws) {
   (void)hipFree(ws->cDegree);
   (void)hipFree(ws->cRowPtr);
   (void)hipFree(ws->cColInd);  // cWeights lies in the same allocation
+  (void)hipFree(ws->cEdge4);
+  ws->cEdge4 = nullptr;
   ws->cDegree = ws->cRowPtr = ws->cColInd = ws->cWeights = nullptr;
   ws->cCapacity = 0;
   ws->cK = 0;
@@ -211,6 +227,25 @@ int ws_init(mosp_ws *ws) {
   }
   local_hops_override(ws->hopsUpdate, ws->hopsSearch);
   ws->listCap = list_cap_override();
+  // MOSP_ASYNC=0: the iteration-synchronous near-far loop everywhere;
+  // MOSP_ASYNC_SPILL=hi,lo: the local queue's spill thresholds.
+  if (const char *env = std::getenv("MOSP_ASYNC")) {
+    ws->async = std::atoi(env) != 0;
+  }
+  if (const char *env = std::getenv("MOSP_ASYNC_SPILL")) {
+    unsigned hi = 0, lo = 0;
+    if (std::sscanf(env, "%u,%u", &hi, &lo) == 2 && lo < hi) {
+      ws->spillHi = hi;
+      ws->spillLo = lo;
+    }
+  }
+  if (const char *env = std::getenv("MOSP_ASYNC_FLAGS")) {
+    ws->asyncFlags = static_cast<unsigned>(std::atoi(env)) & (ASYNC_CARRY | ASYNC_HUNGRY);
+  }
+  if (const char *env = std::getenv("MOSP_ASYNC_KEEP")) {
+    ws->keepHungry = static_cast<unsigned>(std::max(0, std::atoi(env)));
+  }
+  HIP_TRY(hipMalloc(&ws->dAsync, sizeof(AsyncCtl)));
   HIP_TRY(hipMalloc(&ws->dDesc, sizeof(ObjDesc) * (MOSP_MAX_OBJ + 1)));
   HIP_TRY(hipHostMalloc(&ws->hDesc, sizeof(ObjDesc) * (MOSP_MAX_OBJ + 1),
                         hipHostMallocDefault));
@@ -250,13 +285,33 @@ int ws_warm_up(mosp_ws *ws) {
   return run_launch(ws, args, 0, nullptr, nullptr);
 }
 
-size_t state_bytes(int n) {
-  const size_t count = static_cast<size_t>(std::max(n, 1));
-  return align256(count * sizeof(u64)) + 8 * align256(count * sizeof(int)) +
-         align256(2 * static_cast<size_t>(HUB_LIST) * sizeof(int));
+// Entries of one pool segment for graphs of up to n vertices: the segments
+// hold 2n entries in all (a spill that finds its segment full keeps its
+// entries in the local queue).  MOSP_POOL_CAP=c: c entries (tests).
+unsigned pool_seg_cap(int n) {
+  if (const char *env = std::getenv("MOSP_POOL_CAP")) {
+    const int cap = std::atoi(env);
+    if (cap > 0) {
+      return static_cast<unsigned>(cap);
+    }
+  }
+  const size_t entries = 2 * static_cast<size_t>(std::max(n, 1));
+  return static_cast<unsigned>(std::max<size_t>(256, (entries + ASYNC_SEGS - 1) / ASYNC_SEGS));
 }
 
-ObjState carve_state(char *&p, int n) {
+size_t state_bytes(int n, unsigned segCap) {
+  const size_t count = static_cast<size_t>(std::max(n, 1));
+  return align256(count * sizeof(u64)) + 8 * align256(count * sizeof(int)) +
+         align256(2 * static_cast<size_t>(HUB_LIST) * sizeof(int)) +
+         align256(static_cast<size_t>(ASYNC_SEGS) * segCap * sizeof(int));
+}
+
+size_t pool_bytes(unsigned segCap) {
+  return static_cast<size_t>(ASYNC_SEGS) * segCap * sizeof(int);
+}
+
+// segCap 0: no pools (st.pool stays null).
+ObjState carve_state(char *&p, int n, unsigned segCap) {
   const size_t count = static_cast<size_t>(std::max(n, 1));
   const size_t wordBytes = align256(count * sizeof(u64));
   const size_t intBytes = align256(count * sizeof(int));
@@ -271,7 +326,21 @@ ObjState carve_state(char *&p, int n) {
   }
   st.hubs = reinterpret_cast<int *>(p);
   p += align256(2 * static_cast<size_t>(HUB_LIST) * sizeof(int));
+  if (segCap > 0) {
+    st.pool = reinterpret_cast<int *>(p);
+    p += align256(pool_bytes(segCap));
+  }
   return st;
+}
+
+// Pool slots are -1 when free.
+int fill_pools(mosp_ws *ws, hipStream_t stream) {
+  for (const ObjState &st : ws->state) {
+    if (st.pool != nullptr) {
+      HIP_TRY(hipMemsetAsync(st.pool, 0xff, pool_bytes(ws->segCap), stream));
+    }
+  }
+  return 1;
 }
 
 int ws_reserve(mosp_ws *ws, int capacity, int slots) {
@@ -287,17 +356,23 @@ int ws_reserve(mosp_ws *ws, int capacity, int slots) {
   slots = std::max(slots, ws->slots);
   ws_release_slab(ws);
   const int n = std::max(capacity, 1);
-  const size_t perSlot = state_bytes(n);
+  const unsigned segCap = pool_seg_cap(n);
+  const size_t perSlot = state_bytes(n, segCap);
   HIP_TRY(hipMalloc(&ws->slab, perSlot * slots));
   // stamps, far flags and invalidation marks must start at zero
   HIP_TRY(hipMemset(ws->slab, 0, perSlot * slots));
   char *p = static_cast<char *>(ws->slab);
   ws->state.resize(slots);
   for (int s = 0; s < slots; ++s) {
-    ws->state[s] = carve_state(p, n);
+    ws->state[s] = carve_state(p, n, segCap);
   }
   ws->capacity = capacity;
   ws->slots = slots;
+  ws->segCap = segCap;
+  if (!fill_pools(ws, nullptr)) {
+    return 0;
+  }
+  HIP_TRY(hipStreamSynchronize(nullptr));
   return 1;
 }
 
@@ -313,6 +388,9 @@ int ws_reserve_combined(mosp_ws *ws, int n, int K) {
   // staged words per vertex of the fused parent recovery (LaunchArgs::stage).
   HIP_TRY(hipMalloc(&ws->cColInd, 2 * nn * K * sizeof(int)));
   ws->cWeights = ws->cColInd + nn * K;
+  if (ws->async) {
+    HIP_TRY(hipMalloc(&ws->cEdge4, nn * K * sizeof(int4)));
+  }
   ws->cCapacity = n;
   ws->cK = K;
   return 1;
@@ -455,8 +533,12 @@ int run_launch(mosp_ws *ws, LaunchArgs &args, int descCount,
     // The last launch ended early: its stamps, far marks and invalidation
     // marks are not in the state the next launch relies on.
     if (ws->slab != nullptr) {
-      HIP_TRY(hipMemsetAsync(ws->slab, 0, state_bytes(std::max(ws->capacity, 1)) * ws->slots,
+      HIP_TRY(hipMemsetAsync(ws->slab, 0,
+                             state_bytes(std::max(ws->capacity, 1), ws->segCap) * ws->slots,
                              stream));
+      if (!fill_pools(ws, stream)) {
+        return 0;
+      }
     }
     HIP_TRY(hipMemsetAsync(ws->genDev, 0, sizeof(int), stream));
     ws->scrub = false;
@@ -474,6 +556,17 @@ int run_launch(mosp_ws *ws, LaunchArgs &args, int descCount,
   args.hubEdges = hub_edges_override();
   args.cPartials = ws->cPartials;
   args.cDeltaScale = ws->combDeltaScale;
+  // The asynchronous near-far loop runs in the light single-update kernel
+  // when the workspace has pools; spill thresholds from the queue capacity.
+  const unsigned qcap = ws->queueCap >= 256 ? static_cast<unsigned>(ws->queueCap)
+                                            : static_cast<unsigned>(MOSP_QUEUE_CAP_MAX);
+  args.async = nullptr;
+  args.segCap = ws->segCap;
+  args.spillHi = ws->spillHi > 0 ? std::min(ws->spillHi, qcap - 1) : qcap / 4;
+  args.spillLo = ws->spillHi > 0 ? std::min(ws->spillLo, args.spillHi - 1) : qcap / 8;
+  args.asyncFlags = ws->asyncFlags;
+  args.keepHungry = ws->keepHungry > 0 ? ws->keepHungry
+                                       : static_cast<unsigned>(BLOCK >> std::max(0, ws->laneBits));
   if (descCount > 0) {
     HIP_TRY(hipMemcpyAsync(ws->dDesc, ws->hDesc, sizeof(ObjDesc) * descCount,
                            hipMemcpyHostToDevice, stream));
@@ -489,6 +582,10 @@ int run_launch(mosp_ws *ws, LaunchArgs &args, int descCount,
   const KernelShape &shape = kernel_shape(ws->shapeThreads > 0 ? ws->shapeThreads
                                                                 : (hubs ? 512 : 1024));
   const int grid = launch_grid(ws, shape);
+  if (ws->async && !hubs && ws->segCap > 0 && !ws->state.empty() && grid <= ASYNC_MAX_GRID) {
+    args.async = ws->dAsync;
+  }
+  ws->lastAsync = args.async != nullptr;
   HIP_TRY(launch_persistent(shape.persistent(hubs), dim3(grid), dim3(shape.threads), params,
                             stream));
   HIP_TRY(hipEventRecord(ws->ev1, stream));
@@ -505,6 +602,26 @@ int run_launch(mosp_ws *ws, LaunchArgs &args, int descCount,
     const double tick = 1e3 / static_cast<double>(ws->clockKhz);  // us
     const double G = static_cast<double>(ws->gridBlocks);
     const double itU = std::max(1.0, h.prof[11] / G), itC = std::max(1.0, h.prof[15] / G);
+    if (args.async != nullptr) {
+      // Asynchronous near-far: per workgroup and launch (ticks -> us).
+      const double ph = std::max(1.0, h.prof[11] / G);
+      std::fprintf(stderr,
+                   "asyncprof grid=%d barriers=%u phases %.1f: busy %.1f idle %.1f us, phase "
+                   "body %.1f us, barrier %.1f us (per wg, summed over phases); per wg: local "
+                   "batches %.1f (%.1f entries each), pool batches %.1f (%.1f each), seed "
+                   "passes %.1f, idle polls %.1f, spills %.1f (%.1f entries each); lost "
+                   "events %llu; per batch: decide %.2f relax %.2f rest %.2f us\n",
+                   ws->gridBlocks, h.barriers, ph, h.prof[0] * tick / G, h.prof[1] * tick / G,
+                   h.prof[12] * tick / G, h.prof[10] * tick / G, h.prof[4] / G,
+                   h.prof[2] / std::max(1.0, static_cast<double>(h.prof[4])), h.prof[5] / G,
+                   h.prof[3] / std::max(1.0, static_cast<double>(h.prof[5])), h.prof[15] / G,
+                   h.prof[7] / G, h.prof[8] / G,
+                   h.prof[9] / std::max(1.0, static_cast<double>(h.prof[8])), h.prof[13],
+                   h.prof[6] * tick / std::max(1.0, static_cast<double>(h.prof[4] + h.prof[5] + h.prof[15])),
+                   h.prof[14] * tick / std::max(1.0, static_cast<double>(h.prof[4] + h.prof[5] + h.prof[15])),
+                   (h.prof[0] - h.prof[6] - h.prof[14]) * tick /
+                       std::max(1.0, static_cast<double>(h.prof[4] + h.prof[5] + h.prof[15])));
+    }
     std::fprintf(stderr,
                  "subprof grid=%d barriers=%u update (%.0f iters): snapshot %.2f work %.2f "
                  "wait %.2f release %.2f us/iter; push loop %.2f flush %.2f us/call (%.0f calls); "
@@ -519,6 +636,11 @@ int run_launch(mosp_ws *ws, LaunchArgs &args, int descCount,
 #endif
   if (kernelMs != nullptr) {
     *kernelMs = ms;
+  }
+  static const bool asyncDebug = std::getenv("MOSP_ASYNC_DEBUG") != nullptr;
+  if (asyncDebug && args.async != nullptr) {
+    std::fprintf(stderr, "async events: lost batches %u, short spills %u, sweep phases %u\n",
+                 ws->hCtl->asyncLost, ws->hCtl->asyncShortSpills, ws->hCtl->asyncSweeps);
   }
   if (ws->hCtl->gaveUp != 0) {
     ws->scrub = true;
@@ -636,6 +758,7 @@ extern "C" void mosp_ws_destroy(mosp_ws *ws) {
   (void)hipFree(ws->genDev);
   (void)hipFree(ws->degreeWord);
   (void)hipFree(ws->cPartials);
+  (void)hipFree(ws->dAsync);
   if (ws->ev0) (void)hipEventDestroy(ws->ev0);
   if (ws->ev1) (void)hipEventDestroy(ws->ev1);
   delete ws;
@@ -680,6 +803,19 @@ extern "C" int mosp_ws_set_combined_delta_scale(mosp_ws *ws, int32_t scale) {
     return fail("Error: the combined width scale must be >= 1.");
   }
   ws->combDeltaScale = scale;
+  return 1;
+}
+
+extern "C" int mosp_ws_async_events(const mosp_ws *ws, int32_t events[4]) {
+  clear_error();
+  if (ws == nullptr || events == nullptr) {
+    return fail("Error: missing workspace or event array.");
+  }
+  const Ctl *c = ws->hCtl;
+  events[0] = c != nullptr ? static_cast<int32_t>(c->asyncLost) : 0;
+  events[1] = c != nullptr ? static_cast<int32_t>(c->asyncShortSpills) : 0;
+  events[2] = c != nullptr ? static_cast<int32_t>(c->asyncSweeps) : 0;
+  events[3] = ws->lastAsync ? 1 : 0;
   return 1;
 }
 
@@ -920,12 +1056,14 @@ int setup_combined(mosp_ws *ws, LaunchArgs &args, const int32_t *d_parents,
   args.cColInd = ws->cColInd;
   args.cWeights = ws->cWeights;
   args.cEdge = reinterpret_cast<int2 *>(ws->cColInd);  // K * n records over both arrays
+  args.cEdge4 = ws->cEdge4;
   if (!choose_packing(n, base, pc)) {
     return 0;
   }
   mosp_csr view{n, 0, ws->cRowPtr, nullptr, nullptr};
   ObjDesc d = make_desc(view, nullptr, nullptr, pc, 0, ws->state[0]);
   d.outEdge = args.cEdge;
+  d.outEdge4 = args.cEdge4;
   d.distOut = d_dist;
   d.parentOut = d_parent;
   ws->hDesc[descIndex] = d;
@@ -1158,6 +1296,12 @@ extern "C" int mosp_update_fused(const mosp_device_graph *g, int32_t K,
   }
   // The K objectives' descriptors; `optimistic`: packed words also where
   // only their own limit bounds the distances.
+  // Graphs without hubs run the asynchronous loop: it reads the records
+  // with row ranges.
+  const bool rec4 = ws->async && g->maxDegree <= HUB_DEGREE;
+  if (rec4 && !graph_records4(g, as_stream(stream))) {
+    return 0;
+  }
   auto describe = [&](bool optimistic, int &anyFallback, int &anyOptimistic) {
     anyFallback = anyOptimistic = 0;
     for (int k = 0; k < K; ++k) {
@@ -1170,6 +1314,9 @@ extern "C" int mosp_update_fused(const mosp_device_graph *g, int32_t K,
       ObjDesc d = make_desc(out, &in, &changes[k], pc, static_cast<u64>(delta[k]),
                             ws->state[k]);
       graph_edges(g, k, &d.outEdge, &d.inEdge);
+      if (rec4) {
+        d.outEdge4 = g->outEdge4 + static_cast<size_t>(k) * g->stride;
+      }
       const size_t off = static_cast<size_t>(k) * n;
       d.distIn = distIn + off;
       d.parentIn = parIn + off;
@@ -1352,7 +1499,7 @@ extern "C" int mosp_sosp_update_noalloc(const mosp_csr *out, const mosp_csr *in,
   int *genDev = reinterpret_cast<int *>(base + NA_WORDS);
   unsigned *degreeWord = reinterpret_cast<unsigned *>(base + NA_WORDS + 128);
   char *p = base + NA_STATE;
-  const ObjState st = carve_state(p, NOALLOC_MAX_VERTICES);
+  const ObjState st = carve_state(p, NOALLOC_MAX_VERTICES, 0);
 
   // Variant choice as in mosp_sosp_update: any vertex of more than
   // HUB_DEGREE out- or in-edges.

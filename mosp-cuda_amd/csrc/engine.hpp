@@ -72,6 +72,9 @@ struct ObjState {
   int *near[2];    // near frontiers (ping-pong), n each
   int *far[2];     // far piles (ping-pong), n each
   int *hubs;       // grid-relaxed hubs of the next iteration (ping-pong), 2 * HUB_LIST
+  int *pool;       // asynchronous near-far: ASYNC_SEGS shared pool segments of
+                   // LaunchArgs::segCap entries each, all -1 between launches
+                   // (null: the workspace has no pools)
 };
 
 // One objective of a launch (device-resident descriptor).
@@ -82,6 +85,10 @@ struct ObjDesc {
   // when the graph holds them (device graphs, the combined graph); null:
   // the separate arrays above are read.
   const int2 *outEdge, *inEdge;
+  // The out-edges as {column, weight, rowBegin(column), rowEnd(column)}
+  // (asynchronous near-far: a vertex listed by a relaxation carries its own
+  // edge range), or null.
+  const int4 *outEdge4;
   const int *chFrom, *chTo;           // deleted / weight-increased edges
   int nChanged;
   const int *heads;                   // heads of inserted edges
@@ -115,6 +122,9 @@ struct ObjCtl {
   unsigned hubCount[3];    // grid-relaxed hubs listed for an iteration, slot = iteration % 3
   unsigned dropped;        // optimistic packing: a candidate or input distance was past
                            // maxDistance (the launch stops before writing results)
+  unsigned lost[2];        // asynchronous near-far: a near entry of phase p found no room
+                           // (slot p % 2); the next phase lists every vertex under the
+                           // threshold again
   u64 pushes;
 };
 
@@ -146,6 +156,10 @@ struct Ctl {
   u64 clk[16];             // phase ends (PHASE_* below), wall_clock64
   unsigned barriers;       // grid barriers executed by the launch
   unsigned gaveUp;         // a workgroup's barrier wait ran out (the launch ended early)
+  // Asynchronous near-far events: batches that dropped near entries (full
+  // local queue), spills that found less room in their pool segment than
+  // asked, phases that listed every vertex again.
+  unsigned asyncLost, asyncShortSpills, asyncSweeps;
   u64 prof[16];            // sub-phase clock sums (MOSP_SUBPROFILE builds)
   ObjCtl obj[MOSP_MAX_OBJ + 1];
   // Barrier mailboxes (device-only: not part of the host read-back).
@@ -154,6 +168,33 @@ struct Ctl {
 
 // Bytes of Ctl the host reads back after a launch (everything but mail).
 constexpr size_t CTL_READBACK = offsetof(Ctl, mail);
+
+// Asynchronous near-far (the single-update kernel without hub handling):
+// between two grid barriers each workgroup relaxes the vertices of a local
+// LDS queue, spills its surplus into shared pools and takes work from them
+// when its queue runs dry; a phase ends when every appended entry has been
+// processed (counted per workgroup, checked by workgroup 0).
+constexpr int ASYNC_SEGS = 8;         // pool segments per objective (one per XCD)
+constexpr u64 ASYNC_STATUS_INIT = 15ull << 60;  // a status word of no phase
+// LaunchArgs::asyncFlags
+constexpr unsigned ASYNC_CARRY = 1;   // local entries carry the word they were listed with
+constexpr unsigned ASYNC_HUNGRY = 2;  // spill to XCDs with idle workgroups
+constexpr int ASYNC_MAX_GRID = 2048;  // workgroups a launch with pools may have
+
+struct AsyncCtl {
+  // Per workgroup: phase (8 bits) | entries appended (28) | entries processed
+  // (28), cumulative over the launch; written only by its workgroup.
+  u64 status[ASYNC_MAX_GRID];
+  // Pool words {tail << 32 | head}, one 128-byte line each; two sets, used
+  // by alternate phases (a set is cleared while the other is in use).
+  u64 pool[2][MOSP_MAX_OBJ][ASYNC_SEGS][LINE_WORDS / 2];
+  // Phase-done mailboxes: the launch's phase sequence number + 1 once that
+  // phase ended; one line per barrier group.
+  unsigned done[BARRIER_GROUPS][LINE_WORDS];
+  // Idle workgroups per XCD (two sets, as the pool words): busy workgroups
+  // spill into the segment of an XCD that has some.
+  unsigned hungry[2][ASYNC_SEGS][LINE_WORDS];
+};
 
 struct LaunchArgs {
   const ObjDesc *objs;     // nobj descriptors (device memory)
@@ -177,6 +218,14 @@ struct LaunchArgs {
                            // the objective's hub list (HUB_GRID; tests lower it)
   int *genDev;             // device-resident stamp generation (persists)
   Ctl *ctl;
+  // Asynchronous near-far (kernels without hub handling, single update):
+  // its counters, or null for the iteration-synchronous loop; entries per
+  // pool segment; spill thresholds of the local queue (entries).
+  AsyncCtl *async;
+  unsigned segCap;
+  unsigned spillHi, spillLo;
+  unsigned asyncFlags;     // ASYNC_CARRY | ASYNC_HUNGRY
+  unsigned keepHungry;     // entries a spill to idle workgroups leaves in the local queue
   // Bounded barrier waits: a workgroup that waits longer than barrierTicks
   // (wall_clock64 ticks) at a grid barrier, or finds *abortWord set while it
   // waits, sets *abortWord and ends; so does every other workgroup at its
@@ -196,6 +245,8 @@ struct LaunchArgs {
   int *cColInd;            // K * n
   int *cWeights;           // K * n
   int2 *cEdge;             // K * n {column, weight} records (the memory of cColInd + cWeights)
+  int4 *cEdge4;            // K * n records with the column's row range as well, written when
+                           // the launch runs the asynchronous loop (else null)
   int *cPartials;          // one per workgroup (scan)
   u64 *stage;              // nobj * n words (may alias cColInd..cWeights, which are
                            // written only after it is read): the fused parent

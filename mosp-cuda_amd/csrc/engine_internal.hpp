@@ -24,6 +24,13 @@ struct mosp_device_graph {
   // Objective k's edges as {column, weight} records at k * stride, out and
   // in (built after every upload and apply).
   int2 *outEdge = nullptr, *inEdge = nullptr;
+  // Objective k's out-edges as {column, weight, rowBegin(column),
+  // rowEnd(column)} at k * stride, for the asynchronous near-far loop of
+  // graphs without hubs; built on demand (graph_records4) and stale after
+  // every upload and apply.  A cache of the arrays above, so it may be
+  // built through a const graph (one launch on a graph at a time).
+  mutable int4 *outEdge4 = nullptr;
+  mutable bool rec4Valid = false;
   char *scratch = nullptr;  // build / apply temporaries (grow-only)
   size_t scratchBytes = 0;
   unsigned long long *stats = nullptr;  // weight statistics (2 * MOSP_MAX_OBJ words) + degree
@@ -144,8 +151,10 @@ ObjDesc make_desc(const mosp_csr &out, const mosp_csr *in,
                   const ObjState &st);
 // Carves the scratch arrays of one objective for n vertices out of p
 // (advanced past them); state_bytes(n) is their size.
-size_t state_bytes(int n);
-ObjState carve_state(char *&p, int n);
+// The out-edge records with row ranges of g (built when stale).
+int graph_records4(const mosp_device_graph *g, hipStream_t stream);
+size_t state_bytes(int n, unsigned segCap = 0);
+ObjState carve_state(char *&p, int n, unsigned segCap = 0);
 
 }  // namespace detail
 }  // namespace mosp

@@ -73,6 +73,8 @@ __device__ __forceinline__ T *uniform_global(T *p) {
 }
 
 constexpr unsigned QCAP = MOSP_QUEUE_CAP_MAX;  // LDS queue entries per list per workgroup
+// Asynchronous near-far: local queue entries (a power of 2; 20 bytes each).
+constexpr unsigned LQ_CAP = BLOCK >= 1024 ? 2048 : 1024;
 
 // ---------------------------------------------------------------------------
 // Device helpers
@@ -88,6 +90,12 @@ __device__ __forceinline__ void store_relaxed(unsigned *p, unsigned v) {
   __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 __device__ __forceinline__ void store_relaxed(u64 *p, u64 v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ int load_relaxed(const int *p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void store_relaxed(int *p, int v) {
   __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
@@ -558,8 +566,39 @@ struct Shared {
   int aggKey[AGG_SLOTS];         // combined count (hub variant): parent ids ...
   int aggCount[AGG_SLOTS];       // ... and their increments in this workgroup
   u64 redMin[WAVES];             // workgroup minimum
-  BlockQueue near;
+  union {
+    BlockQueue near;
+    // Asynchronous near-far: the local queue, a ring of vertices and the
+    // words they were listed with.
+    struct {
+      int v[LQ_CAP];
+      int b[LQ_CAP];   // first out-edge, or -1: read the row pointers
+      int e[LQ_CAP];   // end of the out-edges
+      u64 w[LQ_CAP];
+    } lq;
+  };
   BlockQueue far;
+  // Asynchronous near-far: the local queue is the ring lq.
+  struct {
+    unsigned lqHead, lqTail;     // entries [lqHead, lqTail) are listed (ring indices mod LQ_CAP)
+    int cur;                     // objective of the local queue and of the far queue
+    int step;                    // what the workgroup does next (STEP_*)
+    unsigned count;              // entries of the current batch
+    int seg;                     // pool segment of a pool batch ...
+    unsigned base;               // ... and its first slot
+    unsigned sent, recv;         // entries this workgroup appended / processed (status word)
+    int seedIdx;                 // index into mine[] of the objective being seeded
+    unsigned seedPos, seedEnd;   // the rest of this workgroup's share of its seeds
+    int exit;                    // the phase is over
+    int lostNear;                // a near entry found no room in this batch
+    unsigned polls;              // consecutive idle polls
+    unsigned batches;
+    u64 farMin;                  // smallest distance in the far queue since its last flush
+    unsigned spillBase, spillKeep;
+    int spillSeg;                // pool segment of the spill
+    int obj;                     // objective of the entries a pool batch took
+    int hungry;                  // this workgroup counts in its XCD's hungry word
+  } as;
 };
 
 // Split the workgroups over the active objectives: one each, the rest in
@@ -1208,6 +1247,759 @@ __device__ int near_far(Desc *objsArg, int nobj, ObjCtl *ctlsArg,
     atomicAdd(&ctls[k].epochs, sh.statEpoch[k]);
     atomicAdd(&ctls[k].pushes, sh.statPush[k]);
   }
+  return generation;
+}
+
+// ---------------------------------------------------------------------------
+// Asynchronous near-far (Step 2 and the searches of the single-update kernel
+// without hub handling).  In near_far() every iteration ends at a grid
+// barrier, and on a road grid ~40 % of an iteration goes to waiting for the
+// last workgroup and the release.  Here only the threshold raises (epochs)
+// are grid-synchronous.  Within the phase between two barriers:
+//   * each workgroup relaxes batches of up to perPass vertices taken from
+//     its own LDS queue (a FIFO ring); the vertices a batch improves below
+//     the threshold are appended to that queue, those at or above it to the
+//     far pile, as in near_far();
+//   * a workgroup whose queue holds more than spillHi entries moves its
+//     newest entries down to spillLo into its XCD's segment of the
+//     objective's shared pool (one atomic reservation); a workgroup whose
+//     queue is empty takes up to perPass entries from a pool (load, then a
+//     compare-and-swap of the pool word), its XCD's segment first;
+//   * pool slots hold -1 when free: a taker waits (bounded) until its slot
+//     is written, and writes -1 back, so the pools are free again when the
+//     phase ends;
+//   * the word of a listed vertex is read with a device-scope load: the
+//     vertex was listed after an atomicMin improved it, and the relaxation
+//     must start from at least that word (a stale copy of its line in this
+//     XCD's L2 would lose the improvement);
+//   * termination: every workgroup publishes (phase, entries appended,
+//     entries processed) after each step; an appended entry is counted
+//     before another workgroup can take it (the pool reservation is issued
+//     after the exchange that published the count has returned).  Idle,
+//     workgroup 0 reads every status word twice; if all carry this phase,
+//     the sums did not change between the reads and appended == processed,
+//     then at an instant between the reads no entry was left -- and none can
+//     be appended any more -- so the phase is over, which it writes to 16
+//     mailboxes that idle workgroups poll.  Each workgroup's seeding token
+//     (one entry appended at the start of the phase, processed once its
+//     share of the seeds is listed) keeps the phase open until all seeds are
+//     listed;
+//   * a near entry that finds no room (full LDS queue) is dropped and the
+//     objective's lost flag set: the next phase lists every vertex under the
+//     threshold again (relaxing a vertex again is always valid).
+// Results are those of near_far(): every vertex improved below the
+// threshold is relaxed again after its last improvement, with a word at
+// least as good, in whatever order.
+// ---------------------------------------------------------------------------
+
+enum : int { MODE_SWEEP = 3 };  // list every vertex under the threshold again
+enum : int { STEP_SEED = 0, STEP_LOCAL = 1, STEP_CLAIM = 2, STEP_POOL = 3, STEP_IDLE = 4 };
+
+// Status word: phase (4 bits: the phase sequence number mod 15) | appended
+// (30) | processed (30).  Phase 15 never occurs: launches start every
+// workgroup's word with it (ASYNC_STATUS_INIT).
+__device__ __forceinline__ unsigned status_phase(unsigned seq) { return seq % 15u; }
+__device__ __forceinline__ u64 status_word(unsigned seq, unsigned sent, unsigned recv) {
+  return (static_cast<u64>(status_phase(seq)) << 60) |
+         (static_cast<u64>(sent & 0x3fffffffu) << 30) | static_cast<u64>(recv & 0x3fffffffu);
+}
+
+__device__ __forceinline__ u64 *pool_word(AsyncCtl *A, unsigned seq, int k, int seg) {
+  return &A->pool[seq & 1][k][seg][0];
+}
+
+// Workgroup-uniform: publish the far queue of objective k (its pile of the
+// epoch it appends to) and its distance minimum.
+__device__ void async_flush_far(Desc *objs, ObjCtl *ctls, Shared &sh, int k, unsigned listCap,
+                                unsigned qcap) {
+  const int e = sh.epoch[k] + (sh.mode[k] == MODE_SPLIT ? 1 : 0);
+  queue_flush(sh.far, objs[k].s.far[e & 1], &ctls[k].farCount[e % 3], listCap, qcap);
+  if (threadIdx.x == 0 && sh.as.farMin != PACKED_INF) {
+    atomicMin(&ctls[k].farMin[e % 3], sh.as.farMin);
+    sh.as.farMin = PACKED_INF;
+  }
+  __syncthreads();
+}
+
+// Thread 0: this workgroup's share of objective k's seeds for the phase.
+__device__ void async_seed_share(Shared &sh, int k, unsigned blockRank, unsigned n,
+                                 unsigned listCap) {
+  const unsigned rank = blockRank - static_cast<unsigned>(sh.blockBegin[k]);
+  const unsigned blocks = static_cast<unsigned>(sh.blockEnd[k] - sh.blockBegin[k]);
+  unsigned count = sh.cnt[k];
+  if (sh.mode[k] == MODE_SWEEP || (sh.mode[k] == MODE_SPLIT && count > listCap)) {
+    count = n;
+  }
+  sh.as.seedPos = static_cast<unsigned>(static_cast<u64>(count) * rank / blocks);
+  sh.as.seedEnd = static_cast<unsigned>(static_cast<u64>(count) * (rank + 1) / blocks);
+}
+
+// near_far() without per-iteration barriers, for objectives objs[0, nobj):
+// same entry state (near list 0 or far pile 0, first threshold initThr),
+// same exit state (final words in the objectives' packed arrays).
+// phaseSeq counts the launch's phases (every workgroup the same).
+__device__ int near_far_async(Desc *objsArg, int nobj, ObjCtl *ctlsArg, AsyncCtl *Aarg,
+                              int source, int generation, u64 initThr,
+                              const u64 *deltaOverride, int laneBits, unsigned n,
+                              unsigned listCap, unsigned segCap, unsigned spillHi,
+                              unsigned spillLo, unsigned flags, unsigned keepHungry,
+                              unsigned *abortWord, unsigned &phaseSeq,
+                              GridBarrier &grid, Threads t, Shared &sh) {
+  Desc *const objs = uniform(objsArg);
+  Ctl *const ctlAll = uniform_global(grid.ctl);
+  ObjCtl *const ctls = uniform_global(ctlsArg);
+  AsyncCtl *const A = uniform_global(Aarg);
+  constexpr int MW = 4;  // per objective: far count, far minimum (two words), lost flag
+  const unsigned qcap = sh.queueCap;                      // far queue
+  const unsigned lcap = qcap < LQ_CAP ? qcap : LQ_CAP;    // local queue
+  const unsigned lanes = 1u << laneBits;
+  const unsigned perPass = BLOCK >> laneBits;
+  const unsigned sub = threadIdx.x & (lanes - 1);
+  const unsigned seedPass = lcap / 2 < static_cast<unsigned>(BLOCK) ? lcap / 2 : BLOCK;
+  spillHi = spillHi < lcap ? spillHi : lcap - 1;
+  spillLo = spillLo < spillHi ? spillLo : spillHi / 2;
+  const bool carry = (flags & ASYNC_CARRY) != 0;
+  const bool useHungry = (flags & ASYNC_HUNGRY) != 0;
+  const int mySeg = static_cast<int>(t.block % ASYNC_SEGS);
+  const u64 limit = grid.limit;
+  if (threadIdx.x < static_cast<unsigned>(nobj)) {
+    const int k = threadIdx.x;
+    sh.thr[k] = initThr;
+    sh.epoch[k] = 0;
+    sh.delta[k] = deltaOverride != nullptr ? *deltaOverride : objs[k].delta;
+    sh.statIter[k] = 0;
+    sh.statEpoch[k] = 0;
+    sh.statPush[k] = 0;
+  }
+  if (threadIdx.x == 0) {
+    sh.as.lqHead = sh.as.lqTail = 0;
+    sh.as.cur = 0;
+    sh.as.farMin = PACKED_INF;
+    sh.as.lostNear = 0;
+    sh.as.batches = 0;
+  }
+  __syncthreads();
+  for (bool first = true;; first = false) {
+    const unsigned seq = phaseSeq;
+    if (threadIdx.x == 0) {
+      sh.any = 0;
+    }
+    __syncthreads();
+    // Snapshot: every workgroup takes the same decisions.
+    if (threadIdx.x < static_cast<unsigned>(nobj)) {
+      const int k = threadIdx.x;
+      ObjCtl &c = ctls[k];
+      const int e = sh.epoch[k];
+      unsigned nc = 0, fc, lost = 0;
+      u64 fm;
+      if (first) {
+        nc = load_relaxed(&c.nearCount[0]);
+        fc = load_relaxed(&c.farCount[e % 3]);
+        fm = load_relaxed(&c.farMin[e % 3]);
+      } else {
+        fc = sh.snap[MW * k];
+        fm = static_cast<u64>(sh.snap[MW * k + 1]) | (static_cast<u64>(sh.snap[MW * k + 2]) << 32);
+        lost = sh.snap[MW * k + 3];
+      }
+      sh.hubs[k] = 0;
+      sh.newThr[k] = sh.thr[k];  // the threshold in force during the phase
+      if (nc > 0) {
+        sh.mode[k] = MODE_PUSH;  // the near list is the seed list
+        sh.cnt[k] = nc;
+        sh.any = 1;
+      } else if (lost != 0) {
+        sh.mode[k] = MODE_SWEEP;
+        sh.cnt[k] = n;
+        sh.any = 1;
+        if (t.block == 0) {
+          atomicAdd(&ctlAll->asyncSweeps, 1u);
+        }
+      } else if (fc > 0) {
+        const u64 base = fm > sh.thr[k] && fm != PACKED_INF ? fm : sh.thr[k];
+        sh.mode[k] = MODE_SPLIT;
+        sh.cnt[k] = fc;
+        sh.newThr[k] = base + sh.delta[k];
+        sh.any = 1;
+      } else {
+        sh.mode[k] = MODE_DONE;
+      }
+    }
+    __syncthreads();
+    if (!sh.any) {
+      break;
+    }
+    ++phaseSeq;
+    const int gen = ++generation;
+    if (threadIdx.x == 0) {
+      partition_blocks(sh, nobj, t.blocks);
+      int mine = 0;
+      for (int k = 0; k < nobj; ++k) {
+        if (sh.mode[k] != MODE_DONE && static_cast<int>(t.block) >= sh.blockBegin[k] &&
+            static_cast<int>(t.block) < sh.blockEnd[k]) {
+          sh.mine[mine++] = k;
+        }
+      }
+      sh.nmine = mine;
+    }
+    if (t.block == 0) {
+      // The other pool set, hungry words and lost slots were last used by the
+      // previous phase, which every workgroup has left: clear them for the
+      // next one.
+      for (unsigned i = threadIdx.x; i < static_cast<unsigned>(MOSP_MAX_OBJ) * ASYNC_SEGS; i += BLOCK) {
+        store_relaxed(pool_word(A, seq + 1, static_cast<int>(i / ASYNC_SEGS),
+                                static_cast<int>(i % ASYNC_SEGS)),
+                      0ull);
+      }
+      if (threadIdx.x < static_cast<unsigned>(ASYNC_SEGS)) {
+        store_relaxed(&A->hungry[(seq + 1) & 1][threadIdx.x][0], 0u);
+      }
+      if (threadIdx.x < static_cast<unsigned>(nobj)) {
+        const int k = threadIdx.x;
+        ObjCtl &c = ctls[k];
+        store_relaxed(&c.lost[(seq + 1) & 1], 0u);
+        if (sh.mode[k] == MODE_SPLIT) {
+          const int e2 = (sh.epoch[k] + 2) % 3;
+          store_relaxed(&c.farCount[e2], 0u);
+          store_relaxed(&c.farMin[e2], PACKED_INF);
+          sh.statEpoch[k] += 1;
+        }
+        if (sh.mode[k] != MODE_DONE) {
+          sh.statIter[k] += 1;
+        }
+      }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      sh.as.seedIdx = 0;
+      if (sh.nmine > 0) {
+        sh.as.cur = sh.mine[0];
+        async_seed_share(sh, sh.mine[0], t.block, n, listCap);
+      }
+      sh.as.sent += 1;  // the seeding token
+      if (sh.nmine == 0) {
+        sh.as.recv += 1;
+      }
+      sh.as.exit = 0;
+      sh.as.polls = 0;
+      sh.as.hungry = 0;
+      store_relaxed(&A->status[t.block], status_word(seq, sh.as.sent, sh.as.recv));
+    }
+    __syncthreads();
+    const u64 phaseStart = wall_clock64();
+    unsigned tailBefore = 0;  // thread 0
+    for (;;) {
+      SUBPROF_MARK(tStep);
+      if (threadIdx.x == 0) {
+        const unsigned have = sh.as.lqTail - sh.as.lqHead;
+        const bool seeding = sh.as.seedIdx < sh.nmine;
+        int step;
+        if (seeding && have < perPass && have + seedPass <= lcap &&
+            (have == 0 || sh.as.cur == sh.mine[sh.as.seedIdx])) {
+          step = STEP_SEED;
+        } else if (have > 0) {
+          step = STEP_LOCAL;
+          sh.as.count = have < perPass ? have : perPass;
+        } else if (seeding) {
+          step = STEP_SEED;
+        } else {
+          step = STEP_CLAIM;
+        }
+        sh.as.step = step;
+        tailBefore = sh.as.lqTail;
+      }
+      __syncthreads();
+      if (sh.as.step == STEP_SEED && sh.mine[sh.as.seedIdx] != sh.as.cur) {
+        async_flush_far(objs, ctls, sh, sh.as.cur, listCap, qcap);
+        if (threadIdx.x == 0) {
+          sh.as.cur = sh.mine[sh.as.seedIdx];
+        }
+        __syncthreads();
+      }
+      if (sh.as.step == STEP_CLAIM) {
+        // Wave 0 looks at two pool segments of every active objective --
+        // its XCD's (lanes 0-31), where spills for its idle workgroups land,
+        // and one other in turn (lanes 32-63) -- and takes up to perPass
+        // entries of one: its own XCD's first, the current objective's first.
+        if (threadIdx.x < 64) {
+          const int lane = static_cast<int>(threadIdx.x);
+          const int kk = lane & 31;
+          const int seg = lane < 32 ? mySeg
+                                    : static_cast<int>((mySeg + 1 + sh.as.polls % (ASYNC_SEGS - 1)) %
+                                                       ASYNC_SEGS);
+          unsigned avail = 0, head = 0;
+          u64 word = 0;
+          u64 *pw = nullptr;
+          if (kk < nobj && sh.mode[kk] != MODE_DONE) {
+            pw = pool_word(A, seq, kk, seg);
+            word = load_relaxed(pw);
+            head = static_cast<unsigned>(word);
+            unsigned tail = static_cast<unsigned>(word >> 32);
+            tail = tail < segCap ? tail : segCap;
+            avail = tail > head ? tail - head : 0;
+          }
+          const u64 some = __ballot(avail > 0);
+          const int cur = sh.as.cur;
+          int pick = -1;
+          if ((some >> cur) & 1ull) {
+            pick = cur;
+          } else if ((some & 0xffffffffull) != 0) {
+            pick = __ffsll(static_cast<long long>(some & 0xffffffffull)) - 1;
+          } else if ((some >> (32 + cur)) & 1ull) {
+            pick = 32 + cur;
+          } else if (some != 0) {
+            pick = __ffsll(static_cast<long long>(some)) - 1;
+          }
+          if (lane == 0) {
+            sh.as.step = STEP_IDLE;
+          }
+          if (lane == pick) {
+            const unsigned c = avail < perPass ? avail : perPass;
+            const u64 old = atomicCAS(pw, word, word + c);
+            if (old == word) {
+              sh.as.step = STEP_POOL;
+              sh.as.count = c;
+              sh.as.seg = seg;
+              sh.as.base = head;
+              sh.as.obj = kk;
+            }
+          }
+        }
+        __syncthreads();
+        if (threadIdx.x == 0 && sh.as.hungry != 0 && sh.as.step == STEP_POOL) {
+          atomicSub(&A->hungry[seq & 1][mySeg][0], 1u);
+          sh.as.hungry = 0;
+        }
+        if (sh.as.step == STEP_POOL) {
+          if (sh.as.obj != sh.as.cur) {
+            async_flush_far(objs, ctls, sh, sh.as.cur, listCap, qcap);
+            if (threadIdx.x == 0) {
+              sh.as.cur = sh.as.obj;
+            }
+          }
+          if (threadIdx.x == 0) {
+            sh.as.polls = 0;
+          }
+          __syncthreads();
+        }
+      }
+      const int step = sh.as.step;
+      SUBPROF_MARK(tDecided);
+      if (step == STEP_IDLE) {
+        // Nothing to take: workgroup 0 checks for the end of the phase, the
+        // others poll its mailbox; every wait is bounded.
+        if (t.block == 0) {
+          if (threadIdx.x < 64) {
+            const unsigned lane = threadIdx.x;
+            bool sameOk = true;
+            u64 s1 = 0, r1 = 0, s2 = 0, r2 = 0;
+            for (int pass = 0; pass < 2; ++pass) {
+              u64 s = 0, r = 0;
+              bool ok = true;
+              for (unsigned j = lane; j < t.blocks; j += 64) {
+                const u64 w = load_relaxed(&A->status[j]);
+                ok = ok && static_cast<unsigned>(w >> 60) == status_phase(seq);
+                s += (w >> 30) & 0x3fffffffu;
+                r += w & 0x3fffffffu;
+              }
+              sameOk = sameOk && __all(ok);
+#pragma unroll
+              for (int offset = 32; offset > 0; offset >>= 1) {
+                s += shfl_u64(s, static_cast<int>(lane ^ offset));
+                r += shfl_u64(r, static_cast<int>(lane ^ offset));
+              }
+              if (pass == 0) {
+                s1 = s;
+                r1 = r;
+              } else {
+                s2 = s;
+                r2 = r;
+              }
+            }
+            if (sameOk && s1 == s2 && r1 == r2 && s2 == r2) {
+              if (lane < static_cast<unsigned>(BARRIER_GROUPS)) {
+                store_relaxed(&A->done[lane][0], seq + 1);
+              }
+              if (lane == 0) {
+                sh.as.exit = 1;
+              }
+            }
+          }
+        } else if (threadIdx.x == 0) {
+          if (load_relaxed(&A->done[t.block % BARRIER_GROUPS][0]) >= seq + 1) {
+            sh.as.exit = 1;
+          }
+        }
+        if (threadIdx.x == 0 && sh.as.exit == 0) {
+          if (useHungry && sh.as.hungry == 0) {
+            atomicAdd(&A->hungry[seq & 1][mySeg][0], 1u);
+            sh.as.hungry = 1;
+          }
+          const unsigned polls = ++sh.as.polls;
+          if ((polls & 15u) == 0 &&
+              (load_relaxed(abortWord) != 0 || wall_clock64() - phaseStart > limit)) {
+            store_relaxed(abortWord, 1u);
+            sh.as.exit = 2;
+          } else if (polls > ASYNC_SEGS) {
+            // Every segment looked at once: back off (~1 us).
+            __builtin_amdgcn_s_sleep(32);
+            __builtin_amdgcn_s_sleep(32);
+          }
+        }
+        __syncthreads();
+        SUBPROF_MARK(tIdle);
+        SUBPROF_ADD(1, tStep, tIdle);
+        SUBPROF_ADD(7, 0, 1);
+        if (sh.as.exit != 0) {
+          break;
+        }
+        continue;
+      }
+      // A batch of the local queue, a batch from a pool, or a seeding pass.
+      const int k = sh.as.cur;
+      Desc &D = objs[k];
+      ObjCtl &c = ctls[k];
+      const int bits = D.parentBits;
+      const u64 noParent = D.noParent;
+      const u64 thr = sh.newThr[k];
+      const u64 maxDist = D.maxDistance;
+      const bool optimistic = D.optimistic != 0;
+      const int ea = sh.epoch[k] + (sh.mode[k] == MODE_SPLIT ? 1 : 0);  // far pile appended to
+      u64 *const packed = uniform_global(D.s.packed);
+      int *const inFar = uniform_global(D.s.inFar);
+      const int *const outRow = uniform_global(D.outRow);
+      const int2 *const outRec = uniform_global(D.outEdge);
+      const int *const outCol = uniform_global(D.outCol);
+      const int *const outW = uniform_global(D.outW);
+      int *const farBuf = uniform_global(D.s.far[ea & 1]);
+      unsigned *const farCnt = &c.farCount[ea % 3];
+      const unsigned headBatch = sh.as.lqHead;
+      // An XCD with idle workgroups, read while the batch runs (thread 0).
+      const int hseg = static_cast<int>((mySeg + sh.as.batches) % ASYNC_SEGS);
+      unsigned hungryThere = 0;
+      if (useHungry && threadIdx.x == 0) {
+        hungryThere = load_relaxed(&A->hungry[seq & 1][hseg][0]);
+      }
+      const int4 *const rec4 = uniform_global(D.outEdge4);
+      auto listNear = [&](int v, u64 word, int begin, int end) {
+        const unsigned slot = wave_reserve(&sh.as.lqTail);
+        if (slot - headBatch < lcap) {
+          const unsigned at = slot & (LQ_CAP - 1);
+          *(__attribute__((address_space(3))) int *)(sh.lq.v + at) = v;
+          *(__attribute__((address_space(3))) int *)(sh.lq.b + at) = begin;
+          *(__attribute__((address_space(3))) int *)(sh.lq.e + at) = end;
+          *(__attribute__((address_space(3))) u64 *)(sh.lq.w + at) = word;
+        } else {
+          sh.as.lostNear = 1;
+        }
+      };
+      auto listFar = [&](int v, u64 d) {
+        slice_push(sh.far, 0, qcap, v, farBuf, farCnt, listCap);
+        atomicMin(&sh.as.farMin, d);
+      };
+      if (step == STEP_SEED) {
+        const unsigned i = sh.as.seedPos + threadIdx.x;
+        if (threadIdx.x < seedPass && i < sh.as.seedEnd) {
+          const int mode = sh.mode[k];
+          if (mode == MODE_PUSH) {
+            const int v = D.s.near[0][i];
+            listNear(v, load_relaxed(&packed[v]), -1, 0);
+          } else if (mode == MODE_SWEEP) {
+            const u64 word = load_relaxed(&packed[i]);
+            if (word != PACKED_INF && dist_of(word, bits) < thr) {
+              listNear(static_cast<int>(i), word, -1, 0);
+            }
+          } else {
+            const bool all = sh.cnt[k] > listCap;
+            const int v = all ? static_cast<int>(i) : D.s.far[sh.epoch[k] & 1][i];
+            if (!all || inFar[v] != 0) {
+              const u64 word = load_relaxed(&packed[v]);
+              const u64 d = word == PACKED_INF ? PACKED_INF : dist_of(word, bits);
+              if (d < thr) {
+                inFar[v] = 0;
+                if (D.s.stamp[v] != gen) {  // duplicates of v in the pile: list once
+                  D.s.stamp[v] = gen;
+                  listNear(v, word, -1, 0);
+                }
+              } else {
+                listFar(v, d);
+              }
+            }
+          }
+        }
+      } else {
+        const unsigned count = sh.as.count;
+        const unsigned i = threadIdx.x >> laneBits;
+        int u = -1, begin = -1, end = 0;
+        u64 word = PACKED_INF;
+        if (i < count) {
+          if (step == STEP_LOCAL) {
+            // The word the vertex was listed with (a later improvement lists
+            // it again with its own word) and its edge range.
+            const unsigned at = (headBatch + i) & (LQ_CAP - 1);
+            u = sh.lq.v[at];
+            word = sh.lq.w[at];
+            begin = sh.lq.b[at];
+            end = sh.lq.e[at];
+          } else {
+            int *const slot = D.s.pool + static_cast<size_t>(sh.as.seg) * segCap + sh.as.base + i;
+            u = load_relaxed(slot);
+            if (u == -1) {
+              const u64 start = wall_clock64();
+              while ((u = load_relaxed(slot)) == -1) {
+                if (wall_clock64() - start > limit) {
+                  sh.as.exit = 2;
+                  break;
+                }
+                __builtin_amdgcn_s_sleep(1);
+              }
+            }
+            if (sub == 0 && u != -1) {
+              store_relaxed(slot, -1);
+            }
+          }
+          if (u < -1 || u >= static_cast<int>(n)) {
+            sh.as.exit = 2;  // not a vertex: never used as an index
+            u = -1;
+          }
+        }
+        if (u >= 0) {
+          if (begin < 0) {
+            begin = outRow[u];
+            end = outRow[u + 1];
+          }
+          // Edge x as {column, weight, rowBegin, rowEnd} (row range -1: not
+          // known), from whichever records the graph holds.
+          auto edgeAt = [&](int x) -> int4 {
+            if (rec4 != nullptr) {
+              return rec4[x];
+            }
+            if (outRec != nullptr) {
+              const int2 r = outRec[x];
+              return make_int4(r.x, r.y, -1, 0);
+            }
+            return make_int4(outCol[x], outW[x], -1, 0);
+          };
+          // The first record is requested before the word is awaited.
+          const int L = static_cast<int>(lanes);
+          int x = begin + static_cast<int>(sub);
+          int4 next = make_int4(-1, 0, -1, 0);
+          if (x < end) {
+            next = edgeAt(x);
+          }
+          if (step == STEP_POOL || !carry) {
+            word = load_relaxed(&packed[u]);
+          }
+          if (word != PACKED_INF) {
+            const u64 du = dist_of(word, bits);
+            const int pu = bits != 0 ? static_cast<int>(word & ((1ull << bits) - 1ull)) : -1;
+            // Two edges per lane and round, both atomics in flight together
+            // (a batch otherwise waits for a second round trip whenever one
+            // of its vertices has more edges than lanes).
+            for (; x < end; x += 2 * L) {
+              const int4 ea = next;
+              const int4 eb = x + L < end ? edgeAt(x + L) : make_int4(-1, 0, -1, 0);
+              if (x + 2 * L < end) {
+                next = edgeAt(x + 2 * L);
+              }
+              const u64 ndA = du + static_cast<u64>(ea.y);
+              const u64 ndB = du + static_cast<u64>(eb.y);
+              bool goA = ea.x != source && ea.x != pu;  // the source is never relaxed; nor is u's parent
+              bool goB = eb.x >= 0 && eb.x != source && eb.x != pu;
+              if ((goA && ndA > maxDist) || (goB && ndB > maxDist)) {
+                if (optimistic) {
+                  store_relaxed(&c.dropped, 1u);
+                }
+                goA = goA && ndA <= maxDist;
+                goB = goB && ndB <= maxDist;
+              }
+              const bool nearA = ndA < thr, nearB = ndB < thr;
+              const int claimA = goA && !nearA ? inFar[ea.x] : 0;
+              const int claimB = goB && !nearB ? inFar[eb.x] : 0;
+              const u64 candA = pack(ndA, u, bits, noParent);
+              const u64 candB = pack(ndB, u, bits, noParent);
+              u64 prevA = 0, prevB = 0;
+              if (goA) {
+                prevA = atomicMin(&packed[ea.x], candA);
+              }
+              if (goB) {
+                prevB = atomicMin(&packed[eb.x], candB);
+              }
+              if (goA && ndA < dist_of(prevA, bits)) {
+                if (nearA) {
+                  listNear(ea.x, candA, ea.z, ea.w);
+                } else if (claimA == 0) {
+                  inFar[ea.x] = 1;
+                  listFar(ea.x, ndA);
+                }
+              }
+              if (goB && ndB < dist_of(prevB, bits)) {
+                if (nearB) {
+                  listNear(eb.x, candB, eb.z, eb.w);
+                } else if (claimB == 0) {
+                  inFar[eb.x] = 1;
+                  listFar(eb.x, ndB);
+                }
+              }
+            }
+          }
+        }
+      }
+      __syncthreads();
+      SUBPROF_MARK(tRelaxed);
+      // Bookkeeping (thread 0): consume the batch, count, publish, spill.
+      if (threadIdx.x == 0) {
+        unsigned tail = sh.as.lqTail;
+        if (tail - headBatch > lcap) {
+          tail = headBatch + lcap;  // appends past the room were dropped
+        }
+        if (sh.as.lostNear != 0) {
+          store_relaxed(&c.lost[seq & 1], 1u);
+          sh.as.lostNear = 0;
+          atomicAdd(&ctlAll->asyncLost, 1u);
+          SUBPROF_ADD(13, 0, 1);
+        }
+        sh.as.lqTail = tail;
+        sh.as.sent += tail - tailBefore;
+        if (step == STEP_SEED) {
+          sh.as.seedPos += seedPass;
+          if (sh.as.seedPos >= sh.as.seedEnd) {
+            if (++sh.as.seedIdx < sh.nmine) {
+              async_seed_share(sh, sh.mine[sh.as.seedIdx], t.block, n, listCap);
+            } else {
+              sh.as.recv += 1;  // the seeding token
+            }
+          }
+        } else {
+          if (step == STEP_LOCAL) {
+            sh.as.lqHead += sh.as.count;
+          }
+          sh.as.recv += sh.as.count;
+          sh.statPush[k] += sh.as.count;
+        }
+        const unsigned have = sh.as.lqTail - sh.as.lqHead;
+        const u64 status = status_word(seq, sh.as.sent, sh.as.recv);
+        sh.as.spillKeep = 0;
+        // Spill: the queue holds more than spillHi entries (down to spillLo,
+        // into this XCD's segment), or an XCD has idle workgroups (all but
+        // keepHungry entries, up to perPass per idle workgroup, into its
+        // segment).
+        unsigned s = 0;
+        int spillSeg = mySeg;
+        if (hungryThere > 0 && have > keepHungry) {
+          s = min(have - keepHungry, hungryThere * perPass);
+          spillSeg = hseg;
+        }
+        if (have > spillHi && have - spillLo > s) {
+          s = have - spillLo;
+          spillSeg = mySeg;
+        }
+        if (s > 0) {
+          // The published count must be seen before the entries can be
+          // taken: wait for the exchange, then reserve.
+          const u64 prev = __hip_atomic_exchange(&A->status[t.block], status, __ATOMIC_RELAXED,
+                                                 __HIP_MEMORY_SCOPE_AGENT);
+          asm volatile("" ::"v"(prev) : "memory");
+          const u64 old = atomicAdd(pool_word(A, seq, k, spillSeg), static_cast<u64>(s) << 32);
+          const unsigned t0 = static_cast<unsigned>(old >> 32);
+          sh.as.spillBase = t0;
+          sh.as.spillSeg = spillSeg;
+          sh.as.spillKeep = t0 < segCap ? min(s, segCap - t0) : 0;
+          if (sh.as.spillKeep < s) {
+            atomicAdd(&ctlAll->asyncShortSpills, 1u);
+          }
+        } else {
+          store_relaxed(&A->status[t.block], status);
+        }
+        if ((++sh.as.batches & 63u) == 0 &&
+            (load_relaxed(abortWord) != 0 || wall_clock64() - phaseStart > limit)) {
+          store_relaxed(abortWord, 1u);
+          sh.as.exit = 2;
+        }
+      }
+      __syncthreads();
+      if (sh.as.exit == 2) {
+        break;
+      }
+      const unsigned keep = sh.as.spillKeep;
+      if (keep > 0) {
+        // The newest `keep` entries go to slots [spillBase, spillBase + keep).
+        const unsigned from = sh.as.lqTail - keep;
+        int *const seg = D.s.pool + static_cast<size_t>(sh.as.spillSeg) * segCap + sh.as.spillBase;
+        for (unsigned j = threadIdx.x; j < keep; j += BLOCK) {
+          store_relaxed(seg + j, sh.lq.v[(from + j) & (LQ_CAP - 1)]);
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+          sh.as.lqTail -= keep;
+        }
+      }
+      if (sh.far.count[0] > qcap / 2) {
+        async_flush_far(objs, ctls, sh, k, listCap, qcap);
+      }
+      __syncthreads();
+      SUBPROF_MARK(tBusy);
+      SUBPROF_ADD(0, tStep, tBusy);
+      if (step != STEP_SEED) {
+        SUBPROF_ADD(step == STEP_LOCAL ? 2 : 3, 0, sh.as.count);
+      }
+      SUBPROF_ADD(6, tStep, tDecided);
+      SUBPROF_ADD(14, tDecided, tRelaxed);
+      SUBPROF_ADD(step == STEP_LOCAL ? 4 : step == STEP_POOL ? 5 : 15, 0, 1);
+      SUBPROF_ADD(8, 0, keep > 0 ? 1 : 0);
+      SUBPROF_ADD(9, 0, keep);
+
+    }
+    if (sh.as.exit == 2) {
+      if (threadIdx.x == 0) {
+        store_relaxed(abortWord, 1u);
+      }
+      grid.aborted = true;
+      return generation;
+    }
+    // The phase is over everywhere: publish the far queue, then the barrier
+    // carries every objective's far count, minimum and lost flag.
+    async_flush_far(objs, ctls, sh, sh.as.cur, listCap, qcap);
+    if (threadIdx.x < static_cast<unsigned>(nobj)) {
+      const int k = threadIdx.x;
+      ObjCtl &c = ctls[k];
+      const int eNext = sh.epoch[k] + (sh.mode[k] == MODE_SPLIT ? 1 : 0);
+      unsigned *fm = reinterpret_cast<unsigned *>(&c.farMin[eNext % 3]);
+      sh.src[MW * k] = &c.farCount[eNext % 3];
+      sh.src[MW * k + 1] = fm;
+      sh.src[MW * k + 2] = fm + 1;
+      sh.src[MW * k + 3] = &c.lost[seq & 1];
+    }
+    SUBPROF_MARK(tEnd);
+    if (!grid.sync(sh.src, MW * nobj, sh.snap)) {
+      return generation;
+    }
+    SUBPROF_MARK(tSynced);
+    SUBPROF_ADD(10, tEnd, tSynced);
+    SUBPROF_ADD(11, 0, 1);
+    SUBPROF_ADD(12, phaseStart, tEnd);
+    if (threadIdx.x < static_cast<unsigned>(nobj)) {
+      const int k = threadIdx.x;
+      if (sh.mode[k] == MODE_SPLIT) {
+        sh.thr[k] = sh.newThr[k];
+        sh.epoch[k] += 1;
+      }
+    }
+  }
+  if (threadIdx.x < static_cast<unsigned>(nobj)) {
+    const int k = threadIdx.x;
+    if (t.block == 0) {
+      atomicAdd(&ctls[k].iterations, sh.statIter[k]);
+      atomicAdd(&ctls[k].epochs, sh.statEpoch[k]);
+    }
+    if (sh.statPush[k] != 0) {
+      atomicAdd(&ctls[k].pushes, sh.statPush[k]);
+    }
+  }
+  // The local queue overlays the near queue: leave that one empty.
+  if (threadIdx.x < WAVES) {
+    sh.near.count[threadIdx.x] = 0;
+  }
+  __syncthreads();
   return generation;
 }
 
@@ -2020,9 +2812,16 @@ __device__ __attribute__((noinline)) void combined_build(Args &args, GridBarrier
     }
     __syncthreads();
   }
+  // Records with the child's row range as well, for the asynchronous loop.
+  int4 *const rec4 = a.async != nullptr ? a.cEdge4 : nullptr;
   for (int v = t.id; v < n; v += t.count) {
     if (v == a.source) {
       continue;
+    }
+    int vBegin = 0, vEnd = 0;
+    if (rec4 != nullptr) {
+      vBegin = a.cRowPtr[v];
+      vEnd = a.cRowPtr[v + 1];
     }
     combined_edges(a, v, [&](int p, int weight) {
       int pos;
@@ -2033,6 +2832,9 @@ __device__ __attribute__((noinline)) void combined_build(Args &args, GridBarrier
         pos = atomicAdd(&a.cDegree[p], 1);
       }
       a.cEdge[pos] = make_int2(v, weight);
+      if (rec4 != nullptr) {
+        rec4[pos] = make_int4(v, weight, vBegin, vEnd);
+      }
     });
   }
   // Near-far width for Step 3: the caller's, or 32 * average weight /
@@ -2108,6 +2910,30 @@ __device__ __forceinline__ int run_phases(Args &a, GridBarrier &grid, const Thre
 #ifdef MOSP_SUBPROFILE
   if (threadIdx.x < 16) subprof_lds()[threadIdx.x] = 0;
 #endif
+  // Asynchronous near-far: every launch starts its counters afresh.  Each
+  // phase is preceded by a grid barrier, which orders these stores before
+  // any use.
+  constexpr bool kAsync = !HUBS && LOCAL == 1;
+  unsigned phaseSeq = 0;
+  if constexpr (kAsync) {
+    if (a.async != nullptr) {
+      AsyncCtl *const A = a.async;
+      if (threadIdx.x == 0) {
+        store_relaxed(&A->status[t.block], ASYNC_STATUS_INIT);
+        sh.as.sent = 0;
+        sh.as.recv = 0;
+      }
+      if (t.block == 0) {
+        for (unsigned i = threadIdx.x; i < static_cast<unsigned>(MOSP_MAX_OBJ) * ASYNC_SEGS;
+             i += BLOCK) {
+          store_relaxed(&A->pool[0][i / ASYNC_SEGS][i % ASYNC_SEGS][0], 0ull);
+        }
+        if (threadIdx.x < static_cast<unsigned>(BARRIER_GROUPS)) {
+          store_relaxed(&A->done[threadIdx.x][0], 0u);
+        }
+      }
+    }
+  }
   // Stamp generations persist in device memory across launches; restart
   // them (and clear the stamps) well before the counter could wrap.
   if (generation > (1 << 30)) {
@@ -2136,9 +2962,16 @@ __device__ __forceinline__ int run_phases(Args &a, GridBarrier &grid, const Thre
     if (grid.aborted) {
       return generation;
     }
-    generation = near_far<HUBS, LOCAL>(desc(a.objs), a.nobj, a.ctl->obj, a.source, generation, 0,
-                                nullptr, a.laneBits, a.hopsUpdate, static_cast<unsigned>(n),
-                                listCap, grid, t, sh, 8);
+    if (kAsync && a.async != nullptr) {
+      generation = near_far_async(desc(a.objs), a.nobj, a.ctl->obj, a.async, a.source, generation,
+                                  0, nullptr, a.laneBits, static_cast<unsigned>(n), listCap,
+                                  a.segCap, a.spillHi, a.spillLo, a.asyncFlags, a.keepHungry, a.abortWord, phaseSeq, grid, t,
+                                  sh);
+    } else {
+      generation = near_far<HUBS, LOCAL>(desc(a.objs), a.nobj, a.ctl->obj, a.source, generation,
+                                         0, nullptr, a.laneBits, a.hopsUpdate,
+                                         static_cast<unsigned>(n), listCap, grid, t, sh, 8);
+    }
     if (grid.aborted || dropped_any(a, a.nobj, t)) {
       return generation;
     }
@@ -2189,9 +3022,16 @@ __device__ __forceinline__ int run_phases(Args &a, GridBarrier &grid, const Thre
     if (!grid.sync()) {
       return generation;
     }
-    generation = near_far<HUBS, LOCAL>(desc(a.objs), 1, a.ctl->obj, a.source, generation,
-                                D.delta, nullptr, a.laneBits, a.hopsSearch,
-                                static_cast<unsigned>(n), listCap, grid, t, sh, 8);
+    if (kAsync && a.async != nullptr) {
+      generation = near_far_async(desc(a.objs), 1, a.ctl->obj, a.async, a.source, generation,
+                                  D.delta, nullptr, a.laneBits, static_cast<unsigned>(n), listCap,
+                                  a.segCap, a.spillHi, a.spillLo, a.asyncFlags, a.keepHungry, a.abortWord, phaseSeq, grid, t,
+                                  sh);
+    } else {
+      generation = near_far<HUBS, LOCAL>(desc(a.objs), 1, a.ctl->obj, a.source, generation,
+                                         D.delta, nullptr, a.laneBits, a.hopsSearch,
+                                         static_cast<unsigned>(n), listCap, grid, t, sh, 8);
+    }
     if (grid.aborted || dropped_any(a, 1, t)) {
       return generation;
     }
@@ -2220,9 +3060,16 @@ __device__ __forceinline__ int run_phases(Args &a, GridBarrier &grid, const Thre
     }
     stamp(a.ctl, PHASE_COMB_FILL, t.id);
     const u64 delta = static_cast<u64>(sh.snap[0]) | (static_cast<u64>(sh.snap[1]) << 32);
-    generation = near_far<HUBS, LOCAL>(desc(a.cDesc), 1, &cc, a.source, generation, delta, &delta,
-                                a.laneBits, a.hopsSearch, static_cast<unsigned>(n), listCap,
-                                grid, t, sh, 12);
+    if (kAsync && a.async != nullptr) {
+      generation = near_far_async(desc(a.cDesc), 1, &cc, a.async, a.source, generation, delta,
+                                  &delta, a.laneBits, static_cast<unsigned>(n), listCap, a.segCap,
+                                  a.spillHi, a.spillLo, a.asyncFlags, a.keepHungry, a.abortWord,
+                                  phaseSeq, grid, t, sh);
+    } else {
+      generation = near_far<HUBS, LOCAL>(desc(a.cDesc), 1, &cc, a.source, generation, delta,
+                                         &delta, a.laneBits, a.hopsSearch,
+                                         static_cast<unsigned>(n), listCap, grid, t, sh, 12);
+    }
     if (grid.aborted) {
       return generation;
     }

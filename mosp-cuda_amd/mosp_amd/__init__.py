@@ -72,7 +72,7 @@ class BatchInstance(C.Structure):
 
 EXPORTS = (
     "mosp_last_error", "mosp_ws_create", "mosp_ws_reserve", "mosp_ws_destroy",
-    "mosp_ws_set_grid", "mosp_ws_grid", "mosp_ws_set_combined_delta_scale", "mosp_ws_last_profile", "mosp_default_delta", "mosp_sosp_update",
+    "mosp_ws_set_grid", "mosp_ws_grid", "mosp_ws_set_combined_delta_scale", "mosp_ws_last_profile", "mosp_ws_async_events", "mosp_default_delta", "mosp_sosp_update",
     "mosp_sosp_update_multi", "mosp_sosp_from_scratch", "mosp_combined_sosp",
     "mosp_update_fused", "mosp_device_graph_upload", "mosp_device_graph_free",
     "mosp_device_graph_view", "mosp_device_graph_objectives", "mosp_path_costs_gpu",
@@ -154,6 +154,7 @@ def lib() -> C.CDLL:
     L.mosp_ws_grid.argtypes = [_VP]
     L.mosp_ws_grid.restype = _I
     L.mosp_ws_last_profile.argtypes = [_VP, _f64p, _I, C.POINTER(C.c_int32)]
+    L.mosp_ws_async_events.argtypes = [_VP, C.POINTER(C.c_int32)]
     L.mosp_default_delta.argtypes = [_L, _I, _L]
     L.mosp_default_delta.restype = _L
     L.mosp_sosp_update.argtypes = [C.POINTER(Csr), C.POINTER(Csr), C.POINTER(Changes), _I, _L,
@@ -569,6 +570,13 @@ class Workspace:
         out = {p: round(float(x), 4) for p, x in zip(self.PHASES, ms)}
         out["barriers"] = b.value
         return out
+
+    def async_events(self) -> dict:
+        """Events of the last launch's asynchronous near-far loop."""
+        ev = (C.c_int32 * 4)()
+        _check(lib().mosp_ws_async_events(self.handle, ev), "mosp_ws_async_events")
+        return {"lost_batches": ev[0], "short_spills": ev[1], "sweep_phases": ev[2],
+                "async": bool(ev[3])}
 
     def close(self) -> None:
         if self.handle:

@@ -425,6 +425,76 @@ def test_full_lists_fall_back_to_sweeps(monkeypatch):
                        tag=f"capped-lists rmat hubs={forced}")
 
 
+ASYNC_KNOBS = [
+    # (label, environment, expected events) -- every path of the asynchronous
+    # near-far loop forced on a small graph, and the iteration-synchronous
+    # loop it replaces.
+    ("default", {}, None),
+    ("iteration-synchronous", {"MOSP_ASYNC": "0"}, None),
+    ("carried-words", {"MOSP_ASYNC_FLAGS": "1"}, None),
+    ("carried-words-hungry", {"MOSP_ASYNC_FLAGS": "3", "MOSP_ASYNC_KEEP": "16"}, None),
+    ("heavy-spills", {"MOSP_ASYNC_SPILL": "64,32", "MOSP_ASYNC_FLAGS": "0"}, None),
+    ("tiny-pools", {"MOSP_ASYNC_SPILL": "64,32", "MOSP_POOL_CAP": "256"}, "short_spills"),
+    # 256-entry local queue kept nearly full, one lane per vertex (batches of
+    # up to 1024 entries): appends past the room are dropped, and the next
+    # phase lists every vertex under the threshold again.
+    ("tiny-local-queue", {"MOSP_QUEUE_CAP": "256", "MOSP_LANE_BITS": "0",
+                          "MOSP_ASYNC_SPILL": "250,200"}, "sweep_phases"),
+    ("two-workgroups", {"MOSP_GRID": "2"}, None),
+    ("one-workgroup", {"MOSP_GRID": "1"}, None),
+    ("512-thread", {"MOSP_WG_THREADS": "512", "MOSP_GRID": "7"}, None),
+]
+
+
+@pytest.mark.parametrize("label,env,event", ASYNC_KNOBS, ids=[k[0] for k in ASYNC_KNOBS])
+def test_async_near_far_paths(monkeypatch, label, env, event):
+    """The asynchronous near-far loop (graphs without hubs, single-update
+    kernel) under every spill, pool, queue and grid setting, against the
+    oracle: K updated trees, the combined graph's distances and MOSP tree.
+    Settings that must take a rare path (short spills, dropped near entries
+    and the sweep phase after them) assert that they did."""
+    for key in ("MOSP_ASYNC", "MOSP_ASYNC_FLAGS", "MOSP_ASYNC_KEEP", "MOSP_ASYNC_SPILL",
+                "MOSP_POOL_CAP", "MOSP_QUEUE_CAP", "MOSP_LANE_BITS", "MOSP_GRID",
+                "MOSP_WG_THREADS"):
+        monkeypatch.delenv(key, raising=False)
+    for key, value in env.items():
+        monkeypatch.setenv(key, value)
+    g = gen.grid_graph(70, 60, 3, 90, 0.2, 301)
+    b = gen.uniform_batch(g, 900, 50, 302)
+    d0, p0 = gen.initial_trees(g, 3)
+    want = O.mosp_update(g, b, d0, p0, 3, 0, [1, 3, 2])
+    assert want["ok"]
+    say(f"async paths: {label} n={g.n} m={g.m} env={env}")
+    sess = M().Session(mgraph(g), mbatch(b), d0, p0, pref=[1, 3, 2])
+    try:
+        for rep in range(2):
+            sess.run()
+            got = sess.results()
+            for k in range(3):
+                assert np.array_equal(got["dist"][k], want["dist"][k]), f"{label} obj{k} dist"
+                assert np.array_equal(got["parents"][k], want["parents"][k]), f"{label} obj{k}"
+            assert np.array_equal(got["comb_dist"], want["comb_dist"]), f"{label} combined"
+            assert np.array_equal(got["comb_parent"], want["comb_parent"]), f"{label} MOSP tree"
+            ev = sess.ws.async_events()
+            assert ev["async"] == (env.get("MOSP_ASYNC") != "0"), ev
+            if event is not None:
+                assert ev[event] > 0, f"{label}: expected {event} ({ev})"
+    finally:
+        sess.close()
+    # From-scratch search of the same engine (single objective, source list).
+    dg = M().DeviceGraph(mgraph(g))
+    ws = M().Workspace(g.n)
+    try:
+        w = g.w.reshape(-1, g.K)[:, 1]
+        d, p, _ = M().from_scratch(dg, 1, 5, M().default_delta(g.m, g.n, int(w.sum())),
+                                   int(w.max()), ws)
+        rd, rp = O.dijkstra(g, 1, 5)
+        assert np.array_equal(d, rd) and np.array_equal(p, rp), f"{label} from scratch"
+    finally:
+        ws.close()
+        dg.close()
+
+
 def test_rmat_large_weights_fallback():
     """R-MAT scale 17, weights up to 2^31 - 1: (n - 1) * maxW does not fit next
     to an 18-bit parent id; the words packed up to their own limit (46 bits)
