@@ -104,6 +104,12 @@ struct mosp_batch {
   size_t usedSlab = 0;         // scratch bytes of the current instances
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   float lastMs = 0;
+  // Asynchronous near-far for graphs without hubs: one control block per
+  // instance; pool segments in the instances' states, whose slots must hold
+  // -1 (refilled after a run gave up).
+  unsigned asyncFlags = ASYNC_HUNGRY;
+  AsyncCtl *dAsync = nullptr;
+  std::vector<std::pair<int *, size_t>> pools;
 };
 
 namespace {
@@ -127,6 +133,7 @@ void release(mosp_batch *b) {
   (void)hipFree(b->dGroups);
   (void)hipFree(b->dGroupOf);
   (void)hipFree(b->dRankOf);
+  (void)hipFree(b->dAsync);
   if (b->hCtl != nullptr) (void)hipHostFree(b->hCtl);
   if (b->stream != nullptr) (void)hipStreamDestroy(b->stream);
   if (b->ev0 != nullptr) (void)hipEventDestroy(b->ev0);
@@ -217,6 +224,9 @@ int setup(mosp_batch *b, int maxInstances, int maxDescs, size_t slabBytes) {
     b->xcds = std::max(1, std::atoi(env));
   }
   local_hops_override(b->hopsUpdate, b->hopsSearch);
+  if (const char *env = std::getenv("MOSP_ASYNC_FLAGS")) {
+    b->asyncFlags = static_cast<unsigned>(std::atoi(env)) & (ASYNC_CARRY | ASYNC_HUNGRY);
+  }
   b->workgroups = b->maxBlocks;  // every co-resident workgroup of the shape
   maxInstances = std::max(maxInstances, 1);
   maxDescs = std::max(maxDescs, 1);
@@ -225,6 +235,7 @@ int setup(mosp_batch *b, int maxInstances, int maxDescs, size_t slabBytes) {
   b->capSlab = std::max<size_t>(slabBytes, 256);
   HIP_TRY(hipMalloc(&b->slab, b->capSlab));
   HIP_TRY(hipMalloc(&b->dCtl, sizeof(Ctl) * maxInstances));
+  HIP_TRY(hipMalloc(&b->dAsync, sizeof(AsyncCtl) * maxInstances));
   HIP_TRY(hipMalloc(&b->dGen, sizeof(int) * maxInstances));
   HIP_TRY(hipMalloc(&b->dAbort, sizeof(unsigned)));
   HIP_TRY(hipMemset(b->dAbort, 0, sizeof(unsigned)));
@@ -242,7 +253,8 @@ int setup(mosp_batch *b, int maxInstances, int maxDescs, size_t slabBytes) {
 
 // Slab bytes of one instance: K objective states + the combined workspace.
 size_t instance_bytes(int n, int K, int partials) {
-  return static_cast<size_t>(K) * state_bytes(n) + combined_bytes(n, K, partials);
+  return static_cast<size_t>(K) * state_bytes(n, pool_seg_cap(n)) +
+         combined_bytes(n, K, partials);
 }
 
 // Describe `count` instances into b (within its capacity) and clear the
@@ -429,13 +441,24 @@ int prepare(mosp_batch *b) {
   b->hDesc.assign(b->descCount, ObjDesc{});
   build_schedule(b);
   b->hArgs.assign(count, LaunchArgs{});
+  // Light instances always run the asynchronous loop: the batched kernel
+  // compiles no other for them.
+  const bool async = b->hubs == 0;
+  const unsigned qcap = b->queueCap >= 256 ? static_cast<unsigned>(b->queueCap)
+                                           : static_cast<unsigned>(MOSP_QUEUE_CAP_MAX);
+  b->pools.clear();
   for (int i = 0; i < count; ++i) {
     Instance &I = b->inst[i];
     const int n = I.n, K = I.K;
     char *p = b->slab + I.slabOffset;
     std::vector<ObjState> states(K);
+    const unsigned segCap = pool_seg_cap(n);
     for (int k = 0; k < K; ++k) {
-      states[k] = carve_state(p, n);
+      states[k] = carve_state(p, n, segCap);
+      b->pools.emplace_back(states[k].pool, pool_bytes(segCap));
+    }
+    if (async && !graph_records4(I.spec.graph, b->stream)) {
+      return 0;
     }
     const size_t nn = static_cast<size_t>(n);
     int *cDegree = reinterpret_cast<int *>(p);
@@ -453,6 +476,9 @@ int prepare(mosp_batch *b) {
       const u64 delta = static_cast<u64>(std::max(1.0, scaled));
       ObjDesc d = make_desc(I.out[k], &I.in[k], &I.changes[k], I.pack[k], delta, states[k]);
       graph_edges(I.spec.graph, k, &d.outEdge, &d.inEdge);
+      if (async) {
+        d.outEdge4 = I.spec.graph->outEdge4 + static_cast<size_t>(k) * I.spec.graph->stride;
+      }
       const size_t off = static_cast<size_t>(k) * nn;
       d.distIn = I.spec.d_dist_in + off;
       d.parentIn = I.spec.d_parents_in + off;
@@ -502,8 +528,21 @@ int prepare(mosp_batch *b) {
     a.cPartials = cPartials;
     a.stage = reinterpret_cast<u64 *>(cColInd);  // cColInd, cWeights: 8 * K * n bytes in a row
     a.cDesc = b->dDesc + I.descOffset + K;
+    if (async) {
+      a.async = b->dAsync + i;
+      a.segCap = segCap;
+      a.spillHi = qcap / 4;
+      a.spillLo = qcap / 8;
+      a.asyncFlags = b->asyncFlags;
+      a.keepHungry = static_cast<unsigned>(b->shapeThreads >> std::max(0, b->laneBits));
+    }
     b->hArgs[b->argsIndex[i]] = a;
   }
+  // Pool slots hold -1 when free.
+  for (const auto &pool : b->pools) {
+    HIP_TRY(hipMemsetAsync(pool.first, 0xff, pool.second, b->stream));
+  }
+  HIP_TRY(hipStreamSynchronize(b->stream));
   const size_t groups = b->hGroups.size();
   HIP_TRY(hipMemcpy(b->dDesc, b->hDesc.data(), sizeof(ObjDesc) * b->descCount,
                     hipMemcpyHostToDevice));
@@ -701,6 +740,9 @@ extern "C" int mosp_batch_run(mosp_batch *b, int32_t use_graph, float *kernel_ms
     // are not in the state the next run relies on.
     HIP_TRY(hipMemsetAsync(b->slab, 0, b->usedSlab, b->stream));
     HIP_TRY(hipMemsetAsync(b->dGen, 0, sizeof(int) * std::max(count, 1), b->stream));
+    for (const auto &pool : b->pools) {
+      HIP_TRY(hipMemsetAsync(pool.first, 0xff, pool.second, b->stream));
+    }
     b->scrub = false;
   }
   if (use_graph == 1) {

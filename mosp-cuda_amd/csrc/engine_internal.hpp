@@ -153,6 +153,10 @@ ObjDesc make_desc(const mosp_csr &out, const mosp_csr *in,
 // (advanced past them); state_bytes(n) is their size.
 // The out-edge records with row ranges of g (built when stale).
 int graph_records4(const mosp_device_graph *g, hipStream_t stream);
+// Entries of one pool segment of the asynchronous near-far loop for n
+// vertices, and the bytes of the ASYNC_SEGS segments.
+unsigned pool_seg_cap(int n);
+size_t pool_bytes(unsigned segCap);
 size_t state_bytes(int n, unsigned segCap = 0);
 ObjState carve_state(char *&p, int n, unsigned segCap = 0);
 

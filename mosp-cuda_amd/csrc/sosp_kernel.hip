@@ -1337,7 +1337,10 @@ __device__ void async_seed_share(Shared &sh, int k, unsigned blockRank, unsigned
 // near_far() without per-iteration barriers, for objectives objs[0, nobj):
 // same entry state (near list 0 or far pile 0, first threshold initThr),
 // same exit state (final words in the objectives' packed arrays).
-// phaseSeq counts the launch's phases (every workgroup the same).
+// phaseSeq counts the launch's phases (every workgroup the same).  LOCAL:
+// the caller's kernel (1 single update, 2 batched), so that each kernel has
+// its own copy and register allocation.
+template <int LOCAL>
 __device__ int near_far_async(Desc *objsArg, int nobj, ObjCtl *ctlsArg, AsyncCtl *Aarg,
                               int source, int generation, u64 initThr,
                               const u64 *deltaOverride, int laneBits, unsigned n,
@@ -2913,7 +2916,11 @@ __device__ __forceinline__ int run_phases(Args &a, GridBarrier &grid, const Thre
   // Asynchronous near-far: every launch starts its counters afresh.  Each
   // phase is preceded by a grid barrier, which orders these stores before
   // any use.
-  constexpr bool kAsync = !HUBS && LOCAL == 1;
+  constexpr bool kAsync = !HUBS && LOCAL != 0;
+  // The batched kernel's light instances run the asynchronous loop only (the
+  // iteration-synchronous one is not compiled into them: both together
+  // spilled 640 bytes per lane).
+  constexpr bool kAsyncOnly = !HUBS && LOCAL == 2;
   unsigned phaseSeq = 0;
   if constexpr (kAsync) {
     if (a.async != nullptr) {
@@ -2930,6 +2937,9 @@ __device__ __forceinline__ int run_phases(Args &a, GridBarrier &grid, const Thre
         }
         if (threadIdx.x < static_cast<unsigned>(BARRIER_GROUPS)) {
           store_relaxed(&A->done[threadIdx.x][0], 0u);
+        }
+        if (threadIdx.x < static_cast<unsigned>(ASYNC_SEGS)) {
+          store_relaxed(&A->hungry[0][threadIdx.x][0], 0u);
         }
       }
     }
@@ -2962,15 +2972,17 @@ __device__ __forceinline__ int run_phases(Args &a, GridBarrier &grid, const Thre
     if (grid.aborted) {
       return generation;
     }
-    if (kAsync && a.async != nullptr) {
-      generation = near_far_async(desc(a.objs), a.nobj, a.ctl->obj, a.async, a.source, generation,
+    if (kAsyncOnly || (kAsync && a.async != nullptr)) {
+      generation = near_far_async<LOCAL>(desc(a.objs), a.nobj, a.ctl->obj, a.async, a.source, generation,
                                   0, nullptr, a.laneBits, static_cast<unsigned>(n), listCap,
                                   a.segCap, a.spillHi, a.spillLo, a.asyncFlags, a.keepHungry, a.abortWord, phaseSeq, grid, t,
                                   sh);
     } else {
-      generation = near_far<HUBS, LOCAL>(desc(a.objs), a.nobj, a.ctl->obj, a.source, generation,
-                                         0, nullptr, a.laneBits, a.hopsUpdate,
-                                         static_cast<unsigned>(n), listCap, grid, t, sh, 8);
+      if constexpr (!kAsyncOnly) {
+        generation = near_far<HUBS, LOCAL>(desc(a.objs), a.nobj, a.ctl->obj, a.source, generation,
+                                           0, nullptr, a.laneBits, a.hopsUpdate,
+                                           static_cast<unsigned>(n), listCap, grid, t, sh, 8);
+      }
     }
     if (grid.aborted || dropped_any(a, a.nobj, t)) {
       return generation;
@@ -3022,15 +3034,17 @@ __device__ __forceinline__ int run_phases(Args &a, GridBarrier &grid, const Thre
     if (!grid.sync()) {
       return generation;
     }
-    if (kAsync && a.async != nullptr) {
-      generation = near_far_async(desc(a.objs), 1, a.ctl->obj, a.async, a.source, generation,
+    if (kAsyncOnly || (kAsync && a.async != nullptr)) {
+      generation = near_far_async<LOCAL>(desc(a.objs), 1, a.ctl->obj, a.async, a.source, generation,
                                   D.delta, nullptr, a.laneBits, static_cast<unsigned>(n), listCap,
                                   a.segCap, a.spillHi, a.spillLo, a.asyncFlags, a.keepHungry, a.abortWord, phaseSeq, grid, t,
                                   sh);
     } else {
-      generation = near_far<HUBS, LOCAL>(desc(a.objs), 1, a.ctl->obj, a.source, generation,
-                                         D.delta, nullptr, a.laneBits, a.hopsSearch,
-                                         static_cast<unsigned>(n), listCap, grid, t, sh, 8);
+      if constexpr (!kAsyncOnly) {
+        generation = near_far<HUBS, LOCAL>(desc(a.objs), 1, a.ctl->obj, a.source, generation,
+                                           D.delta, nullptr, a.laneBits, a.hopsSearch,
+                                           static_cast<unsigned>(n), listCap, grid, t, sh, 8);
+      }
     }
     if (grid.aborted || dropped_any(a, 1, t)) {
       return generation;
@@ -3060,15 +3074,17 @@ __device__ __forceinline__ int run_phases(Args &a, GridBarrier &grid, const Thre
     }
     stamp(a.ctl, PHASE_COMB_FILL, t.id);
     const u64 delta = static_cast<u64>(sh.snap[0]) | (static_cast<u64>(sh.snap[1]) << 32);
-    if (kAsync && a.async != nullptr) {
-      generation = near_far_async(desc(a.cDesc), 1, &cc, a.async, a.source, generation, delta,
+    if (kAsyncOnly || (kAsync && a.async != nullptr)) {
+      generation = near_far_async<LOCAL>(desc(a.cDesc), 1, &cc, a.async, a.source, generation, delta,
                                   &delta, a.laneBits, static_cast<unsigned>(n), listCap, a.segCap,
                                   a.spillHi, a.spillLo, a.asyncFlags, a.keepHungry, a.abortWord,
                                   phaseSeq, grid, t, sh);
     } else {
-      generation = near_far<HUBS, LOCAL>(desc(a.cDesc), 1, &cc, a.source, generation, delta,
-                                         &delta, a.laneBits, a.hopsSearch,
-                                         static_cast<unsigned>(n), listCap, grid, t, sh, 12);
+      if constexpr (!kAsyncOnly) {
+        generation = near_far<HUBS, LOCAL>(desc(a.cDesc), 1, &cc, a.source, generation, delta,
+                                           &delta, a.laneBits, a.hopsSearch,
+                                           static_cast<unsigned>(n), listCap, grid, t, sh, 12);
+      }
     }
     if (grid.aborted) {
       return generation;
