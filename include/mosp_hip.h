@@ -103,6 +103,7 @@ int mosp_ws_reserve(mosp_ws *ws, int32_t capacity, int32_t slots);
 void mosp_ws_destroy(mosp_ws *ws);
 /* Co-resident workgroups of the persistent launch (0 = occupancy maximum). */
 int mosp_ws_set_grid(mosp_ws *ws, int32_t workgroups);
+/* Workgroups of the last launch (before any: of a 1024-thread launch). */
 int32_t mosp_ws_grid(const mosp_ws *ws);
 /* Longest wait of a workgroup at a grid barrier before the persistent
  * launch gives up (ms, default 2000 or MOSP_BARRIER_TIMEOUT_MS).  No

@@ -225,7 +225,7 @@ int setup(mosp_batch *b, int maxInstances, int maxDescs, size_t slabBytes) {
   }
   local_hops_override(b->hopsUpdate, b->hopsSearch);
   if (const char *env = std::getenv("MOSP_ASYNC_FLAGS")) {
-    b->asyncFlags = static_cast<unsigned>(std::atoi(env)) & (ASYNC_CARRY | ASYNC_HUNGRY);
+    b->asyncFlags = static_cast<unsigned>(std::atoi(env)) & ASYNC_FLAG_MASK;
   }
   b->workgroups = b->maxBlocks;  // every co-resident workgroup of the shape
   maxInstances = std::max(maxInstances, 1);
@@ -531,8 +531,8 @@ int prepare(mosp_batch *b) {
     if (async) {
       a.async = b->dAsync + i;
       a.segCap = segCap;
-      a.spillHi = qcap / 4;
-      a.spillLo = qcap / 8;
+      a.spillHi = std::min(qcap / 4, async_ring_entries(b->shapeThreads) / 2);
+      a.spillLo = a.spillHi / 2;
       a.asyncFlags = b->asyncFlags;
       a.keepHungry = static_cast<unsigned>(b->shapeThreads >> std::max(0, b->laneBits));
     }

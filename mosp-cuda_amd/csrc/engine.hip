@@ -123,6 +123,15 @@ struct mosp_ws {
   int *cDegree = nullptr, *cRowPtr = nullptr, *cColInd = nullptr,
       *cWeights = nullptr, *cPartials = nullptr;
   int4 *cEdge4 = nullptr;   // combined records with row ranges (asynchronous loop)
+  // The combined graph as a mask over a device graph's out-edges (fused
+  // launches of graphs without hubs with the asynchronous loop): one record
+  // per edge, and the combined descriptor naming them; the CSR descriptor
+  // is kept for launches that build the combined CSR instead.
+  bool combMask = true;     // MOSP_COMB_MASK=0: always build the combined CSR
+  int4 *cMask = nullptr;
+  size_t cMaskCap = 0;      // records
+  int cMaskIndex = -1;      // descriptor slot of the combined search, or -1
+  ObjDesc cMaskDesc{}, cCsrDesc{};
   double clockKhz = 100000;  // wall_clock64 rate
   int maxBlocks = 0;        // cooperative residency limit (1024-thread shape)
   int maxBlocks512 = 0;     // ... of the 512-thread shape
@@ -151,6 +160,7 @@ struct mosp_ws {
   // work by 12 % for no net gain).
   unsigned asyncFlags = ASYNC_HUNGRY;
   bool lastAsync = false;  // the last launch ran the asynchronous loop
+  int lastGrid = 0;         // workgroups of the last launch
   unsigned keepHungry = 0;            // 0: one batch (BLOCK >> laneBits)
 };
 
@@ -172,6 +182,9 @@ void ws_release_combined(mosp_ws *ws) {
   (void)hipFree(ws->cColInd);  // cWeights lies in the same allocation
   (void)hipFree(ws->cEdge4);
   ws->cEdge4 = nullptr;
+  (void)hipFree(ws->cMask);
+  ws->cMask = nullptr;
+  ws->cMaskCap = 0;
   ws->cDegree = ws->cRowPtr = ws->cColInd = ws->cWeights = nullptr;
   ws->cCapacity = 0;
   ws->cK = 0;
@@ -240,7 +253,10 @@ int ws_init(mosp_ws *ws) {
     }
   }
   if (const char *env = std::getenv("MOSP_ASYNC_FLAGS")) {
-    ws->asyncFlags = static_cast<unsigned>(std::atoi(env)) & (ASYNC_CARRY | ASYNC_HUNGRY);
+    ws->asyncFlags = static_cast<unsigned>(std::atoi(env)) & ASYNC_FLAG_MASK;
+  }
+  if (const char *env = std::getenv("MOSP_COMB_MASK")) {
+    ws->combMask = std::atoi(env) != 0;
   }
   if (const char *env = std::getenv("MOSP_ASYNC_KEEP")) {
     ws->keepHungry = static_cast<unsigned>(std::max(0, std::atoi(env)));
@@ -562,18 +578,7 @@ int run_launch(mosp_ws *ws, LaunchArgs &args, int descCount,
                                             : static_cast<unsigned>(MOSP_QUEUE_CAP_MAX);
   args.async = nullptr;
   args.segCap = ws->segCap;
-  args.spillHi = ws->spillHi > 0 ? std::min(ws->spillHi, qcap - 1) : qcap / 4;
-  args.spillLo = ws->spillHi > 0 ? std::min(ws->spillLo, args.spillHi - 1) : qcap / 8;
   args.asyncFlags = ws->asyncFlags;
-  args.keepHungry = ws->keepHungry > 0 ? ws->keepHungry
-                                       : static_cast<unsigned>(BLOCK >> std::max(0, ws->laneBits));
-  if (descCount > 0) {
-    HIP_TRY(hipMemcpyAsync(ws->dDesc, ws->hDesc, sizeof(ObjDesc) * descCount,
-                           hipMemcpyHostToDevice, stream));
-  }
-  HIP_TRY(launch_ctl_reset(ws->dCtl, stream));
-  void *params[] = {&args};
-  HIP_TRY(hipEventRecord(ws->ev0, stream));
   const int forced = hubs_override();
   const bool hubs = forced >= 0 ? forced != 0 : args.hubs != 0;
   // Shape: the caller's, else 512-thread workgroups two per CU for the hub
@@ -585,7 +590,37 @@ int run_launch(mosp_ws *ws, LaunchArgs &args, int descCount,
   if (ws->async && !hubs && ws->segCap > 0 && !ws->state.empty() && grid <= ASYNC_MAX_GRID) {
     args.async = ws->dAsync;
   }
+  // Spill above a quarter of the queue capacity (at most half the local
+  // ring, so a batch's appends always fit) down to half of that; idle
+  // workgroups elsewhere take all but one batch.
+  const unsigned ring = std::min(qcap, async_ring_entries(shape.threads));
+  args.spillHi = ws->spillHi > 0 ? std::min(ws->spillHi, qcap - 1) : std::min(qcap / 4, ring / 2);
+  args.spillLo = ws->spillHi > 0 ? std::min(ws->spillLo, args.spillHi - 1) : args.spillHi / 2;
+  args.keepHungry = ws->keepHungry > 0
+                        ? ws->keepHungry
+                        : static_cast<unsigned>(shape.threads >> std::max(0, ws->laneBits));
   ws->lastAsync = args.async != nullptr;
+  ws->lastGrid = grid;
+  // The masked combined graph (set up by the fused entry point for this
+  // launch) is searched by the asynchronous loop only; otherwise the launch
+  // builds the combined CSR and searches it.
+  if (args.cMaskRec != nullptr) {
+    const bool inRange = ws->cMaskIndex >= 0 && ws->cMaskIndex < descCount;
+    const bool mask = args.async != nullptr && inRange;
+    if (inRange) {
+      ws->hDesc[ws->cMaskIndex] = mask ? ws->cMaskDesc : ws->cCsrDesc;
+    }
+    if (!mask) {
+      args.cMaskRec = nullptr;
+    }
+  }
+  if (descCount > 0) {
+    HIP_TRY(hipMemcpyAsync(ws->dDesc, ws->hDesc, sizeof(ObjDesc) * descCount,
+                           hipMemcpyHostToDevice, stream));
+  }
+  HIP_TRY(launch_ctl_reset(ws->dCtl, stream));
+  void *params[] = {&args};
+  HIP_TRY(hipEventRecord(ws->ev0, stream));
   HIP_TRY(launch_persistent(shape.persistent(hubs), dim3(grid), dim3(shape.threads), params,
                             stream));
   HIP_TRY(hipEventRecord(ws->ev1, stream));
@@ -774,7 +809,9 @@ extern "C" int mosp_ws_set_grid(mosp_ws *ws, int32_t workgroups) {
   return 1;
 }
 
-extern "C" int32_t mosp_ws_grid(const mosp_ws *ws) { return ws->gridBlocks; }
+extern "C" int32_t mosp_ws_grid(const mosp_ws *ws) {
+  return ws->lastGrid > 0 ? ws->lastGrid : ws->gridBlocks;
+}
 
 extern "C" int mosp_ws_set_shape(mosp_ws *ws, int32_t threads, int32_t queue_cap) {
   clear_error();
@@ -1337,6 +1374,30 @@ extern "C" int mosp_update_fused(const mosp_device_graph *g, int32_t K,
   if (!setup_combined(ws, args, d_parents, n, K, pref, nPref, 0, d_comb_dist,
                       d_comb_parent, K, cstats, cpc)) {
     return 0;
+  }
+  // Graphs without hubs: the combined graph as a mask over the graph's
+  // out-edges (every tree edge is one), searched by the asynchronous loop.
+  ws->cMaskIndex = -1;
+  if (rec4 && ws->combMask && cpc.bits > 0) {
+    const size_t m = static_cast<size_t>(std::max(g->m, 1));
+    if (m > ws->cMaskCap) {
+      (void)hipFree(ws->cMask);
+      ws->cMask = nullptr;
+      ws->cMaskCap = 0;
+      HIP_TRY(hipMalloc(&ws->cMask, m * sizeof(int4)));
+      ws->cMaskCap = m;
+    }
+    mosp_csr out{}, in{};
+    mosp_device_graph_view(g, 0, &out, &in);
+    args.cMaskRec = ws->cMask;
+    args.cMaskSrc = g->outEdge4;
+    args.cMaskRow = out.row_ptr;
+    ws->cCsrDesc = ws->hDesc[K];
+    ws->cMaskDesc = ws->hDesc[K];
+    ws->cMaskDesc.outRow = out.row_ptr;
+    ws->cMaskDesc.outEdge = nullptr;
+    ws->cMaskDesc.outEdge4 = ws->cMask;
+    ws->cMaskIndex = K;
   }
   args.stage = reinterpret_cast<u64 *>(ws->cColInd);  // 8 * K * n bytes, free until Step 3
   args.objs = ws->dDesc;

@@ -179,11 +179,15 @@ constexpr u64 ASYNC_STATUS_INIT = 15ull << 60;  // a status word of no phase
 // LaunchArgs::asyncFlags
 constexpr unsigned ASYNC_CARRY = 1;   // local entries carry the word they were listed with
 constexpr unsigned ASYNC_HUNGRY = 2;  // spill to XCDs with idle workgroups
+constexpr unsigned ASYNC_FLAG_MASK = ASYNC_CARRY | ASYNC_HUNGRY;
 constexpr int ASYNC_MAX_GRID = 2048;  // workgroups a launch with pools may have
+// Entries of a workgroup's local queue (LDS ring) for `threads` threads.
+constexpr unsigned async_ring_entries(int threads) { return threads >= 1024 ? 2048u : 1024u; }
 
 struct AsyncCtl {
-  // Per workgroup: phase (8 bits) | entries appended (28) | entries processed
-  // (28), cumulative over the launch; written only by its workgroup.
+  // Per workgroup: phase (4 bits: the launch's phase sequence number mod 15)
+  // | entries appended (30) | entries processed (30), counted from the start
+  // of the phase; written only by its workgroup.
   u64 status[ASYNC_MAX_GRID];
   // Pool words {tail << 32 | head}, one 128-byte line each; two sets, used
   // by alternate phases (a set is cleared while the other is in use).
@@ -248,6 +252,16 @@ struct LaunchArgs {
   int4 *cEdge4;            // K * n records with the column's row range as well, written when
                            // the launch runs the asynchronous loop (else null)
   int *cPartials;          // one per workgroup (scan)
+  // The combined graph as a mask over the graph's out-edges (fused launches
+  // of graphs without hubs that run the asynchronous loop; null otherwise):
+  // every tree edge (p, v) is an out-edge of p, so the combined row of p is
+  // part of p's row in the graph.  cMaskRec[e] for edge e = (p, v): {v,
+  // combined weight, rowBegin(v), rowEnd(v)} when p is a parent of v (first
+  // such edge of p's row), else {source, 0, 0, 0} (the source is never
+  // relaxed); cDesc then names the graph's rows and these records.
+  int4 *cMaskRec;
+  const int4 *cMaskSrc;    // the graph's out-edge records with row ranges
+  const int *cMaskRow;     // the graph's out-row pointers
   u64 *stage;              // nobj * n words (may alias cColInd..cWeights, which are
                            // written only after it is read): the fused parent
                            // recovery's per-vertex records; nullptr: none

@@ -74,7 +74,7 @@ __device__ __forceinline__ T *uniform_global(T *p) {
 
 constexpr unsigned QCAP = MOSP_QUEUE_CAP_MAX;  // LDS queue entries per list per workgroup
 // Asynchronous near-far: local queue entries (a power of 2; 20 bytes each).
-constexpr unsigned LQ_CAP = BLOCK >= 1024 ? 2048 : 1024;
+constexpr unsigned LQ_CAP = async_ring_entries(BLOCK);
 
 // ---------------------------------------------------------------------------
 // Device helpers
@@ -1478,10 +1478,10 @@ __device__ int near_far_async(Desc *objsArg, int nobj, ObjCtl *ctlsArg, AsyncCtl
         sh.as.cur = sh.mine[0];
         async_seed_share(sh, sh.mine[0], t.block, n, listCap);
       }
-      sh.as.sent += 1;  // the seeding token
-      if (sh.nmine == 0) {
-        sh.as.recv += 1;
-      }
+      // Counts of this phase; the seeding token counts as appended until
+      // this workgroup's seeds are listed.
+      sh.as.sent = 1;
+      sh.as.recv = sh.nmine == 0 ? 1 : 0;
       sh.as.exit = 0;
       sh.as.polls = 0;
       sh.as.hungry = 0;
@@ -2683,6 +2683,98 @@ __device__ __forceinline__ void combined_edges(Args &a, int v, F &&f) {
   }
 }
 
+__device__ __forceinline__ void combined_delta(Args &a, Ctl *ctl, Threads t);
+
+// Sums of the combined edges and weights a thread counted, into the launch's
+// totals (one atomic per wave).
+__device__ __forceinline__ void combined_totals(Ctl *ctl, u64 edges, u64 wsum) {
+#pragma unroll
+  for (int offset = 32; offset > 0; offset >>= 1) {
+    edges += __shfl_xor(edges, offset);
+    wsum += __shfl_xor(wsum, offset);
+  }
+  if (__lane_id() == 0 && edges > 0) {
+    atomicAdd(&ctl->combEdges, edges);
+    atomicAdd(&ctl->combWeightSum, wsum);
+  }
+}
+
+// The combined graph as a mask over the graph's out-edges (cMaskRec): one
+// pass over the rows, no counters, scan or per-edge atomics.  Same edges,
+// weights, edge count and weight sum as combined_build.
+__device__ __attribute__((noinline)) void combined_masked(Args &args, GridBarrier &grid,
+                                                          Threads t) {
+  Args &a = *uniform(&args);
+  const int n = a.n;
+  const int K = a.cK;
+  const int source = a.source;
+  Ctl *ctl = a.ctl;
+  const int *const row = uniform_global(a.cMaskRow);
+  const int4 *const src = uniform_global(a.cMaskSrc);
+  int4 *const dst = uniform_global(a.cMaskRec);
+  const int *const par = uniform_global(a.cParents);
+  const size_t nn = static_cast<size_t>(n);
+  // A row is taken MU edges at a time: their records, then every parent
+  // word they need, are requested together (two round trips per chunk).
+  constexpr int MU = 4;
+  u64 edges = 0, wsum = 0;
+  for (int p = t.id; p < n; p += t.count) {
+    const int b = row[p], e = row[p + 1];
+    for (int x0 = b; x0 < e; x0 += MU) {
+      int4 r[MU];
+#pragma unroll
+      for (int j = 0; j < MU; ++j) {
+        r[j] = x0 + j < e ? src[x0 + j] : make_int4(source, 0, 0, 0);
+      }
+      int w[MU];
+#pragma unroll
+      for (int j = 0; j < MU; ++j) {
+        w[j] = a.cBase;
+      }
+      for (int k = 0; k < K; ++k) {
+        const int *const pk = par + k * nn;
+        const int term = a.cTerms[k];
+#pragma unroll
+        for (int j = 0; j < MU; ++j) {
+          if (r[j].x != source && pk[r[j].x] == p) {
+            w[j] -= term;
+          }
+        }
+      }
+#pragma unroll
+      for (int j = 0; j < MU; ++j) {
+        if (x0 + j >= e) {
+          break;
+        }
+        // A combined edge if some tree's parent of v is p; a repeated (p, v)
+        // edge of the row after the first is masked.
+        bool keep = w[j] != a.cBase;
+#pragma unroll
+        for (int i = 0; i < j; ++i) {
+          keep = keep && r[i].x != r[j].x;
+        }
+        for (int y = b; y < x0 && keep; ++y) {
+          keep = src[y].x != r[j].x;
+        }
+        if (keep) {
+          dst[x0 + j] = make_int4(r[j].x, w[j], r[j].z, r[j].w);
+          edges += 1;
+          wsum += static_cast<u64>(w[j]);
+        } else {
+          dst[x0 + j] = make_int4(source, 0, 0, 0);
+        }
+      }
+    }
+  }
+  combined_totals(ctl, edges, wsum);
+  if (!grid.sync()) {
+    return;
+  }
+  stamp(ctl, PHASE_COMB_COUNT, t.id);
+  stamp(ctl, PHASE_COMB_SCAN, t.id);
+  combined_delta(a, ctl, t);
+}
+
 // A call, not inlined: it then does not raise the register use of the
 // near-far loops around it.
 template <bool HUBS>
@@ -2730,15 +2822,7 @@ __device__ __attribute__((noinline)) void combined_build(Args &args, GridBarrier
       }
     }
   }
-#pragma unroll
-  for (int offset = 32; offset > 0; offset >>= 1) {
-    edges += __shfl_xor(edges, offset);
-    wsum += __shfl_xor(wsum, offset);
-  }
-  if (__lane_id() == 0 && edges > 0) {
-    atomicAdd(&ctl->combEdges, edges);
-    atomicAdd(&ctl->combWeightSum, wsum);
-  }
+  combined_totals(ctl, edges, wsum);
   if (!grid.sync()) {
     return;
   }
@@ -2840,9 +2924,15 @@ __device__ __attribute__((noinline)) void combined_build(Args &args, GridBarrier
       }
     });
   }
-  // Near-far width for Step 3: the caller's, or 32 * average weight /
-  // average out-degree (src/sospUpdateGpu.cu:607-616), in double as there,
-  // times the launch's scale (a tuning factor; outputs do not depend on it).
+  combined_delta(a, ctl, t);
+}
+
+// Near-far width for Step 3: the caller's, or 32 * average weight / average
+// out-degree (src/sospUpdateGpu.cu:607-616), in double as there, times the
+// launch's scale (a tuning factor; outputs do not depend on it).  Needs the
+// edge count and weight sum complete.
+__device__ __forceinline__ void combined_delta(Args &a, Ctl *ctl, Threads t) {
+  const int n = a.n;
   if (t.id == 0) {
     long long delta = a.cDelta;
     if (delta <= 0) {
@@ -3061,7 +3151,11 @@ __device__ __forceinline__ int run_phases(Args &a, GridBarrier &grid, const Thre
   if (a.phases & PH_COMBINED) {
     Desc &C = *desc(a.cDesc);
     ObjCtl &cc = a.ctl->obj[MOSP_MAX_OBJ];
-    combined_build<HUBS>(a, grid, t, sh);
+    if (!HUBS && a.cMaskRec != nullptr) {
+      combined_masked(a, grid, t);
+    } else {
+      combined_build<HUBS>(a, grid, t, sh);
+    }
     if (grid.aborted) {
       return generation;
     }

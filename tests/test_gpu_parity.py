@@ -434,7 +434,11 @@ ASYNC_KNOBS = [
     ("carried-words", {"MOSP_ASYNC_FLAGS": "1"}, None),
     ("carried-words-hungry", {"MOSP_ASYNC_FLAGS": "3", "MOSP_ASYNC_KEEP": "16"}, None),
     ("heavy-spills", {"MOSP_ASYNC_SPILL": "64,32", "MOSP_ASYNC_FLAGS": "0"}, None),
-    ("tiny-pools", {"MOSP_ASYNC_SPILL": "64,32", "MOSP_POOL_CAP": "256"}, "short_spills"),
+    # Two workgroups keep long local queues and spill at least 32 entries at
+    # a time into 32-entry pool segments: every second spill to a segment in
+    # a phase finds it full.
+    ("tiny-pools", {"MOSP_ASYNC_SPILL": "64,32", "MOSP_POOL_CAP": "32", "MOSP_GRID": "2"},
+     "short_spills"),
     # 256-entry local queue kept nearly full, one lane per vertex (batches of
     # up to 1024 entries): appends past the room are dropped, and the next
     # phase lists every vertex under the threshold again.
@@ -443,6 +447,9 @@ ASYNC_KNOBS = [
     ("two-workgroups", {"MOSP_GRID": "2"}, None),
     ("one-workgroup", {"MOSP_GRID": "1"}, None),
     ("512-thread", {"MOSP_WG_THREADS": "512", "MOSP_GRID": "7"}, None),
+    # The combined graph built as a CSR (count, scan, fill) instead of a mask
+    # over the graph's out-edges.
+    ("combined-csr", {"MOSP_COMB_MASK": "0"}, None),
 ]
 
 
@@ -455,7 +462,7 @@ def test_async_near_far_paths(monkeypatch, label, env, event):
     and the sweep phase after them) assert that they did."""
     for key in ("MOSP_ASYNC", "MOSP_ASYNC_FLAGS", "MOSP_ASYNC_KEEP", "MOSP_ASYNC_SPILL",
                 "MOSP_POOL_CAP", "MOSP_QUEUE_CAP", "MOSP_LANE_BITS", "MOSP_GRID",
-                "MOSP_WG_THREADS"):
+                "MOSP_WG_THREADS", "MOSP_COMB_MASK"):
         monkeypatch.delenv(key, raising=False)
     for key, value in env.items():
         monkeypatch.setenv(key, value)
@@ -493,6 +500,31 @@ def test_async_near_far_paths(monkeypatch, label, env, event):
     finally:
         ws.close()
         dg.close()
+
+
+@pytest.mark.parametrize("mask", ["1", "0"])
+def test_repeated_edges(monkeypatch, mask):
+    """Rows that hold the same (u, v) edge more than once, with other
+    weights: the trees take the lightest copy, and the combined graph has one
+    edge per distinct parent (the first copy in the parent's row when the
+    combined graph is a mask over the graph's edges; mask "0" builds its
+    CSR).  Fused pipeline against the oracle, edge count included."""
+    monkeypatch.setenv("MOSP_COMB_MASK", mask)
+    base = gen.grid_graph(45, 40, 3, 60, 0.15, 411)
+    rng = np.random.default_rng(412)
+    edges = []
+    for u in range(base.n):
+        for e in range(int(base.row[u]), int(base.row[u + 1])):
+            w = [base.weight(e, k) for k in range(3)]
+            edges.append((u, int(base.col[e]), w))
+            if rng.random() < 0.3:  # a copy, lighter or heavier per objective
+                edges.append((u, int(base.col[e]),
+                              [max(1, x + int(rng.integers(-20, 21))) for x in w]))
+    g = O.graph_from_edges(base.n, 3, edges)
+    assert g.m > base.m
+    for kind, seed in (("safe", 413), ("mixed", 414)):
+        b = gen.uniform_batch(g, 400, 50, seed, safe=(kind == "safe"))
+        check_pipeline(g, b, pref=[2, 1, 3], tag=f"repeated edges {kind} mask={mask}")
 
 
 def test_rmat_large_weights_fallback():
