@@ -504,6 +504,7 @@ int prepare(mosp_batch *b) {
     a.anyFallback = anyFallback;
     a.laneBits = b->laneBits;
     a.listCap = list_cap_override();
+    a.initPush = init_push_override();
     a.queueCap = b->queueCap;
     a.hubEdges = hub_edges_override();
     a.hopsUpdate = b->hopsUpdate;

@@ -128,6 +128,7 @@ struct mosp_ws {
   // per edge, and the combined descriptor naming them; the CSR descriptor
   // is kept for launches that build the combined CSR instead.
   bool combMask = true;     // MOSP_COMB_MASK=0: always build the combined CSR
+  int initPush = 0;         // LaunchArgs::initPush (MOSP_INIT_PUSH)
   int4 *cMask = nullptr;
   size_t cMaskCap = 0;      // records
   int cMaskIndex = -1;      // descriptor slot of the combined search, or -1
@@ -258,6 +259,7 @@ int ws_init(mosp_ws *ws) {
   if (const char *env = std::getenv("MOSP_COMB_MASK")) {
     ws->combMask = std::atoi(env) != 0;
   }
+  ws->initPush = init_push_override();
   if (const char *env = std::getenv("MOSP_ASYNC_KEEP")) {
     ws->keepHungry = static_cast<unsigned>(std::max(0, std::atoi(env)));
   }
@@ -572,6 +574,7 @@ int run_launch(mosp_ws *ws, LaunchArgs &args, int descCount,
   args.hubEdges = hub_edges_override();
   args.cPartials = ws->cPartials;
   args.cDeltaScale = ws->combDeltaScale;
+  args.initPush = ws->initPush;
   // The asynchronous near-far loop runs in the light single-update kernel
   // when the workspace has pools; spill thresholds from the queue capacity.
   const unsigned qcap = ws->queueCap >= 256 ? static_cast<unsigned>(ws->queueCap)
@@ -727,6 +730,15 @@ void local_hops_override(int &update, int &search) {
     update = std::max(0, std::min(8, u));
     search = std::max(0, std::min(8, c));
   }
+}
+
+// MOSP_INIT_PUSH=0|1|2: Step 1's candidates for invalidated vertices pushed
+// from the valid vertices when more than half are invalidated (0, default),
+// always pulled (1), always pushed (2); results are the same (tests).
+int init_push_override() {
+  const char *env = std::getenv("MOSP_INIT_PUSH");
+  const int v = env != nullptr ? std::atoi(env) : 0;
+  return v == 1 || v == 2 ? v : 0;
 }
 
 // MOSP_LIST_CAP=c: near / far lists of c entries instead of n, so that
@@ -1604,6 +1616,7 @@ extern "C" int mosp_sosp_update_noalloc(const mosp_csr *out, const mosp_csr *in,
     args.hopsSearch = kHopsSearch;
     local_hops_override(args.hopsUpdate, args.hopsSearch);
     args.listCap = list_cap_override();
+    args.initPush = init_push_override();
     args.hubEdges = hub_edges_override();
     args.hubs = hubs;
     args.genDev = genDev;

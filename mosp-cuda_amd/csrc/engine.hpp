@@ -218,6 +218,9 @@ struct LaunchArgs {
   int hubs;                // host only: launch the HUB_DEGREE-aware variant
   int queueCap;            // LDS queue entries the push loop uses: 256..MOSP_QUEUE_CAP_MAX
                            // (0: all of them)
+  int initPush;            // Step 1 candidates of invalidated vertices: 0 pushed from the
+                           // valid vertices when more than half are invalidated (graphs
+                           // without hubs), else pulled; 1 always pulled; 2 always pushed
   int hubEdges;            // hub variant: push vertices of more out-edges go to
                            // the objective's hub list (HUB_GRID; tests lower it)
   int *genDev;             // device-resident stamp generation (persists)

@@ -101,6 +101,8 @@ unsigned long long barrier_ticks(double ms, double clockKhz);
 // makes every such launch plain (the same grid, without the cooperative API).
 hipError_t launch_persistent(const void *kernel, dim3 grid, dim3 block, void **params,
                              hipStream_t stream);
+// MOSP_INIT_PUSH (LaunchArgs::initPush): 0 unset, 1 pull, 2 push.
+int init_push_override();
 // MOSP_LIST_CAP=c caps the near / far lists at c entries instead of n, so
 // that small inputs take the sweep that follows a full list (tests); 0 unset.
 int list_cap_override();

@@ -519,6 +519,33 @@ struct Threads {
   unsigned blocks;  // workgroups in the group
 };
 
+// Dense sweep over the vertices [0, n) of the launch's threads, four
+// vertices per thread in flight: load(v) of the four is issued before any
+// use(v, loaded), so a sweep waits for one round trip per four vertices
+// instead of one per vertex (stores in use() would otherwise keep the next
+// vertex's loads behind them).
+template <typename L, typename U>
+__device__ __forceinline__ void sweep4(int n, Threads t, L &&load, U &&use) {
+  const long long stride = t.count;
+  for (long long v0 = t.id; v0 < n; v0 += 4 * stride) {
+    decltype(load(0)) x[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const long long v = v0 + j * stride;
+      if (v < n) {
+        x[j] = load(static_cast<int>(v));
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const long long v = v0 + j * stride;
+      if (v < n) {
+        use(static_cast<int>(v), x[j]);
+      }
+    }
+  }
+}
+
 // ---------------------------------------------------------------------------
 // Workgroup-shared state
 // ---------------------------------------------------------------------------
@@ -2022,23 +2049,25 @@ __device__ void clear_roots(Desc &D, Threads t) {
 __device__ void unpack(Desc &D, int n, int source, Threads t) {
   const int bits = D.parentBits;
   clear_roots(D, t);
-  for (int v = t.id; v < n; v += t.count) {
-    const u64 word = D.s.packed[v];
+  const u64 *const packed = D.s.packed;
+  int64_t *const distOut = D.distOut;
+  int *const parentOut = D.parentOut;
+  const u64 noParent = D.noParent;
+  sweep4(n, t, [&](int v) { return packed[v]; }, [&](int v, u64 word) {
     if (bits > 0) {
       if (word == PACKED_INF) {
-        D.distOut[v] = DISTANCE_INF;
-        D.parentOut[v] = -1;
+        distOut[v] = DISTANCE_INF;
+        parentOut[v] = -1;
       } else {
-        D.distOut[v] = static_cast<int64_t>(dist_of(word, bits));
-        const u64 p = word & D.noParent;
-        D.parentOut[v] = p == D.noParent ? -1 : static_cast<int>(p);
+        distOut[v] = static_cast<int64_t>(dist_of(word, bits));
+        const u64 p = word & noParent;
+        parentOut[v] = p == noParent ? -1 : static_cast<int>(p);
       }
     } else {
-      D.distOut[v] = word == PACKED_INF ? DISTANCE_INF
-                                        : static_cast<int64_t>(word);
-      D.parentOut[v] = word == PACKED_INF || v == source ? -1 : INT_MAX;
+      distOut[v] = word == PACKED_INF ? DISTANCE_INF : static_cast<int64_t>(word);
+      parentOut[v] = word == PACKED_INF || v == source ? -1 : INT_MAX;
     }
-  }
+  });
 }
 
 // Edge-parallel sweep of this workgroup's even, contiguous share of the m
@@ -2294,9 +2323,15 @@ __device__ __forceinline__ int update_step1(Args &a, GridBarrier &grid, Threads 
     Desc &D = desc(a.objs)[k];
     anyChanged = anyChanged || D.nChanged > 0;
     unsigned overflow = 0;
-    for (int v = t.id; v < n; v += t.count) {
-      const int64_t d = D.distIn[v];
-      int p = D.parentIn[v];
+    const int64_t *const distIn = D.distIn;
+    const int *const parentIn = D.parentIn;
+    struct Old {
+      int64_t d;
+      int p;
+    };
+    sweep4(n, t, [&](int v) { return Old{distIn[v], parentIn[v]}; }, [&](int v, Old old) {
+      const int64_t d = old.d;
+      int p = old.p;
       if (p < -1 || p >= n) {
         overflow |= 2;  // not a vertex: reported, and never used as an index
         p = -1;
@@ -2317,7 +2352,7 @@ __device__ __forceinline__ int update_step1(Args &a, GridBarrier &grid, Threads 
       if (D.nChanged > 0) {
         D.s.ancestor[v] = p + 1;  // jump word: ancestor + 1, no flag
       }
-    }
+    });
     if (overflow) {
       atomicOr(&ctl->obj[k].overflow, overflow);
     }
@@ -2392,12 +2427,14 @@ __device__ __forceinline__ int update_step1(Args &a, GridBarrier &grid, Threads 
     Desc &D = desc(a.objs)[k];
     unsigned count = 0;
     if (D.nChanged > 0) {
-      for (int v = t.id; v < n; v += t.count) {
-        if (D.s.ancestor[v] & JUMP_FLAG) {
-          D.s.packed[v] = PACKED_INF;
+      const int *const anc = D.s.ancestor;
+      u64 *const packed = D.s.packed;
+      sweep4(n, t, [&](int v) { return anc[v]; }, [&](int v, int w) {
+        if (w & JUMP_FLAG) {
+          packed[v] = PACKED_INF;
           ++count;
         }
-      }
+      });
     }
     for (int i = t.id; i < D.nHeads; i += t.count) {
       const int v = D.heads[i];
@@ -2410,7 +2447,10 @@ __device__ __forceinline__ int update_step1(Args &a, GridBarrier &grid, Threads 
       atomicAdd(&ctl->obj[k].invalidated, count);
     }
   }
-  if (!grid.sync()) {
+  if (threadIdx.x < static_cast<unsigned>(a.nobj)) {
+    sh.src[threadIdx.x] = &ctl->obj[threadIdx.x].invalidated;
+  }
+  if (!grid.sync(sh.src, a.nobj, sh.snap)) {
     return generation;
   }
   stamp(ctl, PHASE_INVALIDATE, t.id);
@@ -2433,6 +2473,7 @@ __device__ __forceinline__ int update_step1(Args &a, GridBarrier &grid, Threads 
       }
     }
   };
+  const unsigned listCap = static_cast<unsigned>(a.listCap > 0 && a.listCap < n ? a.listCap : n);
   for (int k = 0; k < a.nobj; ++k) {
     Desc &D = desc(a.objs)[k];
     ObjCtl &c = ctl->obj[k];
@@ -2441,6 +2482,91 @@ __device__ __forceinline__ int update_step1(Args &a, GridBarrier &grid, Threads 
       sh.farMin = PACKED_INF;
     }
     __syncthreads();
+    // Most vertices invalidated (graphs without hubs): the valid vertices
+    // offer their candidates to their invalidated out-neighbours instead,
+    // the same (valid in-neighbour, invalidated vertex) pairs as the pull,
+    // so the same minima; insert heads still pull over all their in-edges.
+    // Improved vertices get their far mark, and the far count is raised past
+    // the list capacity so that epoch 0's split reads the pile from the far
+    // marks (no list appends here).
+    const bool push = !HUBS && D.nChanged > 0 && a.initPush != 1 &&
+                      (a.initPush == 2 || 2ull * sh.snap[k] > static_cast<u64>(n));
+    if (push) {
+      int *const anc = D.s.ancestor;
+      u64 *const packed = D.s.packed;
+      auto offer = [&](int v, u64 cand) {
+        const u64 prev = atomicMin(&packed[v], cand);
+        const u64 cd = dist_of(cand, bits);
+        if (cd < dist_of(prev, bits)) {
+          D.s.inFar[v] = 1;
+          atomicMin(&sh.farMin, cd);
+        }
+      };
+      sweep4(n, t, [&](int u) { return anc[u]; }, [&](int u, int w) {
+        if (w & JUMP_FLAG) {
+          return;
+        }
+        const u64 word = packed[u];
+        if (word == PACKED_INF) {
+          return;
+        }
+        const u64 du = dist_of(word, bits);
+        const int end = D.outRow[u + 1];
+        for (int x = D.outRow[u]; x < end; ++x) {
+          const int2 ew = out_edge(D, x);
+          const int v = ew.x;
+          if (v == a.source || !(anc[v] & JUMP_FLAG)) {
+            continue;
+          }
+          const u64 nd = du + static_cast<u64>(ew.y);
+          if (nd > D.maxDistance) {
+            if (D.optimistic) {
+              store_relaxed(&c.dropped, 1u);
+            }
+            continue;
+          }
+          offer(v, pack(nd, u, bits, D.noParent));
+        }
+      });
+      for (int i = t.id; i < D.nHeads; i += t.count) {
+        const int v = D.heads[i];
+        if (v == a.source) {
+          continue;
+        }
+        const u64 current = packed[v];
+        u64 best = current;
+        const int end = D.inRow[v + 1];
+        for (int x = D.inRow[v]; x < end; ++x) {
+          const int2 ew = in_edge(D, x);
+          const u64 word = packed[ew.x];
+          if (word == PACKED_INF) {
+            continue;
+          }
+          const u64 nd = dist_of(word, bits) + static_cast<u64>(ew.y);
+          if (nd > D.maxDistance) {
+            if (D.optimistic) {
+              store_relaxed(&c.dropped, 1u);
+            }
+            continue;
+          }
+          const u64 cand = pack(nd, ew.x, bits, D.noParent);
+          best = cand < best ? cand : best;
+        }
+        if (best < current) {
+          offer(v, best);
+        }
+      }
+      __syncthreads();
+      if (threadIdx.x == 0) {
+        if (sh.farMin != PACKED_INF) {
+          atomicMin(&c.farMin[0], sh.farMin);
+        }
+        if (t.block == 0) {
+          atomicAdd(&c.farCount[0], listCap + 1);
+        }
+      }
+      continue;
+    }
     auto candidate = [&](int x) -> u64 {
       const int2 ew = in_edge(D, x);
       const int u = ew.x;

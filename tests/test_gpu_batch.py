@@ -102,6 +102,21 @@ def test_batched_full_lists_fall_back_to_sweeps(small, monkeypatch):
     bu.close()
 
 
+@pytest.mark.parametrize("mode", ["1", "2"])
+def test_batched_step1_pull_and_push(small, monkeypatch, mode):
+    """Step 1's candidates for invalidated vertices always pulled (1) or
+    always pushed from the valid vertices (2) in the batched kernel."""
+    specs, built = small
+    monkeypatch.setenv("MOSP_INIT_PUSH", mode)
+    sessions = [s for _, s in built]
+    bu = B().BatchedUpdate(sessions)
+    bu.run(use_graph=True)
+    bu.check()
+    for i, (spec, (host, _)) in enumerate(zip(specs, built)):
+        assert_same(bu.results(i), oracle_of(spec, host), f"instance {i} (Step 1 mode {mode})")
+    bu.close()
+
+
 def test_replays_and_direct_launch_agree(small):
     specs, built = small
     sessions = [s for _, s in built]

@@ -128,6 +128,32 @@ def test_families_and_batch_kinds(gname, kind):
         check_pipeline(g, b, pref, fused=(rep == 0), tag=f"{gname}/{kind}/{rep}")
 
 
+STEP1_MODES = [
+    # (label, environment): Step 1's candidates for the invalidated vertices
+    # pulled over their in-edges, or pushed from the valid vertices (epoch
+    # 0's pile then read from the far marks), under both near-far loops.
+    ("pull", {"MOSP_INIT_PUSH": "1"}),
+    ("push", {"MOSP_INIT_PUSH": "2"}),
+    ("push-synchronous", {"MOSP_INIT_PUSH": "2", "MOSP_ASYNC": "0"}),
+    ("push-capped-lists", {"MOSP_INIT_PUSH": "2", "MOSP_LIST_CAP": "16"}),
+]
+
+
+@pytest.mark.parametrize("label,env", STEP1_MODES, ids=[m[0] for m in STEP1_MODES])
+@pytest.mark.parametrize("kind", gen.BATCH_KINDS)
+def test_step1_push_and_pull(monkeypatch, label, env, kind):
+    """Every batch kind on grids (no hubs) with each way of seeding the
+    invalidated vertices, fused and serial schedules, against the oracle."""
+    for key, value in env.items():
+        monkeypatch.setenv(key, value)
+    for rep in range(2):
+        seed = 500 + 7 * gen.BATCH_KINDS.index(kind) + rep
+        g = gen.grid_graph(40, 30, 3, 100, 0.15, seed)
+        b = gen.make_batch(kind, g, 150, seed)
+        pref = [] if rep == 0 else [1, 2, 2]
+        check_pipeline(g, b, pref, fused=(rep == 0), tag=f"{label}/{kind}/{rep}")
+
+
 def test_fused_equals_serial_and_is_deterministic():
     g = gen.grid_graph(60, 50, 3, 100, 0.2, 5)
     b = gen.uniform_batch(g, 400, 50, 6, wmax=100)
