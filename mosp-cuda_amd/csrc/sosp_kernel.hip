@@ -610,6 +610,7 @@ struct Shared {
     unsigned lqHead, lqTail;     // entries [lqHead, lqTail) are listed (ring indices mod LQ_CAP)
     int cur;                     // objective of the local queue and of the far queue
     int step;                    // what the workgroup does next (STEP_*)
+    int claimed;                 // outcome of a claim: STEP_POOL or STEP_IDLE
     unsigned count;              // entries of the current batch
     int seg;                     // pool segment of a pool batch ...
     unsigned base;               // ... and its first slot
@@ -617,6 +618,11 @@ struct Shared {
     int seedIdx;                 // index into mine[] of the objective being seeded
     unsigned seedPos, seedEnd;   // the rest of this workgroup's share of its seeds
     int exit;                    // the phase is over
+    // Set by thread 0 before the barrier that ends a batch, read by every
+    // thread after it: the next batch may already change exit and the far
+    // queue while slower threads still decide on the last one.
+    int leave;                   // exit was 2: leave the loop
+    int flushFar;                // publish the far queue now
     int lostNear;                // a near entry found no room in this batch
     unsigned polls;              // consecutive idle polls
     unsigned batches;
@@ -1409,6 +1415,24 @@ __device__ int near_far_async(Desc *objsArg, int nobj, ObjCtl *ctlsArg, AsyncCtl
     sh.as.batches = 0;
   }
   __syncthreads();
+  // Thread 0: what the workgroup does next, given `have` listed entries;
+  // published by the barrier that ends the current step.
+  auto decide = [&](unsigned have) {
+    const bool seeding = sh.as.seedIdx < sh.nmine;
+    int step;
+    if (seeding && have < perPass && have + seedPass <= lcap &&
+        (have == 0 || sh.as.cur == sh.mine[sh.as.seedIdx])) {
+      step = STEP_SEED;
+    } else if (have > 0) {
+      step = STEP_LOCAL;
+      sh.as.count = have < perPass ? have : perPass;
+    } else if (seeding) {
+      step = STEP_SEED;
+    } else {
+      step = STEP_CLAIM;
+    }
+    sh.as.step = step;
+  };
   for (bool first = true;; first = false) {
     const unsigned seq = phaseSeq;
     if (threadIdx.x == 0) {
@@ -1513,39 +1537,26 @@ __device__ int near_far_async(Desc *objsArg, int nobj, ObjCtl *ctlsArg, AsyncCtl
       sh.as.polls = 0;
       sh.as.hungry = 0;
       store_relaxed(&A->status[t.block], status_word(seq, sh.as.sent, sh.as.recv));
+      decide(sh.as.lqTail - sh.as.lqHead);
     }
     __syncthreads();
     const u64 phaseStart = wall_clock64();
-    unsigned tailBefore = 0;  // thread 0
+    unsigned tailBefore = sh.as.lqTail;  // thread 0: the ring's tail when a step begins
     for (;;) {
+      // The step was decided by thread 0 before the barrier that ended the
+      // previous one (or the phase's start).  sh.as.step is written only by
+      // thread 0 right before such a barrier, so every thread reads the same
+      // value.
       SUBPROF_MARK(tStep);
-      if (threadIdx.x == 0) {
-        const unsigned have = sh.as.lqTail - sh.as.lqHead;
-        const bool seeding = sh.as.seedIdx < sh.nmine;
-        int step;
-        if (seeding && have < perPass && have + seedPass <= lcap &&
-            (have == 0 || sh.as.cur == sh.mine[sh.as.seedIdx])) {
-          step = STEP_SEED;
-        } else if (have > 0) {
-          step = STEP_LOCAL;
-          sh.as.count = have < perPass ? have : perPass;
-        } else if (seeding) {
-          step = STEP_SEED;
-        } else {
-          step = STEP_CLAIM;
-        }
-        sh.as.step = step;
-        tailBefore = sh.as.lqTail;
-      }
-      __syncthreads();
-      if (sh.as.step == STEP_SEED && sh.mine[sh.as.seedIdx] != sh.as.cur) {
+      const int decided = sh.as.step;
+      if (decided == STEP_SEED && sh.mine[sh.as.seedIdx] != sh.as.cur) {
         async_flush_far(objs, ctls, sh, sh.as.cur, listCap, qcap);
         if (threadIdx.x == 0) {
           sh.as.cur = sh.mine[sh.as.seedIdx];
         }
         __syncthreads();
       }
-      if (sh.as.step == STEP_CLAIM) {
+      if (decided == STEP_CLAIM) {
         // Wave 0 looks at two pool segments of every active objective --
         // its XCD's (lanes 0-31), where spills for its idle workgroups land,
         // and one other in turn (lanes 32-63) -- and takes up to perPass
@@ -1579,14 +1590,16 @@ __device__ int near_far_async(Desc *objsArg, int nobj, ObjCtl *ctlsArg, AsyncCtl
           } else if (some != 0) {
             pick = __ffsll(static_cast<long long>(some)) - 1;
           }
+          // The outcome goes to its own word: slower waves may still be
+          // reading sh.as.step.
           if (lane == 0) {
-            sh.as.step = STEP_IDLE;
+            sh.as.claimed = STEP_IDLE;
           }
           if (lane == pick) {
             const unsigned c = avail < perPass ? avail : perPass;
             const u64 old = atomicCAS(pw, word, word + c);
             if (old == word) {
-              sh.as.step = STEP_POOL;
+              sh.as.claimed = STEP_POOL;
               sh.as.count = c;
               sh.as.seg = seg;
               sh.as.base = head;
@@ -1595,11 +1608,11 @@ __device__ int near_far_async(Desc *objsArg, int nobj, ObjCtl *ctlsArg, AsyncCtl
           }
         }
         __syncthreads();
-        if (threadIdx.x == 0 && sh.as.hungry != 0 && sh.as.step == STEP_POOL) {
+        if (threadIdx.x == 0 && sh.as.hungry != 0 && sh.as.claimed == STEP_POOL) {
           atomicSub(&A->hungry[seq & 1][mySeg][0], 1u);
           sh.as.hungry = 0;
         }
-        if (sh.as.step == STEP_POOL) {
+        if (sh.as.claimed == STEP_POOL) {
           if (sh.as.obj != sh.as.cur) {
             async_flush_far(objs, ctls, sh, sh.as.cur, listCap, qcap);
             if (threadIdx.x == 0) {
@@ -1612,7 +1625,9 @@ __device__ int near_far_async(Desc *objsArg, int nobj, ObjCtl *ctlsArg, AsyncCtl
           __syncthreads();
         }
       }
-      const int step = sh.as.step;
+      const int step = decided == STEP_CLAIM ? sh.as.claimed : decided;
+      const unsigned batchCount = sh.as.count;
+      (void)batchCount;
       SUBPROF_MARK(tDecided);
       if (step == STEP_IDLE) {
         // Nothing to take: workgroup 0 checks for the end of the phase, the
@@ -1660,6 +1675,7 @@ __device__ int near_far_async(Desc *objsArg, int nobj, ObjCtl *ctlsArg, AsyncCtl
           }
         }
         if (threadIdx.x == 0 && sh.as.exit == 0) {
+          sh.as.step = STEP_CLAIM;  // the ring is empty and the seeds are listed
           if (useHungry && sh.as.hungry == 0) {
             atomicAdd(&A->hungry[seq & 1][mySeg][0], 1u);
             sh.as.hungry = 1;
@@ -1945,32 +1961,36 @@ __device__ int near_far_async(Desc *objsArg, int nobj, ObjCtl *ctlsArg, AsyncCtl
           store_relaxed(abortWord, 1u);
           sh.as.exit = 2;
         }
+        // The spilled entries leave the ring (copied out below, before any
+        // later append), then the next step.
+        sh.as.lqTail -= sh.as.spillKeep;
+        decide(sh.as.lqTail - sh.as.lqHead);
+        tailBefore = sh.as.lqTail;
+        sh.as.leave = sh.as.exit == 2 ? 1 : 0;
+        sh.as.flushFar = sh.far.count[0] > qcap / 2 ? 1 : 0;
       }
       __syncthreads();
-      if (sh.as.exit == 2) {
+      if (sh.as.leave) {
         break;
       }
       const unsigned keep = sh.as.spillKeep;
       if (keep > 0) {
-        // The newest `keep` entries go to slots [spillBase, spillBase + keep).
-        const unsigned from = sh.as.lqTail - keep;
+        // The newest `keep` entries, now past the tail, go to slots
+        // [spillBase, spillBase + keep) before the next step appends.
+        const unsigned from = sh.as.lqTail;
         int *const seg = D.s.pool + static_cast<size_t>(sh.as.spillSeg) * segCap + sh.as.spillBase;
         for (unsigned j = threadIdx.x; j < keep; j += BLOCK) {
           store_relaxed(seg + j, sh.lq.v[(from + j) & (LQ_CAP - 1)]);
         }
         __syncthreads();
-        if (threadIdx.x == 0) {
-          sh.as.lqTail -= keep;
-        }
       }
-      if (sh.far.count[0] > qcap / 2) {
+      if (sh.as.flushFar) {
         async_flush_far(objs, ctls, sh, k, listCap, qcap);
       }
-      __syncthreads();
       SUBPROF_MARK(tBusy);
       SUBPROF_ADD(0, tStep, tBusy);
       if (step != STEP_SEED) {
-        SUBPROF_ADD(step == STEP_LOCAL ? 2 : 3, 0, sh.as.count);
+        SUBPROF_ADD(step == STEP_LOCAL ? 2 : 3, 0, batchCount);
       }
       SUBPROF_ADD(6, tStep, tDecided);
       SUBPROF_ADD(14, tDecided, tRelaxed);
