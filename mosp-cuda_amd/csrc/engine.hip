@@ -640,6 +640,12 @@ int run_launch(mosp_ws *ws, LaunchArgs &args, int descCount,
     const double tick = 1e3 / static_cast<double>(ws->clockKhz);  // us
     const double G = static_cast<double>(ws->gridBlocks);
     const double itU = std::max(1.0, h.prof[11] / G), itC = std::max(1.0, h.prof[15] / G);
+    if (h.clk[PHASE_COUNT] > h.clk[PHASE_INVALIDATE] && h.clk[PHASE_PULL] > h.clk[PHASE_COUNT]) {
+      std::fprintf(stderr, "pull split: sweep %.1f us, %llu heavy candidates %.1f us\n",
+                   (h.clk[PHASE_COUNT] - h.clk[PHASE_INVALIDATE]) * tick,
+                   static_cast<unsigned long long>(h.clk[PHASE_COUNT + 1]),
+                   (h.clk[PHASE_PULL] - h.clk[PHASE_COUNT]) * tick);
+    }
     if (args.async != nullptr) {
       // Asynchronous near-far: per workgroup and launch (ticks -> us).
       const double ph = std::max(1.0, h.prof[11] / G);

@@ -119,6 +119,7 @@ struct ObjCtl {
   unsigned epochs;
   unsigned jumpCount[3];   // pointer jumping: still-jumping list, slot = round % 3
   unsigned pullHeavy;      // pull candidates left for whole workgroups (hub variant)
+  unsigned pullCount;      // pull candidates listed for lane groups (hub variant)
   unsigned hubCount[3];    // grid-relaxed hubs listed for an iteration, slot = iteration % 3
   unsigned dropped;        // optimistic packing: a candidate or input distance was past
                            // maxDistance (the launch stops before writing results)

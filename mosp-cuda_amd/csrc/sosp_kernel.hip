@@ -288,7 +288,6 @@ constexpr int AGG_BITS = 10;
 constexpr int AGG_SLOTS = 1 << AGG_BITS;
 constexpr int AGG_PROBES = 8;
 constexpr int STAGE_MAX = 8;    // objectives of a staged parent recovery
-constexpr int PULL_WAVE = 8;       // pull: more in-edges -> the candidate's wave
 constexpr int PULL_BLOCK = 1024;   // pull: more in-edges -> a whole workgroup, after the sweep
 
 __device__ __forceinline__ u64 shfl_u64(u64 x, int src) {
@@ -2624,43 +2623,38 @@ __device__ __forceinline__ int update_step1(Args &a, GridBarrier &grid, Threads 
           best = cand < best ? cand : best;
         }
       } else {
-        if (want && end - begin > PULL_BLOCK) {
-          queue_push(sh.near, v, D.s.near[0], &c.pullHeavy, static_cast<unsigned>(n));
+        // Hub graphs: the sweep only lists the wanted vertices -- more than
+        // PULL_BLOCK in-edges: near[0], for whole workgroups; the others:
+        // near[1] (through the far queue, unused here), for lane groups
+        // over the whole grid (a vertex's pull is a chain of dependent
+        // reads: one lane per vertex, or a wave per vertex in turn, kept
+        // the grid waiting for its slowest wave).
+        (void)best;
+        if (want) {
+          if (end - begin > PULL_BLOCK) {
+            queue_push(sh.near, v, D.s.near[0], &c.pullHeavy, static_cast<unsigned>(n));
+          } else {
+            queue_push(sh.far, v, D.s.near[1], &c.pullCount, static_cast<unsigned>(n));
+          }
           want = false;
-          end = begin;
         }
-        const bool heavy = end - begin > PULL_WAVE;
-        if (!heavy) {
-          for (int x = begin; x < end; ++x) {
-            const u64 cand = candidate(x);
-            best = cand < best ? cand : best;
-          }
-        }
-        wave_each(heavy, [&](int src) {
-          const int hb = __shfl(begin, src), he = __shfl(end, src);
-          u64 part = PACKED_INF;
-          for (int x = hb + static_cast<int>(__lane_id()); x < he; x += 64) {
-            const u64 cand = candidate(x);
-            part = cand < part ? cand : part;
-          }
-          part = wave_min_u64(part);
-          if (static_cast<int>(__lane_id()) == src) {
-            best = part < best ? part : best;
-          }
-        });
       }
       if (want) {
         pulled(D, c, v, best, current);
       }
       __syncthreads();
-      if (sh.far.count[0] > QCAP - BLOCK) queue_flush(sh.far, D.s.far[0], &c.farCount[0], static_cast<unsigned>(n));
       if constexpr (HUBS) {
+        if (sh.far.count[0] > QCAP - BLOCK) queue_flush(sh.far, D.s.near[1], &c.pullCount, static_cast<unsigned>(n));
         if (sh.near.count[0] > QCAP - BLOCK) queue_flush(sh.near, D.s.near[0], &c.pullHeavy, static_cast<unsigned>(n));
+      } else {
+        if (sh.far.count[0] > QCAP - BLOCK) queue_flush(sh.far, D.s.far[0], &c.farCount[0], static_cast<unsigned>(n));
       }
     }
-    queue_flush(sh.far, D.s.far[0], &c.farCount[0], static_cast<unsigned>(n));
     if constexpr (HUBS) {
+      queue_flush(sh.far, D.s.near[1], &c.pullCount, static_cast<unsigned>(n));
       queue_flush(sh.near, D.s.near[0], &c.pullHeavy, static_cast<unsigned>(n));
+    } else {
+      queue_flush(sh.far, D.s.far[0], &c.farCount[0], static_cast<unsigned>(n));
     }
     if (threadIdx.x == 0 && sh.farMin != PACKED_INF) {
       atomicMin(&c.farMin[0], sh.farMin);
@@ -2669,10 +2663,80 @@ __device__ __forceinline__ int update_step1(Args &a, GridBarrier &grid, Threads 
   if constexpr (HUBS) {
     if (threadIdx.x < static_cast<unsigned>(a.nobj)) {
       sh.src[threadIdx.x] = &ctl->obj[threadIdx.x].pullHeavy;
+      sh.src[a.nobj + threadIdx.x] = &ctl->obj[threadIdx.x].pullCount;
     }
-    if (!grid.sync(sh.src, a.nobj, sh.snap)) {
+    if (!grid.sync(sh.src, 2 * a.nobj, sh.snap)) {
       return generation;
     }
+    // The listed vertices: lane groups of PULL_GROUP over the whole grid,
+    // each group one vertex at a time, striding over its in-edges.
+    constexpr unsigned PULL_GROUP = 16;
+    const unsigned gsub = threadIdx.x & (PULL_GROUP - 1);
+    const unsigned gid = t.id / PULL_GROUP;
+    const unsigned groups = t.count / PULL_GROUP;
+    for (int k = 0; k < a.nobj; ++k) {
+      Desc &D = desc(a.objs)[k];
+      ObjCtl &c = ctl->obj[k];
+      const int bits = D.parentBits;
+      if (threadIdx.x == 0) {
+        sh.farMin = PACKED_INF;
+      }
+      __syncthreads();
+      const unsigned cnt = min(sh.snap[a.nobj + k], static_cast<unsigned>(n));
+      for (unsigned base = 0; base < cnt; base += groups) {
+        const unsigned i = base + gid;
+        int v = -1;
+        u64 current = PACKED_INF, part = PACKED_INF;
+        if (i < cnt) {
+          v = D.s.near[1][i];
+          if (v < 0 || v >= n) {
+            v = -1;  // not a vertex: never used as an index
+          } else {
+            current = D.s.packed[v];
+            const int end = D.inRow[v + 1];
+            for (int x = D.inRow[v] + static_cast<int>(gsub); x < end;
+                 x += static_cast<int>(PULL_GROUP)) {
+              const int2 ew = in_edge(D, x);
+              const u64 word = D.s.packed[ew.x];
+              if (word != PACKED_INF) {
+                const u64 nd = dist_of(word, bits) + static_cast<u64>(ew.y);
+                if (nd <= D.maxDistance) {
+                  const u64 cand = pack(nd, ew.x, bits, D.noParent);
+                  part = cand < part ? cand : part;
+                } else if (D.optimistic) {
+                  store_relaxed(&c.dropped, 1u);
+                }
+              }
+            }
+          }
+        }
+        // Group minimum (every lane takes part).
+#pragma unroll
+        for (unsigned off = PULL_GROUP / 2; off > 0; off >>= 1) {
+          const u64 o = shfl_u64(part, static_cast<int>(__lane_id() ^ off));
+          part = o < part ? o : part;
+        }
+        if (gsub == 0 && v >= 0) {
+          pulled(D, c, v, part < current ? part : current, current);
+        }
+        __syncthreads();
+        if (sh.far.count[0] > QCAP - BLOCK) queue_flush(sh.far, D.s.far[0], &c.farCount[0], static_cast<unsigned>(n));
+      }
+      queue_flush(sh.far, D.s.far[0], &c.farCount[0], static_cast<unsigned>(n));
+      if (threadIdx.x == 0 && sh.farMin != PACKED_INF) {
+        atomicMin(&c.farMin[0], sh.farMin);
+      }
+    }
+#ifdef MOSP_SUBPROFILE
+    stamp(ctl, PHASE_COUNT, t.id);  // the pull's sweep is over; the heavy candidates follow
+    if (t.id == 0) {
+      unsigned heavy = 0;
+      for (int k = 0; k < a.nobj; ++k) {
+        heavy += sh.snap[k];
+      }
+      ctl->clk[PHASE_COUNT + 1] = heavy;
+    }
+#endif
     for (int k = 0; k < a.nobj; ++k) {
       Desc &D = desc(a.objs)[k];
       ObjCtl &c = ctl->obj[k];
