@@ -615,6 +615,7 @@ int run_launch(mosp_ws *ws, LaunchArgs &args, int descCount,
     }
     if (!mask) {
       args.cMaskRec = nullptr;
+      args.cPar4 = nullptr;
     }
   }
   if (descCount > 0) {
@@ -1410,6 +1411,8 @@ extern "C" int mosp_update_fused(const mosp_device_graph *g, int32_t K,
     args.cMaskRec = ws->cMask;
     args.cMaskSrc = g->outEdge4;
     args.cMaskRow = out.row_ptr;
+    // The combined CSR's arrays (8 K n bytes) are unused by a masked launch.
+    args.cPar4 = K >= 2 && K <= 4 ? reinterpret_cast<int4 *>(ws->cColInd) : nullptr;
     ws->cCsrDesc = ws->hDesc[K];
     ws->cMaskDesc = ws->hDesc[K];
     ws->cMaskDesc.outRow = out.row_ptr;

@@ -266,6 +266,11 @@ struct LaunchArgs {
   int4 *cMaskRec;
   const int4 *cMaskSrc;    // the graph's out-edge records with row ranges
   const int *cMaskRow;     // the graph's out-row pointers
+  int4 *cPar4;             // with cMaskRec, 2 <= K <= 4 and no distance-only objective:
+                           // each vertex's K new parents side by side (-1 beyond K),
+                           // written by the unpack, so the mask pass reads one 16-byte
+                           // record per edge instead of K words in K arrays (n records;
+                           // may alias cColInd); nullptr: the pass reads cParents
   u64 *stage;              // nobj * n words (may alias cColInd..cWeights, which are
                            // written only after it is read): the fused parent
                            // recovery's per-vertex records; nullptr: none

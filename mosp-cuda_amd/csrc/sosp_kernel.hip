@@ -2089,6 +2089,38 @@ __device__ void unpack(Desc &D, int n, int source, Threads t) {
   });
 }
 
+// unpack for every objective at once (NOBJ = 2-4 objectives, all with
+// packed parents), also writing each vertex's parents side by side for the
+// masked combined graph: par4[v] = {parent in tree 0, 1, 2, 3} (-1 beyond
+// NOBJ).
+template <int NOBJ>
+__device__ __forceinline__ void unpack_interleaved(Desc *objs, int n, Threads t, int4 *par4) {
+  for (int k = 0; k < NOBJ; ++k) {
+    clear_roots(objs[k], t);
+  }
+  for (int v = t.id; v < n; v += t.count) {
+    u64 w[NOBJ];
+#pragma unroll
+    for (int k = 0; k < NOBJ; ++k) {
+      w[k] = objs[k].s.packed[v];
+    }
+    int p[4] = {-1, -1, -1, -1};
+#pragma unroll
+    for (int k = 0; k < NOBJ; ++k) {
+      Desc &D = objs[k];
+      int64_t d = DISTANCE_INF;
+      if (w[k] != PACKED_INF) {
+        d = static_cast<int64_t>(dist_of(w[k], D.parentBits));
+        const u64 q = w[k] & D.noParent;
+        p[k] = q == D.noParent ? -1 : static_cast<int>(q);
+      }
+      D.distOut[v] = d;
+      D.parentOut[v] = p[k];
+    }
+    par4[v] = make_int4(p[0], p[1], p[2], p[3]);
+  }
+}
+
 // Edge-parallel sweep of this workgroup's even, contiguous share of the m
 // = rowPtr[n] edges, BLOCK edges a pass: f(u, e) for edge e of row u.
 // Balanced whatever the degrees (power-law generators put the hubs side by
@@ -2895,6 +2927,13 @@ __device__ __forceinline__ void combined_edges(Args &a, int v, F &&f) {
 
 __device__ __forceinline__ void combined_delta(Args &a, Ctl *ctl, Threads t);
 
+// The unpack writes the side-by-side parents (LaunchArgs::cPar4) and the
+// mask pass reads them.
+__device__ __forceinline__ bool par4_written(Args &a) {
+  return a.cPar4 != nullptr && a.cMaskRec != nullptr && !a.anyFallback &&
+         (a.phases & PH_COMBINED) && a.nobj == a.cK && a.cK >= 2 && a.cK <= 4;
+}
+
 // Sums of the combined edges and weights a thread counted, into the launch's
 // totals (one atomic per wave).
 __device__ __forceinline__ void combined_totals(Ctl *ctl, u64 edges, u64 wsum) {
@@ -2923,6 +2962,7 @@ __device__ __attribute__((noinline)) void combined_masked(Args &args, GridBarrie
   const int4 *const src = uniform_global(a.cMaskSrc);
   int4 *const dst = uniform_global(a.cMaskRec);
   const int *const par = uniform_global(a.cParents);
+  const int4 *const par4 = par4_written(a) ? uniform_global(a.cPar4) : nullptr;
   const size_t nn = static_cast<size_t>(n);
   // A row is taken MU edges at a time: their records, then every parent
   // word they need, are requested together (two round trips per chunk).
@@ -2941,13 +2981,28 @@ __device__ __attribute__((noinline)) void combined_masked(Args &args, GridBarrie
       for (int j = 0; j < MU; ++j) {
         w[j] = a.cBase;
       }
-      for (int k = 0; k < K; ++k) {
-        const int *const pk = par + k * nn;
-        const int term = a.cTerms[k];
+      if (par4 != nullptr) {
+        // One 16-byte record per edge holds the K parents of its head.
+        int4 q[MU];
 #pragma unroll
         for (int j = 0; j < MU; ++j) {
-          if (r[j].x != source && pk[r[j].x] == p) {
-            w[j] -= term;
+          q[j] = r[j].x != source ? par4[r[j].x] : make_int4(-1, -1, -1, -1);
+        }
+#pragma unroll
+        for (int j = 0; j < MU; ++j) {
+          w[j] -= (q[j].x == p ? a.cTerms[0] : 0) + (q[j].y == p ? a.cTerms[1] : 0) +
+                  (K > 2 && q[j].z == p ? a.cTerms[2] : 0) +
+                  (K > 3 && q[j].w == p ? a.cTerms[3] : 0);
+        }
+      } else {
+        for (int k = 0; k < K; ++k) {
+          const int *const pk = par + k * nn;
+          const int term = a.cTerms[k];
+#pragma unroll
+          for (int j = 0; j < MU; ++j) {
+            if (r[j].x != source && pk[r[j].x] == p) {
+              w[j] -= term;
+            }
           }
         }
       }
@@ -3298,6 +3353,14 @@ __device__ __forceinline__ int run_phases(Args &a, GridBarrier &grid, const Thre
     u64 *const stage = fused && a.nobj <= STAGE_MAX ? a.stage : nullptr;
     if (stage != nullptr) {
       unpack_staged(desc(a.objs), a.nobj, n, a.source, t, stage);
+    } else if (!HUBS && par4_written(a)) {
+      if (a.nobj == 2) {
+        unpack_interleaved<2>(desc(a.objs), n, t, a.cPar4);
+      } else if (a.nobj == 3) {
+        unpack_interleaved<3>(desc(a.objs), n, t, a.cPar4);
+      } else {
+        unpack_interleaved<4>(desc(a.objs), n, t, a.cPar4);
+      }
     } else {
       for (int k = 0; k < a.nobj; ++k) {
         unpack(desc(a.objs)[k], n, a.source, t);
