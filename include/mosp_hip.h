@@ -164,8 +164,8 @@ int mosp_sosp_update(const mosp_csr *out, const mosp_csr *in,
  * reference: it is for callers that cannot allocate device memory (serving
  * loops, harnesses that forbid hipMalloc).  Same arguments, semantics and
  * results as mosp_sosp_update, without the workspace: the scratch is one
- * static device array (40 bytes per vertex, for at most
- * MOSP_NOALLOC_MAX_VERTICES = 2,097,152 vertices; about 84 MB that each GPU
+ * static device array (36 bytes per vertex, for at most
+ * MOSP_NOALLOC_MAX_VERTICES = 2,097,152 vertices; about 76 MB that each GPU
  * using the library holds until the process ends).  Larger graphs are
  * refused with an error.  Every call on a GPU uses that GPU's array, so
  * calls that overlap on one GPU race: run them one at a time per device.

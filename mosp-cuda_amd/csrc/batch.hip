@@ -258,7 +258,7 @@ size_t instance_bytes(int n, int K, int partials) {
 }
 
 // Describe `count` instances into b (within its capacity) and clear the
-// scratch they use (stamps, far flags and invalidation marks start at zero;
+// scratch they use (stamps and far marks start at zero;
 // generations restart).
 int load(mosp_batch *b, const mosp_batch_instance *spec, int count) {
   std::vector<Instance> inst(count);
@@ -737,7 +737,7 @@ extern "C" int mosp_batch_run(mosp_batch *b, int32_t use_graph, float *kernel_ms
   if (b->dirty && !prepare(b)) return 0;
   const int count = static_cast<int>(b->inst.size());
   if (b->scrub) {
-    // The last run ended early: its stamps, far marks and invalidation marks
+    // The last run ended early: its stamps and far marks
     // are not in the state the next run relies on.
     HIP_TRY(hipMemsetAsync(b->slab, 0, b->usedSlab, b->stream));
     HIP_TRY(hipMemsetAsync(b->dGen, 0, sizeof(int) * std::max(count, 1), b->stream));

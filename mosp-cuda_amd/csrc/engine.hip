@@ -319,7 +319,7 @@ unsigned pool_seg_cap(int n) {
 
 size_t state_bytes(int n, unsigned segCap) {
   const size_t count = static_cast<size_t>(std::max(n, 1));
-  return align256(count * sizeof(u64)) + 8 * align256(count * sizeof(int)) +
+  return align256(count * sizeof(u64)) + 7 * align256(count * sizeof(int)) +
          align256(2 * static_cast<size_t>(HUB_LIST) * sizeof(int)) +
          align256(static_cast<size_t>(ASYNC_SEGS) * segCap * sizeof(int));
 }
@@ -336,9 +336,9 @@ ObjState carve_state(char *&p, int n, unsigned segCap) {
   ObjState st{};
   st.packed = reinterpret_cast<u64 *>(p);
   p += wordBytes;
-  int **ints[8] = {&st.stamp, &st.inFar, &st.flag, &st.ancestor, &st.near[0],
+  int **ints[7] = {&st.stamp, &st.inFar, &st.ancestor, &st.near[0],
                    &st.near[1], &st.far[0], &st.far[1]};
-  for (int i = 0; i < 8; ++i) {
+  for (int i = 0; i < 7; ++i) {
     *ints[i] = reinterpret_cast<int *>(p);
     p += intBytes;
   }
@@ -377,7 +377,7 @@ int ws_reserve(mosp_ws *ws, int capacity, int slots) {
   const unsigned segCap = pool_seg_cap(n);
   const size_t perSlot = state_bytes(n, segCap);
   HIP_TRY(hipMalloc(&ws->slab, perSlot * slots));
-  // stamps, far flags and invalidation marks must start at zero
+  // stamps and far marks must start at zero
   HIP_TRY(hipMemset(ws->slab, 0, perSlot * slots));
   char *p = static_cast<char *>(ws->slab);
   ws->state.resize(slots);
@@ -1472,7 +1472,7 @@ constexpr size_t NA_WORDS = NA_DESC + (sizeof(ObjDesc) + 255) / 256 * 256;
 constexpr size_t NA_STATE = NA_WORDS + 256;
 constexpr size_t NA_STATE_BYTES =
     (static_cast<size_t>(NOALLOC_MAX_VERTICES) * sizeof(u64) + 255) / 256 * 256 +
-    8 * ((static_cast<size_t>(NOALLOC_MAX_VERTICES) * sizeof(int) + 255) / 256 * 256) +
+    7 * ((static_cast<size_t>(NOALLOC_MAX_VERTICES) * sizeof(int) + 255) / 256 * 256) +
     (2 * static_cast<size_t>(HUB_LIST) * sizeof(int) + 255) / 256 * 256;
 constexpr size_t NA_BYTES = NA_STATE + NA_STATE_BYTES;
 
@@ -1521,15 +1521,14 @@ int noalloc_device(NoAllocDevice **out) {
   return 1;
 }
 
-// Stamps, far marks and invalidation marks of the first n vertices, and the
-// generation word, back to zero: the last call on this device left them
-// as it ended (or in any state, had it given up).
-__global__ void noalloc_clear(int *stamp, int *inFar, int *flag, int n, int *gen) {
+// Stamps and far marks of the first n vertices, and the generation word,
+// back to zero: the last call on this device left them as it ended (or in
+// any state, had it given up).
+__global__ void noalloc_clear(int *stamp, int *inFar, int n, int *gen) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) {
     stamp[i] = 0;
     inFar[i] = 0;
-    flag[i] = 0;
   }
   if (i == 0) {
     *gen = 0;
@@ -1636,7 +1635,7 @@ extern "C" int mosp_sosp_update_noalloc(const mosp_csr *out, const mosp_csr *in,
 
     HIP_TRY(hipMemcpyAsync(dDesc, &d, sizeof(ObjDesc), hipMemcpyHostToDevice, stream));
     hipLaunchKernelGGL(noalloc_clear, dim3((n + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, stream,
-                       st.stamp, st.inFar, st.flag, n, genDev);
+                       st.stamp, st.inFar, n, genDev);
     HIP_TRY(hipGetLastError());
     HIP_TRY(launch_ctl_reset(dCtl, stream));
     void *params[] = {&args};

@@ -34,7 +34,7 @@ constexpr int BLOCK = MOSP_BLOCK;    // threads per workgroup (wave64 multiple)
 #define MOSP_MIN_BLOCKS 1            // co-resident workgroups per CU the code is built for
 #endif
 // Vertices of the largest graph mosp_sosp_update_noalloc takes: its scratch
-// (40 bytes per vertex) is a static device array of that size.
+// (36 bytes per vertex) is a static device array of that size.
 #ifndef MOSP_NOALLOC_MAX_VERTICES
 #define MOSP_NOALLOC_MAX_VERTICES (1 << 21)
 #endif
@@ -67,7 +67,6 @@ struct ObjState {
   u64 *packed;     // (distance << b | parent) words, n
   int *stamp;      // generation a vertex was last listed in, n
   int *inFar;      // vertex is in the far pile, n (left all-zero)
-  int *flag;       // invalidation marks, n (left all-zero)
   int *ancestor;   // pointer-jumping ancestors, n
   int *near[2];    // near frontiers (ping-pong), n each
   int *far[2];     // far piles (ping-pong), n each

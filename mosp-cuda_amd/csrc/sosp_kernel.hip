@@ -2052,22 +2052,10 @@ __device__ int near_far_async(Desc *objsArg, int nobj, ObjCtl *ctlsArg, AsyncCtl
   return generation;
 }
 
-// The root marks of Step 1 back to zero for the next update: they are set
-// only at heads of changed edges.
-__device__ void clear_roots(Desc &D, Threads t) {
-  if (D.s.flag == nullptr) {
-    return;
-  }
-  for (int i = t.id; i < D.nChanged; i += t.count) {
-    D.s.flag[D.chTo[i]] = 0;
-  }
-}
-
 // Words -> (distance, parent); distance-only words get INT_MAX parents for
 // the recovery pass (src/sospUpdateGpu.cu:478-497).
 __device__ void unpack(Desc &D, int n, int source, Threads t) {
   const int bits = D.parentBits;
-  clear_roots(D, t);
   const u64 *const packed = D.s.packed;
   int64_t *const distOut = D.distOut;
   int *const parentOut = D.parentOut;
@@ -2095,9 +2083,6 @@ __device__ void unpack(Desc &D, int n, int source, Threads t) {
 // NOBJ).
 template <int NOBJ>
 __device__ __forceinline__ void unpack_interleaved(Desc *objs, int n, Threads t, int4 *par4) {
-  for (int k = 0; k < NOBJ; ++k) {
-    clear_roots(objs[k], t);
-  }
   for (int v = t.id; v < n; v += t.count) {
     u64 w[NOBJ];
 #pragma unroll
@@ -2199,9 +2184,6 @@ __device__ void recover_edges(Desc *objs, int count, int n, int source, Threads 
 // recover_staged: one record per vertex instead of K arrays.
 __device__ void unpack_staged(Desc *objs, int count, int n, int source, Threads t,
                               u64 *stage) {
-  for (int k = 0; k < count; ++k) {
-    clear_roots(objs[k], t);
-  }
   for (int v = t.id; v < n; v += t.count) {
     for (int k = 0; k < count; ++k) {
       Desc &D = objs[k];
@@ -2320,34 +2302,15 @@ constexpr int JUMP_FLAG = INT_MIN;
 // 0.359 / 0.328 ms for the whole invalidation at 1 / 2 / 3 jumps.
 constexpr int JUMPS = 3;
 
-// Round 0, every vertex: roots[] marks the roots (heads of changed tree
-// edges), which get their flag here.  Returns true if v still jumps.
-__device__ __forceinline__ bool jump_first(int *word, const int *roots, int v) {
-  const int w0 = word[v];
-  if (roots[v]) {
-    word[v] = w0 | JUMP_FLAG;
-    return false;
-  }
-  int w = w0;
-  for (int j = 0; j < JUMPS && w != 0; ++j) {
-    const int x = w - 1;
-    const int wx = word[x];
-    if (roots[x] || (wx & JUMP_FLAG)) {
-      word[v] = w0 | JUMP_FLAG;
-      return false;
-    }
-    w = wx;
-  }
-  if (w != w0) {
-    word[v] = w;
-  }
-  return w != 0;
-}
-
-// Later rounds, vertices still jumping (unflagged, with an ancestor): every
-// root carries its flag by now, so each ancestor's word alone decides.
+// One round for vertex v: every root carries its flag before round 0, so
+// each ancestor's word alone decides.  Round 0 takes every vertex (a root
+// is done at once); later rounds the vertices still jumping (unflagged,
+// with an ancestor).  Returns true if v still jumps.
 __device__ __forceinline__ bool jump_next(int *word, int v) {
   const int w0 = word[v];
+  if (w0 & JUMP_FLAG) {
+    return false;
+  }
   int w = w0;
   for (int j = 0; j < JUMPS && w != 0; ++j) {
     const int wx = word[w - 1];
@@ -2407,15 +2370,26 @@ __device__ __forceinline__ int update_step1(Args &a, GridBarrier &grid, Threads 
     if (overflow) {
       atomicOr(&ctl->obj[k].overflow, overflow);
     }
-    for (int i = t.id; i < D.nChanged; i += t.count) {
-      const int v = D.chTo[i];
-      if (D.parentIn[v] == D.chFrom[i]) {
-        D.s.flag[v] = 1;
-      }
-    }
   }
   if (!grid.sync()) {
     return generation;
+  }
+  // The roots (heads of changed tree edges) carry their flag in their jump
+  // word before any round reads it: the rounds then read words only.
+  if (anyChanged) {
+    for (int k = 0; k < a.nobj; ++k) {
+      Desc &D = desc(a.objs)[k];
+      for (int i = t.id; i < D.nChanged; i += t.count) {
+        const int v = D.chTo[i];
+        const int p = D.parentIn[v];
+        if (p == D.chFrom[i]) {
+          D.s.ancestor[v] = (p + 1) | JUMP_FLAG;
+        }
+      }
+    }
+    if (!grid.sync()) {
+      return generation;
+    }
   }
   stamp(ctl, PHASE_PACK, t.id);
 
@@ -2436,7 +2410,6 @@ __device__ __forceinline__ int update_step1(Args &a, GridBarrier &grid, Threads 
           continue;
         }
         int *anc = D.s.ancestor;
-        int *flag = D.s.flag;
         int *outList = D.s.near[(round + 1) & 1];
         unsigned *outCount = &ctl->obj[k].jumpCount[(round + 1) % 3];
         const unsigned cnt = round == 0 ? static_cast<unsigned>(n) : sh.snap[k];
@@ -2445,7 +2418,7 @@ __device__ __forceinline__ int update_step1(Args &a, GridBarrier &grid, Threads 
           const unsigned i = base + threadIdx.x;
           if (i < cnt) {
             const int v = round == 0 ? static_cast<int>(i) : inList[i];
-            if (round == 0 ? jump_first(anc, flag, v) : jump_next(anc, v)) {
+            if (jump_next(anc, v)) {
               queue_push(sh.near, v, outList, outCount, static_cast<unsigned>(n));
             }
           }
@@ -3237,11 +3210,6 @@ __device__ bool dropped_any(Args &a, int count, Threads t) {
   bool dropped = false;
   for (int k = 0; k < count; ++k) {
     dropped = dropped || load_relaxed(&a.ctl->obj[k].dropped) != 0;
-  }
-  if (dropped) {
-    for (int k = 0; k < count; ++k) {
-      clear_roots(desc(a.objs)[k], t);
-    }
   }
   return dropped;
 }
