@@ -158,7 +158,7 @@ def main() -> int:
                     help="served: HBM budget (0 = resident state + room for every slot)")
     ap.add_argument("--instances", type=int, default=64)
     ap.add_argument("--tune-cache", default=str(ROOT / "profiles" / "autotune_batched.json"),
-                    help="autotuned batched launch choices (read; --retune re-times)")
+                    help="autotuned batched launch choices (read; a workload it lacks, or --retune, is timed and its choice stored there)")
     ap.add_argument("--retune", action="store_true")
     ap.add_argument("--headline-only", action="store_true",
                     help="plain mode: the 1.96M-vertex update alone, without the nested "
@@ -404,7 +404,7 @@ def batched_line(args) -> dict:
     if choice is None:
         tuned = autotune.tune(bu, log=log)
         choice = tuned["best"]
-        autotune.save(ROOT / "gpurun_out" / "autotune_batched.json", key, choice)
+        autotune.save(args.tune_cache, key, choice)
     bu.configure(**choice)
     log(f"batched launch configuration: {choice}")
 
