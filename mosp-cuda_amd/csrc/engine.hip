@@ -120,6 +120,7 @@ struct mosp_ws {
   // combined graph
   int cCapacity = 0;
   int cK = 0;
+  int *cSlot = nullptr;     // combined edges' places in their parents' rows (K n)
   int *cDegree = nullptr, *cRowPtr = nullptr, *cColInd = nullptr,
       *cWeights = nullptr, *cPartials = nullptr;
   int4 *cEdge4 = nullptr;   // combined records with row ranges (asynchronous loop)
@@ -183,6 +184,8 @@ void ws_release_combined(mosp_ws *ws) {
   (void)hipFree(ws->cColInd);  // cWeights lies in the same allocation
   (void)hipFree(ws->cEdge4);
   ws->cEdge4 = nullptr;
+  (void)hipFree(ws->cSlot);
+  ws->cSlot = nullptr;
   (void)hipFree(ws->cMask);
   ws->cMask = nullptr;
   ws->cMaskCap = 0;
@@ -409,6 +412,7 @@ int ws_reserve_combined(mosp_ws *ws, int n, int K) {
   if (ws->async) {
     HIP_TRY(hipMalloc(&ws->cEdge4, nn * K * sizeof(int4)));
   }
+  HIP_TRY(hipMalloc(&ws->cSlot, nn * K * sizeof(int)));
   ws->cCapacity = n;
   ws->cK = K;
   return 1;
@@ -1108,6 +1112,7 @@ int setup_combined(mosp_ws *ws, LaunchArgs &args, const int32_t *d_parents,
   }
   args.cDelta = delta;
   args.cDegree = ws->cDegree;
+  args.cSlot = ws->cSlot;
   args.cRowPtr = ws->cRowPtr;
   args.cColInd = ws->cColInd;
   args.cWeights = ws->cWeights;

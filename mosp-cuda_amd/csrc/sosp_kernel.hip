@@ -2789,7 +2789,10 @@ __device__ __forceinline__ int update_step1(Args &a, GridBarrier &grid, Threads 
 // Combined graph of cK trees (src/combinedGraphGpu.cu:55-121) + its CSR.
 // ---------------------------------------------------------------------------
 
-__device__ __forceinline__ void agg_add(int *keys, int *counts, int *global, int p) {
+// slot != nullptr: a parent that takes the global path gets its place in
+// the parent's row written there.
+__device__ __forceinline__ void agg_add(int *keys, int *counts, int *global, int p,
+                                        int *slot = nullptr) {
   unsigned h = (static_cast<unsigned>(p) * 2654435761u) >> (32 - AGG_BITS);
   for (int probe = 0; probe < AGG_PROBES; ++probe) {
     const int key = keys[h];  // a slot's key only ever goes from -1 to a parent
@@ -2806,7 +2809,10 @@ __device__ __forceinline__ void agg_add(int *keys, int *counts, int *global, int
     }
     h = (h + 1) & (AGG_SLOTS - 1);
   }
-  atomicAdd(&global[p], 1);
+  const int at = atomicAdd(&global[p], 1);
+  if (slot != nullptr) {
+    *slot = at;
+  }
 }
 
 // Slot of key p in the table agg_add filled, or -1 (p took the global
@@ -2853,7 +2859,8 @@ __device__ __forceinline__ bool combined_edge(Args &a, int v, int k,
 
 constexpr int COMB_REG = 8;  // trees whose parents a lane holds in registers
 
-// f(p, weight) for every combined edge (p, v): v's parents are read once
+// f(p, weight, k) for every combined edge (p, v), k the first tree where p is
+// v's parent: v's parents are read once
 // (K <= COMB_REG: into registers, every test from them), in tree order.
 template <typename F>
 __device__ __forceinline__ void combined_edges(Args &a, int v, F &&f) {
@@ -2862,7 +2869,7 @@ __device__ __forceinline__ void combined_edges(Args &a, int v, F &&f) {
     for (int k = 0; k < K; ++k) {
       int p, weight;
       if (combined_edge(a, v, k, p, weight)) {
-        f(p, weight);
+        f(p, weight, k);
       }
     }
     return;
@@ -2894,7 +2901,7 @@ __device__ __forceinline__ void combined_edges(Args &a, int v, F &&f) {
         weight -= a.cTerms[j];
       }
     }
-    f(pv[k], weight);
+    f(pv[k], weight, k);
   }
 }
 
@@ -3037,16 +3044,26 @@ __device__ __attribute__((noinline)) void combined_build(Args &args, GridBarrier
     }
     __syncthreads();
   }
+  // With cSlot, every increment's return value is kept: a direct one's is
+  // the edge's place in its parent's row (cSlot), a workgroup's aggregated
+  // one the base of its run there (aggCount), so the fill needs no atomics
+  // on global counters.
+  int *const slotOf = a.cSlot;
+  const size_t nn = static_cast<size_t>(n);
   u64 edges = 0, wsum = 0;
   for (int v = t.id; v < n; v += t.count) {
     if (v == a.source) {
       continue;
     }
-    combined_edges(a, v, [&](int p, int weight) {
+    combined_edges(a, v, [&](int p, int weight, int k) {
+      int *global = slotOf != nullptr ? &slotOf[k * nn + v] : nullptr;
       if constexpr (HUBS) {
-        agg_add(sh.aggKey, sh.aggCount, a.cDegree, p);
+        agg_add(sh.aggKey, sh.aggCount, a.cDegree, p, global);
       } else {
-        atomicAdd(&a.cDegree[p], 1);
+        const int at = atomicAdd(&a.cDegree[p], 1);
+        if (global != nullptr) {
+          *global = at;
+        }
       }
       edges += 1;
       wsum += static_cast<u64>(weight);
@@ -3056,7 +3073,10 @@ __device__ __attribute__((noinline)) void combined_build(Args &args, GridBarrier
     __syncthreads();
     for (int i = threadIdx.x; i < AGG_SLOTS; i += BLOCK) {
       if (sh.aggCount[i] > 0) {
-        atomicAdd(&a.cDegree[sh.aggKey[i]], sh.aggCount[i]);
+        const int at = atomicAdd(&a.cDegree[sh.aggKey[i]], sh.aggCount[i]);
+        if (slotOf != nullptr) {
+          sh.aggCount[i] = at;  // the run's base; the fill counts from it
+        }
       }
     }
   }
@@ -3130,9 +3150,11 @@ __device__ __attribute__((noinline)) void combined_build(Args &args, GridBarrier
   // have hundreds of thousands of children), and LDS atomics hand out the
   // run's places.  Parents that took the global path count there.
   if constexpr (HUBS) {
-    for (int i = threadIdx.x; i < AGG_SLOTS; i += BLOCK) {
-      if (sh.aggCount[i] > 0) {
-        sh.aggCount[i] = atomicAdd(&a.cDegree[sh.aggKey[i]], sh.aggCount[i]);
+    if (slotOf == nullptr) {
+      for (int i = threadIdx.x; i < AGG_SLOTS; i += BLOCK) {
+        if (sh.aggCount[i] > 0) {
+          sh.aggCount[i] = atomicAdd(&a.cDegree[sh.aggKey[i]], sh.aggCount[i]);
+        }
       }
     }
     __syncthreads();
@@ -3148,9 +3170,20 @@ __device__ __attribute__((noinline)) void combined_build(Args &args, GridBarrier
       vBegin = a.cRowPtr[v];
       vEnd = a.cRowPtr[v + 1];
     }
-    combined_edges(a, v, [&](int p, int weight) {
+    combined_edges(a, v, [&](int p, int weight, int k) {
       int pos;
-      if constexpr (HUBS) {
+      if (slotOf != nullptr) {
+        // The place the count took: the workgroup's run (hub parents in its
+        // table) or the edge's own.
+        int at;
+        if constexpr (HUBS) {
+          const int slot = agg_find(sh.aggKey, p);
+          at = slot >= 0 ? atomicAdd(&sh.aggCount[slot], 1) : slotOf[k * nn + v];
+        } else {
+          at = slotOf[k * nn + v];
+        }
+        pos = a.cRowPtr[p] + at;
+      } else if constexpr (HUBS) {
         const int slot = agg_find(sh.aggKey, p);
         pos = slot >= 0 ? atomicAdd(&sh.aggCount[slot], 1) : atomicAdd(&a.cDegree[p], 1);
       } else {
