@@ -3044,11 +3044,13 @@ __device__ __attribute__((noinline)) void combined_build(Args &args, GridBarrier
     }
     __syncthreads();
   }
-  // With cSlot, every increment's return value is kept: a direct one's is
-  // the edge's place in its parent's row (cSlot), a workgroup's aggregated
-  // one the base of its run there (aggCount), so the fill needs no atomics
-  // on global counters.
-  int *const slotOf = a.cSlot;
+  // With cSlot (hub variant), every increment's return value is kept: a
+  // direct one's is the edge's place in its parent's row (cSlot), a
+  // workgroup's aggregated one the base of its run there (aggCount), so the
+  // fill needs no atomics on global counters.  Graphs without hubs build
+  // this CSR only in batched launches or without the mask, where it did not
+  // pay, and their variants keep their smaller frames.
+  int *const slotOf = HUBS ? a.cSlot : nullptr;
   const size_t nn = static_cast<size_t>(n);
   u64 edges = 0, wsum = 0;
   for (int v = t.id; v < n; v += t.count) {
