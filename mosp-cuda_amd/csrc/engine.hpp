@@ -249,8 +249,9 @@ struct LaunchArgs {
   int cDeltaScale;         // ... times this factor (>= 1)
   int *cDegree;            // n + 1 (counts, then fill cursors)
   int *cSlot;              // K * n, or nullptr: a combined edge's place in its parent's
-                           // row, taken by the count (slot[k * n + v] for the edge from
-                           // v's parent in tree k), so the fill needs no atomics
+                           // row, taken by the hub variant's count (slot[k * n + v] for
+                           // the edge from v's parent in tree k), so its fill needs no
+                           // atomics
   int *cRowPtr;            // n + 1 (written here, read through cDesc->outRow)
   int *cColInd;            // K * n
   int *cWeights;           // K * n
